@@ -1,0 +1,227 @@
+"""Benchmark of the PAR2 Reed-Solomon hot path on one MI355X.
+
+Default workload (plain GPU encode): one 1 GiB set of 1024 input slices of
+1 MiB (uniform random bytes, seeded), 102 recovery slices with exponents
+0..101 over GF(2^16) poly 0x1100B, plus the IFSC MD5 and CRC32 of every input
+slice.  Inputs are resident in HBM before timing starts; one step is one
+par2hip_encode_dev call (coefficients, tables, contraction, both hashes) and
+its completion.  ``--workload repair`` times the plain repair instead
+(2000 x 512 KiB, 100 erasures, 100 recovery slices, verify hashes of all
+slices + Gauss-Jordan + rebuild).
+
+Prints ONE JSON line (rank 0): metric/value/unit, ms_per_step, p50/p99 step
+latency, the dominant kernel's roofline object (HIP-event durations measured
+on the kernel's own stream inside the timed region) and the CPU baseline
+(the oracle's restatement of the reference's AVX2 strategy and hashing, timed
+on this host on a bounded sample).
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "par2-rs_amd"))
+
+import par2hip  # noqa: E402
+
+HBM_PEAK_GBS = 8000.0
+METRIC = "encode/repair GB/s source data + p50/p99 latency vs plain GPU path, bit-exact"
+
+
+def log(*a):
+    print("[bench]", *a, file=sys.stderr, flush=True)
+
+
+def random_bytes(seed: int, n: int) -> np.ndarray:
+    return np.frombuffer(np.random.default_rng(seed).bytes(n), np.uint8)
+
+
+def cpu_baseline_encode(n, s_sample, r, threads):
+    """Reference algorithm restated (AVX2 nibble-shuffle muladd with the
+    StreamingEncoder batching, plus per-slice MD5+CRC32), on n slices of
+    s_sample bytes; returns source GB/s."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle
+
+    data = random_bytes(11, n * s_sample).reshape(n, s_sample)
+    oracle.encode_avx2(data[:, :64].view(np.uint16), np.arange(r), threads=threads)  # warm
+    t0 = time.perf_counter()
+    oracle.encode_avx2(data.view(np.uint16), np.arange(r), threads=threads)
+    oracle.hash_slices_mt(data, n, s_sample, threads=threads)
+    dt = time.perf_counter() - t0
+    return n * s_sample / dt / 1e9, dt
+
+
+def bench_encode(args, ctx):
+    n, s, r = 1024, 1 << 20, 102
+    data = random_bytes(1234, n * s)
+    d_sl = ctx.alloc(n * s)
+    d_rec = ctx.alloc(r * s)
+    d_md5 = ctx.alloc(n * 16)
+    d_crc = ctx.alloc(n * 4)
+    d_sl.upload(data)
+    exps = np.arange(r, dtype=np.uint32)
+    if args.kernel:
+        ctx.set_gf_kernel(args.kernel)
+    for _ in range(args.warmup):
+        ctx.encode_dev(d_sl, n, s, exps, d_rec, d_md5, d_crc)
+    ctx.sync()
+    ctx.profile(True)
+    ctx.profile_read()
+    lat = []
+    ctx.sync()
+    t_start = time.perf_counter()
+    for _ in range(args.steps):
+        t0 = time.perf_counter()
+        ctx.encode_dev(d_sl, n, s, exps, d_rec, d_md5, d_crc)
+        ctx.sync()
+        lat.append(time.perf_counter() - t0)
+    ctx.sync()
+    total = time.perf_counter() - t_start
+    stats = ctx.profile_read()
+    ctx.profile(False)
+    # spot-check the result against the oracle on a column window (bit-exact)
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle
+
+    rec = d_rec.download().reshape(r, s).view(np.uint16)
+    w0 = 4096
+    cols = np.ascontiguousarray(data.reshape(n, s).view(np.uint16)[:, w0:w0 + 32])
+    exact = bool(np.array_equal(rec[:, w0:w0 + 32], oracle.encode(cols, exps)))
+    src_bytes = n * s
+    algo_bytes = n * s + r * s  # read every input once, write every recovery slice once
+    gf_name = "gf_muladd_mfma" if "gf_muladd_mfma" in stats else "gf_muladd_lds"
+    launches, gf_ms = stats.get(gf_name, (0, 0.0))
+    gf_avg = gf_ms / max(launches, 1)
+    md5_avg = stats.get("md5_slices", (1, 0.0))[1] / max(stats.get("md5_slices", (1, 0))[0], 1)
+    crc_avg = stats.get("crc32_slices", (1, 0.0))[1] / max(stats.get("crc32_slices", (1, 0))[0], 1)
+    achieved = algo_bytes / (gf_avg * 1e-3) / 1e9 if gf_avg > 0 else 0.0
+    return {
+        "workload": "plain_encode",
+        "config": {"workload": "plain_encode_1GiB", "n_slices": n, "slice_bytes": s, "recovery_slices": r,
+                   "exponents": "0..101", "poly": "0x1100B", "hashes": "md5+crc32 per slice",
+                   "resident": "hbm"},
+        "src_bytes": src_bytes, "total_s": total, "lat": lat, "exact": exact,
+        "kernel_ms": {k: v[1] / max(v[0], 1) for k, v in stats.items()},
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": args.traffic,
+                     "kernel": gf_name, "kernel_ms": round(gf_avg, 4), "algorithmic_bytes": algo_bytes,
+                     "gf_muladds": n * r * (s // 2), "md5_ms": round(md5_avg, 4), "crc32_ms": round(crc_avg, 4)},
+        "cpu": (n, 128 << 10, r),
+    }
+
+
+def bench_repair(args, ctx):
+    n, s, m = 2000, 512 << 10, 100
+    data = random_bytes(4321, n * s).reshape(n, s)
+    exps = np.arange(m, dtype=np.uint32)
+    rec = ctx.encode(data, exps)
+    rng = np.random.default_rng(77)
+    missing = np.sort(rng.choice(n, m, replace=False)).astype(np.uint32)
+    damaged = data.copy()
+    damaged[missing] = 0
+    d_sl = ctx.alloc(n * s)
+    d_rec = ctx.alloc(m * s)
+    d_out = ctx.alloc(m * s)
+    d_md5 = ctx.alloc(n * 16)
+    d_crc = ctx.alloc(n * 4)
+    d_sl.upload(damaged)
+    d_rec.upload(rec)
+    if args.kernel:
+        ctx.set_gf_kernel(args.kernel)
+
+    def step():
+        ctx.hash_slices_dev(d_sl, n, s, d_md5, d_crc)
+        ctx.reconstruct_dev(d_sl, n, s, missing, d_rec, exps, d_out)
+
+    for _ in range(args.warmup):
+        step()
+    ctx.sync()
+    ctx.profile(True)
+    ctx.profile_read()
+    lat = []
+    t_start = time.perf_counter()
+    for _ in range(args.steps):
+        t0 = time.perf_counter()
+        step()
+        ctx.sync()
+        lat.append(time.perf_counter() - t0)
+    total = time.perf_counter() - t_start
+    stats = ctx.profile_read()
+    ctx.profile(False)
+    out = d_out.download().reshape(m, s)
+    exact = bool(np.array_equal(out, data[missing]))
+    gf_name = "gf_muladd_mfma" if "gf_muladd_mfma" in stats else "gf_muladd_lds"
+    launches, gf_ms = stats.get(gf_name, (0, 0.0))
+    gf_avg = gf_ms / max(launches, 1)
+    algo_bytes = n * s + m * s + m * s
+    achieved = algo_bytes / (gf_avg * 1e-3) / 1e9 if gf_avg > 0 else 0.0
+    return {
+        "workload": "plain_repair",
+        "config": {"workload": "plain_repair_1GiB", "n_slices": n, "slice_bytes": s, "erasures": m,
+                   "recovery_slices": m, "resident": "hbm", "verify": "md5+crc32 of all slices"},
+        "src_bytes": n * s, "total_s": total, "lat": lat, "exact": exact,
+        "kernel_ms": {k: v[1] / max(v[0], 1) for k, v in stats.items()},
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": args.traffic,
+                     "kernel": gf_name, "kernel_ms": round(gf_avg, 4), "algorithmic_bytes": algo_bytes,
+                     "gf_muladds": m * n * (s // 2)},
+        "cpu": (n, 32 << 10, m),
+    }
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--workload", choices=["encode", "repair"], default="encode")
+    ap.add_argument("--kernel", type=int, default=0, help="0 auto, 1 LDS tables, 2 MFMA")
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--traffic", type=float, default=None,
+                    help="HBM bytes per GF launch measured with rocprofv3 --pmc (FETCH_SIZE+WRITE_SIZE)")
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py: this build targets one GPU; multi-GPU runs are not part of it "
+              "(the column split that would shard it is described in DESIGN.md)", file=sys.stderr)
+        sys.exit(2)
+    if par2hip.available() <= 0:
+        print("bench.py: no gfx950 device visible to libpar2hip.so", file=sys.stderr)
+        sys.exit(1)
+    ctx = par2hip.Context(0)
+    log("workload", args.workload, "steps", args.steps, "warmup", args.warmup)
+    res = bench_encode(args, ctx) if args.workload == "encode" else bench_repair(args, ctx)
+    lat = np.array(res["lat"]) * 1e3
+    ms = res["total_s"] / args.steps * 1e3
+    value = res["src_bytes"] / (ms * 1e-3) / 1e9
+    out = {
+        "metric": METRIC, "value": round(value, 3), "unit": "GB/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": round(ms, 4), "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "u16", "data": "synthetic (seeded uniform random bytes)",
+        "config": res["config"],
+        "p50_ms": round(float(np.percentile(lat, 50)), 4), "p99_ms": round(float(np.percentile(lat, 99)), 4),
+        "bit_exact_sample": res["exact"],
+        "kernel_ms": {k: round(v, 4) for k, v in res["kernel_ms"].items()},
+        "roofline": res["roofline"],
+    }
+    if not args.no_cpu:
+        threads = min(16, os.cpu_count() or 1)
+        n, ss, r = res["cpu"]
+        log("cpu baseline", n, ss, r, "threads", threads)
+        gbs, dt = cpu_baseline_encode(n, ss, r, threads) if args.workload == "encode" else \
+            cpu_baseline_encode(n, ss, r, threads)
+        out["cpu_baseline"] = {"value": round(gbs, 4), "unit": "GB/s", "cores": threads, "kind": "port",
+                               "sample": f"{n} slices x {ss} B, R={r}: AVX2 nibble-shuffle encode "
+                                         f"(StreamingEncoder batching) + MD5/CRC32 per slice, {dt:.2f} s"}
+    print(json.dumps(out), flush=True)
+    ctx.close()
+
+
+if __name__ == "__main__":
+    main()

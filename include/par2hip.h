@@ -1,0 +1,178 @@
+/*
+ * par2hip.h -- C ABI of the MI355X (gfx950) Reed-Solomon + slice-hash library.
+ *
+ * Drop-in boundary for par2-rs 0.3.1.  Two levels (paths relative to the
+ * reference crate root):
+ *   1. strategy-shaped entry points, one per method of the plugin trait
+ *      `GaloisSimdStrategy` (src/galois/simd/mod.rs:24-128), so a GPU strategy
+ *      can be registered in `x86::register_strategies`
+ *      (src/galois/simd/x86/mod.rs:13-26);
+ *   2. coarse whole-set entry points replacing the encoder
+ *      (`StreamingEncoder::{process_block,flush_batch,finalize}`,
+ *      src/par2_rs.rs:785-902), the repair transform and chunk loop
+ *      (`compute_reconstruction_transform`, src/par2_rs.rs:319-422;
+ *      `reconstruct_streaming_chunk`, src/par2_rs.rs:442-675) and the IFSC slice
+ *      hashing (src/creator.rs:397-409, src/verify.rs:548-654).
+ *
+ * Conventions
+ *   - every call returns int32_t: PAR2HIP_OK (0) or a negative PAR2HIP_E*;
+ *     nothing throws or aborts across the boundary (the reference builds with
+ *     panic="abort", Cargo.toml:46, so failures must be values);
+ *   - one par2hip_ctx per device; a ctx owns one HIP stream and serialises
+ *     calls made on it (calls from several threads on one ctx are safe);
+ *   - "host" buffers are borrowed for the duration of a synchronous call;
+ *     "_dev" entry points take device pointers obtained from par2hip_dev_alloc
+ *     and are asynchronous on the ctx stream until par2hip_sync;
+ *   - GF data are little-endian interleaved u16 words (the reference's
+ *     non-shuffle2x layout, supports_shuffle2x() == false); lengths of GF
+ *     operands are in words, lengths of hashed data in bytes;
+ *   - coefficient matrices are row-major [dst][src].
+ */
+#ifndef PAR2HIP_H
+#define PAR2HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+    PAR2HIP_OK = 0,
+    PAR2HIP_EINVAL = -1,    /* bad argument (sizes, null pointers, odd slice size) */
+    PAR2HIP_ENODEV = -2,    /* no usable gfx950 device */
+    PAR2HIP_ENOMEM = -3,    /* device allocation failed */
+    PAR2HIP_ESINGULAR = -4, /* erasure matrix singular ("Singular matrix", par2_rs.rs:369) */
+    PAR2HIP_EHIP = -5,      /* HIP runtime error; see par2hip_last_error */
+    PAR2HIP_ERECOVERY = -6  /* fewer recovery slices than erasures (par2_rs.rs:332-338) */
+};
+
+/* Strategy priority, same scale as `Priority` (src/galois/simd/mod.rs:15-21). */
+enum { PAR2HIP_PRIORITY_OPTIMAL = 40 };
+
+typedef struct par2hip_ctx par2hip_ctx;
+
+/* ---------------------------------------------------------------- device -- */
+
+/* GaloisSimdStrategy::is_available (simd/mod.rs:29): number of gfx950
+ * devices (0 if none).  Cheap after the first call. */
+int32_t par2hip_available(void);
+/* GaloisSimdStrategy::name / priority (simd/mod.rs:26, :32). */
+const char *par2hip_strategy_name(void);
+int32_t par2hip_strategy_priority(void);
+
+int32_t par2hip_ctx_create(int32_t device, par2hip_ctx **out);
+int32_t par2hip_ctx_destroy(par2hip_ctx *ctx);
+/* Text of the last error on this ctx (valid until the next call on it). */
+const char *par2hip_last_error(par2hip_ctx *ctx);
+
+/* Device memory owned by the ctx's device (serving keeps sets resident). */
+int32_t par2hip_dev_alloc(par2hip_ctx *ctx, size_t bytes, void **dptr);
+int32_t par2hip_dev_free(par2hip_ctx *ctx, void *dptr);
+int32_t par2hip_h2d(par2hip_ctx *ctx, void *dst, const void *src, size_t bytes);
+int32_t par2hip_d2h(par2hip_ctx *ctx, void *dst, const void *src, size_t bytes);
+int32_t par2hip_sync(par2hip_ctx *ctx);
+
+/* --------------------------------------------- strategy-shaped (host) -- */
+
+/* GaloisSimdStrategy::mul_slice (simd/mod.rs:38): data[i] = data[i]*scalar;
+ * scalar 0 zeroes, 1 is a no-op (src/galois/scalar.rs:9-17). */
+int32_t par2hip_gf_mul_slice(par2hip_ctx *ctx, uint16_t scalar, uint16_t *data, size_t len);
+
+/* GaloisSimdStrategy::muladd (simd/mod.rs:44): dst[i] ^= src[i]*scalar for
+ * i < len; scalar 0 is a no-op (src/galois/scalar.rs:21-28). */
+int32_t par2hip_gf_muladd(par2hip_ctx *ctx, uint16_t *dst, const uint16_t *src, size_t len,
+                          uint16_t scalar);
+
+/* GaloisSimdStrategy::muladd_region (simd/mod.rs:50-57): for every dst i and
+ * src j < nsrc: dst_i[off..off+len) ^= coeff_rows[i][j] * src_j[off..off+len)
+ * (src/galois/scalar.rs:82-113).  coeff_rows[i] points at >= nsrc values. */
+int32_t par2hip_gf_muladd_region(par2hip_ctx *ctx, uint16_t *const *dsts, size_t ndst,
+                                 const uint16_t *const *srcs, size_t nsrc,
+                                 const uint16_t *const *coeff_rows, size_t region_offset,
+                                 size_t region_size);
+
+/* ------------------------------------------------------------ encode -- */
+
+/* Replaces StreamingEncoder (src/par2_rs.rs:687-902) for a whole set.
+ * slices:   n input slices of slice_size bytes each, contiguous (slice d at
+ *           slices + d*slice_size), already zero-padded (creator.rs:393-395).
+ * exps:     r recovery exponents (the reference uses 0..r-1, par2_rs.rs:894-900).
+ * recovery: r*slice_size bytes out; recovery e = XOR_d c_d^exps[e] * slice_d,
+ *           c_d the PAR2 constants (par2_rs.rs:64-76).
+ * md5_out / crc_out (nullable): IFSC MD5 (16 B) and CRC32 of every input
+ *           slice (creator.rs:397-409).
+ * slice_size must be even and non-zero; n <= 32768 (distinct constants). */
+int32_t par2hip_encode(par2hip_ctx *ctx, const uint8_t *slices, size_t n, size_t slice_size,
+                       const uint32_t *exps, size_t r, uint8_t *recovery, uint8_t *md5_out,
+                       uint32_t *crc_out);
+
+/* Same, device-resident and asynchronous on the ctx stream. d_md5 / d_crc
+ * may be NULL. */
+int32_t par2hip_encode_dev(par2hip_ctx *ctx, const void *d_slices, size_t n, size_t slice_size,
+                           const uint32_t *exps, size_t r, void *d_recovery, void *d_md5,
+                           void *d_crc);
+
+/* ------------------------------------------------------------ repair -- */
+
+/* Gauss-Jordan inverse of an m x m GF(2^16) matrix on the device
+ * (replaces the O(m^3) elimination of par2_rs.rs:358-412).  Host buffers.
+ * Returns PAR2HIP_ESINGULAR if no pivot exists in some column. */
+int32_t par2hip_gf_invert(par2hip_ctx *ctx, const uint16_t *a, size_t m, uint16_t *inv);
+
+/* Rebuild the m missing input slices of an n-slice set (replaces
+ * compute_reconstruction_transform + reconstruct_streaming_chunk over all
+ * chunks, repair.rs:848-970).
+ * slices:     n*slice_size bytes; the missing slices' contents are ignored.
+ * missing:    m distinct global slice indices (any order).
+ * recovery:   m recovery slices (m*slice_size bytes) with exponents rec_exps.
+ * out:        m*slice_size bytes, row i = rebuilt slice missing[i].
+ * Errors: PAR2HIP_ESINGULAR, PAR2HIP_EINVAL.                                  */
+int32_t par2hip_reconstruct(par2hip_ctx *ctx, const uint8_t *slices, size_t n, size_t slice_size,
+                            const uint32_t *missing, size_t m, const uint8_t *recovery,
+                            const uint32_t *rec_exps, uint8_t *out);
+
+/* Device-resident form (asynchronous; d_out may alias the missing slices'
+ * rows of d_slices only if out_in_place is non-zero, in which case d_out is
+ * ignored and rebuilt slices are written into d_slices). */
+int32_t par2hip_reconstruct_dev(par2hip_ctx *ctx, void *d_slices, size_t n, size_t slice_size,
+                                const uint32_t *missing, size_t m, const void *d_recovery,
+                                const uint32_t *rec_exps, void *d_out, int32_t out_in_place);
+
+/* -------------------------------------------------------------- hashes -- */
+
+/* IFSC MD5 + CRC32 of n slices, each zero-padded to slice_size
+ * (creator.rs:389-409; verify.rs:592-595).  Slice i is read from
+ * slices + i*slice_stride for lens[i] bytes (lens NULL: slice_size each).
+ * md5_out: n*16 bytes (nullable); crc_out: n u32 (nullable). */
+int32_t par2hip_hash_slices(par2hip_ctx *ctx, const uint8_t *slices, size_t n, size_t slice_size,
+                            size_t slice_stride, const uint64_t *lens, uint8_t *md5_out,
+                            uint32_t *crc_out);
+int32_t par2hip_hash_slices_dev(par2hip_ctx *ctx, const void *d_slices, size_t n,
+                                size_t slice_size, size_t slice_stride, const uint64_t *lens,
+                                void *d_md5, void *d_crc);
+
+/* ------------------------------------------------------------- tuning -- */
+
+/* GF contraction kernel selection: 0 = automatic (the measured winner),
+ * 1 = LDS nibble tables, 2 = bit-sliced MFMA.  Returns the previous value. */
+int32_t par2hip_set_gf_kernel(par2hip_ctx *ctx, int32_t which);
+
+/* Per-kernel device timings, recorded with HIP events on the stream each
+ * kernel runs on, while enabled. */
+typedef struct {
+    char name[48];
+    uint64_t launches;
+    double total_ms;
+} par2hip_kernel_stat;
+int32_t par2hip_profile_enable(par2hip_ctx *ctx, int32_t on);
+/* Copies up to max entries; *count receives the number available.  Reading
+ * synchronises the ctx stream and resets the counters. */
+int32_t par2hip_profile_read(par2hip_ctx *ctx, par2hip_kernel_stat *out, size_t max,
+                             size_t *count);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* PAR2HIP_H */

@@ -1,0 +1,169 @@
+// common.hpp -- shared internals of libpar2hip: context, errors, GF(2^16)
+// host tables and the launch helpers every kernel file uses.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+#include <cstdint>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "../../include/par2hip.h"
+
+namespace par2hip {
+
+// GF(2^16) with the PAR2 polynomial x^16+x^12+x^3+x+1 and generator 2
+// (src/galois/core.rs:12, :49-91).
+constexpr uint32_t kPoly = 0x1100Bu;
+constexpr uint32_t kOrder = 65535u;
+
+struct HostField {
+  uint16_t log[65536];
+  uint16_t exp[2 * kOrder];  // doubled: exp[a+b] needs no reduction
+  HostField();
+  uint16_t mul(uint16_t a, uint16_t b) const {
+    return (a && b) ? exp[uint32_t(log[a]) + log[b]] : 0;
+  }
+};
+const HostField &field();
+
+// Exponent n_d of the d-th PAR2 input constant c_d = 2^n_d: the d-th n >= 1
+// with n mod 3, 5, 17, 257 all non-zero (src/par2_rs.rs:64-76).  Because the
+// generator is 2, log(c_d) = n_d, so coefficients are exp[(n_d*e) mod 65535].
+std::vector<uint32_t> constant_logs(size_t count);
+
+struct KernelStat {
+  std::string name;
+  uint64_t launches = 0;
+  double total_ms = 0;
+};
+
+struct PendingTiming {
+  int stat;
+  hipEvent_t start, stop;
+};
+
+// A growable pair of (pinned host, device) buffers used to ship small
+// per-call parameter blocks (pointer tables, exponent lists).
+struct ParamArena {
+  uint8_t *host = nullptr;
+  uint8_t *dev = nullptr;
+  size_t cap = 0;
+  size_t used = 0;
+  hipEvent_t copied = nullptr;  // the last H2D out of `host` has finished
+};
+
+struct DevBuf {
+  void *ptr = nullptr;
+  size_t bytes = 0;
+};
+
+}  // namespace par2hip
+
+struct par2hip_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;       // everything is ordered on this stream
+  hipStream_t side = nullptr;         // hashing overlaps here, joined per call
+  hipEvent_t fork = nullptr, join = nullptr;
+  std::recursive_mutex mu;
+  std::string err;
+  uint16_t *d_exp = nullptr;          // device copy of the doubled exp table
+  uint16_t *d_log = nullptr;          // device copy of the log table
+  uint32_t *d_crc_tab = nullptr;      // CRC-32 byte table + shift powers
+  int *d_status = nullptr;            // device status word (singular flag)
+  par2hip::ParamArena arena;
+  par2hip::DevBuf scratch[8];         // grow-only device scratch, by role
+  int gf_kernel = 0;                  // 0 auto, 1 LDS tables, 2 MFMA
+  bool profile = false;
+  std::vector<par2hip::KernelStat> stats;
+  std::vector<par2hip::PendingTiming> pending;
+  std::vector<hipEvent_t> event_pool;
+};
+
+namespace par2hip {
+
+enum ScratchRole {
+  kScratchCoeff = 0,   // coefficient matrix
+  kScratchTables = 1,  // per-coefficient lookup tables / MFMA operands
+  kScratchGj = 2,      // Gauss-Jordan workspace
+  kScratchIn = 3,      // host-call staging: inputs
+  kScratchOut = 4,     // host-call staging: outputs
+  kScratchHash = 5,    // host-call staging: digests
+  kScratchAux = 6,
+  kScratchAux2 = 7,
+};
+
+int32_t fail(par2hip_ctx *ctx, int32_t code, const std::string &msg);
+int32_t hip_fail(par2hip_ctx *ctx, hipError_t e, const char *what);
+#define P2H_CHECK(ctx, expr)                                     \
+  do {                                                           \
+    hipError_t e_ = (expr);                                      \
+    if (e_ != hipSuccess) return ::par2hip::hip_fail(ctx, e_, #expr); \
+  } while (0)
+
+// Device scratch of at least `bytes` for a role (contents not preserved on growth).
+int32_t scratch(par2hip_ctx *ctx, int role, size_t bytes, void **out);
+
+// Parameter blocks: begin() waits for the previous upload, alloc() carves
+// 256-byte aligned pieces (returns host and device addresses), commit()
+// uploads everything on the ctx stream.
+int32_t params_begin(par2hip_ctx *ctx, size_t bytes);
+void *params_alloc(par2hip_ctx *ctx, size_t bytes, void **dev);
+int32_t params_commit(par2hip_ctx *ctx);
+
+// Kernel timing (only when profiling is enabled).
+int timing_start(par2hip_ctx *ctx, const char *name, hipStream_t s, hipEvent_t *stop);
+void timing_stop(par2hip_ctx *ctx, int slot, hipStream_t s, hipEvent_t stop);
+
+struct ScopedTiming {
+  par2hip_ctx *ctx;
+  hipStream_t s;
+  int slot;
+  hipEvent_t stop = nullptr;
+  ScopedTiming(par2hip_ctx *c, const char *name, hipStream_t st) : ctx(c), s(st) {
+    slot = timing_start(c, name, st, &stop);
+  }
+  ~ScopedTiming() { timing_stop(ctx, slot, s, stop); }
+};
+
+// ---- kernels (device code in gf_kernels.hip, hash_kernels.hip, gj_kernels.hip)
+
+// coeff[r][d] = exp[(src_log[d] * exps[r]) mod 65535]; ld = nsrc.
+int32_t launch_coeff_from_logs(par2hip_ctx *ctx, hipStream_t s, uint16_t *coeff,
+                               const uint32_t *d_src_log, const uint32_t *d_exps, int nrow,
+                               int nsrc);
+
+// GF contraction: dst_i (=|^=) XOR_j coeff[i][j] * src_j over `words` words.
+struct GfProblem {
+  const uint16_t *const *srcs;  // device array of nsrc device pointers
+  uint16_t *const *dsts;        // device array of ndst device pointers
+  const uint16_t *coeff;        // device [ndst][nsrc]
+  int nsrc, ndst;
+  size_t words;
+  bool accumulate;
+  int align_words;              // every pointer and `words` stride aligned to this many words
+};
+int32_t run_gf(par2hip_ctx *ctx, hipStream_t s, const GfProblem &p);
+
+// Slice hashes.
+int32_t launch_md5(par2hip_ctx *ctx, hipStream_t s, const uint8_t *d_data, size_t n,
+                   size_t slice_size, size_t stride, const uint64_t *d_lens, uint32_t *d_md5);
+int32_t launch_crc32(par2hip_ctx *ctx, hipStream_t s, const uint8_t *d_data, size_t n,
+                     size_t slice_size, size_t stride, const uint64_t *d_lens, uint32_t *d_crc);
+int32_t crc_tables_init(par2hip_ctx *ctx);
+
+// Gauss-Jordan on an m x 2m workspace [A | I] (device, row-major u16);
+// writes 0 or 1 (singular) into *d_status.
+int32_t launch_gauss_jordan(par2hip_ctx *ctx, hipStream_t s, uint16_t *d_aug, int m,
+                            int *d_status);
+// A[i][k] = exp[(miss_log[k]*exps[i]) mod 65535] into the left half of aug, I on the right.
+int32_t launch_build_aug(par2hip_ctx *ctx, hipStream_t s, uint16_t *d_aug, const uint32_t *d_miss_log,
+                         const uint32_t *d_exps, int m);
+// coeff[k][j]   = XOR_i inv[k][i] * c_{present_j}^{e_i}   for j < npresent,
+// coeff[k][np+i] = inv[k][i];  inv is the right half of aug.
+int32_t launch_fold(par2hip_ctx *ctx, hipStream_t s, uint16_t *coeff, const uint16_t *d_aug,
+                    const uint32_t *d_present_log, const uint32_t *d_exps, int m, int npresent);
+
+}  // namespace par2hip

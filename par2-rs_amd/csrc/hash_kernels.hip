@@ -1,0 +1,351 @@
+// hash_kernels.hip -- batched per-slice IFSC hashes for gfx950.
+//
+// The reference hashes every input slice, zero-padded to the slice size, with
+// MD5 (md-5 0.10.6) and CRC-32/IEEE (crc32fast 1.5.0): src/creator.rs:397-409
+// on create, src/verify.rs:548-654 on verify.  Both kernels read slice i at
+// data + i*stride; bytes at or beyond lens[i] read as zero (lens == nullptr:
+// every slice is full), exactly the reference's zero padding.
+//
+// MD5 is a serial chain per slice, so one lane owns one slice (64 slices per
+// wavefront); its latency per 64-byte block bounds the kernel.  CRC-32 is
+// linear, so one wavefront owns one slice: each lane folds a contiguous
+// chunk with slice-by-4 tables held in LDS, and the 64 partial remainders are
+// combined by multiplying with x^(8*bytes_after) mod P.
+#include "common.hpp"
+
+namespace par2hip {
+
+// ---------------------------------------------------------------- MD5 -----
+// RFC 1321 section 3.4 constants: T[i] = floor(2^32 * |sin(i + 1)|).
+__constant__ uint32_t kMd5T[64] = {
+    0xd76aa478, 0xe8c7b756, 0x242070db, 0xc1bdceee, 0xf57c0faf, 0x4787c62a, 0xa8304613, 0xfd469501,
+    0x698098d8, 0x8b44f7af, 0xffff5bb1, 0x895cd7be, 0x6b901122, 0xfd987193, 0xa679438e, 0x49b40821,
+    0xf61e2562, 0xc040b340, 0x265e5a51, 0xe9b6c7aa, 0xd62f105d, 0x02441453, 0xd8a1e681, 0xe7d3fbc8,
+    0x21e1cde6, 0xc33707d6, 0xf4d50d87, 0x455a14ed, 0xa9e3e905, 0xfcefa3f8, 0x676f02d9, 0x8d2a4c8a,
+    0xfffa3942, 0x8771f681, 0x6d9d6122, 0xfde5380c, 0xa4beea44, 0x4bdecfa9, 0xf6bb4b60, 0xbebfbc70,
+    0x289b7ec6, 0xeaa127fa, 0xd4ef3085, 0x04881d05, 0xd9d4d039, 0xe6db99e5, 0x1fa27cf8, 0xc4ac5665,
+    0xf4292244, 0x432aff97, 0xab9423a7, 0xfc93a039, 0x655b59c3, 0x8f0ccc92, 0xffeff47d, 0x85845dd1,
+    0x6fa87e4f, 0xfe2ce6e0, 0xa3014314, 0x4e0811a1, 0xf7537e82, 0xbd3af235, 0x2ad7d2bb, 0xeb86d391};
+
+__device__ __forceinline__ uint32_t rotl(uint32_t x, int s) { return __builtin_amdgcn_alignbit(x, x, 32 - s); }
+
+#define MD5_STEP(f, a, b, c, d, m, t, s) \
+  a = b + rotl(a + (f) + (m) + (t), s);
+
+__device__ __forceinline__ void md5_compress(uint32_t st[4], const uint32_t M[16]) {
+  uint32_t a = st[0], b = st[1], c = st[2], d = st[3];
+#define F1(b, c, d) (((b) & (c)) | (~(b) & (d)))
+#define F2(b, c, d) (((b) & (d)) | ((c) & ~(d)))
+#define F3(b, c, d) ((b) ^ (c) ^ (d))
+#define F4(b, c, d) ((c) ^ ((b) | ~(d)))
+  MD5_STEP(F1(b, c, d), a, b, c, d, M[0], kMd5T[0], 7)
+  MD5_STEP(F1(a, b, c), d, a, b, c, M[1], kMd5T[1], 12)
+  MD5_STEP(F1(d, a, b), c, d, a, b, M[2], kMd5T[2], 17)
+  MD5_STEP(F1(c, d, a), b, c, d, a, M[3], kMd5T[3], 22)
+  MD5_STEP(F1(b, c, d), a, b, c, d, M[4], kMd5T[4], 7)
+  MD5_STEP(F1(a, b, c), d, a, b, c, M[5], kMd5T[5], 12)
+  MD5_STEP(F1(d, a, b), c, d, a, b, M[6], kMd5T[6], 17)
+  MD5_STEP(F1(c, d, a), b, c, d, a, M[7], kMd5T[7], 22)
+  MD5_STEP(F1(b, c, d), a, b, c, d, M[8], kMd5T[8], 7)
+  MD5_STEP(F1(a, b, c), d, a, b, c, M[9], kMd5T[9], 12)
+  MD5_STEP(F1(d, a, b), c, d, a, b, M[10], kMd5T[10], 17)
+  MD5_STEP(F1(c, d, a), b, c, d, a, M[11], kMd5T[11], 22)
+  MD5_STEP(F1(b, c, d), a, b, c, d, M[12], kMd5T[12], 7)
+  MD5_STEP(F1(a, b, c), d, a, b, c, M[13], kMd5T[13], 12)
+  MD5_STEP(F1(d, a, b), c, d, a, b, M[14], kMd5T[14], 17)
+  MD5_STEP(F1(c, d, a), b, c, d, a, M[15], kMd5T[15], 22)
+
+  MD5_STEP(F2(b, c, d), a, b, c, d, M[1], kMd5T[16], 5)
+  MD5_STEP(F2(a, b, c), d, a, b, c, M[6], kMd5T[17], 9)
+  MD5_STEP(F2(d, a, b), c, d, a, b, M[11], kMd5T[18], 14)
+  MD5_STEP(F2(c, d, a), b, c, d, a, M[0], kMd5T[19], 20)
+  MD5_STEP(F2(b, c, d), a, b, c, d, M[5], kMd5T[20], 5)
+  MD5_STEP(F2(a, b, c), d, a, b, c, M[10], kMd5T[21], 9)
+  MD5_STEP(F2(d, a, b), c, d, a, b, M[15], kMd5T[22], 14)
+  MD5_STEP(F2(c, d, a), b, c, d, a, M[4], kMd5T[23], 20)
+  MD5_STEP(F2(b, c, d), a, b, c, d, M[9], kMd5T[24], 5)
+  MD5_STEP(F2(a, b, c), d, a, b, c, M[14], kMd5T[25], 9)
+  MD5_STEP(F2(d, a, b), c, d, a, b, M[3], kMd5T[26], 14)
+  MD5_STEP(F2(c, d, a), b, c, d, a, M[8], kMd5T[27], 20)
+  MD5_STEP(F2(b, c, d), a, b, c, d, M[13], kMd5T[28], 5)
+  MD5_STEP(F2(a, b, c), d, a, b, c, M[2], kMd5T[29], 9)
+  MD5_STEP(F2(d, a, b), c, d, a, b, M[7], kMd5T[30], 14)
+  MD5_STEP(F2(c, d, a), b, c, d, a, M[12], kMd5T[31], 20)
+
+  MD5_STEP(F3(b, c, d), a, b, c, d, M[5], kMd5T[32], 4)
+  MD5_STEP(F3(a, b, c), d, a, b, c, M[8], kMd5T[33], 11)
+  MD5_STEP(F3(d, a, b), c, d, a, b, M[11], kMd5T[34], 16)
+  MD5_STEP(F3(c, d, a), b, c, d, a, M[14], kMd5T[35], 23)
+  MD5_STEP(F3(b, c, d), a, b, c, d, M[1], kMd5T[36], 4)
+  MD5_STEP(F3(a, b, c), d, a, b, c, M[4], kMd5T[37], 11)
+  MD5_STEP(F3(d, a, b), c, d, a, b, M[7], kMd5T[38], 16)
+  MD5_STEP(F3(c, d, a), b, c, d, a, M[10], kMd5T[39], 23)
+  MD5_STEP(F3(b, c, d), a, b, c, d, M[13], kMd5T[40], 4)
+  MD5_STEP(F3(a, b, c), d, a, b, c, M[0], kMd5T[41], 11)
+  MD5_STEP(F3(d, a, b), c, d, a, b, M[3], kMd5T[42], 16)
+  MD5_STEP(F3(c, d, a), b, c, d, a, M[6], kMd5T[43], 23)
+  MD5_STEP(F3(b, c, d), a, b, c, d, M[9], kMd5T[44], 4)
+  MD5_STEP(F3(a, b, c), d, a, b, c, M[12], kMd5T[45], 11)
+  MD5_STEP(F3(d, a, b), c, d, a, b, M[15], kMd5T[46], 16)
+  MD5_STEP(F3(c, d, a), b, c, d, a, M[2], kMd5T[47], 23)
+
+  MD5_STEP(F4(b, c, d), a, b, c, d, M[0], kMd5T[48], 6)
+  MD5_STEP(F4(a, b, c), d, a, b, c, M[7], kMd5T[49], 10)
+  MD5_STEP(F4(d, a, b), c, d, a, b, M[14], kMd5T[50], 15)
+  MD5_STEP(F4(c, d, a), b, c, d, a, M[5], kMd5T[51], 21)
+  MD5_STEP(F4(b, c, d), a, b, c, d, M[12], kMd5T[52], 6)
+  MD5_STEP(F4(a, b, c), d, a, b, c, M[3], kMd5T[53], 10)
+  MD5_STEP(F4(d, a, b), c, d, a, b, M[10], kMd5T[54], 15)
+  MD5_STEP(F4(c, d, a), b, c, d, a, M[1], kMd5T[55], 21)
+  MD5_STEP(F4(b, c, d), a, b, c, d, M[8], kMd5T[56], 6)
+  MD5_STEP(F4(a, b, c), d, a, b, c, M[15], kMd5T[57], 10)
+  MD5_STEP(F4(d, a, b), c, d, a, b, M[6], kMd5T[58], 15)
+  MD5_STEP(F4(c, d, a), b, c, d, a, M[13], kMd5T[59], 21)
+  MD5_STEP(F4(b, c, d), a, b, c, d, M[4], kMd5T[60], 6)
+  MD5_STEP(F4(a, b, c), d, a, b, c, M[11], kMd5T[61], 10)
+  MD5_STEP(F4(d, a, b), c, d, a, b, M[2], kMd5T[62], 15)
+  MD5_STEP(F4(c, d, a), b, c, d, a, M[9], kMd5T[63], 21)
+#undef F1
+#undef F2
+#undef F3
+#undef F4
+  st[0] += a;
+  st[1] += b;
+  st[2] += c;
+  st[3] += d;
+}
+
+// Bytes [pos, pos+64) of a slice whose real length is len, zeros beyond.
+__device__ __forceinline__ void md5_load_partial(const uint8_t *__restrict__ p, uint64_t pos, uint64_t len,
+                                                 uint32_t M[16]) {
+#pragma unroll
+  for (int k = 0; k < 16; k++) {
+    uint32_t w = 0;
+    for (int b = 0; b < 4; b++) {
+      const uint64_t o = pos + 4 * k + b;
+      if (o < len) w |= (uint32_t)p[o] << (8 * b);
+    }
+    M[k] = w;
+  }
+}
+
+// One lane per slice.  kAlign: 16 or 4 = every slice start has that byte
+// alignment; 1 = unaligned (byte loads).
+template <int kAlign>
+__global__ __launch_bounds__(64) void k_md5_slices(const uint8_t *__restrict__ data, long n,
+                                                   uint64_t slice_size, uint64_t stride,
+                                                   const uint64_t *__restrict__ lens,
+                                                   uint32_t *__restrict__ out) {
+  const long i = (long)blockIdx.x * 64 + threadIdx.x;
+  if (i >= n) return;
+  const uint8_t *p = data + (uint64_t)i * stride;
+  uint64_t len = lens ? lens[i] : slice_size;
+  if (len > slice_size) len = slice_size;
+  uint32_t st[4] = {0x67452301u, 0xefcdab89u, 0x98badcfeu, 0x10325476u};
+  uint32_t M[16];
+  const uint64_t full = len & ~(uint64_t)63;  // blocks made entirely of real bytes
+  uint64_t pos = 0;
+  if (kAlign == 16) {
+    // software-pipelined: the next block's loads are in flight during this compress
+    if (full) {
+      const uint4 *q = reinterpret_cast<const uint4 *>(p);
+      uint4 n0 = q[0], n1 = q[1], n2 = q[2], n3 = q[3];
+      for (pos = 0; pos < full; pos += 64) {
+        M[0] = n0.x; M[1] = n0.y; M[2] = n0.z; M[3] = n0.w;
+        M[4] = n1.x; M[5] = n1.y; M[6] = n1.z; M[7] = n1.w;
+        M[8] = n2.x; M[9] = n2.y; M[10] = n2.z; M[11] = n2.w;
+        M[12] = n3.x; M[13] = n3.y; M[14] = n3.z; M[15] = n3.w;
+        if (pos + 64 < full) {
+          const uint4 *r = reinterpret_cast<const uint4 *>(p + pos + 64);
+          n0 = r[0]; n1 = r[1]; n2 = r[2]; n3 = r[3];
+        }
+        md5_compress(st, M);
+      }
+    }
+  } else if (kAlign == 4) {
+    for (pos = 0; pos < full; pos += 64) {
+      const uint32_t *q = reinterpret_cast<const uint32_t *>(p + pos);
+#pragma unroll
+      for (int k = 0; k < 16; k++) M[k] = q[k];
+      md5_compress(st, M);
+    }
+  } else {
+    for (pos = 0; pos < full; pos += 64) {
+#pragma unroll
+      for (int k = 0; k < 16; k++)
+        M[k] = (uint32_t)p[pos + 4 * k] | ((uint32_t)p[pos + 4 * k + 1] << 8) |
+               ((uint32_t)p[pos + 4 * k + 2] << 16) | ((uint32_t)p[pos + 4 * k + 3] << 24);
+      md5_compress(st, M);
+    }
+  }
+  // the block holding the end of the real bytes (if any)
+  const uint64_t total = slice_size;
+  if (pos < len) {
+    md5_load_partial(p, pos, len, M);
+    if (pos + 64 <= total) {
+      md5_compress(st, M);
+      pos += 64;
+    }
+  }
+  // zero blocks of the padding that are followed by at least one more block
+  if (pos + 64 <= total) {
+#pragma unroll
+    for (int k = 0; k < 16; k++) M[k] = 0;
+    for (; pos + 64 <= total; pos += 64) md5_compress(st, M);
+  }
+  // final block(s): remaining real/zero bytes, 0x80, zeros, 64-bit bit length
+  {
+    const uint32_t rem = (uint32_t)(total - pos);  // < 64
+    if (!(pos < len)) {
+#pragma unroll
+      for (int k = 0; k < 16; k++) M[k] = 0;
+    }
+    // M already holds bytes [pos, pos+rem) when pos < len (partial load above)
+    M[rem >> 2] |= 0x80u << (8 * (rem & 3));
+    if (rem >= 56) {
+      md5_compress(st, M);
+#pragma unroll
+      for (int k = 0; k < 16; k++) M[k] = 0;
+    }
+    const uint64_t bits = total * 8u;
+    M[14] = (uint32_t)bits;
+    M[15] = (uint32_t)(bits >> 32);
+    md5_compress(st, M);
+  }
+  uint4 o = make_uint4(st[0], st[1], st[2], st[3]);
+  reinterpret_cast<uint4 *>(out)[i] = o;
+}
+
+int32_t launch_md5(par2hip_ctx *ctx, hipStream_t s, const uint8_t *d_data, size_t n,
+                   size_t slice_size, size_t stride, const uint64_t *d_lens, uint32_t *d_md5) {
+  if (n == 0) return PAR2HIP_OK;
+  const bool a16 = ((uintptr_t)d_data % 16 == 0) && (stride % 16 == 0);
+  const bool a4 = ((uintptr_t)d_data % 4 == 0) && (stride % 4 == 0);
+  ScopedTiming t(ctx, "md5_slices", s);
+  dim3 grid((unsigned)((n + 63) / 64));
+  if (a16)
+    hipLaunchKernelGGL(k_md5_slices<16>, grid, dim3(64), 0, s, d_data, (long)n, (uint64_t)slice_size,
+                       (uint64_t)stride, d_lens, d_md5);
+  else if (a4)
+    hipLaunchKernelGGL(k_md5_slices<4>, grid, dim3(64), 0, s, d_data, (long)n, (uint64_t)slice_size,
+                       (uint64_t)stride, d_lens, d_md5);
+  else
+    hipLaunchKernelGGL(k_md5_slices<1>, grid, dim3(64), 0, s, d_data, (long)n, (uint64_t)slice_size,
+                       (uint64_t)stride, d_lens, d_md5);
+  P2H_CHECK(ctx, hipGetLastError());
+  return PAR2HIP_OK;
+}
+
+// ---------------------------------------------------------------- CRC-32 --
+// Reflected representation: bit (31-k) holds the coefficient of x^k.
+// d_crc_tab: [0..255] byte table for 0xEDB88320, [256..319] x^(8*2^k) mod P.
+__device__ __forceinline__ uint32_t gf2_mulmod(uint32_t a, uint32_t b) {
+  uint32_t r = 0;
+#pragma unroll 8
+  for (int k = 0; k < 32; k++) {
+    r ^= (a & (0x80000000u >> k)) ? b : 0u;
+    b = (b >> 1) ^ ((b & 1u) ? 0xEDB88320u : 0u);
+  }
+  return r;
+}
+
+// x^(8*nbytes) mod P from the power table.
+__device__ __forceinline__ uint32_t x8n_mod(uint64_t nbytes, const uint32_t *__restrict__ pow2) {
+  uint32_t r = 0x80000000u;  // 1
+  int k = 0;
+  while (nbytes) {
+    if (nbytes & 1) r = gf2_mulmod(r, pow2[k]);
+    nbytes >>= 1;
+    k++;
+  }
+  return r;
+}
+
+constexpr int kCrcWaves = 4;  // slices per workgroup (one wavefront each)
+
+__global__ __launch_bounds__(64 * kCrcWaves) void k_crc32_slices(const uint8_t *__restrict__ data, long n,
+                                                                uint64_t slice_size, uint64_t stride,
+                                                                const uint64_t *__restrict__ lens,
+                                                                const uint32_t *__restrict__ tab,
+                                                                uint32_t *__restrict__ out) {
+  __shared__ uint32_t T[4][256];
+  for (int k = threadIdx.x; k < 256; k += blockDim.x) T[0][k] = tab[k];
+  __syncthreads();
+  for (int k = threadIdx.x; k < 256; k += blockDim.x) {
+    uint32_t c = T[0][k];
+    c = (c >> 8) ^ T[0][c & 0xFF]; T[1][k] = c;
+    c = (c >> 8) ^ T[0][c & 0xFF]; T[2][k] = c;
+    c = (c >> 8) ^ T[0][c & 0xFF]; T[3][k] = c;
+  }
+  __syncthreads();
+  const int lane = threadIdx.x & 63;
+  const long i = (long)blockIdx.x * kCrcWaves + (threadIdx.x >> 6);
+  if (i >= n) return;
+  const uint8_t *p = data + (uint64_t)i * stride;
+  uint64_t len = lens ? lens[i] : slice_size;
+  if (len > slice_size) len = slice_size;
+  // lane chunks: 16-byte multiples, contiguous, covering [0, len)
+  uint64_t chunk = (len + 63) / 64;
+  chunk = (chunk + 15) & ~(uint64_t)15;
+  const uint64_t a = (uint64_t)lane * chunk;
+  const uint64_t e = (a + chunk < len) ? a + chunk : len;
+  uint32_t crc = 0;  // raw remainder of this lane's bytes, starting from zero
+  if (a < e) {
+    uint64_t o = a;
+    const bool al4 = (((uintptr_t)p) & 3) == 0;
+    if (al4) {
+      for (; o + 4 <= e; o += 4) {
+        crc ^= *reinterpret_cast<const uint32_t *>(p + o);
+        crc = T[3][crc & 0xFF] ^ T[2][(crc >> 8) & 0xFF] ^ T[1][(crc >> 16) & 0xFF] ^ T[0][crc >> 24];
+      }
+    }
+    for (; o < e; o++) crc = T[0][(crc ^ p[o]) & 0xFF] ^ (crc >> 8);
+  }
+  const uint32_t *pow2 = tab + 256;
+  uint32_t contrib = 0;
+  if (a < e && crc) contrib = gf2_mulmod(crc, x8n_mod(slice_size - e, pow2));
+  // initial register 0xFFFFFFFF shifted over the whole (padded) slice
+  if (lane == 0) contrib ^= gf2_mulmod(0xFFFFFFFFu, x8n_mod(slice_size, pow2));
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) contrib ^= __shfl_xor(contrib, off, 64);
+  if (lane == 0) out[i] = ~contrib;
+}
+
+int32_t crc_tables_init(par2hip_ctx *ctx) {
+  uint32_t h[256 + 64];
+  for (uint32_t k = 0; k < 256; k++) {
+    uint32_t c = k;
+    for (int b = 0; b < 8; b++) c = (c & 1) ? (c >> 1) ^ 0xEDB88320u : c >> 1;
+    h[k] = c;
+  }
+  // host mirror of gf2_mulmod for the power table
+  auto mm = [](uint32_t a, uint32_t b) {
+    uint32_t r = 0;
+    for (int k = 0; k < 32; k++) {
+      if (a & (0x80000000u >> k)) r ^= b;
+      b = (b >> 1) ^ ((b & 1u) ? 0xEDB88320u : 0u);
+    }
+    return r;
+  };
+  uint32_t x8 = 0x80000000u >> 8;  // x^8
+  for (int k = 0; k < 64; k++) {
+    h[256 + k] = x8;
+    x8 = mm(x8, x8);
+  }
+  P2H_CHECK(ctx, hipMalloc(&ctx->d_crc_tab, sizeof h));
+  P2H_CHECK(ctx, hipMemcpy(ctx->d_crc_tab, h, sizeof h, hipMemcpyHostToDevice));
+  return PAR2HIP_OK;
+}
+
+int32_t launch_crc32(par2hip_ctx *ctx, hipStream_t s, const uint8_t *d_data, size_t n,
+                     size_t slice_size, size_t stride, const uint64_t *d_lens, uint32_t *d_crc) {
+  if (n == 0) return PAR2HIP_OK;
+  ScopedTiming t(ctx, "crc32_slices", s);
+  hipLaunchKernelGGL(k_crc32_slices, dim3((unsigned)((n + kCrcWaves - 1) / kCrcWaves)),
+                     dim3(64 * kCrcWaves), 0, s, d_data, (long)n, (uint64_t)slice_size,
+                     (uint64_t)stride, d_lens, ctx->d_crc_tab, d_crc);
+  P2H_CHECK(ctx, hipGetLastError());
+  return PAR2HIP_OK;
+}
+
+}  // namespace par2hip

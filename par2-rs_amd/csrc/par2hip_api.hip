@@ -1,0 +1,734 @@
+// par2hip_api.hip -- the C ABI (include/par2hip.h): context management,
+// parameter upload, and the encode / repair / hash / strategy entry points.
+#include <algorithm>
+#include <cstdio>
+#include <cstring>
+#include <memory>
+
+#include "common.hpp"
+
+namespace par2hip {
+
+HostField::HostField() {
+  uint32_t b = 1;
+  for (uint32_t i = 0; i < kOrder; i++) {
+    exp[i] = (uint16_t)b;
+    exp[i + kOrder] = (uint16_t)b;
+    log[b] = (uint16_t)i;
+    b <<= 1;
+    if (b & 0x10000u) b ^= kPoly;
+  }
+  log[0] = 0;
+}
+
+const HostField &field() {
+  static const HostField *f = new HostField();
+  return *f;
+}
+
+std::vector<uint32_t> constant_logs(size_t count) {
+  static std::mutex mu;
+  static std::vector<uint32_t> logs;
+  std::lock_guard<std::mutex> g(mu);
+  uint32_t n = logs.empty() ? 1 : logs.back() + 1;
+  while (logs.size() < count) {
+    if (n % 3 && n % 5 && n % 17 && n % 257) logs.push_back(n);
+    n++;
+  }
+  return std::vector<uint32_t>(logs.begin(), logs.begin() + count);
+}
+
+int32_t fail(par2hip_ctx *ctx, int32_t code, const std::string &msg) {
+  if (ctx) ctx->err = msg;
+  return code;
+}
+
+int32_t hip_fail(par2hip_ctx *ctx, hipError_t e, const char *what) {
+  char buf[512];
+  snprintf(buf, sizeof buf, "%s: %s", what, hipGetErrorString(e));
+  return fail(ctx, e == hipErrorOutOfMemory ? PAR2HIP_ENOMEM : PAR2HIP_EHIP, buf);
+}
+
+int32_t scratch(par2hip_ctx *ctx, int role, size_t bytes, void **out) {
+  DevBuf &b = ctx->scratch[role];
+  if (b.bytes < bytes) {
+    if (b.ptr) {
+      // the buffer may still be read by queued work
+      P2H_CHECK(ctx, hipStreamSynchronize(ctx->stream));
+      P2H_CHECK(ctx, hipStreamSynchronize(ctx->side));
+      P2H_CHECK(ctx, hipFree(b.ptr));
+      b.ptr = nullptr;
+      b.bytes = 0;
+    }
+    size_t want = std::max(bytes, (size_t)4096);
+    want = (want + 4095) & ~(size_t)4095;
+    P2H_CHECK(ctx, hipMalloc(&b.ptr, want));
+    b.bytes = want;
+  }
+  *out = b.ptr;
+  return PAR2HIP_OK;
+}
+
+int32_t params_begin(par2hip_ctx *ctx, size_t bytes) {
+  ParamArena &a = ctx->arena;
+  if (a.copied) P2H_CHECK(ctx, hipEventSynchronize(a.copied));
+  if (a.cap < bytes) {
+    P2H_CHECK(ctx, hipStreamSynchronize(ctx->stream));
+    P2H_CHECK(ctx, hipStreamSynchronize(ctx->side));
+    if (a.host) (void)hipHostFree(a.host);
+    if (a.dev) (void)hipFree(a.dev);
+    a.host = nullptr;
+    a.dev = nullptr;
+    size_t cap = std::max(bytes, (size_t)1 << 20);
+    P2H_CHECK(ctx, hipHostMalloc((void **)&a.host, cap, hipHostMallocDefault));
+    P2H_CHECK(ctx, hipMalloc((void **)&a.dev, cap));
+    a.cap = cap;
+  }
+  a.used = 0;
+  return PAR2HIP_OK;
+}
+
+void *params_alloc(par2hip_ctx *ctx, size_t bytes, void **dev) {
+  ParamArena &a = ctx->arena;
+  size_t off = (a.used + 255) & ~(size_t)255;
+  a.used = off + bytes;
+  *dev = a.dev + off;
+  return a.host + off;
+}
+
+int32_t params_commit(par2hip_ctx *ctx) {
+  ParamArena &a = ctx->arena;
+  if (a.used) {
+    P2H_CHECK(ctx, hipMemcpyAsync(a.dev, a.host, a.used, hipMemcpyHostToDevice, ctx->stream));
+    P2H_CHECK(ctx, hipEventRecord(a.copied, ctx->stream));
+  }
+  return PAR2HIP_OK;
+}
+
+static size_t params_size(std::initializer_list<size_t> parts) {
+  size_t t = 0;
+  for (size_t p : parts) t += (p + 255) & ~(size_t)255;
+  return t + 256;
+}
+
+static hipEvent_t pool_event(par2hip_ctx *ctx) {
+  if (!ctx->event_pool.empty()) {
+    hipEvent_t e = ctx->event_pool.back();
+    ctx->event_pool.pop_back();
+    return e;
+  }
+  hipEvent_t e = nullptr;
+  if (hipEventCreate(&e) != hipSuccess) return nullptr;
+  return e;
+}
+
+int timing_start(par2hip_ctx *ctx, const char *name, hipStream_t s, hipEvent_t *stop) {
+  *stop = nullptr;
+  if (!ctx->profile) return -1;
+  int slot = -1;
+  for (size_t i = 0; i < ctx->stats.size(); i++)
+    if (ctx->stats[i].name == name) slot = (int)i;
+  if (slot < 0) {
+    ctx->stats.push_back(KernelStat{name, 0, 0});
+    slot = (int)ctx->stats.size() - 1;
+  }
+  hipEvent_t start = pool_event(ctx);
+  *stop = pool_event(ctx);
+  if (!start || !*stop) return -1;
+  (void)hipEventRecord(start, s);
+  ctx->pending.push_back(PendingTiming{slot, start, *stop});
+  return slot;
+}
+
+void timing_stop(par2hip_ctx *ctx, int slot, hipStream_t s, hipEvent_t stop) {
+  if (slot < 0 || !stop) return;
+  (void)hipEventRecord(stop, s);
+}
+
+}  // namespace par2hip
+
+using namespace par2hip;
+
+namespace {
+
+struct Guard {
+  par2hip_ctx *ctx;
+  std::lock_guard<std::recursive_mutex> lk;
+  explicit Guard(par2hip_ctx *c) : ctx(c), lk(c->mu) { (void)hipSetDevice(c->device); }
+};
+
+int g_ndev = -1;
+std::once_flag g_dev_once;
+
+void probe_devices() {
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess) n = 0;
+  int ok = 0;
+  for (int i = 0; i < n; i++) {
+    hipDeviceProp_t p;
+    if (hipGetDeviceProperties(&p, i) == hipSuccess && strncmp(p.gcnArchName, "gfx950", 6) == 0) ok++;
+  }
+  g_ndev = ok;
+}
+
+// fork the side stream off the main stream (it may read what the main stream wrote)
+int32_t fork_side(par2hip_ctx *ctx) {
+  P2H_CHECK(ctx, hipEventRecord(ctx->fork, ctx->stream));
+  P2H_CHECK(ctx, hipStreamWaitEvent(ctx->side, ctx->fork, 0));
+  return PAR2HIP_OK;
+}
+
+int32_t join_side(par2hip_ctx *ctx) {
+  P2H_CHECK(ctx, hipEventRecord(ctx->join, ctx->side));
+  P2H_CHECK(ctx, hipStreamWaitEvent(ctx->stream, ctx->join, 0));
+  return PAR2HIP_OK;
+}
+
+int align_words_of(uintptr_t p) {
+  int a = 8;
+  while (a > 1 && (p % (2 * (uintptr_t)a)) != 0) a >>= 1;
+  return a;
+}
+
+// Common encode body: device buffers in, queued on ctx->stream.
+int32_t encode_dev_impl(par2hip_ctx *ctx, const uint8_t *d_slices, size_t n, size_t slice_size,
+                        const uint32_t *exps, size_t r, uint8_t *d_rec, uint32_t *d_md5,
+                        uint32_t *d_crc) {
+  if (n == 0 || slice_size == 0 || (slice_size & 1))
+    return fail(ctx, PAR2HIP_EINVAL, "need n >= 1 and an even, non-zero slice size");
+  if (n > 32768) return fail(ctx, PAR2HIP_EINVAL, "more than 32768 input slices repeat PAR2 constants");
+  if (r > 65535 || (r && !exps)) return fail(ctx, PAR2HIP_EINVAL, "bad recovery exponent list");
+  const size_t words = slice_size / 2;
+  const std::vector<uint32_t> logs = constant_logs(n);
+  int32_t rc = params_begin(ctx, params_size({n * 4, r * 4, n * 8, r * 8}));
+  if (rc) return rc;
+  void *d_logs, *d_exps, *d_srcs, *d_dsts;
+  uint32_t *h_logs = (uint32_t *)params_alloc(ctx, n * 4, &d_logs);
+  uint32_t *h_exps = (uint32_t *)params_alloc(ctx, std::max<size_t>(r, 1) * 4, &d_exps);
+  const uint16_t **h_srcs = (const uint16_t **)params_alloc(ctx, n * 8, &d_srcs);
+  uint16_t **h_dsts = (uint16_t **)params_alloc(ctx, std::max<size_t>(r, 1) * 8, &d_dsts);
+  memcpy(h_logs, logs.data(), n * 4);
+  if (r) memcpy(h_exps, exps, r * 4);
+  for (size_t d = 0; d < n; d++) h_srcs[d] = (const uint16_t *)(d_slices + d * slice_size);
+  for (size_t i = 0; i < r; i++) h_dsts[i] = (uint16_t *)(d_rec + i * slice_size);
+  if ((rc = params_commit(ctx))) return rc;
+
+  if (d_md5 || d_crc) {
+    if ((rc = fork_side(ctx))) return rc;
+    if (d_md5 && (rc = launch_md5(ctx, ctx->side, d_slices, n, slice_size, slice_size, nullptr, d_md5)))
+      return rc;
+    if (d_crc && (rc = launch_crc32(ctx, ctx->side, d_slices, n, slice_size, slice_size, nullptr, d_crc)))
+      return rc;
+  }
+  if (r) {
+    void *coeff;
+    if ((rc = scratch(ctx, kScratchCoeff, r * n * 2, &coeff))) return rc;
+    if ((rc = launch_coeff_from_logs(ctx, ctx->stream, (uint16_t *)coeff, (const uint32_t *)d_logs,
+                                     (const uint32_t *)d_exps, (int)r, (int)n)))
+      return rc;
+    GfProblem p;
+    p.srcs = (const uint16_t *const *)d_srcs;
+    p.dsts = (uint16_t *const *)d_dsts;
+    p.coeff = (const uint16_t *)coeff;
+    p.nsrc = (int)n;
+    p.ndst = (int)r;
+    p.words = words;
+    p.accumulate = false;
+    p.align_words = std::min({align_words_of((uintptr_t)d_slices), align_words_of((uintptr_t)d_rec),
+                              align_words_of(slice_size)});
+    if ((rc = run_gf(ctx, ctx->stream, p))) return rc;
+  }
+  if (d_md5 || d_crc) return join_side(ctx);
+  return PAR2HIP_OK;
+}
+
+// Repair body on device buffers.  Sources of the contraction are the present
+// data slices followed by the m recovery slices.
+int32_t reconstruct_dev_impl(par2hip_ctx *ctx, uint8_t *d_slices, size_t n, size_t slice_size,
+                             const uint32_t *missing, size_t m, const uint8_t *d_recovery,
+                             const uint32_t *rec_exps, uint8_t *d_out, bool in_place) {
+  if (n == 0 || slice_size == 0 || (slice_size & 1))
+    return fail(ctx, PAR2HIP_EINVAL, "need n >= 1 and an even, non-zero slice size");
+  if (n > 32768) return fail(ctx, PAR2HIP_EINVAL, "more than 32768 input slices repeat PAR2 constants");
+  if (m == 0) return PAR2HIP_OK;
+  if (m > n) return fail(ctx, PAR2HIP_EINVAL, "more erasures than slices");
+  std::vector<char> is_missing(n, 0);
+  for (size_t k = 0; k < m; k++) {
+    if (missing[k] >= n) return fail(ctx, PAR2HIP_EINVAL, "erasure index out of range");
+    if (is_missing[missing[k]]) return fail(ctx, PAR2HIP_EINVAL, "duplicate erasure index");
+    is_missing[missing[k]] = 1;
+  }
+  for (size_t i = 0; i < m; i++)
+    for (size_t j = i + 1; j < m; j++)
+      if (rec_exps[i] == rec_exps[j]) return fail(ctx, PAR2HIP_ESINGULAR, "Singular matrix (repeated recovery exponent)");
+  const size_t np = n - m;
+  const size_t words = slice_size / 2;
+  const std::vector<uint32_t> logs = constant_logs(n);
+  int32_t rc = params_begin(ctx, params_size({m * 4, m * 4, np * 4, (np + m) * 8, m * 8}));
+  if (rc) return rc;
+  void *d_mlog, *d_exps, *d_plog, *d_srcs, *d_dsts;
+  uint32_t *h_mlog = (uint32_t *)params_alloc(ctx, m * 4, &d_mlog);
+  uint32_t *h_exps = (uint32_t *)params_alloc(ctx, m * 4, &d_exps);
+  uint32_t *h_plog = (uint32_t *)params_alloc(ctx, std::max<size_t>(np, 1) * 4, &d_plog);
+  const uint16_t **h_srcs = (const uint16_t **)params_alloc(ctx, (np + m) * 8, &d_srcs);
+  uint16_t **h_dsts = (uint16_t **)params_alloc(ctx, m * 8, &d_dsts);
+  for (size_t k = 0; k < m; k++) h_mlog[k] = logs[missing[k]];
+  memcpy(h_exps, rec_exps, m * 4);
+  size_t j = 0;
+  for (size_t d = 0; d < n; d++)
+    if (!is_missing[d]) {
+      h_plog[j] = logs[d];
+      h_srcs[j] = (const uint16_t *)(d_slices + d * slice_size);
+      j++;
+    }
+  for (size_t i = 0; i < m; i++) h_srcs[np + i] = (const uint16_t *)(d_recovery + i * slice_size);
+  for (size_t k = 0; k < m; k++)
+    h_dsts[k] = (uint16_t *)(in_place ? d_slices + (size_t)missing[k] * slice_size : d_out + k * slice_size);
+  if ((rc = params_commit(ctx))) return rc;
+
+  void *aug, *coeff;
+  if ((rc = scratch(ctx, kScratchGj, m * 2 * m * 2, &aug))) return rc;
+  if ((rc = scratch(ctx, kScratchCoeff, m * (np + m) * 2, &coeff))) return rc;
+  if ((rc = launch_build_aug(ctx, ctx->stream, (uint16_t *)aug, (const uint32_t *)d_mlog,
+                             (const uint32_t *)d_exps, (int)m)))
+    return rc;
+  if ((rc = launch_gauss_jordan(ctx, ctx->stream, (uint16_t *)aug, (int)m, ctx->d_status))) return rc;
+  int status = 0;
+  P2H_CHECK(ctx, hipMemcpyAsync(&status, ctx->d_status, sizeof status, hipMemcpyDeviceToHost, ctx->stream));
+  P2H_CHECK(ctx, hipStreamSynchronize(ctx->stream));
+  if (status) return fail(ctx, PAR2HIP_ESINGULAR, "Singular matrix");
+  if ((rc = launch_fold(ctx, ctx->stream, (uint16_t *)coeff, (const uint16_t *)aug, (const uint32_t *)d_plog,
+                        (const uint32_t *)d_exps, (int)m, (int)np)))
+    return rc;
+  GfProblem p;
+  p.srcs = (const uint16_t *const *)d_srcs;
+  p.dsts = (uint16_t *const *)d_dsts;
+  p.coeff = (const uint16_t *)coeff;
+  p.nsrc = (int)(np + m);
+  p.ndst = (int)m;
+  p.words = words;
+  p.accumulate = false;
+  p.align_words = std::min({align_words_of((uintptr_t)d_slices), align_words_of((uintptr_t)d_recovery),
+                            in_place ? 8 : align_words_of((uintptr_t)d_out), align_words_of(slice_size)});
+  if (in_place) {
+    // sources never include the missing rows, so writing them in place is safe
+  }
+  return run_gf(ctx, ctx->stream, p);
+}
+
+}  // namespace
+
+extern "C" {
+
+int32_t par2hip_available(void) {
+  std::call_once(g_dev_once, probe_devices);
+  return g_ndev;
+}
+
+const char *par2hip_strategy_name(void) { return "HIP-gfx950"; }
+int32_t par2hip_strategy_priority(void) { return PAR2HIP_PRIORITY_OPTIMAL; }
+
+int32_t par2hip_ctx_create(int32_t device, par2hip_ctx **out) {
+  if (!out) return PAR2HIP_EINVAL;
+  *out = nullptr;
+  if (par2hip_available() <= 0) return PAR2HIP_ENODEV;
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess || device < 0 || device >= n) return PAR2HIP_ENODEV;
+  std::unique_ptr<par2hip_ctx> c(new par2hip_ctx());
+  c->device = device;
+  if (hipSetDevice(device) != hipSuccess) return PAR2HIP_ENODEV;
+  par2hip_ctx *ctx = c.get();
+  P2H_CHECK(ctx, hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking));
+  P2H_CHECK(ctx, hipStreamCreateWithFlags(&ctx->side, hipStreamNonBlocking));
+  P2H_CHECK(ctx, hipEventCreateWithFlags(&ctx->fork, hipEventDisableTiming));
+  P2H_CHECK(ctx, hipEventCreateWithFlags(&ctx->join, hipEventDisableTiming));
+  P2H_CHECK(ctx, hipEventCreateWithFlags(&ctx->arena.copied, hipEventDisableTiming));
+  const HostField &f = field();
+  P2H_CHECK(ctx, hipMalloc(&ctx->d_exp, sizeof f.exp));
+  P2H_CHECK(ctx, hipMalloc(&ctx->d_log, sizeof f.log));
+  P2H_CHECK(ctx, hipMemcpy(ctx->d_exp, f.exp, sizeof f.exp, hipMemcpyHostToDevice));
+  P2H_CHECK(ctx, hipMemcpy(ctx->d_log, f.log, sizeof f.log, hipMemcpyHostToDevice));
+  P2H_CHECK(ctx, hipMalloc(&ctx->d_status, 256));
+  int32_t rc = crc_tables_init(ctx);
+  if (rc) return rc;
+  *out = c.release();
+  return PAR2HIP_OK;
+}
+
+int32_t par2hip_ctx_destroy(par2hip_ctx *ctx) {
+  if (!ctx) return PAR2HIP_EINVAL;
+  {
+    Guard g(ctx);
+    (void)hipStreamSynchronize(ctx->stream);
+    (void)hipStreamSynchronize(ctx->side);
+    for (auto &b : ctx->scratch)
+      if (b.ptr) (void)hipFree(b.ptr);
+    if (ctx->arena.host) (void)hipHostFree(ctx->arena.host);
+    if (ctx->arena.dev) (void)hipFree(ctx->arena.dev);
+    if (ctx->d_exp) (void)hipFree(ctx->d_exp);
+    if (ctx->d_log) (void)hipFree(ctx->d_log);
+    if (ctx->d_crc_tab) (void)hipFree(ctx->d_crc_tab);
+    if (ctx->d_status) (void)hipFree(ctx->d_status);
+    for (auto &p : ctx->pending) {
+      (void)hipEventDestroy(p.start);
+      (void)hipEventDestroy(p.stop);
+    }
+    for (auto e : ctx->event_pool) (void)hipEventDestroy(e);
+    (void)hipEventDestroy(ctx->fork);
+    (void)hipEventDestroy(ctx->join);
+    (void)hipEventDestroy(ctx->arena.copied);
+    (void)hipStreamDestroy(ctx->stream);
+    (void)hipStreamDestroy(ctx->side);
+  }
+  delete ctx;
+  return PAR2HIP_OK;
+}
+
+const char *par2hip_last_error(par2hip_ctx *ctx) { return ctx ? ctx->err.c_str() : "null context"; }
+
+int32_t par2hip_dev_alloc(par2hip_ctx *ctx, size_t bytes, void **dptr) {
+  if (!ctx || !dptr) return PAR2HIP_EINVAL;
+  Guard g(ctx);
+  *dptr = nullptr;
+  P2H_CHECK(ctx, hipMalloc(dptr, bytes ? bytes : 1));
+  return PAR2HIP_OK;
+}
+
+int32_t par2hip_dev_free(par2hip_ctx *ctx, void *dptr) {
+  if (!ctx) return PAR2HIP_EINVAL;
+  Guard g(ctx);
+  P2H_CHECK(ctx, hipStreamSynchronize(ctx->stream));
+  P2H_CHECK(ctx, hipFree(dptr));
+  return PAR2HIP_OK;
+}
+
+int32_t par2hip_h2d(par2hip_ctx *ctx, void *dst, const void *src, size_t bytes) {
+  if (!ctx || (bytes && (!dst || !src))) return PAR2HIP_EINVAL;
+  Guard g(ctx);
+  P2H_CHECK(ctx, hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, ctx->stream));
+  P2H_CHECK(ctx, hipStreamSynchronize(ctx->stream));
+  return PAR2HIP_OK;
+}
+
+int32_t par2hip_d2h(par2hip_ctx *ctx, void *dst, const void *src, size_t bytes) {
+  if (!ctx || (bytes && (!dst || !src))) return PAR2HIP_EINVAL;
+  Guard g(ctx);
+  P2H_CHECK(ctx, hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, ctx->stream));
+  P2H_CHECK(ctx, hipStreamSynchronize(ctx->stream));
+  return PAR2HIP_OK;
+}
+
+int32_t par2hip_sync(par2hip_ctx *ctx) {
+  if (!ctx) return PAR2HIP_EINVAL;
+  Guard g(ctx);
+  P2H_CHECK(ctx, hipStreamSynchronize(ctx->stream));
+  P2H_CHECK(ctx, hipStreamSynchronize(ctx->side));
+  return PAR2HIP_OK;
+}
+
+// ---------------------------------------------------- strategy-shaped --
+int32_t par2hip_gf_muladd_region(par2hip_ctx *ctx, uint16_t *const *dsts, size_t ndst,
+                                 const uint16_t *const *srcs, size_t nsrc,
+                                 const uint16_t *const *coeff_rows, size_t region_offset,
+                                 size_t region_size) {
+  if (!ctx) return PAR2HIP_EINVAL;
+  if (ndst == 0 || region_size == 0) return PAR2HIP_OK;
+  if (!dsts || (nsrc && (!srcs || !coeff_rows))) return PAR2HIP_EINVAL;
+  if (nsrc == 0) return PAR2HIP_OK;
+  if (ndst > 65535 || nsrc > 65535) return fail(ctx, PAR2HIP_EINVAL, "too many rows");
+  Guard g(ctx);
+  const size_t rb = region_size * 2;
+  const size_t pitch = (rb + 255) & ~(size_t)255;
+  void *din, *dout;
+  int32_t rc;
+  if ((rc = scratch(ctx, kScratchIn, nsrc * pitch, &din))) return rc;
+  if ((rc = scratch(ctx, kScratchOut, ndst * pitch, &dout))) return rc;
+  if ((rc = params_begin(ctx, params_size({ndst * nsrc * 2, nsrc * 8, ndst * 8})))) return rc;
+  void *d_coeff, *d_srcs, *d_dsts;
+  uint16_t *h_coeff = (uint16_t *)params_alloc(ctx, ndst * nsrc * 2, &d_coeff);
+  const uint16_t **h_srcs = (const uint16_t **)params_alloc(ctx, nsrc * 8, &d_srcs);
+  uint16_t **h_dsts = (uint16_t **)params_alloc(ctx, ndst * 8, &d_dsts);
+  for (size_t i = 0; i < ndst; i++) memcpy(h_coeff + i * nsrc, coeff_rows[i], nsrc * 2);
+  for (size_t j = 0; j < nsrc; j++) h_srcs[j] = (const uint16_t *)((uint8_t *)din + j * pitch);
+  for (size_t i = 0; i < ndst; i++) h_dsts[i] = (uint16_t *)((uint8_t *)dout + i * pitch);
+  if ((rc = params_commit(ctx))) return rc;
+  for (size_t j = 0; j < nsrc; j++)
+    P2H_CHECK(ctx, hipMemcpyAsync((uint8_t *)din + j * pitch, srcs[j] + region_offset, rb,
+                                  hipMemcpyHostToDevice, ctx->stream));
+  for (size_t i = 0; i < ndst; i++)
+    P2H_CHECK(ctx, hipMemcpyAsync((uint8_t *)dout + i * pitch, dsts[i] + region_offset, rb,
+                                  hipMemcpyHostToDevice, ctx->stream));
+  GfProblem p;
+  p.srcs = (const uint16_t *const *)d_srcs;
+  p.dsts = (uint16_t *const *)d_dsts;
+  p.coeff = (const uint16_t *)d_coeff;
+  p.nsrc = (int)nsrc;
+  p.ndst = (int)ndst;
+  p.words = region_size;
+  p.accumulate = true;
+  p.align_words = 8;
+  if ((rc = run_gf(ctx, ctx->stream, p))) return rc;
+  for (size_t i = 0; i < ndst; i++)
+    P2H_CHECK(ctx, hipMemcpyAsync(dsts[i] + region_offset, (uint8_t *)dout + i * pitch, rb,
+                                  hipMemcpyDeviceToHost, ctx->stream));
+  P2H_CHECK(ctx, hipStreamSynchronize(ctx->stream));
+  return PAR2HIP_OK;
+}
+
+int32_t par2hip_gf_muladd(par2hip_ctx *ctx, uint16_t *dst, const uint16_t *src, size_t len,
+                          uint16_t scalar) {
+  if (!ctx) return PAR2HIP_EINVAL;
+  if (len == 0 || scalar == 0) return PAR2HIP_OK;
+  if (!dst || !src) return PAR2HIP_EINVAL;
+  const uint16_t *cr[1] = {&scalar};
+  uint16_t *d[1] = {dst};
+  const uint16_t *s[1] = {src};
+  return par2hip_gf_muladd_region(ctx, d, 1, s, 1, cr, 0, len);
+}
+
+int32_t par2hip_gf_mul_slice(par2hip_ctx *ctx, uint16_t scalar, uint16_t *data, size_t len) {
+  if (!ctx) return PAR2HIP_EINVAL;
+  if (len == 0 || scalar == 1) return PAR2HIP_OK;
+  if (!data) return PAR2HIP_EINVAL;
+  if (scalar == 0) {
+    memset(data, 0, len * 2);
+    return PAR2HIP_OK;
+  }
+  // data = scalar*data: overwrite form of the contraction with one source
+  Guard g(ctx);
+  const size_t rb = len * 2;
+  void *din, *dout;
+  int32_t rc;
+  if ((rc = scratch(ctx, kScratchIn, rb, &din))) return rc;
+  if ((rc = scratch(ctx, kScratchOut, rb, &dout))) return rc;
+  if ((rc = params_begin(ctx, params_size({2, 8, 8})))) return rc;
+  void *d_coeff, *d_srcs, *d_dsts;
+  uint16_t *h_coeff = (uint16_t *)params_alloc(ctx, 2, &d_coeff);
+  const uint16_t **h_srcs = (const uint16_t **)params_alloc(ctx, 8, &d_srcs);
+  uint16_t **h_dsts = (uint16_t **)params_alloc(ctx, 8, &d_dsts);
+  h_coeff[0] = scalar;
+  h_srcs[0] = (const uint16_t *)din;
+  h_dsts[0] = (uint16_t *)dout;
+  if ((rc = params_commit(ctx))) return rc;
+  P2H_CHECK(ctx, hipMemcpyAsync(din, data, rb, hipMemcpyHostToDevice, ctx->stream));
+  GfProblem p;
+  p.srcs = (const uint16_t *const *)d_srcs;
+  p.dsts = (uint16_t *const *)d_dsts;
+  p.coeff = (const uint16_t *)d_coeff;
+  p.nsrc = 1;
+  p.ndst = 1;
+  p.words = len;
+  p.accumulate = false;
+  p.align_words = 8;
+  if ((rc = run_gf(ctx, ctx->stream, p))) return rc;
+  P2H_CHECK(ctx, hipMemcpyAsync(data, dout, rb, hipMemcpyDeviceToHost, ctx->stream));
+  P2H_CHECK(ctx, hipStreamSynchronize(ctx->stream));
+  return PAR2HIP_OK;
+}
+
+// ------------------------------------------------------------- encode --
+int32_t par2hip_encode_dev(par2hip_ctx *ctx, const void *d_slices, size_t n, size_t slice_size,
+                           const uint32_t *exps, size_t r, void *d_recovery, void *d_md5, void *d_crc) {
+  if (!ctx || !d_slices || (r && !d_recovery)) return PAR2HIP_EINVAL;
+  Guard g(ctx);
+  return encode_dev_impl(ctx, (const uint8_t *)d_slices, n, slice_size, exps, r, (uint8_t *)d_recovery,
+                         (uint32_t *)d_md5, (uint32_t *)d_crc);
+}
+
+int32_t par2hip_encode(par2hip_ctx *ctx, const uint8_t *slices, size_t n, size_t slice_size,
+                       const uint32_t *exps, size_t r, uint8_t *recovery, uint8_t *md5_out,
+                       uint32_t *crc_out) {
+  if (!ctx || !slices || (r && !recovery)) return PAR2HIP_EINVAL;
+  Guard g(ctx);
+  if (n == 0 || slice_size == 0 || (slice_size & 1))
+    return fail(ctx, PAR2HIP_EINVAL, "need n >= 1 and an even, non-zero slice size");
+  void *din, *dout, *dh = nullptr;
+  int32_t rc;
+  if ((rc = scratch(ctx, kScratchIn, n * slice_size, &din))) return rc;
+  if ((rc = scratch(ctx, kScratchOut, std::max<size_t>(r, 1) * slice_size, &dout))) return rc;
+  if ((md5_out || crc_out) && (rc = scratch(ctx, kScratchHash, n * 20, &dh))) return rc;
+  P2H_CHECK(ctx, hipMemcpyAsync(din, slices, n * slice_size, hipMemcpyHostToDevice, ctx->stream));
+  uint32_t *dmd5 = md5_out ? (uint32_t *)dh : nullptr;
+  uint32_t *dcrc = crc_out ? (uint32_t *)dh + 4 * n : nullptr;
+  if ((rc = encode_dev_impl(ctx, (const uint8_t *)din, n, slice_size, exps, r, (uint8_t *)dout, dmd5, dcrc)))
+    return rc;
+  if (r) P2H_CHECK(ctx, hipMemcpyAsync(recovery, dout, r * slice_size, hipMemcpyDeviceToHost, ctx->stream));
+  if (md5_out) P2H_CHECK(ctx, hipMemcpyAsync(md5_out, dmd5, n * 16, hipMemcpyDeviceToHost, ctx->stream));
+  if (crc_out) P2H_CHECK(ctx, hipMemcpyAsync(crc_out, dcrc, n * 4, hipMemcpyDeviceToHost, ctx->stream));
+  P2H_CHECK(ctx, hipStreamSynchronize(ctx->stream));
+  return PAR2HIP_OK;
+}
+
+// ------------------------------------------------------------- repair --
+int32_t par2hip_gf_invert(par2hip_ctx *ctx, const uint16_t *a, size_t m, uint16_t *inv) {
+  if (!ctx || (m && (!a || !inv))) return PAR2HIP_EINVAL;
+  if (m == 0) return PAR2HIP_OK;
+  Guard g(ctx);
+  void *aug;
+  int32_t rc;
+  if ((rc = scratch(ctx, kScratchGj, m * 2 * m * 2, &aug))) return rc;
+  std::vector<uint16_t> h(m * 2 * m, 0);
+  for (size_t i = 0; i < m; i++) {
+    memcpy(&h[i * 2 * m], a + i * m, m * 2);
+    h[i * 2 * m + m + i] = 1;
+  }
+  P2H_CHECK(ctx, hipMemcpyAsync(aug, h.data(), h.size() * 2, hipMemcpyHostToDevice, ctx->stream));
+  if ((rc = launch_gauss_jordan(ctx, ctx->stream, (uint16_t *)aug, (int)m, ctx->d_status))) return rc;
+  int status = 0;
+  P2H_CHECK(ctx, hipMemcpyAsync(&status, ctx->d_status, sizeof status, hipMemcpyDeviceToHost, ctx->stream));
+  P2H_CHECK(ctx, hipMemcpyAsync(h.data(), aug, h.size() * 2, hipMemcpyDeviceToHost, ctx->stream));
+  P2H_CHECK(ctx, hipStreamSynchronize(ctx->stream));
+  if (status) return fail(ctx, PAR2HIP_ESINGULAR, "Singular matrix");
+  for (size_t i = 0; i < m; i++) memcpy(inv + i * m, &h[i * 2 * m + m], m * 2);
+  return PAR2HIP_OK;
+}
+
+int32_t par2hip_reconstruct_dev(par2hip_ctx *ctx, void *d_slices, size_t n, size_t slice_size,
+                                const uint32_t *missing, size_t m, const void *d_recovery,
+                                const uint32_t *rec_exps, void *d_out, int32_t out_in_place) {
+  if (!ctx || !d_slices || (m && (!missing || !d_recovery || !rec_exps))) return PAR2HIP_EINVAL;
+  if (m && !out_in_place && !d_out) return PAR2HIP_EINVAL;
+  Guard g(ctx);
+  return reconstruct_dev_impl(ctx, (uint8_t *)d_slices, n, slice_size, missing, m, (const uint8_t *)d_recovery,
+                              rec_exps, (uint8_t *)d_out, out_in_place != 0);
+}
+
+int32_t par2hip_reconstruct(par2hip_ctx *ctx, const uint8_t *slices, size_t n, size_t slice_size,
+                            const uint32_t *missing, size_t m, const uint8_t *recovery,
+                            const uint32_t *rec_exps, uint8_t *out) {
+  if (!ctx || !slices || (m && (!missing || !recovery || !rec_exps || !out))) return PAR2HIP_EINVAL;
+  if (m == 0) return PAR2HIP_OK;
+  Guard g(ctx);
+  if (n == 0 || slice_size == 0 || (slice_size & 1))
+    return fail(ctx, PAR2HIP_EINVAL, "need n >= 1 and an even, non-zero slice size");
+  void *din, *drec, *dout;
+  int32_t rc;
+  if ((rc = scratch(ctx, kScratchIn, n * slice_size, &din))) return rc;
+  if ((rc = scratch(ctx, kScratchAux, m * slice_size, &drec))) return rc;
+  if ((rc = scratch(ctx, kScratchOut, m * slice_size, &dout))) return rc;
+  P2H_CHECK(ctx, hipMemcpyAsync(din, slices, n * slice_size, hipMemcpyHostToDevice, ctx->stream));
+  P2H_CHECK(ctx, hipMemcpyAsync(drec, recovery, m * slice_size, hipMemcpyHostToDevice, ctx->stream));
+  if ((rc = reconstruct_dev_impl(ctx, (uint8_t *)din, n, slice_size, missing, m, (const uint8_t *)drec,
+                                 rec_exps, (uint8_t *)dout, false)))
+    return rc;
+  P2H_CHECK(ctx, hipMemcpyAsync(out, dout, m * slice_size, hipMemcpyDeviceToHost, ctx->stream));
+  P2H_CHECK(ctx, hipStreamSynchronize(ctx->stream));
+  return PAR2HIP_OK;
+}
+
+// ------------------------------------------------------------- hashes --
+int32_t par2hip_hash_slices_dev(par2hip_ctx *ctx, const void *d_slices, size_t n, size_t slice_size,
+                                size_t slice_stride, const uint64_t *lens, void *d_md5, void *d_crc) {
+  if (!ctx || (n && !d_slices)) return PAR2HIP_EINVAL;
+  if (n == 0) return PAR2HIP_OK;
+  Guard g(ctx);
+  if (slice_stride == 0) slice_stride = slice_size;
+  if (slice_stride < slice_size && !lens) return fail(ctx, PAR2HIP_EINVAL, "stride shorter than the slice size");
+  const uint64_t *d_lens = nullptr;
+  int32_t rc;
+  if (lens) {
+    for (size_t i = 0; i < n; i++)
+      if (lens[i] > slice_stride && i + 1 < n)
+        return fail(ctx, PAR2HIP_EINVAL, "slice length beyond its stride");
+    if ((rc = params_begin(ctx, params_size({n * 8})))) return rc;
+    void *dl;
+    uint64_t *hl = (uint64_t *)params_alloc(ctx, n * 8, &dl);
+    memcpy(hl, lens, n * 8);
+    if ((rc = params_commit(ctx))) return rc;
+    d_lens = (const uint64_t *)dl;
+  }
+  if (d_md5 && (rc = launch_md5(ctx, ctx->stream, (const uint8_t *)d_slices, n, slice_size, slice_stride,
+                                d_lens, (uint32_t *)d_md5)))
+    return rc;
+  if (d_crc && (rc = launch_crc32(ctx, ctx->stream, (const uint8_t *)d_slices, n, slice_size, slice_stride,
+                                  d_lens, (uint32_t *)d_crc)))
+    return rc;
+  return PAR2HIP_OK;
+}
+
+int32_t par2hip_hash_slices(par2hip_ctx *ctx, const uint8_t *slices, size_t n, size_t slice_size,
+                            size_t slice_stride, const uint64_t *lens, uint8_t *md5_out, uint32_t *crc_out) {
+  if (!ctx || (n && !slices)) return PAR2HIP_EINVAL;
+  if (n == 0) return PAR2HIP_OK;
+  Guard g(ctx);
+  if (slice_stride == 0) slice_stride = slice_size;
+  // bytes actually present in the caller's buffer
+  size_t span = (n - 1) * slice_stride + (lens ? (size_t)lens[n - 1] : slice_size);
+  if (lens)
+    for (size_t i = 0; i < n; i++) span = std::max(span, i * slice_stride + std::min<size_t>(lens[i], slice_size));
+  const size_t dstride = (slice_stride + 15) & ~(size_t)15;
+  void *din, *dh;
+  int32_t rc;
+  if ((rc = scratch(ctx, kScratchIn, n * dstride + 16, &din))) return rc;
+  if ((rc = scratch(ctx, kScratchHash, n * 20, &dh))) return rc;
+  if (dstride == slice_stride) {
+    P2H_CHECK(ctx, hipMemcpyAsync(din, slices, span, hipMemcpyHostToDevice, ctx->stream));
+  } else {
+    if (n > 1)
+      P2H_CHECK(ctx, hipMemcpy2DAsync(din, dstride, slices, slice_stride, std::min(slice_stride, slice_size), n - 1,
+                                      hipMemcpyHostToDevice, ctx->stream));
+    const size_t last = lens ? std::min<size_t>(lens[n - 1], slice_size) : slice_size;
+    P2H_CHECK(ctx, hipMemcpyAsync((uint8_t *)din + (n - 1) * dstride, slices + (n - 1) * slice_stride,
+                                  std::min(last, slice_stride), hipMemcpyHostToDevice, ctx->stream));
+  }
+  std::vector<uint64_t> lv;
+  const uint64_t *lp = lens;
+  if (!lens && slice_stride < slice_size) return fail(ctx, PAR2HIP_EINVAL, "stride shorter than the slice size");
+  if (lens && dstride != slice_stride) {
+    lv.assign(lens, lens + n);
+    for (auto &x : lv) x = std::min<uint64_t>(x, std::min(slice_stride, slice_size));
+    lp = lv.data();
+  }
+  uint32_t *dmd5 = (uint32_t *)dh, *dcrc = (uint32_t *)dh + 4 * n;
+  if ((rc = par2hip_hash_slices_dev(ctx, din, n, slice_size, dstride, lp, md5_out ? dmd5 : nullptr,
+                                    crc_out ? dcrc : nullptr)))
+    return rc;
+  if (md5_out) P2H_CHECK(ctx, hipMemcpyAsync(md5_out, dmd5, n * 16, hipMemcpyDeviceToHost, ctx->stream));
+  if (crc_out) P2H_CHECK(ctx, hipMemcpyAsync(crc_out, dcrc, n * 4, hipMemcpyDeviceToHost, ctx->stream));
+  P2H_CHECK(ctx, hipStreamSynchronize(ctx->stream));
+  return PAR2HIP_OK;
+}
+
+// ------------------------------------------------------------- tuning --
+int32_t par2hip_set_gf_kernel(par2hip_ctx *ctx, int32_t which) {
+  if (!ctx || which < 0 || which > 2) return PAR2HIP_EINVAL;
+  Guard g(ctx);
+  int32_t prev = ctx->gf_kernel;
+  ctx->gf_kernel = which;
+  return prev;
+}
+
+int32_t par2hip_profile_enable(par2hip_ctx *ctx, int32_t on) {
+  if (!ctx) return PAR2HIP_EINVAL;
+  Guard g(ctx);
+  ctx->profile = on != 0;
+  return PAR2HIP_OK;
+}
+
+int32_t par2hip_profile_read(par2hip_ctx *ctx, par2hip_kernel_stat *out, size_t max, size_t *count) {
+  if (!ctx) return PAR2HIP_EINVAL;
+  Guard g(ctx);
+  P2H_CHECK(ctx, hipStreamSynchronize(ctx->stream));
+  P2H_CHECK(ctx, hipStreamSynchronize(ctx->side));
+  for (auto &p : ctx->pending) {
+    float ms = 0;
+    if (hipEventElapsedTime(&ms, p.start, p.stop) == hipSuccess) {
+      ctx->stats[p.stat].launches++;
+      ctx->stats[p.stat].total_ms += ms;
+    }
+    ctx->event_pool.push_back(p.start);
+    ctx->event_pool.push_back(p.stop);
+  }
+  ctx->pending.clear();
+  if (count) *count = ctx->stats.size();
+  for (size_t i = 0; i < ctx->stats.size() && i < max; i++) {
+    memset(out[i].name, 0, sizeof out[i].name);
+    strncpy(out[i].name, ctx->stats[i].name.c_str(), sizeof out[i].name - 1);
+    out[i].launches = ctx->stats[i].launches;
+    out[i].total_ms = ctx->stats[i].total_ms;
+  }
+  ctx->stats.clear();
+  return PAR2HIP_OK;
+}
+
+}  // extern "C"

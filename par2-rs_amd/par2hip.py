@@ -1,0 +1,355 @@
+"""Python binding of libpar2hip.so (the C ABI in include/par2hip.h).
+
+Mirrors the reference's plugin interface for the Reed-Solomon hot path:
+
+* :class:`HipStrategy` -- the methods of ``GaloisSimdStrategy``
+  (src/galois/simd/mod.rs:24-128): ``name``, ``is_available``, ``priority``,
+  ``mul_slice``, ``muladd``, ``muladd_region``, plus the shuffle2x hooks, which
+  are identity/fallbacks here (``supports_shuffle2x`` is False, as the GPU
+  consumes the interleaved layout).
+* :class:`Context` -- the coarse whole-set hooks: :meth:`Context.encode`
+  (StreamingEncoder, src/par2_rs.rs:687-902), :meth:`Context.reconstruct`
+  (compute_reconstruction_transform + reconstruct_streaming_chunk,
+  src/par2_rs.rs:319-675), :meth:`Context.hash_slices` (IFSC,
+  src/creator.rs:397-409), :meth:`Context.gf_invert`, and device-resident
+  variants for callers that keep sets in HBM.
+
+Errors surface as the reference's error kinds (src/error.rs:5-39).  The
+library is required: importing succeeds without it, but every compute call
+raises if ``libpar2hip.so`` is missing -- there is no CPU fallback here.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+import threading
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libpar2hip.so")
+
+OK, EINVAL, ENODEV, ENOMEM, ESINGULAR, EHIP, ERECOVERY = 0, -1, -2, -3, -4, -5, -6
+PRIORITY_OPTIMAL = 40  # Priority::Optimal (src/galois/simd/mod.rs:20)
+
+
+class Par2Error(Exception):
+    """Base error (src/error.rs)."""
+
+
+class InvalidFormat(Par2Error):
+    pass
+
+
+class RepairFailed(Par2Error):
+    pass
+
+
+class InsufficientRecovery(Par2Error):
+    pass
+
+
+class DeviceError(Par2Error):
+    pass
+
+
+class KernelStat(C.Structure):
+    _fields_ = [("name", C.c_char * 48), ("launches", C.c_uint64), ("total_ms", C.c_double)]
+
+
+_lib = None
+_lib_lock = threading.Lock()
+
+
+def build() -> None:
+    """Compile libpar2hip.so for gfx950 in-tree."""
+    subprocess.run(["make", "-s", "-C", os.path.join(HERE, "csrc")], check=True)
+
+
+def lib():
+    global _lib
+    with _lib_lock:
+        if _lib is None:
+            if not os.path.exists(LIB_PATH):
+                raise DeviceError(f"{LIB_PATH} is not built (run par2hip.build())")
+            L = C.CDLL(LIB_PATH)
+            P, S, I32 = C.c_void_p, C.c_size_t, C.c_int32
+            sig = {
+                "par2hip_available": ([], I32),
+                "par2hip_strategy_name": ([], C.c_char_p),
+                "par2hip_strategy_priority": ([], I32),
+                "par2hip_ctx_create": ([I32, C.POINTER(P)], I32),
+                "par2hip_ctx_destroy": ([P], I32),
+                "par2hip_last_error": ([P], C.c_char_p),
+                "par2hip_dev_alloc": ([P, S, C.POINTER(P)], I32),
+                "par2hip_dev_free": ([P, P], I32),
+                "par2hip_h2d": ([P, P, P, S], I32),
+                "par2hip_d2h": ([P, P, P, S], I32),
+                "par2hip_sync": ([P], I32),
+                "par2hip_gf_mul_slice": ([P, C.c_uint16, P, S], I32),
+                "par2hip_gf_muladd": ([P, P, P, S, C.c_uint16], I32),
+                "par2hip_gf_muladd_region": ([P, P, S, P, S, P, S, S], I32),
+                "par2hip_encode": ([P, P, S, S, P, S, P, P, P], I32),
+                "par2hip_encode_dev": ([P, P, S, S, P, S, P, P, P], I32),
+                "par2hip_gf_invert": ([P, P, S, P], I32),
+                "par2hip_reconstruct": ([P, P, S, S, P, S, P, P, P], I32),
+                "par2hip_reconstruct_dev": ([P, P, S, S, P, S, P, P, P, I32], I32),
+                "par2hip_hash_slices": ([P, P, S, S, S, P, P, P], I32),
+                "par2hip_hash_slices_dev": ([P, P, S, S, S, P, P, P], I32),
+                "par2hip_set_gf_kernel": ([P, I32], I32),
+                "par2hip_profile_enable": ([P, I32], I32),
+                "par2hip_profile_read": ([P, P, S, C.POINTER(S)], I32),
+            }
+            for name, (args, res) in sig.items():
+                f = getattr(L, name)
+                f.argtypes = args
+                f.restype = res
+            _lib = L
+    return _lib
+
+
+def _p(a):
+    return None if a is None else a.ctypes.data_as(C.c_void_p)
+
+
+def available() -> int:
+    """Number of gfx950 devices (GaloisSimdStrategy::is_available)."""
+    try:
+        return int(lib().par2hip_available())
+    except (DeviceError, OSError):
+        return 0
+
+
+class DeviceBuffer:
+    """Device memory owned by a Context's device (freed with the buffer)."""
+
+    def __init__(self, ctx: "Context", nbytes: int):
+        self.ctx = ctx
+        self.nbytes = int(nbytes)
+        ptr = C.c_void_p()
+        ctx._check(lib().par2hip_dev_alloc(ctx.handle, self.nbytes, C.byref(ptr)))
+        self.ptr = ptr
+
+    @property
+    def addr(self) -> int:
+        return int(self.ptr.value)
+
+    def upload(self, arr: np.ndarray, offset: int = 0) -> None:
+        arr = np.ascontiguousarray(arr)
+        if offset + arr.nbytes > self.nbytes:
+            raise InvalidFormat("upload beyond buffer")
+        self.ctx._check(lib().par2hip_h2d(self.ctx.handle, C.c_void_p(self.addr + offset), _p(arr), arr.nbytes))
+
+    def download(self, nbytes: int | None = None, offset: int = 0) -> np.ndarray:
+        nbytes = self.nbytes - offset if nbytes is None else nbytes
+        out = np.empty(nbytes, np.uint8)
+        self.ctx._check(lib().par2hip_d2h(self.ctx.handle, _p(out), C.c_void_p(self.addr + offset), nbytes))
+        return out
+
+    def free(self) -> None:
+        if self.ptr is not None and self.ptr.value:
+            lib().par2hip_dev_free(self.ctx.handle, self.ptr)
+        self.ptr = None
+
+    def __del__(self):
+        try:
+            if self.ctx.handle is not None:
+                self.free()
+        except Exception:
+            pass
+
+
+class Context:
+    """One device; wraps par2hip_ctx (calls on one Context are serialised)."""
+
+    def __init__(self, device: int = 0):
+        h = C.c_void_p()
+        rc = lib().par2hip_ctx_create(device, C.byref(h))
+        if rc != OK:
+            raise DeviceError(f"par2hip_ctx_create failed ({rc}): no gfx950 device {device}")
+        self.handle = h
+
+    def close(self) -> None:
+        if self.handle is not None:
+            lib().par2hip_ctx_destroy(self.handle)
+            self.handle = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _check(self, rc: int) -> None:
+        if rc == OK:
+            return
+        msg = (lib().par2hip_last_error(self.handle) or b"").decode(errors="replace")
+        if rc == ESINGULAR:
+            raise RepairFailed(msg or "Singular matrix")
+        if rc == ERECOVERY:
+            raise InsufficientRecovery(msg)
+        if rc == EINVAL:
+            raise InvalidFormat(msg or "invalid argument")
+        raise DeviceError(f"{rc}: {msg}")
+
+    # ---- memory
+    def alloc(self, nbytes: int) -> DeviceBuffer:
+        return DeviceBuffer(self, nbytes)
+
+    def sync(self) -> None:
+        self._check(lib().par2hip_sync(self.handle))
+
+    # ---- coarse hooks
+    def encode(self, slices: np.ndarray, exponents, hashes: bool = False):
+        """slices: (n, slice_size) uint8, zero-padded.  Returns recovery (r, slice_size)
+        [, md5 (n, 16), crc32 (n,)]."""
+        slices = np.ascontiguousarray(slices, np.uint8)
+        n, s = slices.shape
+        exps = np.ascontiguousarray(exponents, np.uint32)
+        r = len(exps)
+        rec = np.zeros((max(r, 1), s), np.uint8)
+        md5 = np.zeros((n, 16), np.uint8) if hashes else None
+        crc = np.zeros(n, np.uint32) if hashes else None
+        self._check(lib().par2hip_encode(self.handle, _p(slices), n, s, _p(exps), r, _p(rec), _p(md5), _p(crc)))
+        rec = rec[:r]
+        return (rec, md5, crc) if hashes else rec
+
+    def encode_dev(self, d_slices: DeviceBuffer, n: int, slice_size: int, exponents, d_rec: DeviceBuffer,
+                   d_md5: DeviceBuffer | None = None, d_crc: DeviceBuffer | None = None) -> None:
+        exps = np.ascontiguousarray(exponents, np.uint32)
+        self._check(lib().par2hip_encode_dev(self.handle, d_slices.ptr, n, slice_size, _p(exps), len(exps),
+                                             d_rec.ptr, d_md5.ptr if d_md5 else None,
+                                             d_crc.ptr if d_crc else None))
+
+    def reconstruct(self, slices: np.ndarray, missing, recovery: np.ndarray, rec_exps) -> np.ndarray:
+        slices = np.ascontiguousarray(slices, np.uint8)
+        n, s = slices.shape
+        missing = np.ascontiguousarray(missing, np.uint32)
+        m = len(missing)
+        recovery = np.ascontiguousarray(recovery, np.uint8).reshape(-1, s)
+        rec_exps = np.ascontiguousarray(rec_exps, np.uint32)
+        if m == 0:
+            return np.zeros((0, s), np.uint8)
+        if recovery.shape[0] < m or len(rec_exps) < m:
+            raise InsufficientRecovery(f"Need {m} recovery blocks, have {min(recovery.shape[0], len(rec_exps))}")
+        out = np.zeros((m, s), np.uint8)
+        self._check(lib().par2hip_reconstruct(self.handle, _p(slices), n, s, _p(missing), m,
+                                              _p(np.ascontiguousarray(recovery[:m])), _p(rec_exps[:m].copy()),
+                                              _p(out)))
+        return out
+
+    def reconstruct_dev(self, d_slices: DeviceBuffer, n: int, slice_size: int, missing,
+                        d_recovery: DeviceBuffer, rec_exps, d_out: DeviceBuffer | None = None,
+                        in_place: bool = False) -> None:
+        missing = np.ascontiguousarray(missing, np.uint32)
+        rec_exps = np.ascontiguousarray(rec_exps, np.uint32)
+        self._check(lib().par2hip_reconstruct_dev(self.handle, d_slices.ptr, n, slice_size, _p(missing),
+                                                  len(missing), d_recovery.ptr, _p(rec_exps),
+                                                  d_out.ptr if d_out else None, 1 if in_place else 0))
+
+    def gf_invert(self, a: np.ndarray) -> np.ndarray:
+        a = np.ascontiguousarray(a, np.uint16)
+        m = a.shape[0]
+        inv = np.zeros_like(a)
+        self._check(lib().par2hip_gf_invert(self.handle, _p(a), m, _p(inv)))
+        return inv
+
+    def hash_slices(self, data: np.ndarray, n: int, slice_size: int, slice_stride: int = 0, lens=None):
+        data = np.ascontiguousarray(data).view(np.uint8).reshape(-1)
+        md5 = np.zeros((max(n, 1), 16), np.uint8)
+        crc = np.zeros(max(n, 1), np.uint32)
+        lv = None if lens is None else np.ascontiguousarray(lens, np.uint64)
+        self._check(lib().par2hip_hash_slices(self.handle, _p(data), n, slice_size, slice_stride, _p(lv),
+                                              _p(md5), _p(crc)))
+        return md5[:n], crc[:n]
+
+    def hash_slices_dev(self, d_data: DeviceBuffer, n: int, slice_size: int, d_md5: DeviceBuffer | None,
+                        d_crc: DeviceBuffer | None, slice_stride: int = 0) -> None:
+        self._check(lib().par2hip_hash_slices_dev(self.handle, d_data.ptr, n, slice_size, slice_stride, None,
+                                                  d_md5.ptr if d_md5 else None, d_crc.ptr if d_crc else None))
+
+    # ---- tuning / measurement
+    def set_gf_kernel(self, which: int) -> int:
+        rc = lib().par2hip_set_gf_kernel(self.handle, which)
+        if rc < 0:
+            self._check(rc)
+        return rc
+
+    def profile(self, on: bool = True) -> None:
+        self._check(lib().par2hip_profile_enable(self.handle, 1 if on else 0))
+
+    def profile_read(self) -> dict:
+        arr = (KernelStat * 64)()
+        cnt = C.c_size_t(0)
+        self._check(lib().par2hip_profile_read(self.handle, arr, 64, C.byref(cnt)))
+        return {arr[i].name.decode(): (int(arr[i].launches), float(arr[i].total_ms))
+                for i in range(min(cnt.value, 64))}
+
+
+class HipStrategy:
+    """GaloisSimdStrategy implemented on the GPU (src/galois/simd/mod.rs:24-128).
+
+    Arrays are numpy uint16 (the reference's ``&mut [u16]`` slices) and are
+    updated in place, as the trait does.
+    """
+
+    def __init__(self, ctx: Context | None = None):
+        self._ctx = ctx
+
+    @property
+    def ctx(self) -> Context:
+        if self._ctx is None:
+            self._ctx = Context(0)
+        return self._ctx
+
+    def name(self) -> str:
+        return lib().par2hip_strategy_name().decode()
+
+    def is_available(self) -> bool:
+        return available() > 0
+
+    def priority(self) -> int:
+        return int(lib().par2hip_strategy_priority())
+
+    def mul_slice(self, scalar: int, data: np.ndarray) -> None:
+        assert data.dtype == np.uint16 and data.flags.c_contiguous
+        self.ctx._check(lib().par2hip_gf_mul_slice(self.ctx.handle, scalar, _p(data), data.size))
+
+    def muladd(self, dst: np.ndarray, src: np.ndarray, scalar: int) -> None:
+        assert dst.dtype == np.uint16 and src.dtype == np.uint16
+        self.ctx._check(lib().par2hip_gf_muladd(self.ctx.handle, _p(dst), _p(src), min(dst.size, src.size), scalar))
+
+    def muladd_region(self, destinations, sources, coefficients, region_offset: int, region_size: int) -> None:
+        nd, ns = len(destinations), len(sources)
+        if nd == 0:
+            return
+        dp = (C.c_void_p * nd)(*[d.ctypes.data for d in destinations])
+        sp = (C.c_void_p * max(ns, 1))(*[s.ctypes.data for s in sources])
+        rows = [np.ascontiguousarray(c, np.uint16) for c in coefficients[:nd]]
+        cp = (C.c_void_p * nd)(*[r.ctypes.data for r in rows])
+        self.ctx._check(lib().par2hip_gf_muladd_region(self.ctx.handle, dp, nd, sp, ns, cp, region_offset,
+                                                       region_size))
+
+    # shuffle2x hooks: the GPU consumes interleaved words, so these are the
+    # trait's default no-op conversions and plain muladd (simd/mod.rs:67-128)
+    def supports_shuffle2x(self) -> bool:
+        return False
+
+    def prepare_shuffle2x(self, data) -> None:
+        return None
+
+    def finish_shuffle2x(self, data) -> None:
+        return None
+
+    def muladd_shuffle2x(self, dst, src, scalar) -> None:
+        self.muladd(dst, src, scalar)
+
+    def muladd_region_shuffle2x(self, destinations, sources, coefficients, region_offset, region_size) -> None:
+        self.muladd_region(destinations, sources, coefficients, region_offset, region_size)

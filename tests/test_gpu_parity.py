@@ -1,0 +1,279 @@
+"""GPU parity: libpar2hip.so (through the C ABI) against the oracle and the
+reference's golden fixture.  Every comparison is bit-exact.
+
+Case names are printed before each GPU step so a failing run names its case.
+"""
+from __future__ import annotations
+
+import hashlib
+import json
+import os
+import sys
+import zlib
+
+import numpy as np
+import pytest
+
+import oracle
+import par2hip
+from fixtures import GOLDEN, quickpar
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print("[case]", *a, flush=True)
+    sys.stdout.flush()
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    if par2hip.available() <= 0:
+        pytest.fail("no gfx950 device visible to libpar2hip.so")
+    c = par2hip.Context(0)
+    yield c
+    c.close()
+
+
+# ------------------------------------------------------------ golden set --
+def test_quickpar_encode_and_ifsc(ctx):
+    qp = quickpar()
+    say("quickpar encode n=200 S=5376 r=60")
+    rec, md5, crc = ctx.encode(qp.data, qp.exponents, hashes=True)
+    assert np.array_equal(rec, qp.recovery), "recovery slices differ from QuickPar"
+    for i, (m, c) in enumerate(qp.ifsc):
+        assert md5[i].tobytes() == m and int(crc[i]) == c, i
+
+
+@pytest.mark.parametrize("missing", [[0, 1], [3, 4, 5], [16, 17], list(range(103, 136)), [199]])
+def test_quickpar_repair(ctx, missing):
+    # tests/external_multiblock_repair.rs:49-173 shapes
+    qp = quickpar()
+    m = len(missing)
+    damaged = qp.data.copy()
+    damaged[missing] = 0
+    say("quickpar repair", missing[:4], "m", m)
+    out = ctx.reconstruct(damaged, missing, qp.recovery[:m], qp.exponents[:m])
+    assert np.array_equal(out, qp.data[missing])
+
+
+def test_quickpar_repair_exponent_gaps(ctx):
+    qp = quickpar()
+    missing, exps = [10, 50, 90, 150], [0, 1, 3, 5]
+    say("quickpar repair with exponent gaps")
+    out = ctx.reconstruct(qp.data, missing, qp.recovery[exps], exps)
+    assert np.array_equal(out, qp.data[missing])
+
+
+# ----------------------------------------------------- encode vs oracle --
+ENCODE_SHAPES = [
+    (1, 2, 1), (1, 6, 3), (2, 18, 5), (3, 34, 4), (17, 1000, 9), (37, 4098, 17),
+    (16, 4096, 16), (64, 65536, 4), (5, 65538, 33), (200, 5376, 1), (130, 2050, 21), (8, 16, 0),
+]
+
+
+@pytest.mark.parametrize("n,s,r", ENCODE_SHAPES)
+def test_encode_matches_oracle(ctx, n, s, r):
+    rng = np.random.default_rng(n * 1000 + s + r)
+    data = rng.integers(0, 256, (n, s), dtype=np.uint8)
+    say("encode", n, s, r)
+    rec, md5, crc = ctx.encode(data, np.arange(r), hashes=True)
+    want = oracle.encode(data.view(np.uint16), np.arange(r)).view(np.uint8)
+    assert np.array_equal(rec, want.reshape(r, s))
+    wm, wc = oracle.hash_slices(data, n, s)
+    assert np.array_equal(md5, wm) and np.array_equal(crc, wc)
+
+
+def test_encode_arbitrary_exponents(ctx):
+    rng = np.random.default_rng(99)
+    data = rng.integers(0, 256, (29, 1026), dtype=np.uint8)
+    exps = [0, 3, 7, 100, 4096, 65534, 65535, 70000]
+    say("encode exps", exps)
+    rec = ctx.encode(data, exps)
+    assert np.array_equal(rec, oracle.encode(data.view(np.uint16), exps).view(np.uint8).reshape(len(exps), -1))
+
+
+def test_encode_rejects_bad_sizes(ctx):
+    say("encode invalid")
+    with pytest.raises(par2hip.InvalidFormat):
+        ctx.encode(np.zeros((2, 7), np.uint8), [0])
+    with pytest.raises(par2hip.InvalidFormat):
+        ctx.encode(np.zeros((32769, 2), np.uint8), [0])
+
+
+def test_encode_column_sample_at_full_slice_size(ctx):
+    """1024 x 1 MiB slices, R=102 (the plain-encode workload): the contraction
+    is column independent, so a seeded column window checks against the oracle."""
+    n, s, r = 1024, 1 << 20, 102
+    rng = np.random.default_rng(2)
+    data = rng.integers(0, 256, (n, s), dtype=np.uint8)
+    say("encode full size", n, s, r)
+    rec = ctx.encode(data, np.arange(r))
+    for w0 in (0, 77777, (s // 2) - 64):
+        cols = data.view(np.uint16)[:, w0:w0 + 64]
+        want = oracle.encode(np.ascontiguousarray(cols), np.arange(r))
+        assert np.array_equal(rec.view(np.uint16)[:, w0:w0 + 64], want), w0
+
+
+# --------------------------------------------------------------- hashes --
+def test_hash_golden_vectors(ctx):
+    with open(os.path.join(GOLDEN, "hash_vectors.json")) as f:
+        g = json.load(f)
+    rng = np.random.default_rng(20261203)
+    for v in g["vectors"]:
+        b = rng.integers(0, 256, v["seed_len"], dtype=np.uint8)
+        say("hash len", len(b))
+        buf = b if len(b) else np.zeros(1, np.uint8)
+        md5, crc = ctx.hash_slices(buf, 1, len(b), lens=[len(b)])
+        assert md5[0].tobytes().hex() == v["md5"] and int(crc[0]) == v["crc32"], len(b)
+    rng2 = np.random.default_rng(7)
+    ss = g["slice_size"]
+    buf = np.zeros((len(g["slices"]), ss), np.uint8)
+    lens = []
+    for i, s in enumerate(g["slices"]):
+        buf[i, : s["len"]] = rng2.integers(0, 256, s["len"], dtype=np.uint8)
+        lens.append(s["len"])
+    say("hash padded slices")
+    md5, crc = ctx.hash_slices(buf, len(lens), ss, lens=lens)
+    for i, s in enumerate(g["slices"]):
+        assert md5[i].tobytes().hex() == s["md5"] and int(crc[i]) == s["crc32"]
+
+
+@pytest.mark.parametrize("n,s,stride", [(1, 64, 0), (3, 55, 60), (65, 120, 0), (130, 4096, 4100), (7, 1 << 20, 0)])
+def test_hash_shapes_vs_stdlib(ctx, n, s, stride):
+    rng = np.random.default_rng(n + s)
+    st = stride or s
+    buf = rng.integers(0, 256, n * st, dtype=np.uint8)
+    say("hash", n, s, st)
+    md5, crc = ctx.hash_slices(buf, n, s, slice_stride=st)
+    for i in range(n):
+        b = buf[i * st:i * st + s].tobytes()
+        assert md5[i].tobytes() == hashlib.md5(b).digest() and int(crc[i]) == zlib.crc32(b)
+
+
+def test_hash_ragged_lengths_zero_padded(ctx):
+    rng = np.random.default_rng(3)
+    s = 5376
+    lens = [0, 1, 55, 56, 63, 64, 65, 5375, 5376, 2688]
+    buf = rng.integers(0, 256, len(lens) * s, dtype=np.uint8)
+    say("hash ragged")
+    md5, crc = ctx.hash_slices(buf, len(lens), s, lens=lens)
+    for i, ln in enumerate(lens):
+        b = buf[i * s:i * s + ln].tobytes() + bytes(s - ln)
+        assert md5[i].tobytes() == hashlib.md5(b).digest() and int(crc[i]) == zlib.crc32(b), ln
+
+
+# --------------------------------------------------------------- repair --
+@pytest.mark.parametrize("m", [1, 2, 17, 100, 300])
+def test_gf_invert_vs_oracle(ctx, m):
+    rng = np.random.default_rng(m)
+    c = oracle.par2_constants(2000)
+    cols = rng.choice(2000, m, replace=False)
+    exps = rng.choice(3000, m, replace=False)
+    A = np.array([[oracle.gf_pow(int(c[j]), int(e)) for j in cols] for e in exps], np.uint16)
+    say("gf_invert m", m)
+    assert np.array_equal(ctx.gf_invert(A), oracle.gf_invert(A))
+
+
+def test_gf_invert_singular(ctx):
+    A = np.array([[1, 2, 1], [3, 4, 3], [5, 6, 5]], np.uint16)
+    say("gf_invert singular")
+    with pytest.raises(par2hip.RepairFailed):
+        ctx.gf_invert(A)
+    A = np.array([[0, 1], [0, 7]], np.uint16)
+    with pytest.raises(par2hip.RepairFailed):
+        ctx.gf_invert(A)
+
+
+@pytest.mark.parametrize("n,s,r,m", [(10, 64, 3, 3), (64, 1000, 8, 5), (300, 4098, 40, 40), (33, 2, 4, 2)])
+def test_encode_erase_reconstruct(ctx, n, s, r, m):
+    rng = np.random.default_rng(n + s + r + m)
+    data = rng.integers(0, 256, (n, s), dtype=np.uint8)
+    say("roundtrip", n, s, r, m)
+    rec = ctx.encode(data, np.arange(r))
+    missing = sorted(rng.choice(n, m, replace=False).tolist())
+    use = sorted(rng.choice(r, m, replace=False).tolist())
+    damaged = data.copy()
+    damaged[missing] = 0xA5
+    out = ctx.reconstruct(damaged, missing, rec[use], use)
+    assert np.array_equal(out, data[missing])
+    # the oracle's faithful transform+replay agrees
+    present = [d for d in range(n) if d not in missing]
+    want, _ = oracle.reconstruct(n, missing, present, use, rec[use].view(np.uint16), data[present].view(np.uint16))
+    assert np.array_equal(out.view(np.uint16), want)
+
+
+def test_reconstruct_singular_and_insufficient(ctx):
+    rng = np.random.default_rng(1)
+    data = rng.integers(0, 256, (8, 32), dtype=np.uint8)
+    rec = ctx.encode(data, [0, 1, 2])
+    say("reconstruct errors")
+    with pytest.raises(par2hip.RepairFailed):
+        ctx.reconstruct(data, [1, 2], rec[[0, 0]], [0, 0])
+    with pytest.raises(par2hip.InsufficientRecovery):
+        ctx.reconstruct(data, [1, 2, 3], rec[:2], [0, 1])
+
+
+def test_reconstruct_dev_in_place(ctx):
+    rng = np.random.default_rng(5)
+    n, s, r = 50, 4096, 6
+    data = rng.integers(0, 256, (n, s), dtype=np.uint8)
+    rec = ctx.encode(data, np.arange(r))
+    missing = [0, 7, 49]
+    damaged = data.copy()
+    damaged[missing] = 0
+    d_sl = ctx.alloc(n * s)
+    d_rec = ctx.alloc(3 * s)
+    d_sl.upload(damaged)
+    d_rec.upload(rec[[1, 3, 5]])
+    say("reconstruct_dev in place")
+    ctx.reconstruct_dev(d_sl, n, s, missing, d_rec, [1, 3, 5], in_place=True)
+    ctx.sync()
+    assert np.array_equal(d_sl.download().reshape(n, s), data)
+
+
+# ----------------------------------------------------- strategy methods --
+SIZES = [0, 1, 7, 15, 16, 17, 31, 33, 100, 1000]
+SCALARS = [0, 1, 2, 12345, 65535]
+
+
+def test_strategy_identity(ctx):
+    s = par2hip.HipStrategy(ctx)
+    assert s.name() == "HIP-gfx950" and s.priority() == 40 and s.is_available()
+    assert not s.supports_shuffle2x()
+
+
+@pytest.mark.parametrize("size", SIZES)
+def test_strategy_mul_slice_and_muladd(ctx, size):
+    # src/galois/tests.rs:255-298 sweep, GPU strategy against the scalar oracle
+    s = par2hip.HipStrategy(ctx)
+    rng = np.random.default_rng(size)
+    for k in SCALARS:
+        src = rng.integers(0, 65536, size).astype(np.uint16)
+        dst = rng.integers(0, 65536, size).astype(np.uint16)
+        want = dst.copy()
+        oracle.muladd(want, src, k)
+        got = dst.copy()
+        say("muladd", size, k)
+        s.muladd(got, src, k)
+        assert np.array_equal(got, want)
+        d1, d2 = src.copy(), src.copy()
+        oracle.mul_slice(k, d1)
+        s.mul_slice(k, d2)
+        assert np.array_equal(d1, d2)
+
+
+def test_strategy_muladd_region(ctx):
+    s = par2hip.HipStrategy(ctx)
+    rng = np.random.default_rng(3)
+    nd, ns, L = 5, 9, 1031
+    srcs = [rng.integers(0, 65536, L).astype(np.uint16) for _ in range(ns)]
+    dsts = [rng.integers(0, 65536, L).astype(np.uint16) for _ in range(nd)]
+    coeff = rng.integers(0, 65536, (nd, ns + 3)).astype(np.uint16)
+    coeff[1, 2] = 0
+    want = [d.copy() for d in dsts]
+    oracle.muladd_region(want, srcs, coeff, 13, 1000)
+    say("muladd_region")
+    s.muladd_region(dsts, srcs, list(coeff), 13, 1000)
+    for a, b in zip(dsts, want):
+        assert np.array_equal(a, b)
