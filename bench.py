@@ -45,7 +45,7 @@ def random_bytes(seed: int, n: int) -> np.ndarray:
 def cpu_baseline_encode(n, s_sample, r, threads):
     """Reference algorithm restated (AVX2 nibble-shuffle muladd with the
     StreamingEncoder batching, plus per-slice MD5+CRC32), on n slices of
-    s_sample bytes; returns source GB/s."""
+    s_sample bytes; returns source GB/s and the wall time."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle
 
@@ -54,6 +54,21 @@ def cpu_baseline_encode(n, s_sample, r, threads):
     t0 = time.perf_counter()
     oracle.encode_avx2(data.view(np.uint16), np.arange(r), threads=threads)
     oracle.hash_slices_mt(data, n, s_sample, threads=threads)
+    dt = time.perf_counter() - t0
+    return n * s_sample / dt / 1e9, dt
+
+
+def cpu_baseline_repair(n, s_sample, m, threads):
+    """Repair on the CPU with the reference's algorithm restated: CRC32+MD5
+    verify of every slice, then the rebuild contraction of m slices from n
+    sources (same GF work as m recovery slices over n inputs)."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle
+
+    data = random_bytes(12, n * s_sample).reshape(n, s_sample)
+    t0 = time.perf_counter()
+    oracle.hash_slices_mt(data, n, s_sample, threads=threads)
+    oracle.encode_avx2(data.view(np.uint16), np.arange(m), threads=threads)
     dt = time.perf_counter() - t0
     return n * s_sample / dt / 1e9, dt
 
@@ -113,8 +128,44 @@ def bench_encode(args, ctx):
                      "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": args.traffic,
                      "kernel": gf_name, "kernel_ms": round(gf_avg, 4), "algorithmic_bytes": algo_bytes,
                      "gf_muladds": n * r * (s // 2), "md5_ms": round(md5_avg, 4), "crc32_ms": round(crc_avg, 4)},
-        "cpu": (n, 128 << 10, r),
+        "cpu": (n, s, r),
     }
+
+
+def bench_micro(args, ctx):
+    """Each kernel of the encode step alone on the plain-encode data (device-resident)."""
+    n, s, r = 1024, 1 << 20, 102
+    data = random_bytes(1234, n * s)
+    d_sl = ctx.alloc(n * s)
+    d_rec = ctx.alloc(r * s)
+    d_md5 = ctx.alloc(n * 16)
+    d_crc = ctx.alloc(n * 4)
+    d_sl.upload(data)
+    exps = np.arange(r, dtype=np.uint32)
+    out = {}
+
+    def timed(name, fn, reps):
+        fn()
+        ctx.sync()
+        ctx.profile(True)
+        ctx.profile_read()
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            fn()
+        ctx.sync()
+        wall = (time.perf_counter() - t0) / reps * 1e3
+        st = ctx.profile_read()
+        ctx.profile(False)
+        out[name] = {"wall_ms": round(wall, 4), **{k: round(v[1] / max(v[0], 1), 4) for k, v in st.items()}}
+        log(name, out[name])
+
+    for k, nm in ((1, "gf_lds"), (2, "gf_mfma")):
+        ctx.set_gf_kernel(k)
+        timed(nm, lambda: ctx.encode_dev(d_sl, n, s, exps, d_rec), args.steps)
+    ctx.set_gf_kernel(0)
+    timed("md5_only", lambda: ctx.hash_slices_dev(d_sl, n, s, d_md5, None), max(2, args.steps // 4))
+    timed("crc32_only", lambda: ctx.hash_slices_dev(d_sl, n, s, None, d_crc), args.steps)
+    return out
 
 
 def bench_repair(args, ctx):
@@ -172,7 +223,7 @@ def bench_repair(args, ctx):
                      "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": args.traffic,
                      "kernel": gf_name, "kernel_ms": round(gf_avg, 4), "algorithmic_bytes": algo_bytes,
                      "gf_muladds": m * n * (s // 2)},
-        "cpu": (n, 32 << 10, m),
+        "cpu": (n, s, m),
     }
 
 
@@ -181,7 +232,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--workload", choices=["encode", "repair"], default="encode")
+    ap.add_argument("--workload", choices=["encode", "repair", "micro"], default="encode")
     ap.add_argument("--kernel", type=int, default=0, help="0 auto, 1 LDS tables, 2 MFMA")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--traffic", type=float, default=None,
@@ -196,6 +247,10 @@ def main():
         sys.exit(1)
     ctx = par2hip.Context(0)
     log("workload", args.workload, "steps", args.steps, "warmup", args.warmup)
+    if args.workload == "micro":
+        print(json.dumps({"micro": bench_micro(args, ctx)}), flush=True)
+        ctx.close()
+        return
     res = bench_encode(args, ctx) if args.workload == "encode" else bench_repair(args, ctx)
     lat = np.array(res["lat"]) * 1e3
     ms = res["total_s"] / args.steps * 1e3
@@ -214,11 +269,16 @@ def main():
         threads = min(16, os.cpu_count() or 1)
         n, ss, r = res["cpu"]
         log("cpu baseline", n, ss, r, "threads", threads)
-        gbs, dt = cpu_baseline_encode(n, ss, r, threads) if args.workload == "encode" else \
-            cpu_baseline_encode(n, ss, r, threads)
+        if args.workload == "encode":
+            gbs, dt = cpu_baseline_encode(n, ss, r, threads)
+            what = (f"{n} slices x {ss} B, R={r}: AVX2 nibble-shuffle encode (StreamingEncoder batching) "
+                    f"+ MD5/CRC32 per slice, one pass, {dt:.2f} s wall")
+        else:
+            gbs, dt = cpu_baseline_repair(n, ss, r, threads)
+            what = (f"{n} slices x {ss} B: MD5/CRC32 verify of every slice + rebuild contraction of {r} slices "
+                    f"(AVX2 nibble-shuffle), one pass, {dt:.2f} s wall")
         out["cpu_baseline"] = {"value": round(gbs, 4), "unit": "GB/s", "cores": threads, "kind": "port",
-                               "sample": f"{n} slices x {ss} B, R={r}: AVX2 nibble-shuffle encode "
-                                         f"(StreamingEncoder batching) + MD5/CRC32 per slice, {dt:.2f} s"}
+                               "sample": what}
     print(json.dumps(out), flush=True)
     ctx.close()
 

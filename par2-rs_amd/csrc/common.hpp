@@ -146,6 +146,10 @@ struct GfProblem {
   int align_words;              // every pointer and `words` stride aligned to this many words
 };
 int32_t run_gf(par2hip_ctx *ctx, hipStream_t s, const GfProblem &p);
+int32_t run_gf_lds(par2hip_ctx *ctx, hipStream_t s, const GfProblem &p);
+int32_t run_gf_mfma(par2hip_ctx *ctx, hipStream_t s, const GfProblem &p);
+// Kernel used when the caller leaves the choice to the library.
+constexpr int kAutoGfKernel = 1;
 
 // Slice hashes.
 int32_t launch_md5(par2hip_ctx *ctx, hipStream_t s, const uint8_t *d_data, size_t n,

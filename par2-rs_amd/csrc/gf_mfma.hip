@@ -1,3 +1,222 @@
-// gf_mfma.hip -- placeholder translation unit for the bit-sliced MFMA
-// contraction (filled in once the LDS-table variant is parity-green).
+// gf_mfma.hip -- bit-sliced GF(2^16) contraction on the gfx950 matrix cores.
+//
+// Multiplication by a constant c is GF(2)-linear: bit j of c*x is the parity
+// of sum_i M_c[j][i] x_i with M_c[j][i] = bit j of (c * 2^i).  The whole
+// contraction dst_r = XOR_d coeff[r][d] * src_d is therefore one GF(2) matrix
+// product
+//     C[(r,j)][w] = parity( sum_{d,i} A[(r,j)][(d,i)] * B[(d,i)][w] )
+// with A = the stacked bit matrices (16R x 16N) and B = the bits of the input
+// words (16N x W).  Integer sums of 0/1 products are exact in fp32 up to
+// 2^24 > 16N, so the product runs on v_mfma_scale_f32_32x32x64_f8f6f4 with
+// FP4 (e2m1) operands, 1.0 = 0b0010, and the parity is read off the fp32
+// accumulator at the end.
+//
+// One 32x32x64 MFMA covers 32 A-rows = 2 output slices x 16 bits, 64 K = 4
+// sources x 16 bits, and 32 words.  Lane layout (checked on the device):
+//   A/B operand lane l: row/column l%32, K half l/32 (same nibble order in A
+//   and B, so the K order is ours to choose);  C lane l, accumulator q:
+//   column l%32, row 8*(q/4) + 4*(l/32) + q%4.
+// A-rows are ordered so that lane l's 16 accumulators of a fragment are the
+// 16 bits of ONE output slice (output 2*frag + l/32): row m <-> (h, j) with
+// h = (m/4)%2, j = 4*(m/8) + m%4, so accumulator q is bit q.
+//
+// K order inside one 64-wide step (sources 4ks..4ks+3): lane half g holds
+// sources 4ks+2g+s, s in {0,1}; with y = x_s0 | x_s1 << 16 the four operand
+// registers are ((y << 1), y, (y >> 1), (y >> 2)) & 0x22222222, i.e. nibble p
+// of register t carries bit 4*(p%4)+t of source s = p/4.  A is pre-packed in
+// the same shape (one u32 per lane per fragment per step, expanded on chip).
 #include "common.hpp"
+
+namespace par2hip {
+
+typedef int v8i __attribute__((ext_vector_type(8)));
+typedef float v16f __attribute__((ext_vector_type(16)));
+
+constexpr int kMfWaves = 8;              // waves per workgroup
+constexpr int kMfRows = 4;               // A fragments (output pairs) per workgroup -> 8 outputs
+constexpr int kMfCols = 2;               // B fragments (32 words) per wave
+constexpr int kMfWgWords = kMfWaves * kMfCols * 32;  // 512 words per workgroup
+constexpr uint32_t kNib = 0x22222222u;
+
+__device__ __forceinline__ uint16_t mul_x(uint16_t v) {
+  return (uint16_t)((v << 1) ^ ((v & 0x8000u) ? (kPoly & 0xFFFFu) : 0u));
+}
+
+// Apk[mf][ks][lane]: the A operand of fragment mf (outputs 2mf, 2mf+1) at
+// K-step ks, packed as two 16-bit rows of the transposed bit matrices.
+__global__ void k_pack_a(uint32_t *__restrict__ apk, const uint16_t *__restrict__ coeff, int nrow,
+                         int nsrc, int nmf, int nks) {
+  const long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= (long)nmf * nks * 64) return;
+  const int lane = (int)(idx & 63);
+  const long fk = idx >> 6;
+  const int ks = (int)(fk % nks), mf = (int)(fk / nks);
+  const int m = lane & 31, g = lane >> 5;
+  const int h = (m >> 2) & 1, j = 4 * (m >> 3) + (m & 3);
+  const int r = 2 * mf + h;
+  uint32_t out = 0;
+#pragma unroll
+  for (int s = 0; s < 2; s++) {
+    const int d = 4 * ks + 2 * g + s;
+    uint32_t t = 0;
+    if (r < nrow && d < nsrc) {
+      uint16_t v = coeff[(long)r * nsrc + d];
+#pragma unroll
+      for (int i = 0; i < 16; i++) {
+        t |= (uint32_t)((v >> j) & 1u) << i;
+        v = mul_x(v);
+      }
+    }
+    out |= t << (16 * s);
+  }
+  apk[idx] = out;
+}
+
+__device__ __forceinline__ v8i expand_fp4(uint32_t y) {
+  v8i o = {0, 0, 0, 0, 0, 0, 0, 0};
+  o[0] = (int)((y << 1) & kNib);
+  o[1] = (int)(y & kNib);
+  o[2] = (int)((y >> 1) & kNib);
+  o[3] = (int)((y >> 2) & kNib);
+  return o;
+}
+
+// two consecutive words of a source (0 beyond `words` or for absent sources)
+__device__ __forceinline__ uint32_t load_pair(const uint16_t *__restrict__ p, long w, long words) {
+  if (w + 1 < words) return *reinterpret_cast<const uint32_t *>(p + w);
+  if (w < words) return p[w];
+  return 0u;
+}
+
+__global__ __launch_bounds__(64 * kMfWaves, 1) void k_gf_mfma(const uint16_t *const *__restrict__ srcs, int nsrc,
+                                                              uint16_t *const *__restrict__ dsts, int ndst,
+                                                              const uint32_t *__restrict__ apk, int nmf, int nks,
+                                                              long words, int accumulate, int ntiles,
+                                                              int nrowblk) {
+  __shared__ uint4 lds_a[2][kMfRows][64];  // 8 KiB, double-buffered A fragments
+
+  const int L = blockIdx.x;
+  const int xcd = L & 7, q = L >> 3;
+  const int tiles_per_xcd = (ntiles + 7) >> 3;
+  const int local_tile = q / nrowblk, rowblk = q % nrowblk;
+  const int tile = local_tile * 8 + xcd;
+  if (local_tile >= tiles_per_xcd || tile >= ntiles) return;
+
+  const int tid = threadIdx.x;
+  const int wave = tid >> 6, lane = tid & 63;
+  const int n = lane & 31, g = lane >> 5;
+  const int mf0 = rowblk * kMfRows;
+  const long w0 = (long)tile * kMfWgWords + wave * (kMfCols * 32) + 2 * n;  // this lane's two words
+
+  // A staging role: (fragment, lane, half)
+  const int st_frag = tid >> 7, st_lane = (tid >> 1) & 63, st_half = tid & 1;
+  const bool st_live = (mf0 + st_frag) < nmf;
+  const uint32_t *st_src = apk + ((long)(mf0 + st_frag) * nks) * 64 + st_lane;
+
+  auto stage = [&](int ks, int buf) {
+    const uint32_t y = st_live ? st_src[(long)ks * 64] : 0u;
+    uint2 v;
+    if (st_half == 0) {
+      v.x = (y << 1) & kNib;
+      v.y = y & kNib;
+    } else {
+      v.x = (y >> 1) & kNib;
+      v.y = (y >> 2) & kNib;
+    }
+    reinterpret_cast<uint2 *>(&lds_a[buf][st_frag][st_lane])[st_half] = v;
+  };
+  auto load_b = [&](int ks, uint32_t &x0, uint32_t &x1) {
+    const int d0 = 4 * ks + 2 * g;
+    x0 = (d0 < nsrc) ? load_pair(srcs[d0], w0, words) : 0u;
+    x1 = (d0 + 1 < nsrc) ? load_pair(srcs[d0 + 1], w0, words) : 0u;
+  };
+
+  v16f acc[kMfRows][kMfCols];
+#pragma unroll
+  for (int i = 0; i < kMfRows; i++)
+#pragma unroll
+    for (int f = 0; f < kMfCols; f++)
+#pragma unroll
+      for (int k = 0; k < 16; k++) acc[i][f][k] = 0.f;
+
+  uint32_t x0, x1;
+  load_b(0, x0, x1);
+  stage(0, 0);
+  __syncthreads();
+  for (int ks = 0; ks < nks; ks++) {
+    const int buf = ks & 1;
+    uint32_t nx0 = 0, nx1 = 0;
+    if (ks + 1 < nks) {
+      stage(ks + 1, buf ^ 1);
+      load_b(ks + 1, nx0, nx1);
+    }
+    v8i a[kMfRows];
+#pragma unroll
+    for (int i = 0; i < kMfRows; i++) {
+      const uint4 t = lds_a[buf][i][lane];
+      a[i] = v8i{(int)t.x, (int)t.y, (int)t.z, (int)t.w, 0, 0, 0, 0};
+    }
+    // word f of the pair: source 2g in the low half, source 2g+1 in the high half
+    const v8i b0 = expand_fp4((x0 & 0xFFFFu) | (x1 << 16));
+    const v8i b1 = expand_fp4((x0 >> 16) | (x1 & 0xFFFF0000u));
+#pragma unroll
+    for (int i = 0; i < kMfRows; i++) {
+      acc[i][0] = __builtin_amdgcn_mfma_scale_f32_32x32x64_f8f6f4(a[i], b0, acc[i][0], 4, 4, 0, 127, 0, 127);
+      acc[i][1] = __builtin_amdgcn_mfma_scale_f32_32x32x64_f8f6f4(a[i], b1, acc[i][1], 4, 4, 0, 127, 0, 127);
+    }
+    x0 = nx0;
+    x1 = nx1;
+    __syncthreads();
+  }
+
+  // parity of each accumulator = bit q of output slice 2*(mf0+i) + g at words w0, w0+1
+  if (w0 >= words) return;
+#pragma unroll
+  for (int i = 0; i < kMfRows; i++) {
+    const int r = 2 * (mf0 + i) + g;
+    if (r >= ndst) continue;
+    uint32_t v[kMfCols];
+#pragma unroll
+    for (int f = 0; f < kMfCols; f++) {
+      uint32_t word = 0;
+#pragma unroll
+      for (int k = 0; k < 16; k++) word |= ((uint32_t)acc[i][f][k] & 1u) << k;
+      v[f] = word;
+    }
+    uint16_t *p = dsts[r];
+    if (w0 + 1 < words) {
+      uint32_t out = v[0] | (v[1] << 16);
+      if (accumulate) out ^= *reinterpret_cast<const uint32_t *>(p + w0);
+      *reinterpret_cast<uint32_t *>(p + w0) = out;
+    } else {
+      p[w0] = accumulate ? (uint16_t)(p[w0] ^ v[0]) : (uint16_t)v[0];
+    }
+  }
+}
+
+int32_t run_gf_mfma(par2hip_ctx *ctx, hipStream_t s, const GfProblem &p) {
+  const int nmf = (p.ndst + 1) / 2;
+  const int nks = (p.nsrc + 3) / 4;
+  const size_t apk_bytes = (size_t)nmf * nks * 64 * sizeof(uint32_t);
+  void *apk = nullptr;
+  int32_t rc = scratch(ctx, kScratchTables, apk_bytes, &apk);
+  if (rc) return rc;
+  {
+    const long total = (long)nmf * nks * 64;
+    ScopedTiming t(ctx, "pack_bitmatrices", s);
+    hipLaunchKernelGGL(k_pack_a, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, s, (uint32_t *)apk, p.coeff,
+                       p.ndst, p.nsrc, nmf, nks);
+    P2H_CHECK(ctx, hipGetLastError());
+  }
+  const int ntiles = (int)((p.words + kMfWgWords - 1) / kMfWgWords);
+  const int nrowblk = (nmf + kMfRows - 1) / kMfRows;
+  const long nwg = 8L * ((ntiles + 7) / 8) * nrowblk;
+  if (nwg > 0x7fffffffL) return fail(ctx, PAR2HIP_EINVAL, "GF problem too large for one launch");
+  ScopedTiming t(ctx, "gf_muladd_mfma", s);
+  hipLaunchKernelGGL(k_gf_mfma, dim3((unsigned)nwg), dim3(64 * kMfWaves), 0, s, p.srcs, p.nsrc, p.dsts, p.ndst,
+                     (const uint32_t *)apk, nmf, nks, (long)p.words, p.accumulate ? 1 : 0, ntiles, nrowblk);
+  P2H_CHECK(ctx, hipGetLastError());
+  return PAR2HIP_OK;
+}
+
+}  // namespace par2hip

@@ -213,13 +213,14 @@ int32_t encode_dev_impl(par2hip_ctx *ctx, const uint8_t *d_slices, size_t n, siz
   for (size_t i = 0; i < r; i++) h_dsts[i] = (uint16_t *)(d_rec + i * slice_size);
   if ((rc = params_commit(ctx))) return rc;
 
-  if (d_md5 || d_crc) {
+  // MD5 is a long serial chain per slice: it runs on the side stream from the
+  // start and the contraction and CRC-32 share the rest of the device.
+  if (d_md5) {
     if ((rc = fork_side(ctx))) return rc;
-    if (d_md5 && (rc = launch_md5(ctx, ctx->side, d_slices, n, slice_size, slice_size, nullptr, d_md5)))
-      return rc;
-    if (d_crc && (rc = launch_crc32(ctx, ctx->side, d_slices, n, slice_size, slice_size, nullptr, d_crc)))
-      return rc;
+    if ((rc = launch_md5(ctx, ctx->side, d_slices, n, slice_size, slice_size, nullptr, d_md5))) return rc;
   }
+  if (d_crc && (rc = launch_crc32(ctx, ctx->stream, d_slices, n, slice_size, slice_size, nullptr, d_crc)))
+    return rc;
   if (r) {
     void *coeff;
     if ((rc = scratch(ctx, kScratchCoeff, r * n * 2, &coeff))) return rc;
@@ -238,7 +239,7 @@ int32_t encode_dev_impl(par2hip_ctx *ctx, const uint8_t *d_slices, size_t n, siz
                               align_words_of(slice_size)});
     if ((rc = run_gf(ctx, ctx->stream, p))) return rc;
   }
-  if (d_md5 || d_crc) return join_side(ctx);
+  if (d_md5) return join_side(ctx);
   return PAR2HIP_OK;
 }
 

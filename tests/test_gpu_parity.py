@@ -35,8 +35,20 @@ def ctx():
     c.close()
 
 
+KERNELS = {"lds": 1, "mfma": 2}
+
+
+@pytest.fixture(params=list(KERNELS))
+def kctx(request, ctx):
+    """The context with one contraction kernel forced (LDS tables / bit-sliced MFMA)."""
+    prev = ctx.set_gf_kernel(KERNELS[request.param])
+    yield ctx
+    ctx.set_gf_kernel(prev)
+
+
 # ------------------------------------------------------------ golden set --
-def test_quickpar_encode_and_ifsc(ctx):
+def test_quickpar_encode_and_ifsc(kctx):
+    ctx = kctx
     qp = quickpar()
     say("quickpar encode n=200 S=5376 r=60")
     rec, md5, crc = ctx.encode(qp.data, qp.exponents, hashes=True)
@@ -46,7 +58,8 @@ def test_quickpar_encode_and_ifsc(ctx):
 
 
 @pytest.mark.parametrize("missing", [[0, 1], [3, 4, 5], [16, 17], list(range(103, 136)), [199]])
-def test_quickpar_repair(ctx, missing):
+def test_quickpar_repair(kctx, missing):
+    ctx = kctx
     # tests/external_multiblock_repair.rs:49-173 shapes
     qp = quickpar()
     m = len(missing)
@@ -73,7 +86,8 @@ ENCODE_SHAPES = [
 
 
 @pytest.mark.parametrize("n,s,r", ENCODE_SHAPES)
-def test_encode_matches_oracle(ctx, n, s, r):
+def test_encode_matches_oracle(kctx, n, s, r):
+    ctx = kctx
     rng = np.random.default_rng(n * 1000 + s + r)
     data = rng.integers(0, 256, (n, s), dtype=np.uint8)
     say("encode", n, s, r)
@@ -84,7 +98,8 @@ def test_encode_matches_oracle(ctx, n, s, r):
     assert np.array_equal(md5, wm) and np.array_equal(crc, wc)
 
 
-def test_encode_arbitrary_exponents(ctx):
+def test_encode_arbitrary_exponents(kctx):
+    ctx = kctx
     rng = np.random.default_rng(99)
     data = rng.integers(0, 256, (29, 1026), dtype=np.uint8)
     exps = [0, 3, 7, 100, 4096, 65534, 65535, 70000]
@@ -101,9 +116,10 @@ def test_encode_rejects_bad_sizes(ctx):
         ctx.encode(np.zeros((32769, 2), np.uint8), [0])
 
 
-def test_encode_column_sample_at_full_slice_size(ctx):
+def test_encode_column_sample_at_full_slice_size(kctx):
     """1024 x 1 MiB slices, R=102 (the plain-encode workload): the contraction
     is column independent, so a seeded column window checks against the oracle."""
+    ctx = kctx
     n, s, r = 1024, 1 << 20, 102
     rng = np.random.default_rng(2)
     data = rng.integers(0, 256, (n, s), dtype=np.uint8)
@@ -186,7 +202,8 @@ def test_gf_invert_singular(ctx):
 
 
 @pytest.mark.parametrize("n,s,r,m", [(10, 64, 3, 3), (64, 1000, 8, 5), (300, 4098, 40, 40), (33, 2, 4, 2)])
-def test_encode_erase_reconstruct(ctx, n, s, r, m):
+def test_encode_erase_reconstruct(kctx, n, s, r, m):
+    ctx = kctx
     rng = np.random.default_rng(n + s + r + m)
     data = rng.integers(0, 256, (n, s), dtype=np.uint8)
     say("roundtrip", n, s, r, m)
@@ -214,7 +231,8 @@ def test_reconstruct_singular_and_insufficient(ctx):
         ctx.reconstruct(data, [1, 2, 3], rec[:2], [0, 1])
 
 
-def test_reconstruct_dev_in_place(ctx):
+def test_reconstruct_dev_in_place(kctx):
+    ctx = kctx
     rng = np.random.default_rng(5)
     n, s, r = 50, 4096, 6
     data = rng.integers(0, 256, (n, s), dtype=np.uint8)
@@ -263,7 +281,8 @@ def test_strategy_mul_slice_and_muladd(ctx, size):
         assert np.array_equal(d1, d2)
 
 
-def test_strategy_muladd_region(ctx):
+def test_strategy_muladd_region(kctx):
+    ctx = kctx
     s = par2hip.HipStrategy(ctx)
     rng = np.random.default_rng(3)
     nd, ns, L = 5, 9, 1031
