@@ -32,10 +32,12 @@ namespace par2hip {
 typedef int v8i __attribute__((ext_vector_type(8)));
 typedef float v16f __attribute__((ext_vector_type(16)));
 
-constexpr int kMfWaves = 8;              // waves per workgroup
+constexpr int kMfWaves = 8;              // waves per workgroup (two per SIMD)
 constexpr int kMfRows = 4;               // A fragments (output pairs) per workgroup -> 8 outputs
 constexpr int kMfCols = 2;               // B fragments (32 words) per wave
 constexpr int kMfWgWords = kMfWaves * kMfCols * 32;  // 512 words per workgroup
+constexpr int kMfKc = 8;                 // K-steps (4 sources each) per LDS stage -> 32 sources
+constexpr int kMfThreads = 64 * kMfWaves;
 constexpr uint32_t kNib = 0x22222222u;
 
 __device__ __forceinline__ uint16_t mul_x(uint16_t v) {
@@ -81,19 +83,20 @@ __device__ __forceinline__ v8i expand_fp4(uint32_t y) {
   return o;
 }
 
-// two consecutive words of a source (0 beyond `words` or for absent sources)
-__device__ __forceinline__ uint32_t load_pair(const uint16_t *__restrict__ p, long w, long words) {
-  if (w + 1 < words) return *reinterpret_cast<const uint32_t *>(p + w);
-  if (w < words) return p[w];
-  return 0u;
-}
-
-__global__ __launch_bounds__(64 * kMfWaves, 1) void k_gf_mfma(const uint16_t *const *__restrict__ srcs, int nsrc,
-                                                              uint16_t *const *__restrict__ dsts, int ndst,
-                                                              const uint32_t *__restrict__ apk, int nmf, int nks,
-                                                              long words, int accumulate, int ntiles,
-                                                              int nrowblk) {
-  __shared__ uint4 lds_a[2][kMfRows][64];  // 8 KiB, double-buffered A fragments
+// Workgroup: 8 outputs (4 A fragments) x 512 words (8 waves x 2 B fragments).
+// Sources are staged 32 at a time: the 32 KiB slab of input words and the
+// packed A of 8 K-steps go global -> registers -> LDS one stage ahead of the
+// MFMAs that consume them (one barrier per 64 MFMAs per wave).  A is shared
+// by every wave, so it is expanded to FP4 once, by the staging threads; each
+// wave expands only its own B columns.  Two waves per SIMD let one wave's LDS
+// reads and expansion overlap the other's MFMAs.
+// Lane (n, g) of wave v owns words tile + 64v + 2n + f for B fragment f.
+__global__ __launch_bounds__(kMfThreads, 1) void k_gf_mfma(const uint16_t *const *__restrict__ srcs, int nsrc,
+                                                           uint16_t *const *__restrict__ dsts, int ndst,
+                                                           const uint32_t *__restrict__ apk, int nmf, int nks,
+                                                           long words, int accumulate, int ntiles, int nrowblk) {
+  __shared__ uint4 sB[2][4 * kMfKc][kMfWgWords / 8];   // [buf][source][16-byte unit], 64 KiB
+  __shared__ uint4 sA[2][kMfKc][kMfRows][64];         // [buf][ks][fragment][lane], FP4 operands, 64 KiB
 
   const int L = blockIdx.x;
   const int xcd = L & 7, q = L >> 3;
@@ -106,29 +109,49 @@ __global__ __launch_bounds__(64 * kMfWaves, 1) void k_gf_mfma(const uint16_t *co
   const int wave = tid >> 6, lane = tid & 63;
   const int n = lane & 31, g = lane >> 5;
   const int mf0 = rowblk * kMfRows;
-  const long w0 = (long)tile * kMfWgWords + wave * (kMfCols * 32) + 2 * n;  // this lane's two words
+  const long tile_w0 = (long)tile * kMfWgWords;
+  const int nchunks = (nks + kMfKc - 1) / kMfKc;
 
-  // A staging role: (fragment, lane, half)
-  const int st_frag = tid >> 7, st_lane = (tid >> 1) & 63, st_half = tid & 1;
-  const bool st_live = (mf0 + st_frag) < nmf;
-  const uint32_t *st_src = apk + ((long)(mf0 + st_frag) * nks) * 64 + st_lane;
-
-  auto stage = [&](int ks, int buf) {
-    const uint32_t y = st_live ? st_src[(long)ks * 64] : 0u;
-    uint2 v;
-    if (st_half == 0) {
-      v.x = (y << 1) & kNib;
-      v.y = y & kNib;
-    } else {
-      v.x = (y >> 1) & kNib;
-      v.y = (y >> 2) & kNib;
+  // staging roles.  B: unit tid + 512k -> source (tid>>6) + 8k, 16-byte column tid&63.
+  // A: thread tid owns packed words 4*tid..4*tid+3 of the stage: fragment tid>>7,
+  // K-step (tid>>4)&7, lanes 4*(tid&15)..+3.
+  uint4 rb[4], ra;
+  auto load_stage = [&](int c) {
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+      const int d = 32 * c + (tid >> 6) + 8 * k;
+      const long w = tile_w0 + 8 * (tid & 63);
+      uint4 v = make_uint4(0, 0, 0, 0);
+      if (d < nsrc) {
+        const uint16_t *p = srcs[d];
+        if (w + 8 <= words) {
+          v = *reinterpret_cast<const uint4 *>(p + w);
+        } else if (w < words) {
+          uint16_t t[8];
+#pragma unroll
+          for (int e = 0; e < 8; e++) t[e] = (w + e < words) ? p[w + e] : 0;
+          v = make_uint4(t[0] | (uint32_t)t[1] << 16, t[2] | (uint32_t)t[3] << 16, t[4] | (uint32_t)t[5] << 16,
+                         t[6] | (uint32_t)t[7] << 16);
+        }
+      }
+      rb[k] = v;
     }
-    reinterpret_cast<uint2 *>(&lds_a[buf][st_frag][st_lane])[st_half] = v;
+    {
+      const int frag = tid >> 7, ksl = (tid >> 4) & 7, l4 = 4 * (tid & 15);
+      const int ks = kMfKc * c + ksl;
+      ra = make_uint4(0, 0, 0, 0);
+      if (mf0 + frag < nmf && ks < nks)
+        ra = *reinterpret_cast<const uint4 *>(apk + ((long)(mf0 + frag) * nks + ks) * 64 + l4);
+    }
   };
-  auto load_b = [&](int ks, uint32_t &x0, uint32_t &x1) {
-    const int d0 = 4 * ks + 2 * g;
-    x0 = (d0 < nsrc) ? load_pair(srcs[d0], w0, words) : 0u;
-    x1 = (d0 + 1 < nsrc) ? load_pair(srcs[d0 + 1], w0, words) : 0u;
+  auto store_stage = [&](int buf) {
+#pragma unroll
+    for (int k = 0; k < 4; k++) sB[buf][(tid >> 6) + 8 * k][tid & 63] = rb[k];
+    const int frag = tid >> 7, ksl = (tid >> 4) & 7, l4 = 4 * (tid & 15);
+    const uint32_t y[4] = {ra.x, ra.y, ra.z, ra.w};
+#pragma unroll
+    for (int e = 0; e < 4; e++)
+      sA[buf][ksl][frag][l4 + e] = make_uint4((y[e] << 1) & kNib, y[e] & kNib, (y[e] >> 1) & kNib, (y[e] >> 2) & kNib);
   };
 
   v16f acc[kMfRows][kMfCols];
@@ -139,38 +162,39 @@ __global__ __launch_bounds__(64 * kMfWaves, 1) void k_gf_mfma(const uint16_t *co
 #pragma unroll
       for (int k = 0; k < 16; k++) acc[i][f][k] = 0.f;
 
-  uint32_t x0, x1;
-  load_b(0, x0, x1);
-  stage(0, 0);
+  load_stage(0);
+  store_stage(0);
   __syncthreads();
-  for (int ks = 0; ks < nks; ks++) {
-    const int buf = ks & 1;
-    uint32_t nx0 = 0, nx1 = 0;
-    if (ks + 1 < nks) {
-      stage(ks + 1, buf ^ 1);
-      load_b(ks + 1, nx0, nx1);
-    }
-    v8i a[kMfRows];
+  const int bcol = wave * (kMfCols * 32) + 2 * n;  // this lane's first word inside the tile
+  for (int c = 0; c < nchunks; c++) {
+    const int buf = c & 1;
+    if (c + 1 < nchunks) load_stage(c + 1);
+    // all 8 K-steps of the stage (K-steps past the last source carry zero
+    // operands); each wave expands its own B columns, A comes pre-expanded
+    for (int ksl = 0; ksl < kMfKc; ksl++) {
+      const uint32_t x0 = reinterpret_cast<const uint32_t *>(&sB[buf][4 * ksl + 2 * g][0])[bcol >> 1];
+      const uint32_t x1 = reinterpret_cast<const uint32_t *>(&sB[buf][4 * ksl + 2 * g + 1][0])[bcol >> 1];
+      v8i b[kMfCols];
+      b[0] = expand_fp4(__builtin_amdgcn_perm(x1, x0, 0x05040100u));  // word 0: src 2g low, 2g+1 high
+      b[1] = expand_fp4(__builtin_amdgcn_perm(x1, x0, 0x07060302u));  // word 1
 #pragma unroll
-    for (int i = 0; i < kMfRows; i++) {
-      const uint4 t = lds_a[buf][i][lane];
-      a[i] = v8i{(int)t.x, (int)t.y, (int)t.z, (int)t.w, 0, 0, 0, 0};
-    }
-    // word f of the pair: source 2g in the low half, source 2g+1 in the high half
-    const v8i b0 = expand_fp4((x0 & 0xFFFFu) | (x1 << 16));
-    const v8i b1 = expand_fp4((x0 >> 16) | (x1 & 0xFFFF0000u));
+      for (int i = 0; i < kMfRows; i++) {
+        const uint4 t = sA[buf][ksl][i][lane];
+        const v8i a = v8i{(int)t.x, (int)t.y, (int)t.z, (int)t.w, 0, 0, 0, 0};
 #pragma unroll
-    for (int i = 0; i < kMfRows; i++) {
-      acc[i][0] = __builtin_amdgcn_mfma_scale_f32_32x32x64_f8f6f4(a[i], b0, acc[i][0], 4, 4, 0, 127, 0, 127);
-      acc[i][1] = __builtin_amdgcn_mfma_scale_f32_32x32x64_f8f6f4(a[i], b1, acc[i][1], 4, 4, 0, 127, 0, 127);
+        for (int f = 0; f < kMfCols; f++)
+          acc[i][f] = __builtin_amdgcn_mfma_scale_f32_32x32x64_f8f6f4(a, b[f], acc[i][f], 4, 4, 0, 127, 0, 127);
+      }
     }
-    x0 = nx0;
-    x1 = nx1;
+    // the other buffer was last read in stage c-1, which every wave finished
+    // before the previous barrier
+    if (c + 1 < nchunks) store_stage(buf ^ 1);
     __syncthreads();
   }
 
-  // parity of each accumulator = bit q of output slice 2*(mf0+i) + g at words w0, w0+1
-  if (w0 >= words) return;
+  // parity of each accumulator = bit q of output slice 2*(mf0+i) + g at words w, w+1
+  const long w = tile_w0 + bcol;
+  if (w >= words) return;
 #pragma unroll
   for (int i = 0; i < kMfRows; i++) {
     const int r = 2 * (mf0 + i) + g;
@@ -184,12 +208,12 @@ __global__ __launch_bounds__(64 * kMfWaves, 1) void k_gf_mfma(const uint16_t *co
       v[f] = word;
     }
     uint16_t *p = dsts[r];
-    if (w0 + 1 < words) {
+    if (w + 2 <= words) {
       uint32_t out = v[0] | (v[1] << 16);
-      if (accumulate) out ^= *reinterpret_cast<const uint32_t *>(p + w0);
-      *reinterpret_cast<uint32_t *>(p + w0) = out;
+      if (accumulate) out ^= *reinterpret_cast<const uint32_t *>(p + w);
+      *reinterpret_cast<uint32_t *>(p + w) = out;
     } else {
-      p[w0] = accumulate ? (uint16_t)(p[w0] ^ v[0]) : (uint16_t)v[0];
+      p[w] = accumulate ? (uint16_t)(p[w] ^ v[0]) : (uint16_t)v[0];
     }
   }
 }
@@ -213,7 +237,7 @@ int32_t run_gf_mfma(par2hip_ctx *ctx, hipStream_t s, const GfProblem &p) {
   const long nwg = 8L * ((ntiles + 7) / 8) * nrowblk;
   if (nwg > 0x7fffffffL) return fail(ctx, PAR2HIP_EINVAL, "GF problem too large for one launch");
   ScopedTiming t(ctx, "gf_muladd_mfma", s);
-  hipLaunchKernelGGL(k_gf_mfma, dim3((unsigned)nwg), dim3(64 * kMfWaves), 0, s, p.srcs, p.nsrc, p.dsts, p.ndst,
+  hipLaunchKernelGGL(k_gf_mfma, dim3((unsigned)nwg), dim3(kMfThreads), 0, s, p.srcs, p.nsrc, p.dsts, p.ndst,
                      (const uint32_t *)apk, nmf, nks, (long)p.words, p.accumulate ? 1 : 0, ntiles, nrowblk);
   P2H_CHECK(ctx, hipGetLastError());
   return PAR2HIP_OK;

@@ -34,10 +34,12 @@ __device__ __forceinline__ uint32_t rotl(uint32_t x, int s) { return __builtin_a
 
 __device__ __forceinline__ void md5_compress(uint32_t st[4], const uint32_t M[16]) {
   uint32_t a = st[0], b = st[1], c = st[2], d = st[3];
-#define F1(b, c, d) (((b) & (c)) | (~(b) & (d)))
-#define F2(b, c, d) (((b) & (d)) | ((c) & ~(d)))
-#define F3(b, c, d) ((b) ^ (c) ^ (d))
-#define F4(b, c, d) ((c) ^ ((b) | ~(d)))
+  // the four round functions, each one v_bitop3_b32 (truth tables over
+  // operands (b, c, d) = (0xF0, 0xCC, 0xAA))
+#define F1(b, c, d) __builtin_amdgcn_bitop3_b32((b), (c), (d), 0xCA)  // (b & c) | (~b & d)
+#define F2(b, c, d) __builtin_amdgcn_bitop3_b32((b), (c), (d), 0xE4)  // (b & d) | (c & ~d)
+#define F3(b, c, d) __builtin_amdgcn_bitop3_b32((b), (c), (d), 0x96)  // b ^ c ^ d
+#define F4(b, c, d) __builtin_amdgcn_bitop3_b32((b), (c), (d), 0x39)  // c ^ (b | ~d)
   MD5_STEP(F1(b, c, d), a, b, c, d, M[0], kMd5T[0], 7)
   MD5_STEP(F1(a, b, c), d, a, b, c, M[1], kMd5T[1], 12)
   MD5_STEP(F1(d, a, b), c, d, a, b, M[2], kMd5T[2], 17)
@@ -136,6 +138,9 @@ __global__ __launch_bounds__(64) void k_md5_slices(const uint8_t *__restrict__ d
                                                    uint64_t slice_size, uint64_t stride,
                                                    const uint64_t *__restrict__ lens,
                                                    uint32_t *__restrict__ out) {
+  // the chain latency of these few waves bounds the whole step: let them
+  // issue ahead of co-resident waves of other kernels
+  __builtin_amdgcn_s_setprio(3);
   const long i = (long)blockIdx.x * 64 + threadIdx.x;
   if (i >= n) return;
   const uint8_t *p = data + (uint64_t)i * stride;
@@ -146,22 +151,29 @@ __global__ __launch_bounds__(64) void k_md5_slices(const uint8_t *__restrict__ d
   const uint64_t full = len & ~(uint64_t)63;  // blocks made entirely of real bytes
   uint64_t pos = 0;
   if (kAlign == 16) {
-    // software-pipelined: the next block's loads are in flight during this compress
-    if (full) {
-      const uint4 *q = reinterpret_cast<const uint4 *>(p);
-      uint4 n0 = q[0], n1 = q[1], n2 = q[2], n3 = q[3];
-      for (pos = 0; pos < full; pos += 64) {
-        M[0] = n0.x; M[1] = n0.y; M[2] = n0.z; M[3] = n0.w;
-        M[4] = n1.x; M[5] = n1.y; M[6] = n1.z; M[7] = n1.w;
-        M[8] = n2.x; M[9] = n2.y; M[10] = n2.z; M[11] = n2.w;
-        M[12] = n3.x; M[13] = n3.y; M[14] = n3.z; M[15] = n3.w;
-        if (pos + 64 < full) {
-          const uint4 *r = reinterpret_cast<const uint4 *>(p + pos + 64);
-          n0 = r[0]; n1 = r[1]; n2 = r[2]; n3 = r[3];
-        }
-        md5_compress(st, M);
-      }
+    // Two register sets alternate: while one block is compressed the next
+    // one's loads are in flight, with no copies between the sets.
+    const uint4 *q = reinterpret_cast<const uint4 *>(p);
+    const uint64_t nblk = full >> 6;
+    uint32_t A[16], B[16];
+    auto fetch = [&](uint32_t (&D)[16], uint64_t blk) {
+      const uint4 *r = q + 4 * blk;
+      const uint4 t0 = r[0], t1 = r[1], t2 = r[2], t3 = r[3];
+      D[0] = t0.x; D[1] = t0.y; D[2] = t0.z; D[3] = t0.w;
+      D[4] = t1.x; D[5] = t1.y; D[6] = t1.z; D[7] = t1.w;
+      D[8] = t2.x; D[9] = t2.y; D[10] = t2.z; D[11] = t2.w;
+      D[12] = t3.x; D[13] = t3.y; D[14] = t3.z; D[15] = t3.w;
+    };
+    uint64_t blk = 0;
+    if (nblk) fetch(A, 0);
+    for (; blk + 2 <= nblk; blk += 2) {
+      fetch(B, blk + 1);
+      md5_compress(st, A);
+      if (blk + 2 < nblk) fetch(A, blk + 2);
+      md5_compress(st, B);
     }
+    if (blk < nblk) md5_compress(st, A);
+    pos = full;
   } else if (kAlign == 4) {
     for (pos = 0; pos < full; pos += 64) {
       const uint32_t *q = reinterpret_cast<const uint32_t *>(p + pos);
@@ -238,7 +250,7 @@ int32_t launch_md5(par2hip_ctx *ctx, hipStream_t s, const uint8_t *d_data, size_
 
 // ---------------------------------------------------------------- CRC-32 --
 // Reflected representation: bit (31-k) holds the coefficient of x^k.
-// d_crc_tab: [0..255] byte table for 0xEDB88320, [256..319] x^(8*2^k) mod P.
+// d_crc_tab: the 256-entry byte table for 0xEDB88320.
 __device__ __forceinline__ uint32_t gf2_mulmod(uint32_t a, uint32_t b) {
   uint32_t r = 0;
 #pragma unroll 8
@@ -249,101 +261,145 @@ __device__ __forceinline__ uint32_t gf2_mulmod(uint32_t a, uint32_t b) {
   return r;
 }
 
-// x^(8*nbytes) mod P from the power table.
-__device__ __forceinline__ uint32_t x8n_mod(uint64_t nbytes, const uint32_t *__restrict__ pow2) {
-  uint32_t r = 0x80000000u;  // 1
-  int k = 0;
-  while (nbytes) {
-    if (nbytes & 1) r = gf2_mulmod(r, pow2[k]);
-    nbytes >>= 1;
-    k++;
-  }
-  return r;
+// One workgroup of 256 lanes per slice.  The (zero-padded) slice is viewed as
+// pad leading zero bytes followed by its slice_size bytes, pad chosen so that
+// 256 equal chunks of `chunk` bytes (a multiple of 16) cover it; leading
+// zeros leave a remainder that starts at zero unchanged.  Each lane folds its
+// chunk from a zero register, then a binary tree joins neighbours with
+// left * x^(8 * chunk * 2^k) + right (level constants from the host), and the
+// initial register 0xFFFFFFFF contributes init_term = ~0 * x^(8*slice_size).
+constexpr int kCrcLanes = 256;
+
+struct CrcParams {
+  uint64_t chunk;       // bytes per lane (multiple of 16)
+  uint64_t pad;         // leading virtual zero bytes
+  uint32_t level[8];    // x^(8*chunk*2^k) mod P, k = 0..7
+  uint32_t init_term;   // 0xFFFFFFFF * x^(8*slice_size) mod P
+};
+
+__device__ __forceinline__ uint32_t crc_word(uint32_t crc, uint32_t w, const uint32_t (*T)[256]) {
+  crc ^= w;
+  return T[3][crc & 0xFF] ^ T[2][(crc >> 8) & 0xFF] ^ T[1][(crc >> 16) & 0xFF] ^ T[0][crc >> 24];
 }
 
-constexpr int kCrcWaves = 4;  // slices per workgroup (one wavefront each)
-
-__global__ __launch_bounds__(64 * kCrcWaves) void k_crc32_slices(const uint8_t *__restrict__ data, long n,
-                                                                uint64_t slice_size, uint64_t stride,
-                                                                const uint64_t *__restrict__ lens,
-                                                                const uint32_t *__restrict__ tab,
-                                                                uint32_t *__restrict__ out) {
+template <bool kAligned16>
+__global__ __launch_bounds__(kCrcLanes) void k_crc32_slices(const uint8_t *__restrict__ data, uint64_t slice_size,
+                                                            uint64_t stride, const uint64_t *__restrict__ lens,
+                                                            const uint32_t *__restrict__ tab, CrcParams prm,
+                                                            uint32_t *__restrict__ out) {
   __shared__ uint32_t T[4][256];
-  for (int k = threadIdx.x; k < 256; k += blockDim.x) T[0][k] = tab[k];
-  __syncthreads();
-  for (int k = threadIdx.x; k < 256; k += blockDim.x) {
-    uint32_t c = T[0][k];
-    c = (c >> 8) ^ T[0][c & 0xFF]; T[1][k] = c;
-    c = (c >> 8) ^ T[0][c & 0xFF]; T[2][k] = c;
-    c = (c >> 8) ^ T[0][c & 0xFF]; T[3][k] = c;
+  __shared__ uint32_t part[kCrcLanes / 64];
+  for (int k = threadIdx.x; k < 256; k += kCrcLanes) {
+    uint32_t c = tab[k];
+    T[0][k] = c;
+    // T_j[i]: byte i followed by j zero bytes
+    uint32_t c1 = (c >> 8) ^ tab[c & 0xFF];
+    uint32_t c2 = (c1 >> 8) ^ tab[c1 & 0xFF];
+    uint32_t c3 = (c2 >> 8) ^ tab[c2 & 0xFF];
+    T[1][k] = c1;
+    T[2][k] = c2;
+    T[3][k] = c3;
   }
   __syncthreads();
-  const int lane = threadIdx.x & 63;
-  const long i = (long)blockIdx.x * kCrcWaves + (threadIdx.x >> 6);
-  if (i >= n) return;
+  const long i = blockIdx.x;
+  const int lane = threadIdx.x;
   const uint8_t *p = data + (uint64_t)i * stride;
   uint64_t len = lens ? lens[i] : slice_size;
   if (len > slice_size) len = slice_size;
-  // lane chunks: 16-byte multiples, contiguous, covering [0, len)
-  uint64_t chunk = (len + 63) / 64;
-  chunk = (chunk + 15) & ~(uint64_t)15;
-  const uint64_t a = (uint64_t)lane * chunk;
-  const uint64_t e = (a + chunk < len) ? a + chunk : len;
-  uint32_t crc = 0;  // raw remainder of this lane's bytes, starting from zero
-  if (a < e) {
-    uint64_t o = a;
-    const bool al4 = (((uintptr_t)p) & 3) == 0;
-    if (al4) {
-      for (; o + 4 <= e; o += 4) {
-        crc ^= *reinterpret_cast<const uint32_t *>(p + o);
-        crc = T[3][crc & 0xFF] ^ T[2][(crc >> 8) & 0xFF] ^ T[1][(crc >> 16) & 0xFF] ^ T[0][crc >> 24];
-      }
+  // this lane's bytes in slice coordinates: [a, e) real or zero-padding
+  const int64_t va = (int64_t)lane * (int64_t)prm.chunk - (int64_t)prm.pad;
+  const int64_t ve = va + (int64_t)prm.chunk;
+  const uint64_t a = va < 0 ? 0 : (uint64_t)va;
+  const uint64_t e = ve < 0 ? 0 : (uint64_t)ve;   // e <= slice_size by construction
+  const uint64_t real_end = e < len ? e : len;
+  uint32_t crc = 0;
+  uint64_t o = a;
+  if (kAligned16) {
+    // a is a multiple of 16 here (chunk, pad multiples of 16, aligned base)
+    for (; o + 64 <= real_end; o += 64) {
+      const uint4 *q = reinterpret_cast<const uint4 *>(p + o);
+      const uint4 v0 = q[0], v1 = q[1], v2 = q[2], v3 = q[3];
+      crc = crc_word(crc, v0.x, T); crc = crc_word(crc, v0.y, T); crc = crc_word(crc, v0.z, T); crc = crc_word(crc, v0.w, T);
+      crc = crc_word(crc, v1.x, T); crc = crc_word(crc, v1.y, T); crc = crc_word(crc, v1.z, T); crc = crc_word(crc, v1.w, T);
+      crc = crc_word(crc, v2.x, T); crc = crc_word(crc, v2.y, T); crc = crc_word(crc, v2.z, T); crc = crc_word(crc, v2.w, T);
+      crc = crc_word(crc, v3.x, T); crc = crc_word(crc, v3.y, T); crc = crc_word(crc, v3.z, T); crc = crc_word(crc, v3.w, T);
     }
-    for (; o < e; o++) crc = T[0][(crc ^ p[o]) & 0xFF] ^ (crc >> 8);
+    for (; o + 4 <= real_end; o += 4) crc = crc_word(crc, *reinterpret_cast<const uint32_t *>(p + o), T);
   }
-  const uint32_t *pow2 = tab + 256;
-  uint32_t contrib = 0;
-  if (a < e && crc) contrib = gf2_mulmod(crc, x8n_mod(slice_size - e, pow2));
-  // initial register 0xFFFFFFFF shifted over the whole (padded) slice
-  if (lane == 0) contrib ^= gf2_mulmod(0xFFFFFFFFu, x8n_mod(slice_size, pow2));
+  for (; o < real_end; o++) crc = T[0][(crc ^ p[o]) & 0xFF] ^ (crc >> 8);
+  // zero padding inside this lane's range (only when the slice is short)
+  for (; o + 4 <= e; o += 4) crc = crc_word(crc, 0u, T);
+  for (; o < e; o++) crc = T[0][crc & 0xFF] ^ (crc >> 8);
+
+  // tree join: level k merges block [l, l+2^k) with its right neighbour
+  uint32_t v = crc;
 #pragma unroll
-  for (int off = 32; off > 0; off >>= 1) contrib ^= __shfl_xor(contrib, off, 64);
-  if (lane == 0) out[i] = ~contrib;
+  for (int k = 0; k < 6; k++) {
+    const uint32_t right = __shfl_down(v, 1 << k, 64);
+    if ((lane & ((2 << k) - 1)) == 0) v = gf2_mulmod(v, prm.level[k]) ^ right;
+  }
+  if ((lane & 63) == 0) part[lane >> 6] = v;
+  __syncthreads();
+  if (lane == 0) {
+    uint32_t w01 = gf2_mulmod(part[0], prm.level[6]) ^ part[1];
+    uint32_t w23 = gf2_mulmod(part[2], prm.level[6]) ^ part[3];
+    const uint32_t tot = gf2_mulmod(w01, prm.level[7]) ^ w23;
+    out[i] = ~(prm.init_term ^ tot);
+  }
 }
 
 int32_t crc_tables_init(par2hip_ctx *ctx) {
-  uint32_t h[256 + 64];
+  uint32_t h[256];
   for (uint32_t k = 0; k < 256; k++) {
     uint32_t c = k;
     for (int b = 0; b < 8; b++) c = (c & 1) ? (c >> 1) ^ 0xEDB88320u : c >> 1;
     h[k] = c;
-  }
-  // host mirror of gf2_mulmod for the power table
-  auto mm = [](uint32_t a, uint32_t b) {
-    uint32_t r = 0;
-    for (int k = 0; k < 32; k++) {
-      if (a & (0x80000000u >> k)) r ^= b;
-      b = (b >> 1) ^ ((b & 1u) ? 0xEDB88320u : 0u);
-    }
-    return r;
-  };
-  uint32_t x8 = 0x80000000u >> 8;  // x^8
-  for (int k = 0; k < 64; k++) {
-    h[256 + k] = x8;
-    x8 = mm(x8, x8);
   }
   P2H_CHECK(ctx, hipMalloc(&ctx->d_crc_tab, sizeof h));
   P2H_CHECK(ctx, hipMemcpy(ctx->d_crc_tab, h, sizeof h, hipMemcpyHostToDevice));
   return PAR2HIP_OK;
 }
 
+static uint32_t host_mulmod(uint32_t a, uint32_t b) {
+  uint32_t r = 0;
+  for (int k = 0; k < 32; k++) {
+    if (a & (0x80000000u >> k)) r ^= b;
+    b = (b >> 1) ^ ((b & 1u) ? 0xEDB88320u : 0u);
+  }
+  return r;
+}
+
+static uint32_t host_x8n(uint64_t nbytes) {
+  uint32_t r = 0x80000000u, sq = 0x80000000u >> 8;  // 1, x^8
+  while (nbytes) {
+    if (nbytes & 1) r = host_mulmod(r, sq);
+    sq = host_mulmod(sq, sq);
+    nbytes >>= 1;
+  }
+  return r;
+}
+
 int32_t launch_crc32(par2hip_ctx *ctx, hipStream_t s, const uint8_t *d_data, size_t n,
                      size_t slice_size, size_t stride, const uint64_t *d_lens, uint32_t *d_crc) {
   if (n == 0) return PAR2HIP_OK;
+  CrcParams prm;
+  prm.chunk = ((slice_size + kCrcLanes - 1) / kCrcLanes + 15) & ~(uint64_t)15;
+  if (prm.chunk == 0) prm.chunk = 16;
+  prm.pad = prm.chunk * kCrcLanes - slice_size;
+  uint32_t lv = host_x8n(prm.chunk);
+  for (int k = 0; k < 8; k++) {
+    prm.level[k] = lv;
+    lv = host_mulmod(lv, lv);
+  }
+  prm.init_term = host_mulmod(0xFFFFFFFFu, host_x8n(slice_size));
+  const bool a16 = ((uintptr_t)d_data % 16 == 0) && (stride % 16 == 0) && (prm.pad % 16 == 0);
   ScopedTiming t(ctx, "crc32_slices", s);
-  hipLaunchKernelGGL(k_crc32_slices, dim3((unsigned)((n + kCrcWaves - 1) / kCrcWaves)),
-                     dim3(64 * kCrcWaves), 0, s, d_data, (long)n, (uint64_t)slice_size,
-                     (uint64_t)stride, d_lens, ctx->d_crc_tab, d_crc);
+  if (a16)
+    hipLaunchKernelGGL(k_crc32_slices<true>, dim3((unsigned)n), dim3(kCrcLanes), 0, s, d_data, (uint64_t)slice_size,
+                       (uint64_t)stride, d_lens, ctx->d_crc_tab, prm, d_crc);
+  else
+    hipLaunchKernelGGL(k_crc32_slices<false>, dim3((unsigned)n), dim3(kCrcLanes), 0, s, d_data, (uint64_t)slice_size,
+                       (uint64_t)stride, d_lens, ctx->d_crc_tab, prm, d_crc);
   P2H_CHECK(ctx, hipGetLastError());
   return PAR2HIP_OK;
 }
