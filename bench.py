@@ -164,32 +164,50 @@ def bench_micro(args, ctx):
         timed(nm, lambda: ctx.encode_dev(d_sl, n, s, exps, d_rec), args.steps)
     ctx.set_gf_kernel(0)
     timed("md5_only", lambda: ctx.hash_slices_dev(d_sl, n, s, d_md5, None), max(2, args.steps // 4))
+    # the repair-shaped contraction: 2000 sources x 100 outputs x 256 Ki words
+    n3, s3, r3 = 2000, 512 << 10, 100
+    d_sl3 = ctx.alloc(n3 * s3)
+    d_rec3 = ctx.alloc(r3 * s3)
+    d_sl3.upload(random_bytes(99, n3 * s3))
+    exps3 = np.arange(r3, dtype=np.uint32)
+    for k, nm in ((1, "gf_lds_2000x100"), (2, "gf_mfma_2000x100")):
+        ctx.set_gf_kernel(k)
+        timed(nm, lambda: ctx.encode_dev(d_sl3, n3, s3, exps3, d_rec3), args.steps)
+    ctx.set_gf_kernel(0)
+    d_sl3.free()
+    d_rec3.free()
     timed("crc32_only", lambda: ctx.hash_slices_dev(d_sl, n, s, None, d_crc), args.steps)
     return out
 
 
 def bench_repair(args, ctx):
+    """Plain repair: 2000 x 512 KiB slices (~1 GiB), 100 erased at seeded
+    positions, 100 recovery slices present.  One step = par2hip_repair_set_dev:
+    CRC32 + MD5 verify of every slice against its IFSC entry, Gauss-Jordan of
+    the 100 x 100 erasure matrix, fold, and the rebuild contraction."""
     n, s, m = 2000, 512 << 10, 100
     data = random_bytes(4321, n * s).reshape(n, s)
     exps = np.arange(m, dtype=np.uint32)
-    rec = ctx.encode(data, exps)
-    rng = np.random.default_rng(77)
-    missing = np.sort(rng.choice(n, m, replace=False)).astype(np.uint32)
-    damaged = data.copy()
-    damaged[missing] = 0
     d_sl = ctx.alloc(n * s)
     d_rec = ctx.alloc(m * s)
     d_out = ctx.alloc(m * s)
     d_md5 = ctx.alloc(n * 16)
     d_crc = ctx.alloc(n * 4)
+    d_sl.upload(data)
+    ctx.encode_dev(d_sl, n, s, exps, d_rec, d_md5, d_crc)  # IFSC entries + recovery of the intact set
+    ctx.sync()
+    md5 = d_md5.download().reshape(n, 16)
+    crc = d_crc.download().view(np.uint32)
+    rng = np.random.default_rng(77)
+    missing = np.sort(rng.choice(n, m, replace=False)).astype(np.uint32)
+    damaged = data.copy()
+    damaged[missing] = 0
     d_sl.upload(damaged)
-    d_rec.upload(rec)
     if args.kernel:
         ctx.set_gf_kernel(args.kernel)
 
     def step():
-        ctx.hash_slices_dev(d_sl, n, s, d_md5, d_crc)
-        ctx.reconstruct_dev(d_sl, n, s, missing, d_rec, exps, d_out)
+        return ctx.repair_set_dev(d_sl, n, s, md5, crc, d_rec, exps, d_out)
 
     for _ in range(args.warmup):
         step()
@@ -200,14 +218,14 @@ def bench_repair(args, ctx):
     t_start = time.perf_counter()
     for _ in range(args.steps):
         t0 = time.perf_counter()
-        step()
+        got = step()
         ctx.sync()
         lat.append(time.perf_counter() - t0)
     total = time.perf_counter() - t_start
     stats = ctx.profile_read()
     ctx.profile(False)
     out = d_out.download().reshape(m, s)
-    exact = bool(np.array_equal(out, data[missing]))
+    exact = bool(np.array_equal(got, missing) and np.array_equal(out, data[missing]))
     gf_name = "gf_muladd_mfma" if "gf_muladd_mfma" in stats else "gf_muladd_lds"
     launches, gf_ms = stats.get(gf_name, (0, 0.0))
     gf_avg = gf_ms / max(launches, 1)
@@ -216,7 +234,7 @@ def bench_repair(args, ctx):
     return {
         "workload": "plain_repair",
         "config": {"workload": "plain_repair_1GiB", "n_slices": n, "slice_bytes": s, "erasures": m,
-                   "recovery_slices": m, "resident": "hbm", "verify": "md5+crc32 of all slices"},
+                   "recovery_slices": m, "resident": "hbm", "verify": "crc32+md5 of every slice vs IFSC"},
         "src_bytes": n * s, "total_s": total, "lat": lat, "exact": exact,
         "kernel_ms": {k: v[1] / max(v[0], 1) for k, v in stats.items()},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",

@@ -133,12 +133,36 @@ int32_t par2hip_reconstruct(par2hip_ctx *ctx, const uint8_t *slices, size_t n, s
                             const uint32_t *missing, size_t m, const uint8_t *recovery,
                             const uint32_t *rec_exps, uint8_t *out);
 
-/* Device-resident form (asynchronous; d_out may alias the missing slices'
- * rows of d_slices only if out_in_place is non-zero, in which case d_out is
- * ignored and rebuilt slices are written into d_slices). */
+/* Device-resident form.  The call waits for the (short) inverse so it can
+ * report PAR2HIP_ESINGULAR, then queues the rebuild on the ctx stream
+ * (complete at par2hip_sync).  With out_in_place non-zero the rebuilt slices
+ * are written into their rows of d_slices and d_out is ignored. */
 int32_t par2hip_reconstruct_dev(par2hip_ctx *ctx, void *d_slices, size_t n, size_t slice_size,
                                 const uint32_t *missing, size_t m, const void *d_recovery,
                                 const uint32_t *rec_exps, void *d_out, int32_t out_in_place);
+
+/* Verify a device-resident set against its IFSC checksums and rebuild the
+ * damaged slices (replaces verify_file_blocks, src/verify.rs:548-654, the
+ * damaged-slice collection and recovery choice of src/repair.rs:536-604, and
+ * the transform + chunk loop, repair.rs:848-970).
+ * d_slices:      n zero-padded slices of slice_size bytes.
+ * expected_md5 / expected_crc: host arrays, the IFSC entries (n*16 B, n u32).
+ * d_recovery:    nrec recovery slices (contiguous), exponents rec_exps[nrec].
+ * d_out:         NULL = rebuild in place in d_slices; otherwise room for
+ *                min(n, nrec) slices, row k = rebuilt damaged_out[k].
+ * damaged_out:   host, capacity n: the damaged indices, ascending
+ *                (damaged = CRC32 or MD5 mismatch, verify.rs:621-645);
+ *                *ndamaged receives their count.
+ * The ndamaged lowest recovery exponents are used (repair.rs:584-604).
+ * CRC32 picks the erasures first and the rebuild starts while the MD5
+ * confirmation still runs; a slice whose CRC matches but MD5 does not is
+ * added and the rebuild redone.  Errors: PAR2HIP_ERECOVERY (more damaged
+ * slices than recovery slices; damaged_out is still filled),
+ * PAR2HIP_ESINGULAR. */
+int32_t par2hip_repair_set_dev(par2hip_ctx *ctx, void *d_slices, size_t n, size_t slice_size,
+                               const uint8_t *expected_md5, const uint32_t *expected_crc,
+                               const void *d_recovery, const uint32_t *rec_exps, size_t nrec,
+                               void *d_out, uint32_t *damaged_out, size_t *ndamaged);
 
 /* -------------------------------------------------------------- hashes -- */
 

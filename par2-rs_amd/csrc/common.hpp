@@ -65,8 +65,11 @@ struct DevBuf {
 struct par2hip_ctx {
   int device = 0;
   hipStream_t stream = nullptr;       // everything is ordered on this stream
-  hipStream_t side = nullptr;         // hashing overlaps here, joined per call
-  hipEvent_t fork = nullptr, join = nullptr;
+  hipStream_t side = nullptr;         // MD5 chains overlap here (a few reserved CUs), joined per call
+  hipStream_t rest = nullptr;         // the other CUs, for work that overlaps the side stream
+  hipStream_t cur = nullptr;          // stream the current call's main work goes to
+  bool masked = false;                // side/rest carry complementary CU masks
+  hipEvent_t fork = nullptr, join = nullptr, join2 = nullptr;
   std::recursive_mutex mu;
   std::string err;
   uint16_t *d_exp = nullptr;          // device copy of the doubled exp table
@@ -74,8 +77,9 @@ struct par2hip_ctx {
   uint32_t *d_crc_tab = nullptr;      // CRC-32 byte table + shift powers
   int *d_status = nullptr;            // device status word (singular flag)
   par2hip::ParamArena arena;
-  par2hip::DevBuf scratch[8];         // grow-only device scratch, by role
+  par2hip::DevBuf scratch[10];        // grow-only device scratch, by role
   int gf_kernel = 0;                  // 0 auto, 1 LDS tables, 2 MFMA
+  int num_cus = 256;                  // compute units of the device
   bool profile = false;
   std::vector<par2hip::KernelStat> stats;
   std::vector<par2hip::PendingTiming> pending;
@@ -93,6 +97,8 @@ enum ScratchRole {
   kScratchHash = 5,    // host-call staging: digests
   kScratchAux = 6,
   kScratchAux2 = 7,
+  kScratchVand = 8,    // repair: V rows c_present^e (fold sources)
+  kScratchHashD = 9,   // repair: digests of the verify pass
 };
 
 int32_t fail(par2hip_ctx *ctx, int32_t code, const std::string &msg);
@@ -139,11 +145,13 @@ int32_t launch_coeff_from_logs(par2hip_ctx *ctx, hipStream_t s, uint16_t *coeff,
 struct GfProblem {
   const uint16_t *const *srcs;  // device array of nsrc device pointers
   uint16_t *const *dsts;        // device array of ndst device pointers
-  const uint16_t *coeff;        // device [ndst][nsrc]
+  const uint16_t *coeff;        // device [ndst][ldc]
   int nsrc, ndst;
+  int ldc = 0;                  // row stride of coeff in entries (0: nsrc)
   size_t words;
   bool accumulate;
   int align_words;              // every pointer and `words` stride aligned to this many words
+  const char *tag = nullptr;    // kernel name for timing (default: gf_muladd_<variant>)
 };
 int32_t run_gf(par2hip_ctx *ctx, hipStream_t s, const GfProblem &p);
 int32_t run_gf_lds(par2hip_ctx *ctx, hipStream_t s, const GfProblem &p);
@@ -152,8 +160,11 @@ int32_t run_gf_mfma(par2hip_ctx *ctx, hipStream_t s, const GfProblem &p);
 constexpr int kAutoGfKernel = 1;
 
 // Slice hashes.
+// high_priority: the MD5 chains are the critical path of the caller, so their
+// waves issue ahead of co-resident waves of other kernels.
 int32_t launch_md5(par2hip_ctx *ctx, hipStream_t s, const uint8_t *d_data, size_t n,
-                   size_t slice_size, size_t stride, const uint64_t *d_lens, uint32_t *d_md5);
+                   size_t slice_size, size_t stride, const uint64_t *d_lens, uint32_t *d_md5,
+                   bool high_priority = false);
 int32_t launch_crc32(par2hip_ctx *ctx, hipStream_t s, const uint8_t *d_data, size_t n,
                      size_t slice_size, size_t stride, const uint64_t *d_lens, uint32_t *d_crc);
 int32_t crc_tables_init(par2hip_ctx *ctx);
@@ -165,9 +176,16 @@ int32_t launch_gauss_jordan(par2hip_ctx *ctx, hipStream_t s, uint16_t *d_aug, in
 // A[i][k] = exp[(miss_log[k]*exps[i]) mod 65535] into the left half of aug, I on the right.
 int32_t launch_build_aug(par2hip_ctx *ctx, hipStream_t s, uint16_t *d_aug, const uint32_t *d_miss_log,
                          const uint32_t *d_exps, int m);
-// coeff[k][j]   = XOR_i inv[k][i] * c_{present_j}^{e_i}   for j < npresent,
-// coeff[k][np+i] = inv[k][i];  inv is the right half of aug.
-int32_t launch_fold(par2hip_ctx *ctx, hipStream_t s, uint16_t *coeff, const uint16_t *d_aug,
-                    const uint32_t *d_present_log, const uint32_t *d_exps, int m, int npresent);
+// Closed-form inverse for consecutive exponents e0, e0+1, ...: writes A^-1
+// into the right half of the m x 2m workspace.
+constexpr int kVandMaxM = 512;
+int32_t launch_vandermonde_inverse(par2hip_ctx *ctx, hipStream_t s, uint16_t *d_aug, const uint32_t *d_miss_log,
+                                   uint32_t e0, int m);
+// vand[i][j] = c_{present_j}^{e_i} (row stride vs words).
+int32_t launch_build_vand(par2hip_ctx *ctx, hipStream_t s, uint16_t *vand, size_t vs,
+                          const uint32_t *d_present_log, const uint32_t *d_exps, int m, int npresent);
+// coeff[k][np + i] = inv[k][i] (inv = right half of aug); row stride cs.
+int32_t launch_copy_inverse(par2hip_ctx *ctx, hipStream_t s, uint16_t *coeff, size_t cs, const uint16_t *d_aug,
+                            int m, int npresent);
 
 }  // namespace par2hip

@@ -24,8 +24,11 @@ __global__ void k_coeff_from_logs(uint16_t *__restrict__ coeff, const uint32_t *
   const long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (idx >= (long)nrow * nsrc) return;
   const int r = (int)(idx / nsrc), d = (int)(idx % nsrc);
-  // c_d^e = 2^(n_d * e); exponents reduce mod 65535 as in gf_pow (core.rs:256)
-  const uint64_t l = ((uint64_t)src_log[d] * exps[r]) % kOrder;
+  // c_d^e = 2^(n_d * e) with e already reduced mod 65535 (as gf_pow does,
+  // core.rs:256): n_d * e < 2^32, folded twice into [0, 65535]
+  uint32_t l = src_log[d] * exps[r];
+  l = (l & 0xFFFFu) + (l >> 16);
+  l = (l & 0xFFFFu) + (l >> 16);
   coeff[idx] = exp_tab[l];
 }
 
@@ -46,7 +49,7 @@ int32_t launch_coeff_from_logs(par2hip_ctx *ctx, hipStream_t s, uint16_t *coeff,
 // d = source, i = nibble position, v = nibble value; entry packs the four
 // rows' products c_{4g+j,d} * (v << 4i) at bits [16j, 16j+16).
 __global__ void k_nibble_tables(uint64_t *__restrict__ tabs, const uint16_t *__restrict__ coeff,
-                                int nrow, int nsrc, const uint16_t *__restrict__ log_tab,
+                                int nrow, int nsrc, int ldc, const uint16_t *__restrict__ log_tab,
                                 const uint16_t *__restrict__ exp_tab) {
   const long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
   const int ngroups = (nrow + 3) / 4;
@@ -63,7 +66,7 @@ __global__ void k_nibble_tables(uint64_t *__restrict__ tabs, const uint16_t *__r
     for (int j = 0; j < 4; j++) {
       const int r = 4 * g + j;
       if (r < nrow) {
-        const uint16_t c = coeff[(long)r * nsrc + d];
+        const uint16_t c = coeff[(long)r * ldc + d];
         if (c) e |= (uint64_t)exp_tab[lx + log_tab[c]] << (16 * j);
       }
     }
@@ -210,7 +213,7 @@ static int32_t launch_lds_variant(par2hip_ctx *ctx, hipStream_t s, const GfProbl
   const int tiles_per_xcd = (ntiles + 7) / 8;
   const long nwg = 8L * tiles_per_xcd * nrowblk;
   if (nwg > 0x7fffffffL) return fail(ctx, PAR2HIP_EINVAL, "GF problem too large for one launch");
-  ScopedTiming t(ctx, "gf_muladd_lds", s);
+  ScopedTiming t(ctx, p.tag ? p.tag : "gf_muladd_lds", s);
   hipLaunchKernelGGL(k_gf_lds<V>, dim3((unsigned)nwg), dim3(kTpb), 0, s, p.srcs, p.nsrc, p.dsts,
                      p.ndst, tabs, (long)p.words, p.accumulate ? 1 : 0, ntiles, nrowblk);
   P2H_CHECK(ctx, hipGetLastError());
@@ -227,12 +230,20 @@ int32_t run_gf_lds(par2hip_ctx *ctx, hipStream_t s, const GfProblem &p) {
     const long total = (long)ngroups * p.nsrc * 64;
     ScopedTiming t(ctx, "nibble_tables", s);
     hipLaunchKernelGGL(k_nibble_tables, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, s,
-                       (uint64_t *)tabs, p.coeff, p.ndst, p.nsrc, ctx->d_log, ctx->d_exp);
+                       (uint64_t *)tabs, p.coeff, p.ndst, p.nsrc, p.ldc ? p.ldc : p.nsrc, ctx->d_log, ctx->d_exp);
     P2H_CHECK(ctx, hipGetLastError());
   }
-  if (p.align_words % 8 == 0) return launch_lds_variant<8>(ctx, s, p, (const uint64_t *)tabs);
-  if (p.align_words % 4 == 0) return launch_lds_variant<4>(ctx, s, p, (const uint64_t *)tabs);
-  if (p.align_words % 2 == 0) return launch_lds_variant<2>(ctx, s, p, (const uint64_t *)tabs);
+  // words per thread: the widest the alignment allows (per-workgroup table
+  // staging is amortised over more words); narrower only when the grid would
+  // not even give every resident workgroup slot (2 per CU) one workgroup
+  const int nrowblk = (ngroups + kGpw - 1) / kGpw;
+  const long slots = 2L * ctx->num_cus;
+  auto nwg = [&](int v) { return (long)((p.words + (size_t)kTpb * v - 1) / ((size_t)kTpb * v)) * nrowblk; };
+  int v = p.align_words % 8 == 0 ? 8 : p.align_words % 4 == 0 ? 4 : p.align_words % 2 == 0 ? 2 : 1;
+  while (v > 1 && nwg(v) < slots) v /= 2;
+  if (v == 8) return launch_lds_variant<8>(ctx, s, p, (const uint64_t *)tabs);
+  if (v == 4) return launch_lds_variant<4>(ctx, s, p, (const uint64_t *)tabs);
+  if (v == 2) return launch_lds_variant<2>(ctx, s, p, (const uint64_t *)tabs);
   return launch_lds_variant<1>(ctx, s, p, (const uint64_t *)tabs);
 }
 

@@ -47,7 +47,7 @@ __device__ __forceinline__ uint16_t mul_x(uint16_t v) {
 // Apk[mf][ks][lane]: the A operand of fragment mf (outputs 2mf, 2mf+1) at
 // K-step ks, packed as two 16-bit rows of the transposed bit matrices.
 __global__ void k_pack_a(uint32_t *__restrict__ apk, const uint16_t *__restrict__ coeff, int nrow,
-                         int nsrc, int nmf, int nks) {
+                         int nsrc, int ldc, int nmf, int nks) {
   const long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (idx >= (long)nmf * nks * 64) return;
   const int lane = (int)(idx & 63);
@@ -62,7 +62,7 @@ __global__ void k_pack_a(uint32_t *__restrict__ apk, const uint16_t *__restrict_
     const int d = 4 * ks + 2 * g + s;
     uint32_t t = 0;
     if (r < nrow && d < nsrc) {
-      uint16_t v = coeff[(long)r * nsrc + d];
+      uint16_t v = coeff[(long)r * ldc + d];
 #pragma unroll
       for (int i = 0; i < 16; i++) {
         t |= (uint32_t)((v >> j) & 1u) << i;
@@ -170,7 +170,9 @@ __global__ __launch_bounds__(kMfThreads, 1) void k_gf_mfma(const uint16_t *const
     const int buf = c & 1;
     if (c + 1 < nchunks) load_stage(c + 1);
     // all 8 K-steps of the stage (K-steps past the last source carry zero
-    // operands); each wave expands its own B columns, A comes pre-expanded
+    // operands); each wave expands its own B columns, A comes pre-expanded.
+    // Kept rolled: unrolled, the operands of 8 steps spill at 2 waves/SIMD.
+#pragma unroll 1
     for (int ksl = 0; ksl < kMfKc; ksl++) {
       const uint32_t x0 = reinterpret_cast<const uint32_t *>(&sB[buf][4 * ksl + 2 * g][0])[bcol >> 1];
       const uint32_t x1 = reinterpret_cast<const uint32_t *>(&sB[buf][4 * ksl + 2 * g + 1][0])[bcol >> 1];
@@ -229,14 +231,14 @@ int32_t run_gf_mfma(par2hip_ctx *ctx, hipStream_t s, const GfProblem &p) {
     const long total = (long)nmf * nks * 64;
     ScopedTiming t(ctx, "pack_bitmatrices", s);
     hipLaunchKernelGGL(k_pack_a, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, s, (uint32_t *)apk, p.coeff,
-                       p.ndst, p.nsrc, nmf, nks);
+                       p.ndst, p.nsrc, p.ldc ? p.ldc : p.nsrc, nmf, nks);
     P2H_CHECK(ctx, hipGetLastError());
   }
   const int ntiles = (int)((p.words + kMfWgWords - 1) / kMfWgWords);
   const int nrowblk = (nmf + kMfRows - 1) / kMfRows;
   const long nwg = 8L * ((ntiles + 7) / 8) * nrowblk;
   if (nwg > 0x7fffffffL) return fail(ctx, PAR2HIP_EINVAL, "GF problem too large for one launch");
-  ScopedTiming t(ctx, "gf_muladd_mfma", s);
+  ScopedTiming t(ctx, p.tag ? p.tag : "gf_muladd_mfma", s);
   hipLaunchKernelGGL(k_gf_mfma, dim3((unsigned)nwg), dim3(kMfThreads), 0, s, p.srcs, p.nsrc, p.dsts, p.ndst,
                      (const uint32_t *)apk, nmf, nks, (long)p.words, p.accumulate ? 1 : 0, ntiles, nrowblk);
   P2H_CHECK(ctx, hipGetLastError());

@@ -137,10 +137,10 @@ template <int kAlign>
 __global__ __launch_bounds__(64) void k_md5_slices(const uint8_t *__restrict__ data, long n,
                                                    uint64_t slice_size, uint64_t stride,
                                                    const uint64_t *__restrict__ lens,
-                                                   uint32_t *__restrict__ out) {
-  // the chain latency of these few waves bounds the whole step: let them
-  // issue ahead of co-resident waves of other kernels
-  __builtin_amdgcn_s_setprio(3);
+                                                   uint32_t *__restrict__ out, int high_priority) {
+  // when these few chains bound the caller's step, let them issue ahead of
+  // co-resident waves of other kernels
+  if (high_priority) __builtin_amdgcn_s_setprio(3);
   const long i = (long)blockIdx.x * 64 + threadIdx.x;
   if (i >= n) return;
   const uint8_t *p = data + (uint64_t)i * stride;
@@ -229,7 +229,8 @@ __global__ __launch_bounds__(64) void k_md5_slices(const uint8_t *__restrict__ d
 }
 
 int32_t launch_md5(par2hip_ctx *ctx, hipStream_t s, const uint8_t *d_data, size_t n,
-                   size_t slice_size, size_t stride, const uint64_t *d_lens, uint32_t *d_md5) {
+                   size_t slice_size, size_t stride, const uint64_t *d_lens, uint32_t *d_md5,
+                   bool high_priority) {
   if (n == 0) return PAR2HIP_OK;
   const bool a16 = ((uintptr_t)d_data % 16 == 0) && (stride % 16 == 0);
   const bool a4 = ((uintptr_t)d_data % 4 == 0) && (stride % 4 == 0);
@@ -237,13 +238,13 @@ int32_t launch_md5(par2hip_ctx *ctx, hipStream_t s, const uint8_t *d_data, size_
   dim3 grid((unsigned)((n + 63) / 64));
   if (a16)
     hipLaunchKernelGGL(k_md5_slices<16>, grid, dim3(64), 0, s, d_data, (long)n, (uint64_t)slice_size,
-                       (uint64_t)stride, d_lens, d_md5);
+                       (uint64_t)stride, d_lens, d_md5, high_priority ? 1 : 0);
   else if (a4)
     hipLaunchKernelGGL(k_md5_slices<4>, grid, dim3(64), 0, s, d_data, (long)n, (uint64_t)slice_size,
-                       (uint64_t)stride, d_lens, d_md5);
+                       (uint64_t)stride, d_lens, d_md5, high_priority ? 1 : 0);
   else
     hipLaunchKernelGGL(k_md5_slices<1>, grid, dim3(64), 0, s, d_data, (long)n, (uint64_t)slice_size,
-                       (uint64_t)stride, d_lens, d_md5);
+                       (uint64_t)stride, d_lens, d_md5, high_priority ? 1 : 0);
   P2H_CHECK(ctx, hipGetLastError());
   return PAR2HIP_OK;
 }

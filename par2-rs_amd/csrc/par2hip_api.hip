@@ -56,6 +56,7 @@ int32_t scratch(par2hip_ctx *ctx, int role, size_t bytes, void **out) {
       // the buffer may still be read by queued work
       P2H_CHECK(ctx, hipStreamSynchronize(ctx->stream));
       P2H_CHECK(ctx, hipStreamSynchronize(ctx->side));
+      P2H_CHECK(ctx, hipStreamSynchronize(ctx->rest));
       P2H_CHECK(ctx, hipFree(b.ptr));
       b.ptr = nullptr;
       b.bytes = 0;
@@ -75,6 +76,7 @@ int32_t params_begin(par2hip_ctx *ctx, size_t bytes) {
   if (a.cap < bytes) {
     P2H_CHECK(ctx, hipStreamSynchronize(ctx->stream));
     P2H_CHECK(ctx, hipStreamSynchronize(ctx->side));
+    P2H_CHECK(ctx, hipStreamSynchronize(ctx->rest));
     if (a.host) (void)hipHostFree(a.host);
     if (a.dev) (void)hipFree(a.dev);
     a.host = nullptr;
@@ -99,8 +101,8 @@ void *params_alloc(par2hip_ctx *ctx, size_t bytes, void **dev) {
 int32_t params_commit(par2hip_ctx *ctx) {
   ParamArena &a = ctx->arena;
   if (a.used) {
-    P2H_CHECK(ctx, hipMemcpyAsync(a.dev, a.host, a.used, hipMemcpyHostToDevice, ctx->stream));
-    P2H_CHECK(ctx, hipEventRecord(a.copied, ctx->stream));
+    P2H_CHECK(ctx, hipMemcpyAsync(a.dev, a.host, a.used, hipMemcpyHostToDevice, ctx->cur));
+    P2H_CHECK(ctx, hipEventRecord(a.copied, ctx->cur));
   }
   return PAR2HIP_OK;
 }
@@ -171,18 +173,35 @@ void probe_devices() {
   g_ndev = ok;
 }
 
-// fork the side stream off the main stream (it may read what the main stream wrote)
+// Overlapped section: the side stream (MD5 chains, a few reserved CUs) and
+// the rest stream (all other CUs) both start after everything queued on the
+// main stream; the call's main work then goes to `rest` until join.
 int32_t fork_side(par2hip_ctx *ctx) {
   P2H_CHECK(ctx, hipEventRecord(ctx->fork, ctx->stream));
   P2H_CHECK(ctx, hipStreamWaitEvent(ctx->side, ctx->fork, 0));
+  if (ctx->rest != ctx->stream) P2H_CHECK(ctx, hipStreamWaitEvent(ctx->rest, ctx->fork, 0));
+  ctx->cur = ctx->rest;
   return PAR2HIP_OK;
 }
 
 int32_t join_side(par2hip_ctx *ctx) {
   P2H_CHECK(ctx, hipEventRecord(ctx->join, ctx->side));
   P2H_CHECK(ctx, hipStreamWaitEvent(ctx->stream, ctx->join, 0));
+  if (ctx->rest != ctx->stream) {
+    P2H_CHECK(ctx, hipEventRecord(ctx->join2, ctx->rest));
+    P2H_CHECK(ctx, hipStreamWaitEvent(ctx->stream, ctx->join2, 0));
+  }
+  ctx->cur = ctx->stream;
   return PAR2HIP_OK;
 }
+
+// Forks on construction, joins on destruction (also on early error returns).
+struct Overlap {
+  par2hip_ctx *ctx;
+  int32_t rc;
+  explicit Overlap(par2hip_ctx *c) : ctx(c) { rc = fork_side(c); }
+  ~Overlap() { (void)join_side(ctx); }
+};
 
 int align_words_of(uintptr_t p) {
   int a = 8;
@@ -200,31 +219,34 @@ int32_t encode_dev_impl(par2hip_ctx *ctx, const uint8_t *d_slices, size_t n, siz
   if (r > 65535 || (r && !exps)) return fail(ctx, PAR2HIP_EINVAL, "bad recovery exponent list");
   const size_t words = slice_size / 2;
   const std::vector<uint32_t> logs = constant_logs(n);
-  int32_t rc = params_begin(ctx, params_size({n * 4, r * 4, n * 8, r * 8}));
-  if (rc) return rc;
+  void *coeff = nullptr;
+  int32_t rc;
+  if (r && (rc = scratch(ctx, kScratchCoeff, r * n * 2, &coeff))) return rc;
+  if ((rc = params_begin(ctx, params_size({n * 4, r * 4, n * 8, r * 8})))) return rc;
   void *d_logs, *d_exps, *d_srcs, *d_dsts;
   uint32_t *h_logs = (uint32_t *)params_alloc(ctx, n * 4, &d_logs);
   uint32_t *h_exps = (uint32_t *)params_alloc(ctx, std::max<size_t>(r, 1) * 4, &d_exps);
   const uint16_t **h_srcs = (const uint16_t **)params_alloc(ctx, n * 8, &d_srcs);
   uint16_t **h_dsts = (uint16_t **)params_alloc(ctx, std::max<size_t>(r, 1) * 8, &d_dsts);
   memcpy(h_logs, logs.data(), n * 4);
-  if (r) memcpy(h_exps, exps, r * 4);
+  for (size_t i = 0; i < r; i++) h_exps[i] = exps[i] % kOrder;  // c^e = c^(e mod 65535)
   for (size_t d = 0; d < n; d++) h_srcs[d] = (const uint16_t *)(d_slices + d * slice_size);
   for (size_t i = 0; i < r; i++) h_dsts[i] = (uint16_t *)(d_rec + i * slice_size);
   if ((rc = params_commit(ctx))) return rc;
 
-  // MD5 is a long serial chain per slice: it runs on the side stream from the
-  // start and the contraction and CRC-32 share the rest of the device.
+  // MD5 is a long serial chain per slice: it runs on the side stream's CUs
+  // from the start while CRC-32 and the contraction use the other CUs.
+  std::unique_ptr<Overlap> ov;
   if (d_md5) {
-    if ((rc = fork_side(ctx))) return rc;
-    if ((rc = launch_md5(ctx, ctx->side, d_slices, n, slice_size, slice_size, nullptr, d_md5))) return rc;
+    ov.reset(new Overlap(ctx));
+    if (ov->rc) return ov->rc;
+    if ((rc = launch_md5(ctx, ctx->side, d_slices, n, slice_size, slice_size, nullptr, d_md5, !ctx->masked)))
+      return rc;
   }
-  if (d_crc && (rc = launch_crc32(ctx, ctx->stream, d_slices, n, slice_size, slice_size, nullptr, d_crc)))
+  if (d_crc && (rc = launch_crc32(ctx, ctx->cur, d_slices, n, slice_size, slice_size, nullptr, d_crc)))
     return rc;
   if (r) {
-    void *coeff;
-    if ((rc = scratch(ctx, kScratchCoeff, r * n * 2, &coeff))) return rc;
-    if ((rc = launch_coeff_from_logs(ctx, ctx->stream, (uint16_t *)coeff, (const uint32_t *)d_logs,
+    if ((rc = launch_coeff_from_logs(ctx, ctx->cur, (uint16_t *)coeff, (const uint32_t *)d_logs,
                                      (const uint32_t *)d_exps, (int)r, (int)n)))
       return rc;
     GfProblem p;
@@ -237,17 +259,19 @@ int32_t encode_dev_impl(par2hip_ctx *ctx, const uint8_t *d_slices, size_t n, siz
     p.accumulate = false;
     p.align_words = std::min({align_words_of((uintptr_t)d_slices), align_words_of((uintptr_t)d_rec),
                               align_words_of(slice_size)});
-    if ((rc = run_gf(ctx, ctx->stream, p))) return rc;
+    if ((rc = run_gf(ctx, ctx->cur, p))) return rc;
   }
-  if (d_md5) return join_side(ctx);
   return PAR2HIP_OK;
 }
 
 // Repair body on device buffers.  Sources of the contraction are the present
 // data slices followed by the m recovery slices.
+// rec_rows (nullable): which rows of d_recovery hold the m recovery slices
+// with exponents rec_exps[0..m) (default: rows 0..m-1).
 int32_t reconstruct_dev_impl(par2hip_ctx *ctx, uint8_t *d_slices, size_t n, size_t slice_size,
                              const uint32_t *missing, size_t m, const uint8_t *d_recovery,
-                             const uint32_t *rec_exps, uint8_t *d_out, bool in_place) {
+                             const uint32_t *rec_exps, uint8_t *d_out, bool in_place,
+                             const uint32_t *rec_rows = nullptr) {
   if (n == 0 || slice_size == 0 || (slice_size & 1))
     return fail(ctx, PAR2HIP_EINVAL, "need n >= 1 and an even, non-zero slice size");
   if (n > 32768) return fail(ctx, PAR2HIP_EINVAL, "more than 32768 input slices repeat PAR2 constants");
@@ -265,16 +289,26 @@ int32_t reconstruct_dev_impl(par2hip_ctx *ctx, uint8_t *d_slices, size_t n, size
   const size_t np = n - m;
   const size_t words = slice_size / 2;
   const std::vector<uint32_t> logs = constant_logs(n);
-  int32_t rc = params_begin(ctx, params_size({m * 4, m * 4, np * 4, (np + m) * 8, m * 8}));
-  if (rc) return rc;
-  void *d_mlog, *d_exps, *d_plog, *d_srcs, *d_dsts;
+  // device workspaces first: the one parameter block of this call holds
+  // pointers into them.  C = [A^-1 V | A^-1] over [present | recovery], rows
+  // padded to a multiple of 8 entries; V rows padded likewise.
+  const size_t cs = (np + m + 7) & ~(size_t)7, vs = (std::max<size_t>(np, 1) + 7) & ~(size_t)7;
+  void *aug, *coeff, *vand;
+  int32_t rc;
+  if ((rc = scratch(ctx, kScratchGj, m * 2 * m * 2, &aug))) return rc;
+  if ((rc = scratch(ctx, kScratchCoeff, m * cs * 2, &coeff))) return rc;
+  if ((rc = scratch(ctx, kScratchVand, m * vs * 2, &vand))) return rc;
+  if ((rc = params_begin(ctx, params_size({m * 4, m * 4, np * 4, (np + m) * 8, m * 8, m * 8, m * 8})))) return rc;
+  void *d_mlog, *d_exps, *d_plog, *d_srcs, *d_dsts, *d_fsrc, *d_fdst;
   uint32_t *h_mlog = (uint32_t *)params_alloc(ctx, m * 4, &d_mlog);
   uint32_t *h_exps = (uint32_t *)params_alloc(ctx, m * 4, &d_exps);
   uint32_t *h_plog = (uint32_t *)params_alloc(ctx, std::max<size_t>(np, 1) * 4, &d_plog);
   const uint16_t **h_srcs = (const uint16_t **)params_alloc(ctx, (np + m) * 8, &d_srcs);
   uint16_t **h_dsts = (uint16_t **)params_alloc(ctx, m * 8, &d_dsts);
+  const uint16_t **h_fsrc = (const uint16_t **)params_alloc(ctx, m * 8, &d_fsrc);
+  uint16_t **h_fdst = (uint16_t **)params_alloc(ctx, m * 8, &d_fdst);
   for (size_t k = 0; k < m; k++) h_mlog[k] = logs[missing[k]];
-  memcpy(h_exps, rec_exps, m * 4);
+  for (size_t i = 0; i < m; i++) h_exps[i] = rec_exps[i] % kOrder;
   size_t j = 0;
   for (size_t d = 0; d < n; d++)
     if (!is_missing[d]) {
@@ -282,39 +316,146 @@ int32_t reconstruct_dev_impl(par2hip_ctx *ctx, uint8_t *d_slices, size_t n, size
       h_srcs[j] = (const uint16_t *)(d_slices + d * slice_size);
       j++;
     }
-  for (size_t i = 0; i < m; i++) h_srcs[np + i] = (const uint16_t *)(d_recovery + i * slice_size);
+  for (size_t i = 0; i < m; i++)
+    h_srcs[np + i] = (const uint16_t *)(d_recovery + (size_t)(rec_rows ? rec_rows[i] : i) * slice_size);
   for (size_t k = 0; k < m; k++)
     h_dsts[k] = (uint16_t *)(in_place ? d_slices + (size_t)missing[k] * slice_size : d_out + k * slice_size);
+  for (size_t i = 0; i < m; i++) {
+    h_fsrc[i] = (const uint16_t *)vand + i * vs;
+    h_fdst[i] = (uint16_t *)coeff + i * cs;
+  }
   if ((rc = params_commit(ctx))) return rc;
 
-  void *aug, *coeff;
-  if ((rc = scratch(ctx, kScratchGj, m * 2 * m * 2, &aug))) return rc;
-  if ((rc = scratch(ctx, kScratchCoeff, m * (np + m) * 2, &coeff))) return rc;
-  if ((rc = launch_build_aug(ctx, ctx->stream, (uint16_t *)aug, (const uint32_t *)d_mlog,
-                             (const uint32_t *)d_exps, (int)m)))
+  if ((rc = launch_build_vand(ctx, ctx->cur, (uint16_t *)vand, vs, (const uint32_t *)d_plog,
+                              (const uint32_t *)d_exps, (int)m, (int)np)))
     return rc;
-  if ((rc = launch_gauss_jordan(ctx, ctx->stream, (uint16_t *)aug, (int)m, ctx->d_status))) return rc;
-  int status = 0;
-  P2H_CHECK(ctx, hipMemcpyAsync(&status, ctx->d_status, sizeof status, hipMemcpyDeviceToHost, ctx->stream));
-  P2H_CHECK(ctx, hipStreamSynchronize(ctx->stream));
-  if (status) return fail(ctx, PAR2HIP_ESINGULAR, "Singular matrix");
-  if ((rc = launch_fold(ctx, ctx->stream, (uint16_t *)coeff, (const uint16_t *)aug, (const uint32_t *)d_plog,
-                        (const uint32_t *)d_exps, (int)m, (int)np)))
+  // consecutive exponents (the lowest ones, as the reference picks them when
+  // no volume is missing) have a closed-form inverse; otherwise Gauss-Jordan
+  bool consecutive = m <= (size_t)kVandMaxM;
+  for (size_t i = 1; i < m && consecutive; i++) consecutive = rec_exps[i] == rec_exps[0] + i;
+  if (consecutive && rec_exps[0] + m - 1 >= kOrder) consecutive = false;
+  if (consecutive) {
+    if ((rc = launch_vandermonde_inverse(ctx, ctx->cur, (uint16_t *)aug, (const uint32_t *)d_mlog, rec_exps[0],
+                                         (int)m)))
+      return rc;
+  } else {
+    if ((rc = launch_build_aug(ctx, ctx->cur, (uint16_t *)aug, (const uint32_t *)d_mlog,
+                               (const uint32_t *)d_exps, (int)m)))
+      return rc;
+    if ((rc = launch_gauss_jordan(ctx, ctx->cur, (uint16_t *)aug, (int)m, ctx->d_status))) return rc;
+    int status = 0;
+    P2H_CHECK(ctx, hipMemcpyAsync(&status, ctx->d_status, sizeof status, hipMemcpyDeviceToHost, ctx->cur));
+    P2H_CHECK(ctx, hipStreamSynchronize(ctx->cur));
+    if (status) return fail(ctx, PAR2HIP_ESINGULAR, "Singular matrix");
+  }
+  if ((rc = launch_copy_inverse(ctx, ctx->cur, (uint16_t *)coeff, cs, (const uint16_t *)aug, (int)m, (int)np)))
     return rc;
+  if (np) {
+    // fold: row k of A^-1 V = XOR_i inv[k][i] * V_i -- the contraction itself,
+    // with V's rows as m sources of np words and C's rows as outputs
+    GfProblem f;
+    f.srcs = (const uint16_t *const *)d_fsrc;
+    f.dsts = (uint16_t *const *)d_fdst;
+    f.coeff = (const uint16_t *)aug + m;  // A^-1, row stride 2m
+    f.ldc = (int)(2 * m);
+    f.nsrc = (int)m;
+    f.ndst = (int)m;
+    f.words = np;
+    f.accumulate = false;
+    f.align_words = 8;
+    f.tag = "gf_fold";
+    if ((rc = run_gf(ctx, ctx->cur, f))) return rc;
+  }
   GfProblem p;
   p.srcs = (const uint16_t *const *)d_srcs;
   p.dsts = (uint16_t *const *)d_dsts;
   p.coeff = (const uint16_t *)coeff;
+  p.ldc = (int)cs;
   p.nsrc = (int)(np + m);
   p.ndst = (int)m;
   p.words = words;
   p.accumulate = false;
   p.align_words = std::min({align_words_of((uintptr_t)d_slices), align_words_of((uintptr_t)d_recovery),
                             in_place ? 8 : align_words_of((uintptr_t)d_out), align_words_of(slice_size)});
-  if (in_place) {
-    // sources never include the missing rows, so writing them in place is safe
+  // sources never include the missing rows, so writing them in place is safe
+  return run_gf(ctx, ctx->cur, p);
+}
+
+int32_t repair_set_impl(par2hip_ctx *ctx, uint8_t *d_slices, size_t n, size_t slice_size,
+                        const uint8_t *expected_md5, const uint32_t *expected_crc, const uint8_t *d_recovery,
+                        const uint32_t *rec_exps, size_t nrec, uint8_t *d_out, uint32_t *damaged_out,
+                        size_t *ndamaged) {
+  if (n == 0 || slice_size == 0 || (slice_size & 1))
+    return fail(ctx, PAR2HIP_EINVAL, "need n >= 1 and an even, non-zero slice size");
+  int32_t rc;
+  void *dh;
+  if ((rc = scratch(ctx, kScratchHashD, n * 20 + 16, &dh))) return rc;
+  uint32_t *d_md5 = (uint32_t *)dh, *d_crc = (uint32_t *)dh + 4 * n;
+  // MD5 (long serial chains) on the side stream's CUs, CRC-32 and the
+  // rebuild on the other CUs
+  Overlap ov(ctx);
+  if (ov.rc) return ov.rc;
+  if ((rc = launch_md5(ctx, ctx->side, d_slices, n, slice_size, slice_size, nullptr, d_md5))) return rc;
+  if ((rc = launch_crc32(ctx, ctx->cur, d_slices, n, slice_size, slice_size, nullptr, d_crc))) return rc;
+  std::vector<uint32_t> crc(n);
+  P2H_CHECK(ctx, hipMemcpyAsync(crc.data(), d_crc, n * 4, hipMemcpyDeviceToHost, ctx->cur));
+  P2H_CHECK(ctx, hipStreamSynchronize(ctx->cur));
+  std::vector<char> bad(n, 0);
+  std::vector<uint32_t> dmg;
+  for (size_t i = 0; i < n; i++)
+    if (crc[i] != expected_crc[i]) {
+      bad[i] = 1;
+      dmg.push_back((uint32_t)i);
+    }
+  // recovery slices in ascending exponent order (repair.rs:584-604)
+  std::vector<uint32_t> order(nrec);
+  for (size_t i = 0; i < nrec; i++) order[i] = (uint32_t)i;
+  std::stable_sort(order.begin(), order.end(), [&](uint32_t a, uint32_t b) { return rec_exps[a] < rec_exps[b]; });
+  auto rebuild = [&](const std::vector<uint32_t> &d) -> int32_t {
+    if (d.empty()) return PAR2HIP_OK;
+    const size_t m = d.size();
+    std::vector<uint32_t> rows(order.begin(), order.begin() + m), exps(m);
+    for (size_t k = 0; k < m; k++) exps[k] = rec_exps[rows[k]];
+    return reconstruct_dev_impl(ctx, d_slices, n, slice_size, d.data(), m, d_recovery, exps.data(), d_out,
+                                d_out == nullptr, rows.data());
+  };
+  bool rebuilt = false;
+  if (dmg.size() <= nrec) {
+    // speculative: MD5 may still add slices.  An in-place rebuild rewrites the
+    // CRC-damaged rows while the MD5 pass still reads them; their digests are
+    // never used (only slices whose CRC matched are confirmed by MD5).
+    if ((rc = rebuild(dmg))) return rc;
+    rebuilt = true;
   }
-  return run_gf(ctx, ctx->stream, p);
+  // MD5 confirmation of every slice whose CRC matched (verify.rs:631-645)
+  std::vector<uint8_t> md5(n * 16);
+  P2H_CHECK(ctx, hipMemcpyAsync(md5.data(), d_md5, n * 16, hipMemcpyDeviceToHost, ctx->side));
+  P2H_CHECK(ctx, hipStreamSynchronize(ctx->side));
+  bool more = false;
+  for (size_t i = 0; i < n; i++)
+    if (!bad[i] && memcmp(&md5[i * 16], expected_md5 + i * 16, 16) != 0) {
+      bad[i] = 1;
+      more = true;
+    }
+  if (more) {
+    dmg.clear();
+    for (size_t i = 0; i < n; i++)
+      if (bad[i]) dmg.push_back((uint32_t)i);
+  }
+  for (size_t k = 0; k < dmg.size(); k++) damaged_out[k] = dmg[k];
+  *ndamaged = dmg.size();
+  if (dmg.size() > nrec) {
+    P2H_CHECK(ctx, hipStreamSynchronize(ctx->cur));
+    char buf[128];
+    snprintf(buf, sizeof buf, "Need %zu recovery blocks, have %zu", dmg.size(), nrec);
+    return fail(ctx, PAR2HIP_ERECOVERY, buf);
+  }
+  if (more || !rebuilt) {
+    // the first rebuild (if any) treated a damaged slice as intact: redo it
+    if ((rc = rebuild(dmg))) return rc;
+  }
+  P2H_CHECK(ctx, hipStreamSynchronize(ctx->cur));
+  return PAR2HIP_OK;
 }
 
 }  // namespace
@@ -339,10 +480,33 @@ int32_t par2hip_ctx_create(int32_t device, par2hip_ctx **out) {
   c->device = device;
   if (hipSetDevice(device) != hipSuccess) return PAR2HIP_ENODEV;
   par2hip_ctx *ctx = c.get();
+  hipDeviceProp_t prop;
+  if (hipGetDeviceProperties(&prop, device) == hipSuccess && prop.multiProcessorCount > 0)
+    ctx->num_cus = prop.multiProcessorCount;
   P2H_CHECK(ctx, hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking));
-  P2H_CHECK(ctx, hipStreamCreateWithFlags(&ctx->side, hipStreamNonBlocking));
+  ctx->cur = ctx->stream;
+  {
+    // MD5 chains get a few CUs of their own (enough for 64 chains per SIMD-wave,
+    // 2048 slices); overlapped work uses the complementary mask.  Without CU
+    // masks the side stream is an ordinary stream and `rest` is the main stream.
+    const int ncu = ctx->num_cus, hash_cus = ncu >= 64 ? 8 : 0;
+    std::vector<uint32_t> side_mask((ncu + 31) / 32, 0u), rest_mask((ncu + 31) / 32, 0u);
+    for (int c = 0; c < ncu; c++) (c < hash_cus ? side_mask : rest_mask)[c / 32] |= 1u << (c % 32);
+    bool ok = hash_cus > 0 &&
+              hipExtStreamCreateWithCUMask(&ctx->side, (uint32_t)side_mask.size(), side_mask.data()) == hipSuccess;
+    if (ok && hipExtStreamCreateWithCUMask(&ctx->rest, (uint32_t)rest_mask.size(), rest_mask.data()) != hipSuccess) {
+      (void)hipStreamDestroy(ctx->side);
+      ok = false;
+    }
+    ctx->masked = ok;
+    if (!ok) {
+      P2H_CHECK(ctx, hipStreamCreateWithFlags(&ctx->side, hipStreamNonBlocking));
+      ctx->rest = ctx->stream;
+    }
+  }
   P2H_CHECK(ctx, hipEventCreateWithFlags(&ctx->fork, hipEventDisableTiming));
   P2H_CHECK(ctx, hipEventCreateWithFlags(&ctx->join, hipEventDisableTiming));
+  P2H_CHECK(ctx, hipEventCreateWithFlags(&ctx->join2, hipEventDisableTiming));
   P2H_CHECK(ctx, hipEventCreateWithFlags(&ctx->arena.copied, hipEventDisableTiming));
   const HostField &f = field();
   P2H_CHECK(ctx, hipMalloc(&ctx->d_exp, sizeof f.exp));
@@ -362,6 +526,7 @@ int32_t par2hip_ctx_destroy(par2hip_ctx *ctx) {
     Guard g(ctx);
     (void)hipStreamSynchronize(ctx->stream);
     (void)hipStreamSynchronize(ctx->side);
+    if (ctx->rest != ctx->stream) (void)hipStreamSynchronize(ctx->rest);
     for (auto &b : ctx->scratch)
       if (b.ptr) (void)hipFree(b.ptr);
     if (ctx->arena.host) (void)hipHostFree(ctx->arena.host);
@@ -377,6 +542,8 @@ int32_t par2hip_ctx_destroy(par2hip_ctx *ctx) {
     for (auto e : ctx->event_pool) (void)hipEventDestroy(e);
     (void)hipEventDestroy(ctx->fork);
     (void)hipEventDestroy(ctx->join);
+    (void)hipEventDestroy(ctx->join2);
+    if (ctx->rest != ctx->stream) (void)hipStreamDestroy(ctx->rest);
     (void)hipEventDestroy(ctx->arena.copied);
     (void)hipStreamDestroy(ctx->stream);
     (void)hipStreamDestroy(ctx->side);
@@ -424,6 +591,7 @@ int32_t par2hip_sync(par2hip_ctx *ctx) {
   Guard g(ctx);
   P2H_CHECK(ctx, hipStreamSynchronize(ctx->stream));
   P2H_CHECK(ctx, hipStreamSynchronize(ctx->side));
+  P2H_CHECK(ctx, hipStreamSynchronize(ctx->rest));
   return PAR2HIP_OK;
 }
 
@@ -617,6 +785,19 @@ int32_t par2hip_reconstruct(par2hip_ctx *ctx, const uint8_t *slices, size_t n, s
   return PAR2HIP_OK;
 }
 
+int32_t par2hip_repair_set_dev(par2hip_ctx *ctx, void *d_slices, size_t n, size_t slice_size,
+                               const uint8_t *expected_md5, const uint32_t *expected_crc,
+                               const void *d_recovery, const uint32_t *rec_exps, size_t nrec,
+                               void *d_out, uint32_t *damaged_out, size_t *ndamaged) {
+  if (!ctx || !d_slices || !expected_md5 || !expected_crc || !damaged_out || !ndamaged ||
+      (nrec && (!d_recovery || !rec_exps)))
+    return PAR2HIP_EINVAL;
+  Guard g(ctx);
+  *ndamaged = 0;
+  return repair_set_impl(ctx, (uint8_t *)d_slices, n, slice_size, expected_md5, expected_crc,
+                         (const uint8_t *)d_recovery, rec_exps, nrec, (uint8_t *)d_out, damaged_out, ndamaged);
+}
+
 // ------------------------------------------------------------- hashes --
 int32_t par2hip_hash_slices_dev(par2hip_ctx *ctx, const void *d_slices, size_t n, size_t slice_size,
                                 size_t slice_stride, const uint64_t *lens, void *d_md5, void *d_crc) {
@@ -711,6 +892,7 @@ int32_t par2hip_profile_read(par2hip_ctx *ctx, par2hip_kernel_stat *out, size_t 
   Guard g(ctx);
   P2H_CHECK(ctx, hipStreamSynchronize(ctx->stream));
   P2H_CHECK(ctx, hipStreamSynchronize(ctx->side));
+  P2H_CHECK(ctx, hipStreamSynchronize(ctx->rest));
   for (auto &p : ctx->pending) {
     float ms = 0;
     if (hipEventElapsedTime(&ms, p.start, p.stop) == hipSuccess) {

@@ -95,6 +95,7 @@ def lib():
                 "par2hip_gf_invert": ([P, P, S, P], I32),
                 "par2hip_reconstruct": ([P, P, S, S, P, S, P, P, P], I32),
                 "par2hip_reconstruct_dev": ([P, P, S, S, P, S, P, P, P, I32], I32),
+                "par2hip_repair_set_dev": ([P, P, S, S, P, P, P, P, S, P, P, C.POINTER(S)], I32),
                 "par2hip_hash_slices": ([P, P, S, S, S, P, P, P], I32),
                 "par2hip_hash_slices_dev": ([P, P, S, S, S, P, P, P], I32),
                 "par2hip_set_gf_kernel": ([P, I32], I32),
@@ -253,6 +254,24 @@ class Context:
         self._check(lib().par2hip_reconstruct_dev(self.handle, d_slices.ptr, n, slice_size, _p(missing),
                                                   len(missing), d_recovery.ptr, _p(rec_exps),
                                                   d_out.ptr if d_out else None, 1 if in_place else 0))
+
+    def repair_set_dev(self, d_slices: DeviceBuffer, n: int, slice_size: int, expected_md5: np.ndarray,
+                       expected_crc: np.ndarray, d_recovery: DeviceBuffer | None, rec_exps,
+                       d_out: DeviceBuffer | None = None):
+        """Verify n device slices against their IFSC entries and rebuild the damaged
+        ones (in place unless d_out is given).  Returns the damaged indices."""
+        md5 = np.ascontiguousarray(expected_md5, np.uint8).reshape(n, 16)
+        crc = np.ascontiguousarray(expected_crc, np.uint32)
+        exps = np.ascontiguousarray(rec_exps if rec_exps is not None else [], np.uint32)
+        dmg = np.zeros(max(n, 1), np.uint32)
+        cnt = C.c_size_t(0)
+        rc = lib().par2hip_repair_set_dev(self.handle, d_slices.ptr, n, slice_size, _p(md5), _p(crc),
+                                          d_recovery.ptr if d_recovery else None, _p(exps) if len(exps) else None,
+                                          len(exps), d_out.ptr if d_out else None, _p(dmg), C.byref(cnt))
+        if rc == ERECOVERY:
+            raise InsufficientRecovery(f"Need {cnt.value} recovery blocks, have {len(exps)}")
+        self._check(rc)
+        return dmg[:cnt.value].copy()
 
     def gf_invert(self, a: np.ndarray) -> np.ndarray:
         a = np.ascontiguousarray(a, np.uint16)

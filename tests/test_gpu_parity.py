@@ -296,3 +296,124 @@ def test_strategy_muladd_region(kctx):
     s.muladd_region(dsts, srcs, list(coeff), 13, 1000)
     for a, b in zip(dsts, want):
         assert np.array_equal(a, b)
+
+
+# ---------------------------------------------------- verify + repair set --
+CRC_SILENT = bytes.fromhex("410671db01")  # the CRC-32 polynomial: XOR-ing it never changes CRC-32
+
+
+def _set_with_damage(n, s, r, seed):
+    rng = np.random.default_rng(seed)
+    data = rng.integers(0, 256, (n, s), dtype=np.uint8)
+    md5, crc = oracle.hash_slices(data, n, s)
+    rec = oracle.encode(data.view(np.uint16), np.arange(r)).view(np.uint8).reshape(r, s)
+    return rng, data, md5, crc, rec
+
+
+@pytest.mark.parametrize("inplace", [True, False])
+def test_repair_set_crc_detected(kctx, inplace):
+    ctx = kctx
+    n, s, r = 120, 4096, 12
+    rng, data, md5, crc, rec = _set_with_damage(n, s, r, 31)
+    erased = sorted(rng.choice(n, 7, replace=False).tolist())
+    damaged = data.copy()
+    damaged[erased] = 0
+    perm = rng.permutation(r)  # recovery rows in shuffled order: the lowest exponents must be chosen
+    d_sl, d_rec = ctx.alloc(n * s), ctx.alloc(r * s)
+    d_sl.upload(damaged)
+    d_rec.upload(rec[perm])
+    d_out = None if inplace else ctx.alloc(r * s)
+    say("repair_set crc", erased, "inplace", inplace)
+    got = ctx.repair_set_dev(d_sl, n, s, md5, crc, d_rec, perm, d_out)
+    assert got.tolist() == erased
+    if inplace:
+        assert np.array_equal(d_sl.download().reshape(n, s), data)
+    else:
+        out = d_out.download().reshape(r, s)[: len(erased)]
+        assert np.array_equal(out, data[erased])
+        assert np.array_equal(d_sl.download().reshape(n, s), damaged)
+
+
+def test_repair_set_md5_only_damage(kctx):
+    """A CRC-silent corruption is caught by the MD5 confirmation and the
+    speculative rebuild is redone with the extra slice (verify.rs:621-645)."""
+    ctx = kctx
+    n, s, r = 64, 2048, 6
+    rng, data, md5, crc, rec = _set_with_damage(n, s, r, 32)
+    damaged = data.copy()
+    damaged[[5, 40]] = 0
+    silent = 17
+    damaged[silent, 100:105] ^= np.frombuffer(CRC_SILENT, np.uint8)
+    assert zlib.crc32(damaged[silent].tobytes()) == int(crc[silent])
+    d_sl, d_rec = ctx.alloc(n * s), ctx.alloc(r * s)
+    d_sl.upload(damaged)
+    d_rec.upload(rec)
+    say("repair_set md5-only damage")
+    got = ctx.repair_set_dev(d_sl, n, s, md5, crc, d_rec, np.arange(r))
+    assert got.tolist() == [5, silent, 40]
+    assert np.array_equal(d_sl.download().reshape(n, s), data)
+
+
+def test_repair_set_clean_and_insufficient(ctx):
+    n, s, r = 40, 1024, 3
+    rng, data, md5, crc, rec = _set_with_damage(n, s, r, 33)
+    d_sl, d_rec = ctx.alloc(n * s), ctx.alloc(r * s)
+    d_sl.upload(data)
+    d_rec.upload(rec)
+    say("repair_set clean")
+    assert ctx.repair_set_dev(d_sl, n, s, md5, crc, d_rec, np.arange(r)).tolist() == []
+    damaged = data.copy()
+    damaged[[1, 2, 3, 4]] = 0
+    d_sl.upload(damaged)
+    say("repair_set insufficient")
+    with pytest.raises(par2hip.InsufficientRecovery):
+        ctx.repair_set_dev(d_sl, n, s, md5, crc, d_rec, np.arange(r))
+    # nothing was written back
+    assert np.array_equal(d_sl.download().reshape(n, s), damaged)
+
+
+def test_repair_set_quickpar(ctx):
+    """The golden set: delete a whole 33-slice file and two slices elsewhere,
+    repair from the QuickPar recovery volumes using their IFSC entries."""
+    qp = quickpar()
+    n, s = qp.data.shape
+    md5 = np.frombuffer(b"".join(m for m, _ in qp.ifsc), np.uint8).reshape(n, 16)
+    crc = np.array([c for _, c in qp.ifsc], np.uint32)
+    lost = list(range(103, 136)) + [0, 199]
+    damaged = qp.data.copy()
+    damaged[lost] = 0
+    d_sl, d_rec = ctx.alloc(n * s), ctx.alloc(qp.recovery.nbytes)
+    d_sl.upload(damaged)
+    d_rec.upload(qp.recovery)
+    say("repair_set quickpar", len(lost))
+    got = ctx.repair_set_dev(d_sl, n, s, md5, crc, d_rec, qp.exponents)
+    assert got.tolist() == sorted(lost)
+    assert np.array_equal(d_sl.download().reshape(n, s), qp.data)
+
+
+@pytest.mark.parametrize("exps", [[0, 1, 2, 3, 4], [3, 4, 5, 6, 7], [10, 11], [0, 2, 3, 4, 5], [65530, 65531, 65532]])
+def test_reconstruct_inverse_paths(kctx, exps):
+    """Consecutive exponents take the closed-form (Lagrange) inverse, others
+    Gauss-Jordan; both must rebuild the same bytes."""
+    ctx = kctx
+    rng = np.random.default_rng(sum(exps))
+    n, s = 90, 1026
+    data = rng.integers(0, 256, (n, s), dtype=np.uint8)
+    rec = oracle.encode(data.view(np.uint16), exps).view(np.uint8).reshape(len(exps), s)
+    missing = sorted(rng.choice(n, len(exps), replace=False).tolist())
+    say("inverse path", exps)
+    out = ctx.reconstruct(data, missing, rec, exps)
+    assert np.array_equal(out, data[missing])
+
+
+def test_reconstruct_many_erasures(kctx):
+    """m = 300 erasures: closed-form inverse beyond the LDS Gauss-Jordan size."""
+    ctx = kctx
+    rng = np.random.default_rng(300)
+    n, s, m = 700, 512, 300
+    data = rng.integers(0, 256, (n, s), dtype=np.uint8)
+    rec = ctx.encode(data, np.arange(m))
+    missing = sorted(rng.choice(n, m, replace=False).tolist())
+    say("reconstruct m=300")
+    out = ctx.reconstruct(data, missing, rec, np.arange(m))
+    assert np.array_equal(out, data[missing])
