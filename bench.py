@@ -245,12 +245,120 @@ def bench_repair(args, ctx):
     }
 
 
+def draw_sets(seed, count=256, cap_bytes=48 << 30):
+    """Batched workload: count sets, N ~ logU[8,4096], S ~ logU[4 KiB,1 MiB] (multiple
+    of 4 B), redundancy ~ U[1,20]% with R by the creator's rule
+    (creator.rs:324-331), half repairs with m = ceil(U[1,10]% * N) clamped to R;
+    redrawn until the source total fits cap_bytes."""
+    rng = np.random.default_rng(seed)
+    while True:
+        sets = []
+        for i in range(count):
+            n = int(round(np.exp(rng.uniform(np.log(8), np.log(4096)))))
+            s = int(round(np.exp(rng.uniform(np.log(4096), np.log(1 << 20))) / 4)) * 4
+            pct = rng.uniform(1, 20)
+            r = -(-int(n * s * pct / 100.0) // s)
+            r = max(r, 1)
+            kind = "repair" if i % 2 else "encode"
+            m = 0
+            if kind == "repair":
+                m = min(r, max(1, int(np.ceil(rng.uniform(0.01, 0.10) * n))))
+            sets.append({"n": n, "s": s, "r": r, "kind": kind, "m": m})
+        total = sum(x["n"] * x["s"] for x in sets)
+        if total <= cap_bytes:
+            return sets, total
+
+
+def bench_batched(args, ctx):
+    """256 PAR2 sets of uneven shapes (half create, half repair) in one batched
+    run vs the plain path (one set at a time), both with IFSC hashes."""
+    sets, total = draw_sets(args.seed)
+    pool = random_bytes(5, 1 << 30)  # host pool; each set takes a window of it
+    rng = np.random.default_rng(6)
+    jobs = []
+    log("batched workload:", len(sets), "sets,", round(total / 2**30, 2), "GiB of source")
+    for x in sets:
+        n, s = x["n"], x["s"]
+        nb = n * s
+        off = int(rng.integers(0, len(pool) - min(nb, len(pool)) + 1))
+        d_sl = ctx.alloc(nb)
+        if nb <= len(pool):
+            d_sl.upload(pool[off:off + nb])
+        else:
+            for o in range(0, nb, len(pool)):
+                d_sl.upload(pool[: min(len(pool), nb - o)], offset=o)
+        job = {"kind": x["kind"], "slices": d_sl, "n": n, "slice_size": s,
+               "md5": ctx.alloc(n * 16), "crc": ctx.alloc(n * 4)}
+        if x["kind"] == "encode":
+            job.update(exps=np.arange(x["r"], dtype=np.uint32), recovery=ctx.alloc(x["r"] * s))
+        else:
+            m = x["m"]
+            d_rec = ctx.alloc(m * s)
+            ctx.encode_dev(d_sl, n, s, np.arange(m, dtype=np.uint32), d_rec)
+            missing = np.sort(rng.choice(n, m, replace=False)).astype(np.uint32)
+            job.update(exps=np.arange(m, dtype=np.uint32), recovery=d_rec, missing=missing, out=ctx.alloc(m * s))
+        jobs.append(job)
+    ctx.sync()
+    # the erased slices of repair jobs are left in place: the rebuild ignores them
+
+    def plain():
+        for j in jobs:
+            if j["kind"] == "encode":
+                ctx.encode_dev(j["slices"], j["n"], j["slice_size"], j["exps"], j["recovery"], j["md5"], j["crc"])
+            else:
+                ctx.hash_slices_dev(j["slices"], j["n"], j["slice_size"], j["md5"], j["crc"])
+                ctx.reconstruct_dev(j["slices"], j["n"], j["slice_size"], j["missing"], j["recovery"], j["exps"],
+                                    j["out"])
+            ctx.sync()
+
+    def batched():
+        ctx.batch_run_dev(jobs)
+        ctx.sync()
+
+    def timed(fn, reps):
+        fn()
+        lat = []
+        for _ in range(reps):
+            t0 = time.perf_counter()
+            fn()
+            lat.append(time.perf_counter() - t0)
+        return lat
+
+    # bit-exactness: batched outputs equal the plain path's for every job
+    plain()
+    ref = [(j["recovery"].download() if j["kind"] == "encode" else j["out"].download(), j["md5"].download(),
+            j["crc"].download()) for j in jobs]
+    for j in jobs:
+        (j["recovery"] if j["kind"] == "encode" else j["out"]).upload(np.zeros(16, np.uint8))
+    batched()
+    exact = all(np.array_equal(a, (j["recovery"].download() if j["kind"] == "encode" else j["out"].download()))
+                and np.array_equal(b, j["md5"].download()) and np.array_equal(c, j["crc"].download())
+                for j, (a, b, c) in zip(jobs, ref))
+    ctx.profile(True)
+    ctx.profile_read()
+    lat_b = timed(batched, args.steps)
+    stats = ctx.profile_read()
+    lat_p = timed(plain, max(1, args.steps // 2))
+    ctx.profile(False)
+    return {
+        "workload": "batched",
+        "config": {"workload": "batched_256_sets", "sets": len(sets), "source_GiB": round(total / 2**30, 3),
+                   "encode_sets": sum(x["kind"] == "encode" for x in sets),
+                   "repair_sets": sum(x["kind"] == "repair" for x in sets), "seed": args.seed,
+                   "hashes": "md5+crc32 of every input slice", "resident": "hbm"},
+        "src_bytes": total, "total_s": float(np.sum(lat_b)), "lat": lat_b, "exact": bool(exact),
+        "kernel_ms": {k: v[1] / max(v[0], 1) for k, v in stats.items()},
+        "plain_ms": float(np.mean(lat_p) * 1e3), "plain_gbs": total / float(np.mean(lat_p)) / 1e9,
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--workload", choices=["encode", "repair", "micro"], default="encode")
+    ap.add_argument("--workload", choices=["encode", "repair", "micro", "batched"], default="encode")
+    ap.add_argument("--seed", type=int, default=4)
     ap.add_argument("--kernel", type=int, default=0, help="0 auto, 1 LDS tables, 2 MFMA")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--traffic", type=float, default=None,
@@ -264,12 +372,15 @@ def main():
         print("bench.py: no gfx950 device visible to libpar2hip.so", file=sys.stderr)
         sys.exit(1)
     ctx = par2hip.Context(0)
-    log("workload", args.workload, "steps", args.steps, "warmup", args.warmup)
+    log("workload", args.workload, "steps", args.steps, "warmup", args.warmup, "device", ctx.device_info())
     if args.workload == "micro":
         print(json.dumps({"micro": bench_micro(args, ctx)}), flush=True)
         ctx.close()
         return
-    res = bench_encode(args, ctx) if args.workload == "encode" else bench_repair(args, ctx)
+    if args.workload == "batched":
+        res = bench_batched(args, ctx)
+    else:
+        res = bench_encode(args, ctx) if args.workload == "encode" else bench_repair(args, ctx)
     lat = np.array(res["lat"]) * 1e3
     ms = res["total_s"] / args.steps * 1e3
     value = res["src_bytes"] / (ms * 1e-3) / 1e9
@@ -280,10 +391,15 @@ def main():
         "config": res["config"],
         "p50_ms": round(float(np.percentile(lat, 50)), 4), "p99_ms": round(float(np.percentile(lat, 99)), 4),
         "bit_exact_sample": res["exact"],
+        "device": ctx.device_info(),
         "kernel_ms": {k: round(v, 4) for k, v in res["kernel_ms"].items()},
-        "roofline": res["roofline"],
     }
-    if not args.no_cpu:
+    if "roofline" in res:
+        out["roofline"] = res["roofline"]
+    if "plain_ms" in res:
+        out["plain_path"] = {"ms_per_batch": round(res["plain_ms"], 3), "value": round(res["plain_gbs"], 3),
+                             "unit": "GB/s", "speedup_batched": round(value / res["plain_gbs"], 3)}
+    if not args.no_cpu and "cpu" in res:
         threads = min(16, os.cpu_count() or 1)
         n, ss, r = res["cpu"]
         log("cpu baseline", n, ss, r, "threads", threads)

@@ -67,6 +67,15 @@ int32_t par2hip_ctx_destroy(par2hip_ctx *ctx);
 /* Text of the last error on this ctx (valid until the next call on it). */
 const char *par2hip_last_error(par2hip_ctx *ctx);
 
+/* What the ctx runs on: compute units, the CUs reserved for the MD5 side
+ * stream (0 when CU masks are unavailable), and the gfx arch name. */
+typedef struct {
+    int32_t compute_units;
+    int32_t hash_cus;
+    char arch[32];
+} par2hip_device_info;
+int32_t par2hip_get_device_info(par2hip_ctx *ctx, par2hip_device_info *out);
+
 /* Device memory owned by the ctx's device (serving keeps sets resident). */
 int32_t par2hip_dev_alloc(par2hip_ctx *ctx, size_t bytes, void **dptr);
 int32_t par2hip_dev_free(par2hip_ctx *ctx, void *dptr);
@@ -163,6 +172,34 @@ int32_t par2hip_repair_set_dev(par2hip_ctx *ctx, void *d_slices, size_t n, size_
                                const uint8_t *expected_md5, const uint32_t *expected_crc,
                                const void *d_recovery, const uint32_t *rec_exps, size_t nrec,
                                void *d_out, uint32_t *damaged_out, size_t *ndamaged);
+
+/* ------------------------------------------------------------- batched -- */
+
+/* One job of a batch: a whole PAR2 set to encode, or one to repair. */
+enum { PAR2HIP_JOB_ENCODE = 0, PAR2HIP_JOB_REPAIR = 1 };
+typedef struct {
+    int32_t kind;            /* PAR2HIP_JOB_ENCODE or PAR2HIP_JOB_REPAIR */
+    int32_t reserved;
+    void *d_slices;          /* n zero-padded slices of slice_size bytes (device) */
+    uint64_t n;
+    uint64_t slice_size;     /* even; any multiple of 2 */
+    const uint32_t *exps;    /* host. encode: the count recovery exponents;
+                                repair: exponents of the count recovery rows */
+    uint64_t count;          /* encode: recovery slices out; repair: erasures */
+    void *d_recovery;        /* encode: out, count rows; repair: in, count rows */
+    const uint32_t *missing; /* repair: count erased slice indices (host) */
+    void *d_out;             /* repair: count rebuilt rows out, row k = missing[k] */
+    void *d_md5;             /* nullable: IFSC MD5 of every input slice (n*16 B) */
+    void *d_crc;             /* nullable: CRC32 of every input slice (n u32) */
+} par2hip_job;
+
+/* Run many jobs as one batch (no counterpart in the reference, which handles
+ * one set per Par2Creator / Par2Repairer): coefficients, inverses (closed form
+ * for consecutive exponents), tables and hashes of all jobs in one launch each
+ * and every contraction in one persistent launch over a cost-sorted work list.
+ * Asynchronous on the ctx stream; complete at par2hip_sync.  Repair jobs with
+ * non-consecutive exponents take the Gauss-Jordan path after the batch. */
+int32_t par2hip_batch_run_dev(par2hip_ctx *ctx, const par2hip_job *jobs, size_t njobs);
 
 /* -------------------------------------------------------------- hashes -- */
 

@@ -1,0 +1,372 @@
+// batch.hip -- many PAR2 sets (encode and repair jobs of uneven sizes) in one
+// pass: every job's coefficients, inverses, nibble tables and hashes are
+// produced by one launch each over all jobs, and all contractions run in one
+// persistent launch over a cost-sorted work list.
+#include <algorithm>
+#include <cstring>
+#include <map>
+
+#include "common.hpp"
+
+namespace par2hip {
+
+namespace {
+
+struct Arena {
+  size_t used = 0;
+  size_t take(size_t bytes) {
+    size_t off = (used + 255) & ~(size_t)255;
+    used = off + bytes;
+    return off;
+  }
+};
+
+bool consecutive_exps(const uint32_t *e, size_t m) {
+  if (m == 0 || m > (size_t)kVandMaxM) return false;
+  for (size_t i = 1; i < m; i++)
+    if (e[i] != e[0] + i) return false;
+  return e[0] + m - 1 < kOrder;
+}
+
+}  // namespace
+
+// Implemented in par2hip_api.hip (the single-set repair path).
+int32_t reconstruct_single(par2hip_ctx *ctx, uint8_t *d_slices, size_t n, size_t slice_size, const uint32_t *missing,
+                           size_t m, const uint8_t *d_recovery, const uint32_t *rec_exps, uint8_t *d_out);
+int32_t fork_overlap(par2hip_ctx *ctx);
+int32_t join_overlap(par2hip_ctx *ctx);
+
+int32_t batch_run(par2hip_ctx *ctx, const par2hip_job *jobs, size_t njobs) {
+  // ---- validate and classify
+  std::vector<size_t> enc, rep, rep_slow;
+  for (size_t j = 0; j < njobs; j++) {
+    const par2hip_job &J = jobs[j];
+    if (!J.d_slices || J.n == 0 || J.slice_size == 0 || (J.slice_size & 1))
+      return fail(ctx, PAR2HIP_EINVAL, "batch job: need slices, n >= 1 and an even slice size");
+    if (J.n > 32768) return fail(ctx, PAR2HIP_EINVAL, "batch job: more than 32768 input slices");
+    if (J.kind == PAR2HIP_JOB_ENCODE) {
+      if (J.count && (!J.exps || !J.d_recovery)) return fail(ctx, PAR2HIP_EINVAL, "batch encode job: exponents and recovery buffer");
+      if (J.count > 65535) return fail(ctx, PAR2HIP_EINVAL, "batch encode job: too many recovery slices");
+      if (J.count) enc.push_back(j);
+    } else if (J.kind == PAR2HIP_JOB_REPAIR) {
+      if (J.count == 0) continue;
+      if (!J.exps || !J.d_recovery || !J.missing || !J.d_out || J.count > J.n)
+        return fail(ctx, PAR2HIP_EINVAL, "batch repair job: exponents, recovery, erasures and output");
+      std::vector<char> seen(J.n, 0);
+      for (size_t k = 0; k < J.count; k++) {
+        if (J.missing[k] >= J.n || seen[J.missing[k]]) return fail(ctx, PAR2HIP_EINVAL, "batch repair job: bad erasure list");
+        seen[J.missing[k]] = 1;
+      }
+      (consecutive_exps(J.exps, J.count) ? rep : rep_slow).push_back(j);
+    } else {
+      return fail(ctx, PAR2HIP_EINVAL, "batch job: unknown kind");
+    }
+  }
+
+  // ---- device arena layout
+  Arena ar;
+  struct EncPlan { size_t coeff; };
+  struct RepPlan { size_t aug, vand, coeff; size_t vs, cs, np; };
+  std::vector<EncPlan> ep(enc.size());
+  std::vector<RepPlan> rp(rep.size());
+  for (size_t k = 0; k < enc.size(); k++) {
+    const par2hip_job &J = jobs[enc[k]];
+    ep[k].coeff = ar.take(J.count * J.n * 2);
+  }
+  for (size_t k = 0; k < rep.size(); k++) {
+    const par2hip_job &J = jobs[rep[k]];
+    const size_t m = J.count, np = J.n - m;
+    rp[k].np = np;
+    rp[k].vs = (std::max<size_t>(np, 1) + 7) & ~(size_t)7;
+    rp[k].cs = (np + m + 7) & ~(size_t)7;
+    rp[k].aug = ar.take(m * 2 * m * 2);
+    rp[k].vand = ar.take(m * rp[k].vs * 2);
+    rp[k].coeff = ar.take(m * rp[k].cs * 2);
+  }
+  // tasks: one fold per repair job (if any present slice), one main per job
+  struct TaskPlan { size_t tabs; GfTaskDev t; size_t srcs_at, dsts_at; };
+  std::vector<TaskPlan> fold_tasks, main_tasks;
+  auto tab_bytes = [](int ndst, int nsrc) { return (size_t)((ndst + 3) / 4) * nsrc * 64 * 8; };
+  for (size_t k = 0; k < rep.size(); k++) {
+    const par2hip_job &J = jobs[rep[k]];
+    if (rp[k].np) {
+      TaskPlan tp{};
+      tp.t.nsrc = (int)J.count;
+      tp.t.ndst = (int)J.count;
+      tp.tabs = ar.take(tab_bytes(tp.t.ndst, tp.t.nsrc));
+      fold_tasks.push_back(tp);
+    }
+  }
+  for (size_t k = 0; k < enc.size(); k++) {
+    const par2hip_job &J = jobs[enc[k]];
+    TaskPlan tp{};
+    tp.t.nsrc = (int)J.n;
+    tp.t.ndst = (int)J.count;
+    tp.tabs = ar.take(tab_bytes(tp.t.ndst, tp.t.nsrc));
+    main_tasks.push_back(tp);
+  }
+  for (size_t k = 0; k < rep.size(); k++) {
+    const par2hip_job &J = jobs[rep[k]];
+    TaskPlan tp{};
+    tp.t.nsrc = (int)J.n;
+    tp.t.ndst = (int)J.count;
+    tp.tabs = ar.take(tab_bytes(tp.t.ndst, tp.t.nsrc));
+    main_tasks.push_back(tp);
+  }
+  const size_t cursor_off = ar.take(512);
+  void *arena_v = nullptr;
+  int32_t rc;
+  if ((rc = scratch(ctx, kScratchBatch, std::max<size_t>(ar.used, 4096), &arena_v))) return rc;
+  uint8_t *arena = (uint8_t *)arena_v;
+
+  // ---- work items (sorted by cost: sources per task, descending)
+  auto make_items = [&](std::vector<TaskPlan> &tasks, std::vector<GfItem> &items, const std::vector<long> &words) {
+    std::vector<int> order(tasks.size());
+    for (size_t i = 0; i < tasks.size(); i++) order[i] = (int)i;
+    std::stable_sort(order.begin(), order.end(),
+                     [&](int a, int b) { return (long)tasks[a].t.nsrc * words[a] > (long)tasks[b].t.nsrc * words[b]; });
+    for (int ti : order) {
+      const int ntiles = (int)((words[ti] + kBatchWordsPerWg - 1) / kBatchWordsPerWg);
+      const int nrb = gf_task_rowblks(tasks[ti].t.ndst);
+      for (int tile = 0; tile < ntiles; tile++)
+        for (int rb = 0; rb < nrb; rb++) items.push_back(GfItem{ti, tile, rb, 0});
+    }
+  };
+  std::vector<long> fold_words, main_words;
+  {
+    size_t f = 0;
+    for (size_t k = 0; k < rep.size(); k++)
+      if (rp[k].np) {
+        fold_words.push_back((long)rp[k].np);
+        fold_tasks[f++].t.words = (long)rp[k].np;
+      }
+    for (size_t k = 0; k < enc.size(); k++) main_words.push_back((long)(jobs[enc[k]].slice_size / 2));
+    for (size_t k = 0; k < rep.size(); k++) main_words.push_back((long)(jobs[rep[k]].slice_size / 2));
+    for (size_t i = 0; i < main_tasks.size(); i++) main_tasks[i].t.words = main_words[i];
+  }
+  std::vector<GfItem> fold_items, main_items;
+  make_items(fold_tasks, fold_items, fold_words);
+  make_items(main_tasks, main_items, main_words);
+  if (main_items.size() > 0x7fffffff) return fail(ctx, PAR2HIP_EINVAL, "batch too large");
+
+  // ---- hash lists
+  std::vector<HashEntry> md5_list, crc_list, crc_small;
+  std::map<uint64_t, uint32_t> crc_idx;
+  std::vector<CrcParams> crc_params;
+  bool aligned4 = true;
+  for (size_t j = 0; j < njobs; j++) {
+    const par2hip_job &J = jobs[j];
+    if (!J.d_md5 && !J.d_crc) continue;
+    uint32_t ci = 0;
+    if (J.d_crc) {
+      auto it = crc_idx.find(J.slice_size);
+      if (it == crc_idx.end()) {
+        ci = (uint32_t)crc_params.size();
+        crc_idx[J.slice_size] = ci;
+        crc_params.push_back(crc_params_for(J.slice_size));
+      } else {
+        ci = it->second;
+      }
+    }
+    for (uint64_t i = 0; i < J.n; i++) {
+      HashEntry e{};
+      e.data = (const uint8_t *)J.d_slices + i * J.slice_size;
+      e.len = e.size = J.slice_size;
+      e.md5 = J.d_md5 ? (uint32_t *)J.d_md5 + 4 * i : nullptr;
+      e.crc = J.d_crc ? (uint32_t *)J.d_crc + i : nullptr;
+      e.crc_params = ci;
+      if (((uintptr_t)e.data & 3) != 0) aligned4 = false;
+      if (e.md5) md5_list.push_back(e);
+      if (e.crc) (J.slice_size < kCrcLaneMaxBytes ? crc_small : crc_list).push_back(e);
+    }
+  }
+  std::stable_sort(md5_list.begin(), md5_list.end(), [](const HashEntry &a, const HashEntry &b) { return a.size > b.size; });
+
+  // ---- one parameter block for the whole batch
+  size_t words_needed = 0;
+  for (size_t j : enc) words_needed += (jobs[j].n + jobs[j].count) * 8 + jobs[j].n * 4 + jobs[j].count * 4 + 1024;
+  for (size_t k = 0; k < rep.size(); k++) {
+    const par2hip_job &J = jobs[rep[k]];
+    words_needed += (J.n + 4 * J.count) * 8 + (J.n + J.count) * 4 + 2048;
+  }
+  const size_t pbytes = words_needed + (fold_tasks.size() + main_tasks.size()) * sizeof(GfTaskDev) +
+                        (fold_items.size() + main_items.size()) * sizeof(GfItem) +
+                        enc.size() * sizeof(CoeffJob) + rep.size() * sizeof(InvJob) +
+                        (md5_list.size() + crc_list.size() + crc_small.size()) * sizeof(HashEntry) +
+                        crc_params.size() * sizeof(CrcParams) +
+                        64 * 256;
+  if ((rc = params_begin(ctx, pbytes))) return rc;
+  const std::vector<uint32_t> logs_all = constant_logs(32768);
+  std::vector<CoeffJob> cjobs;
+  long max_coeff = 0;
+  for (size_t k = 0; k < enc.size(); k++) {
+    const par2hip_job &J = jobs[enc[k]];
+    void *d_logs, *d_exps, *d_srcs, *d_dsts;
+    uint32_t *h_logs = (uint32_t *)params_alloc(ctx, J.n * 4, &d_logs);
+    uint32_t *h_exps = (uint32_t *)params_alloc(ctx, J.count * 4, &d_exps);
+    const uint16_t **h_srcs = (const uint16_t **)params_alloc(ctx, J.n * 8, &d_srcs);
+    uint16_t **h_dsts = (uint16_t **)params_alloc(ctx, J.count * 8, &d_dsts);
+    memcpy(h_logs, logs_all.data(), J.n * 4);
+    for (size_t i = 0; i < J.count; i++) h_exps[i] = J.exps[i] % kOrder;
+    for (size_t d = 0; d < J.n; d++) h_srcs[d] = (const uint16_t *)((const uint8_t *)J.d_slices + d * J.slice_size);
+    for (size_t i = 0; i < J.count; i++) h_dsts[i] = (uint16_t *)((uint8_t *)J.d_recovery + i * J.slice_size);
+    uint16_t *coeff = (uint16_t *)(arena + ep[k].coeff);
+    cjobs.push_back(CoeffJob{coeff, (const uint32_t *)d_logs, (const uint32_t *)d_exps, (int)J.count, (int)J.n});
+    max_coeff = std::max(max_coeff, (long)(J.count * J.n));
+    GfTaskDev &t = main_tasks[k].t;
+    t.srcs = (const uint16_t *const *)d_srcs;
+    t.dsts = (uint16_t *const *)d_dsts;
+    t.coeff = coeff;
+    t.ldc = (int)J.n;
+    t.tabs = (uint64_t *)(arena + main_tasks[k].tabs);
+    t.accumulate = 0;
+  }
+  std::vector<InvJob> ijobs;
+  int max_m = 0;
+  {
+    size_t f = 0;
+    for (size_t k = 0; k < rep.size(); k++) {
+      const par2hip_job &J = jobs[rep[k]];
+      const size_t m = J.count, np = rp[k].np;
+      std::vector<char> miss(J.n, 0);
+      for (size_t q = 0; q < m; q++) miss[J.missing[q]] = 1;
+      void *d_mlog, *d_exps, *d_plog, *d_srcs, *d_dsts, *d_fsrc, *d_fdst;
+      uint32_t *h_mlog = (uint32_t *)params_alloc(ctx, m * 4, &d_mlog);
+      uint32_t *h_exps = (uint32_t *)params_alloc(ctx, m * 4, &d_exps);
+      uint32_t *h_plog = (uint32_t *)params_alloc(ctx, std::max<size_t>(np, 1) * 4, &d_plog);
+      const uint16_t **h_srcs = (const uint16_t **)params_alloc(ctx, (np + m) * 8, &d_srcs);
+      uint16_t **h_dsts = (uint16_t **)params_alloc(ctx, m * 8, &d_dsts);
+      const uint16_t **h_fsrc = (const uint16_t **)params_alloc(ctx, m * 8, &d_fsrc);
+      uint16_t **h_fdst = (uint16_t **)params_alloc(ctx, m * 8, &d_fdst);
+      for (size_t q = 0; q < m; q++) h_mlog[q] = logs_all[J.missing[q]];
+      for (size_t i = 0; i < m; i++) h_exps[i] = J.exps[i] % kOrder;
+      size_t c = 0;
+      for (size_t d = 0; d < J.n; d++)
+        if (!miss[d]) {
+          h_plog[c] = logs_all[d];
+          h_srcs[c] = (const uint16_t *)((const uint8_t *)J.d_slices + d * J.slice_size);
+          c++;
+        }
+      for (size_t i = 0; i < m; i++) h_srcs[np + i] = (const uint16_t *)((const uint8_t *)J.d_recovery + i * J.slice_size);
+      for (size_t q = 0; q < m; q++) h_dsts[q] = (uint16_t *)((uint8_t *)J.d_out + q * J.slice_size);
+      uint16_t *aug = (uint16_t *)(arena + rp[k].aug);
+      uint16_t *vand = (uint16_t *)(arena + rp[k].vand);
+      uint16_t *coeff = (uint16_t *)(arena + rp[k].coeff);
+      for (size_t i = 0; i < m; i++) {
+        h_fsrc[i] = vand + i * rp[k].vs;
+        h_fdst[i] = coeff + i * rp[k].cs;
+      }
+      InvJob ij{};
+      ij.aug = aug;
+      ij.miss_log = (const uint32_t *)d_mlog;
+      ij.plog = (const uint32_t *)d_plog;
+      ij.exps = (const uint32_t *)d_exps;
+      ij.vand = vand;
+      ij.coeff = coeff;
+      ij.vs = rp[k].vs;
+      ij.cs = rp[k].cs;
+      ij.e0 = J.exps[0] % kOrder;
+      ij.m = (int)m;
+      ij.np = (int)np;
+      ijobs.push_back(ij);
+      max_m = std::max(max_m, (int)m);
+      if (np) {
+        GfTaskDev &t = fold_tasks[f].t;
+        t.srcs = (const uint16_t *const *)d_fsrc;
+        t.dsts = (uint16_t *const *)d_fdst;
+        t.coeff = aug + m;  // A^-1, row stride 2m
+        t.ldc = (int)(2 * m);
+        t.tabs = (uint64_t *)(arena + fold_tasks[f].tabs);
+        t.accumulate = 0;
+        f++;
+      }
+      GfTaskDev &t = main_tasks[enc.size() + k].t;
+      t.srcs = (const uint16_t *const *)d_srcs;
+      t.dsts = (uint16_t *const *)d_dsts;
+      t.coeff = coeff;
+      t.ldc = (int)rp[k].cs;
+      t.tabs = (uint64_t *)(arena + main_tasks[enc.size() + k].tabs);
+      t.accumulate = 0;
+    }
+  }
+  auto put = [&](const void *src, size_t bytes) -> void * {
+    void *dev = nullptr;
+    void *h = params_alloc(ctx, std::max<size_t>(bytes, 8), &dev);
+    if (bytes) memcpy(h, src, bytes);
+    return dev;
+  };
+  std::vector<GfTaskDev> ft(fold_tasks.size()), mt(main_tasks.size());
+  long max_ft = 0, max_mt = 0;
+  for (size_t i = 0; i < ft.size(); i++) {
+    ft[i] = fold_tasks[i].t;
+    max_ft = std::max(max_ft, (long)((ft[i].ndst + 3) / 4) * ft[i].nsrc * 64);
+  }
+  for (size_t i = 0; i < mt.size(); i++) {
+    mt[i] = main_tasks[i].t;
+    max_mt = std::max(max_mt, (long)((mt[i].ndst + 3) / 4) * mt[i].nsrc * 64);
+  }
+  const GfTaskDev *d_ft = (const GfTaskDev *)put(ft.data(), ft.size() * sizeof(GfTaskDev));
+  const GfTaskDev *d_mt = (const GfTaskDev *)put(mt.data(), mt.size() * sizeof(GfTaskDev));
+  const GfItem *d_fi = (const GfItem *)put(fold_items.data(), fold_items.size() * sizeof(GfItem));
+  const GfItem *d_mi = (const GfItem *)put(main_items.data(), main_items.size() * sizeof(GfItem));
+  const CoeffJob *d_cj = (const CoeffJob *)put(cjobs.data(), cjobs.size() * sizeof(CoeffJob));
+  const InvJob *d_ij = (const InvJob *)put(ijobs.data(), ijobs.size() * sizeof(InvJob));
+  const HashEntry *d_md5l = (const HashEntry *)put(md5_list.data(), md5_list.size() * sizeof(HashEntry));
+  const HashEntry *d_crcl = (const HashEntry *)put(crc_list.data(), crc_list.size() * sizeof(HashEntry));
+  const HashEntry *d_crcs = (const HashEntry *)put(crc_small.data(), crc_small.size() * sizeof(HashEntry));
+  const CrcParams *d_cp = (const CrcParams *)put(crc_params.data(), crc_params.size() * sizeof(CrcParams));
+  if ((rc = params_commit(ctx))) return rc;
+
+  // ---- launches.  MD5 chains overlap everything else: on the reserved side
+  // CUs when they fit there (one wave per SIMD), otherwise -- a throughput
+  // load of many chains -- on an unmasked stream over the whole device at
+  // normal issue priority (boosted, they would starve the CRC and contraction
+  // waves that share their SIMDs); the longest chains are listed first.
+  const size_t md5_waves = (md5_list.size() + 63) / 64;
+  const bool side_fits = ctx->masked && md5_waves <= (size_t)(4 * ctx->hash_cus);
+  const bool overlap = !md5_list.empty() && side_fits;
+  hipEvent_t wide_done = nullptr;
+  if (overlap) {
+    if ((rc = fork_overlap(ctx))) return rc;
+    if ((rc = launch_md5_list(ctx, ctx->side, d_md5l, md5_list.size(), aligned4, false))) return rc;
+  } else if (!md5_list.empty()) {
+    P2H_CHECK(ctx, hipEventRecord(ctx->fork, ctx->stream));
+    P2H_CHECK(ctx, hipStreamWaitEvent(ctx->wide, ctx->fork, 0));
+    if ((rc = launch_md5_list(ctx, ctx->wide, d_md5l, md5_list.size(), aligned4, false))) return rc;
+    wide_done = ctx->join;
+  }
+  hipStream_t s = ctx->cur;
+  unsigned *cursors = (unsigned *)(arena + cursor_off);
+  if ((rc = launch_crc32_list(ctx, s, d_crcl, crc_list.size(), d_cp))) return rc;
+  if ((rc = launch_crc32_list_lane(ctx, s, d_crcs, crc_small.size()))) return rc;
+  if ((rc = launch_coeff_batch(ctx, s, d_cj, (int)cjobs.size(), max_coeff))) return rc;
+  if ((rc = launch_inverse_batch(ctx, s, d_ij, (int)ijobs.size(), max_m, 0))) return rc;
+  if ((rc = launch_tables_batch(ctx, s, d_ft, (int)ft.size(), max_ft))) return rc;
+  if ((rc = launch_gf_batch(ctx, s, d_ft, d_fi, (int)fold_items.size(), cursors, "gf_fold"))) return rc;
+  if ((rc = launch_tables_batch(ctx, s, d_mt, (int)mt.size(), max_mt))) return rc;
+  if ((rc = launch_gf_batch(ctx, s, d_mt, d_mi, (int)main_items.size(), cursors + 64, "gf_batch"))) return rc;
+  if (overlap && (rc = join_overlap(ctx))) return rc;
+  if (wide_done) {
+    P2H_CHECK(ctx, hipEventRecord(wide_done, ctx->wide));
+    P2H_CHECK(ctx, hipStreamWaitEvent(ctx->stream, wide_done, 0));
+  }
+
+  // repair jobs whose exponents are not consecutive take the Gauss-Jordan path
+  for (size_t j : rep_slow) {
+    const par2hip_job &J = jobs[j];
+    if ((rc = reconstruct_single(ctx, (uint8_t *)J.d_slices, J.n, J.slice_size, J.missing, J.count,
+                                 (const uint8_t *)J.d_recovery, J.exps, (uint8_t *)J.d_out)))
+      return rc;
+  }
+  return PAR2HIP_OK;
+}
+
+}  // namespace par2hip
+
+extern "C" int32_t par2hip_batch_run_dev(par2hip_ctx *ctx, const par2hip_job *jobs, size_t njobs) {
+  if (!ctx || (njobs && !jobs)) return PAR2HIP_EINVAL;
+  if (njobs == 0) return PAR2HIP_OK;
+  std::lock_guard<std::recursive_mutex> lk(ctx->mu);
+  (void)hipSetDevice(ctx->device);
+  return par2hip::batch_run(ctx, jobs, njobs);
+}

@@ -69,15 +69,18 @@ struct par2hip_ctx {
   hipStream_t rest = nullptr;         // the other CUs, for work that overlaps the side stream
   hipStream_t cur = nullptr;          // stream the current call's main work goes to
   bool masked = false;                // side/rest carry complementary CU masks
+  int hash_cus = 0;                   // CUs of the side stream's mask
+  hipStream_t wide = nullptr;         // unmasked helper stream (hash loads too big for the side CUs)
+  char arch[32] = {0};
   hipEvent_t fork = nullptr, join = nullptr, join2 = nullptr;
   std::recursive_mutex mu;
   std::string err;
   uint16_t *d_exp = nullptr;          // device copy of the doubled exp table
   uint16_t *d_log = nullptr;          // device copy of the log table
-  uint32_t *d_crc_tab = nullptr;      // CRC-32 byte table + shift powers
+  uint32_t *d_crc_tab = nullptr;      // CRC-32 slice-by-4 tables T0..T3 (4 x 256)
   int *d_status = nullptr;            // device status word (singular flag)
   par2hip::ParamArena arena;
-  par2hip::DevBuf scratch[10];        // grow-only device scratch, by role
+  par2hip::DevBuf scratch[11];        // grow-only device scratch, by role
   int gf_kernel = 0;                  // 0 auto, 1 LDS tables, 2 MFMA
   int num_cus = 256;                  // compute units of the device
   bool profile = false;
@@ -99,6 +102,7 @@ enum ScratchRole {
   kScratchAux2 = 7,
   kScratchVand = 8,    // repair: V rows c_present^e (fold sources)
   kScratchHashD = 9,   // repair: digests of the verify pass
+  kScratchBatch = 10,  // batched runs: all per-set workspaces in one arena
 };
 
 int32_t fail(par2hip_ctx *ctx, int32_t code, const std::string &msg);
@@ -154,12 +158,80 @@ struct GfProblem {
   const char *tag = nullptr;    // kernel name for timing (default: gf_muladd_<variant>)
 };
 int32_t run_gf(par2hip_ctx *ctx, hipStream_t s, const GfProblem &p);
+
+// ---- batched contraction (many sets in one persistent launch)
+struct GfTaskDev {
+  const uint16_t *const *srcs;
+  uint16_t *const *dsts;
+  const uint16_t *coeff;  // [ndst][ldc]
+  uint64_t *tabs;         // nibble tables of this task, ngroups*nsrc*64 entries
+  long words;
+  int nsrc, ndst, ldc, accumulate;
+};
+struct GfItem {
+  int task, tile, rowblk, pad_;
+};
+constexpr int kBatchWordsPerWg = 2048;  // 256 threads x 8 words
+inline int gf_task_rowblks(int ndst) { return ((ndst + 3) / 4 + 3) / 4; }
+// Nibble tables of every task (grid: elements x tasks), then the persistent
+// kernel works through `items` (sorted by cost) with an atomic cursor.
+int32_t launch_tables_batch(par2hip_ctx *ctx, hipStream_t s, const GfTaskDev *d_tasks, int ntasks,
+                            long max_entries);
+int32_t launch_gf_batch(par2hip_ctx *ctx, hipStream_t s, const GfTaskDev *d_tasks, const GfItem *d_items,
+                        int nitems, unsigned *d_cursor, const char *tag);
+// coefficient matrices of many encode jobs: coeff[r][d] = exp[(log_d * e_r) mod 65535]
+struct CoeffJob {
+  uint16_t *coeff;
+  const uint32_t *src_log, *exps;
+  int nrow, nsrc;
+};
+int32_t launch_coeff_batch(par2hip_ctx *ctx, hipStream_t s, const CoeffJob *d_jobs, int njobs, long max_entries);
+// repair jobs: closed-form inverse, V rows and the inverse copy, many at once
+struct InvJob {
+  uint16_t *aug;          // m x 2m workspace; A^-1 lands in the right half
+  const uint32_t *miss_log;
+  const uint32_t *plog, *exps;
+  uint16_t *vand;         // m rows of np entries, stride vs
+  uint16_t *coeff;        // m rows, stride cs: [ fold output | A^-1 ]
+  uint64_t vs, cs;
+  uint32_t e0;
+  int m, np;
+};
+int32_t launch_inverse_batch(par2hip_ctx *ctx, hipStream_t s, const InvJob *d_jobs, int njobs, int max_m,
+                             long max_vand);
 int32_t run_gf_lds(par2hip_ctx *ctx, hipStream_t s, const GfProblem &p);
 int32_t run_gf_mfma(par2hip_ctx *ctx, hipStream_t s, const GfProblem &p);
 // Kernel used when the caller leaves the choice to the library.
 constexpr int kAutoGfKernel = 1;
 
 // Slice hashes.
+constexpr int kCrcLanes = 256;  // CRC-32: lanes (one workgroup) per slice
+// Per-slice-size constants of the CRC-32 tree combine.
+struct CrcParams {
+  uint64_t chunk;       // bytes per lane (multiple of 16)
+  uint64_t pad;         // leading virtual zero bytes
+  uint32_t level[8];    // x^(8*chunk*2^k) mod P, k = 0..7
+  uint32_t init_term;   // 0xFFFFFFFF * x^(8*slice_size) mod P
+};
+CrcParams crc_params_for(uint64_t slice_size);
+// One slice of a batched hash pass (device-resident list).
+struct HashEntry {
+  const uint8_t *data;  // slice bytes
+  uint64_t len;         // real bytes (zeros follow up to size)
+  uint64_t size;        // padded slice size
+  uint32_t *md5;        // 4 words out (nullable for CRC-only lists)
+  uint32_t *crc;        // 1 word out
+  uint32_t crc_params;  // index into the CrcParams table
+  uint32_t pad_;
+};
+int32_t launch_md5_list(par2hip_ctx *ctx, hipStream_t s, const HashEntry *d_list, size_t n, bool aligned4,
+                        bool high_priority);
+int32_t launch_crc32_list(par2hip_ctx *ctx, hipStream_t s, const HashEntry *d_list, size_t n,
+                          const CrcParams *d_params);
+// entries below this size take the lane-per-slice CRC kernel
+constexpr uint64_t kCrcLaneMaxBytes = 64 * 1024;
+int32_t launch_crc32_list_lane(par2hip_ctx *ctx, hipStream_t s, const HashEntry *d_list, size_t n);
+
 // high_priority: the MD5 chains are the critical path of the caller, so their
 // waves issue ahead of co-resident waves of other kernels.
 int32_t launch_md5(par2hip_ctx *ctx, hipStream_t s, const uint8_t *d_data, size_t n,

@@ -13,7 +13,9 @@
 // 64-bit entry, so one ds_read_b64 yields four partial products.  A 16-entry
 // table of 8-byte entries covers all 32 LDS banks exactly once, so any lane
 // pattern is conflict-free (equal addresses broadcast).
-#include "common.hpp"
+#include <algorithm>
+
+#include "gf_device.hpp"
 
 namespace par2hip {
 
@@ -74,56 +76,6 @@ __global__ void k_nibble_tables(uint64_t *__restrict__ tabs, const uint16_t *__r
   tabs[idx] = e;
 }
 
-constexpr int kTpb = 256;  // threads per workgroup
-constexpr int kGpw = 4;    // output groups (of 4 rows) per workgroup -> 16 rows
-constexpr int kDch = 16;   // sources per LDS table chunk
-
-template <int V>
-struct WordVec;
-template <>
-struct WordVec<8> { using T = uint4; };
-template <>
-struct WordVec<4> { using T = uint2; };
-template <>
-struct WordVec<2> { using T = uint32_t; };
-template <>
-struct WordVec<1> { using T = uint16_t; };
-
-template <int V>
-__device__ inline void load_words(const uint16_t *__restrict__ p, long w0, long words, uint32_t (&x)[V]) {
-  if (w0 + V <= words) {
-    typename WordVec<V>::T t = *reinterpret_cast<const typename WordVec<V>::T *>(p + w0);
-    const uint16_t *h = reinterpret_cast<const uint16_t *>(&t);
-#pragma unroll
-    for (int k = 0; k < V; k++) x[k] = h[k];
-  } else {
-#pragma unroll
-    for (int k = 0; k < V; k++) x[k] = (w0 + k < words) ? p[w0 + k] : 0u;
-  }
-}
-
-template <int V>
-__device__ inline void store_words(uint16_t *__restrict__ p, long w0, long words, const uint32_t (&x)[V],
-                                   bool accumulate) {
-  if (w0 + V <= words) {
-    typename WordVec<V>::T t;
-    uint16_t *h = reinterpret_cast<uint16_t *>(&t);
-    if (accumulate) {
-      t = *reinterpret_cast<const typename WordVec<V>::T *>(p + w0);
-#pragma unroll
-      for (int k = 0; k < V; k++) h[k] ^= (uint16_t)x[k];
-    } else {
-#pragma unroll
-      for (int k = 0; k < V; k++) h[k] = (uint16_t)x[k];
-    }
-    *reinterpret_cast<typename WordVec<V>::T *>(p + w0) = t;
-  } else {
-#pragma unroll
-    for (int k = 0; k < V; k++)
-      if (w0 + k < words) p[w0 + k] = accumulate ? (uint16_t)(p[w0 + k] ^ x[k]) : (uint16_t)x[k];
-  }
-}
-
 // Workgroup w covers words [tile*kTpb*V, +kTpb*V) of output rows
 // [16*rowblk, +16).  The 1-D grid is remapped so that the row blocks of one
 // word tile run on the same XCD back to back (workgroups are dealt to the 8
@@ -134,8 +86,6 @@ __global__ __launch_bounds__(kTpb, 2) void k_gf_lds(const uint16_t *const *__res
                                                    const uint64_t *__restrict__ tabs, long words,
                                                    int accumulate, int ntiles, int nrowblk) {
   __shared__ uint64_t lds[kDch * kGpw * 64];  // [d][g][i*16+v], 32 KiB
-
-  // --- XCD-aware remap of the linear workgroup id
   const int L = blockIdx.x;
   const int xcd = L & 7, q = L >> 3;
   const int tiles_per_xcd = (ntiles + 7) >> 3;
@@ -143,64 +93,7 @@ __global__ __launch_bounds__(kTpb, 2) void k_gf_lds(const uint16_t *const *__res
   const int local_tile = q / nrowblk, rowblk = q % nrowblk;
   const int tile = local_tile * 8 + xcd;
   if (local_tile >= tiles_per_xcd || tile >= ntiles) return;
-
-  const int tid = threadIdx.x;
-  const int ngroups = (ndst + 3) / 4;
-  const int g0 = rowblk * kGpw;
-  const long w0 = ((long)tile * kTpb + tid) * V;
-
-  uint64_t acc[kGpw][V];
-#pragma unroll
-  for (int g = 0; g < kGpw; g++)
-#pragma unroll
-    for (int k = 0; k < V; k++) acc[g][k] = 0;
-
-  for (int d0 = 0; d0 < nsrc; d0 += kDch) {
-    const int dn = min(kDch, nsrc - d0);
-    __syncthreads();
-    // stage the tables of sources d0..d0+dn-1 for this workgroup's groups (16-byte units)
-    const int units = kGpw * dn * 32;
-    for (int u = tid; u < units; u += kTpb) {
-      const int g = u / (dn * 32), rem = u - g * dn * 32, d = rem >> 5, k = rem & 31;
-      uint4 val = make_uint4(0, 0, 0, 0);
-      if (g0 + g < ngroups)
-        val = reinterpret_cast<const uint4 *>(tabs + ((long)(g0 + g) * nsrc + d0 + d) * 64)[k];
-      reinterpret_cast<uint4 *>(lds + (d * kGpw + g) * 64)[k] = val;
-    }
-    __syncthreads();
-    if (w0 < words) {
-      for (int d = 0; d < dn; d++) {
-        uint32_t x[V];
-        load_words<V>(srcs[d0 + d], w0, words, x);
-        const uint64_t *t = lds + d * kGpw * 64;
-#pragma unroll
-        for (int k = 0; k < V; k++) {
-          const uint32_t a0 = x[k] & 15u, a1 = 16u + ((x[k] >> 4) & 15u);
-          const uint32_t a2 = 32u + ((x[k] >> 8) & 15u), a3 = 48u + (x[k] >> 12);
-#pragma unroll
-          for (int g = 0; g < kGpw; g++) {
-            const uint64_t *tg = t + g * 64;
-            acc[g][k] ^= tg[a0] ^ tg[a1] ^ tg[a2] ^ tg[a3];
-          }
-        }
-      }
-    }
-  }
-
-  if (w0 >= words) return;
-#pragma unroll
-  for (int g = 0; g < kGpw; g++) {
-#pragma unroll
-    for (int j = 0; j < 4; j++) {
-      const int r = 4 * (g0 + g) + j;
-      if (r < ndst) {
-        uint32_t y[V];
-#pragma unroll
-        for (int k = 0; k < V; k++) y[k] = (uint32_t)(acc[g][k] >> (16 * j)) & 0xFFFFu;
-        store_words<V>(dsts[r], w0, words, y, accumulate != 0);
-      }
-    }
-  }
+  gf_lds_tile<V, false>(lds, srcs, nsrc, dsts, ndst, tabs, words, accumulate, tile, rowblk);
 }
 
 template <int V>
@@ -259,6 +152,96 @@ int32_t run_gf(par2hip_ctx *ctx, hipStream_t s, const GfProblem &p) {
   if (which == 0) which = kAutoGfKernel;
   if (which == 2 && mfma_ok) return run_gf_mfma(ctx, s, p);
   return run_gf_lds(ctx, s, p);
+}
+
+// ---------------------------------------------------------------- batched --
+__global__ void k_coeff_batch(const CoeffJob *__restrict__ jobs, const uint16_t *__restrict__ exp_tab) {
+  const CoeffJob j = jobs[blockIdx.y];
+  const long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= (long)j.nrow * j.nsrc) return;
+  const int r = (int)(idx / j.nsrc), d = (int)(idx % j.nsrc);
+  uint32_t l = j.src_log[d] * j.exps[r];  // exponents arrive reduced mod 65535
+  l = (l & 0xFFFFu) + (l >> 16);
+  l = (l & 0xFFFFu) + (l >> 16);
+  j.coeff[idx] = exp_tab[l];
+}
+
+int32_t launch_coeff_batch(par2hip_ctx *ctx, hipStream_t s, const CoeffJob *d_jobs, int njobs, long max_entries) {
+  if (njobs == 0 || max_entries == 0) return PAR2HIP_OK;
+  ScopedTiming t(ctx, "coeff_from_logs", s);
+  hipLaunchKernelGGL(k_coeff_batch, dim3((unsigned)((max_entries + 255) / 256), (unsigned)njobs), dim3(256), 0, s,
+                     d_jobs, ctx->d_exp);
+  P2H_CHECK(ctx, hipGetLastError());
+  return PAR2HIP_OK;
+}
+
+__global__ void k_tables_batch(const GfTaskDev *__restrict__ tasks, const uint16_t *__restrict__ log_tab,
+                               const uint16_t *__restrict__ exp_tab) {
+  const GfTaskDev t = tasks[blockIdx.y];
+  const int ngroups = (t.ndst + 3) / 4;
+  const long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= (long)ngroups * t.nsrc * 64) return;
+  const int iv = (int)(idx & 63);
+  const long gd = idx >> 6;
+  const int d = (int)(gd % t.nsrc), g = (int)(gd / t.nsrc);
+  const int i = iv >> 4, v = iv & 15;
+  const uint32_t x = (uint32_t)v << (4 * i);
+  uint64_t e = 0;
+  if (x) {
+    const uint32_t lx = log_tab[x];
+#pragma unroll
+    for (int jj = 0; jj < 4; jj++) {
+      const int r = 4 * g + jj;
+      if (r < t.ndst) {
+        const uint16_t c = t.coeff[(long)r * t.ldc + d];
+        if (c) e |= (uint64_t)exp_tab[lx + log_tab[c]] << (16 * jj);
+      }
+    }
+  }
+  t.tabs[idx] = e;
+}
+
+int32_t launch_tables_batch(par2hip_ctx *ctx, hipStream_t s, const GfTaskDev *d_tasks, int ntasks,
+                            long max_entries) {
+  if (ntasks == 0 || max_entries == 0) return PAR2HIP_OK;
+  ScopedTiming t(ctx, "nibble_tables", s);
+  hipLaunchKernelGGL(k_tables_batch, dim3((unsigned)((max_entries + 255) / 256), (unsigned)ntasks), dim3(256), 0, s,
+                     d_tasks, ctx->d_log, ctx->d_exp);
+  P2H_CHECK(ctx, hipGetLastError());
+  return PAR2HIP_OK;
+}
+
+__global__ void k_reset_cursor(unsigned *cursor) { *cursor = 0u; }
+
+// Persistent: every workgroup takes the next work item from the cursor until
+// none is left (every wave reaches the exit because the cursor only grows).
+// Items are sorted by cost (sources per task, descending) on the host.
+__global__ __launch_bounds__(kTpb, 2) void k_gf_lds_batch(const GfTaskDev *__restrict__ tasks,
+                                                         const GfItem *__restrict__ items, int nitems,
+                                                         unsigned *__restrict__ cursor) {
+  __shared__ uint64_t lds[kDch * kGpw * 64];
+  __shared__ int s_item;
+  for (;;) {
+    if (threadIdx.x == 0) s_item = (int)atomicAdd(cursor, 1u);
+    __syncthreads();
+    const int it = s_item;
+    __syncthreads();  // all lanes hold `it` before lane 0 may overwrite s_item
+    if (it >= nitems) break;
+    const GfItem w = items[it];
+    const GfTaskDev t = tasks[w.task];
+    gf_lds_tile<8, true>(lds, t.srcs, t.nsrc, t.dsts, t.ndst, t.tabs, t.words, t.accumulate, w.tile, w.rowblk);
+  }
+}
+
+int32_t launch_gf_batch(par2hip_ctx *ctx, hipStream_t s, const GfTaskDev *d_tasks, const GfItem *d_items,
+                        int nitems, unsigned *d_cursor, const char *tag) {
+  if (nitems == 0) return PAR2HIP_OK;
+  hipLaunchKernelGGL(k_reset_cursor, dim3(1), dim3(1), 0, s, d_cursor);
+  ScopedTiming t(ctx, tag, s);
+  const int grid = std::min(nitems, 2 * ctx->num_cus);
+  hipLaunchKernelGGL(k_gf_lds_batch, dim3(grid), dim3(kTpb), 0, s, d_tasks, d_items, nitems, d_cursor);
+  P2H_CHECK(ctx, hipGetLastError());
+  return PAR2HIP_OK;
 }
 
 }  // namespace par2hip

@@ -277,4 +277,84 @@ int32_t launch_copy_inverse(par2hip_ctx *ctx, hipStream_t s, uint16_t *coeff, si
   return PAR2HIP_OK;
 }
 
+// ---------------------------------------------------------------- batched --
+// One workgroup per repair job: the closed-form inverse (consecutive
+// exponents), then V rows and the inverse copy into the coefficient matrix.
+__global__ __launch_bounds__(1024) void k_inverse_batch(const InvJob *__restrict__ jobs,
+                                                       const uint16_t *__restrict__ log_tab,
+                                                       const uint16_t *__restrict__ exp_tab) {
+  extern __shared__ uint16_t sh16[];
+  const InvJob J = jobs[blockIdx.x];
+  const int m = J.m, tid = threadIdx.x;
+  uint16_t *tabs = sh16;
+  uint16_t *poly = sh16 + (size_t)m * 64;
+  for (int e = tid; e < m * 64; e += blockDim.x) {
+    const int k = e >> 6, iv = e & 63, i = iv >> 4, v = iv & 15;
+    const uint32_t x = (uint32_t)v << (4 * i);
+    tabs[e] = x ? exp_tab[log_tab[x] + J.miss_log[k]] : 0;
+  }
+  for (int i = tid; i <= m; i += blockDim.x) poly[i] = (i == 0) ? 1 : 0;
+  __syncthreads();
+  if (tid < 64) {
+    for (int k = 0; k < m; k++) {
+      const uint16_t *tk = tabs + (size_t)k * 64;
+      uint32_t nv[9];
+      int u = 0;
+      for (int i = tid; i <= k + 1; i += 64, u++) {
+        const uint32_t lo = i ? poly[i - 1] : 0u;
+        const uint32_t hi = (i <= k) ? poly[i] : 0u;
+        nv[u] = lo ^ mul_tab(tk, hi);
+      }
+      __builtin_amdgcn_wave_barrier();
+      u = 0;
+      for (int i = tid; i <= k + 1; i += 64, u++) poly[i] = (uint16_t)nv[u];
+      __builtin_amdgcn_wave_barrier();
+    }
+  }
+  __syncthreads();
+  for (int j = tid; j < m; j += blockDim.x) {
+    const uint16_t *tj = tabs + (size_t)j * 64;
+    uint16_t *row = J.aug + (size_t)j * 2 * m + m;
+    uint32_t q = poly[m];
+    row[m - 1] = (uint16_t)q;
+    uint32_t h = q;
+    for (int i = m - 1; i >= 1; i--) {
+      q = poly[i] ^ mul_tab(tj, q);
+      row[i - 1] = (uint16_t)q;
+      h = mul_tab(tj, h) ^ q;
+    }
+    const uint32_t lx = J.miss_log[j];
+    const uint32_t ld = (log_tab[h] + (uint32_t)(((uint64_t)lx * J.e0) % kOrder)) % kOrder;
+    const uint32_t linv = kOrder - ld;
+    for (int i = 0; i < m; i++) {
+      const uint32_t v = row[i];
+      const uint16_t w = v ? exp_tab[log_tab[v] + linv] : 0;
+      row[i] = w;
+      J.coeff[(size_t)j * J.cs + J.np + i] = w;  // inverse columns of C
+    }
+  }
+  // V rows: V[i][c] = c_{present_c}^{e_i}
+  for (long e = tid; e < (long)m * J.np; e += blockDim.x) {
+    const int i = (int)(e / J.np), c = (int)(e % J.np);
+    J.vand[(size_t)i * J.vs + c] = exp_tab[fold_mod65535(J.plog[c] * J.exps[i])];
+  }
+}
+
+int32_t launch_inverse_batch(par2hip_ctx *ctx, hipStream_t s, const InvJob *d_jobs, int njobs, int max_m,
+                             long max_vand) {
+  (void)max_vand;
+  if (njobs == 0) return PAR2HIP_OK;
+  if (max_m > kVandMaxM) return fail(ctx, PAR2HIP_EINVAL, "too many erasures for the closed-form inverse");
+  static std::once_flag once;
+  std::call_once(once, [] {
+    (void)hipFuncSetAttribute(reinterpret_cast<const void *>(&k_inverse_batch),
+                              hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024 - 1024);
+  });
+  ScopedTiming t(ctx, "vandermonde_inverse", s);
+  hipLaunchKernelGGL(k_inverse_batch, dim3((unsigned)njobs), dim3(1024), (size_t)(max_m * 64 + max_m + 1) * 2, s,
+                     d_jobs, ctx->d_log, ctx->d_exp);
+  P2H_CHECK(ctx, hipGetLastError());
+  return PAR2HIP_OK;
+}
+
 }  // namespace par2hip

@@ -131,21 +131,12 @@ __device__ __forceinline__ void md5_load_partial(const uint8_t *__restrict__ p, 
   }
 }
 
-// One lane per slice.  kAlign: 16 or 4 = every slice start has that byte
-// alignment; 1 = unaligned (byte loads).
+// MD5 of one slice: bytes [0, len) of p followed by zeros up to `total`
+// (the IFSC zero padding), digest to out[0..3].  kAlign: 16 or 4 = p has that
+// byte alignment; 1 = unaligned (byte loads).
 template <int kAlign>
-__global__ __launch_bounds__(64) void k_md5_slices(const uint8_t *__restrict__ data, long n,
-                                                   uint64_t slice_size, uint64_t stride,
-                                                   const uint64_t *__restrict__ lens,
-                                                   uint32_t *__restrict__ out, int high_priority) {
-  // when these few chains bound the caller's step, let them issue ahead of
-  // co-resident waves of other kernels
-  if (high_priority) __builtin_amdgcn_s_setprio(3);
-  const long i = (long)blockIdx.x * 64 + threadIdx.x;
-  if (i >= n) return;
-  const uint8_t *p = data + (uint64_t)i * stride;
-  uint64_t len = lens ? lens[i] : slice_size;
-  if (len > slice_size) len = slice_size;
+__device__ __forceinline__ void md5_slice(const uint8_t *__restrict__ p, uint64_t len, uint64_t total,
+                                          uint32_t *__restrict__ out) {
   uint32_t st[4] = {0x67452301u, 0xefcdab89u, 0x98badcfeu, 0x10325476u};
   uint32_t M[16];
   const uint64_t full = len & ~(uint64_t)63;  // blocks made entirely of real bytes
@@ -191,7 +182,6 @@ __global__ __launch_bounds__(64) void k_md5_slices(const uint8_t *__restrict__ d
     }
   }
   // the block holding the end of the real bytes (if any)
-  const uint64_t total = slice_size;
   if (pos < len) {
     md5_load_partial(p, pos, len, M);
     if (pos + 64 <= total) {
@@ -224,8 +214,47 @@ __global__ __launch_bounds__(64) void k_md5_slices(const uint8_t *__restrict__ d
     M[15] = (uint32_t)(bits >> 32);
     md5_compress(st, M);
   }
-  uint4 o = make_uint4(st[0], st[1], st[2], st[3]);
-  reinterpret_cast<uint4 *>(out)[i] = o;
+  *reinterpret_cast<uint4 *>(out) = make_uint4(st[0], st[1], st[2], st[3]);
+}
+
+// One lane per slice (64 slices per wavefront).
+template <int kAlign>
+__global__ __launch_bounds__(64) void k_md5_slices(const uint8_t *__restrict__ data, long n,
+                                                   uint64_t slice_size, uint64_t stride,
+                                                   const uint64_t *__restrict__ lens,
+                                                   uint32_t *__restrict__ out, int high_priority) {
+  // when these few chains bound the caller's step, let them issue ahead of
+  // co-resident waves of other kernels
+  if (high_priority) __builtin_amdgcn_s_setprio(3);
+  const long i = (long)blockIdx.x * 64 + threadIdx.x;
+  if (i >= n) return;
+  uint64_t len = lens ? lens[i] : slice_size;
+  if (len > slice_size) len = slice_size;
+  md5_slice<kAlign>(data + (uint64_t)i * stride, len, slice_size, out + 4 * i);
+}
+
+// Batched form: one lane per entry of a slice list (slices of many sets,
+// longest chains first so the waves stay even).
+__global__ __launch_bounds__(64) void k_md5_list(const HashEntry *__restrict__ list, long n, int aligned4,
+                                                 int high_priority) {
+  if (high_priority) __builtin_amdgcn_s_setprio(3);
+  const long i = (long)blockIdx.x * 64 + threadIdx.x;
+  if (i >= n) return;
+  const HashEntry e = list[i];
+  if (aligned4)
+    md5_slice<4>(e.data, e.len, e.size, e.md5);
+  else
+    md5_slice<1>(e.data, e.len, e.size, e.md5);
+}
+
+int32_t launch_md5_list(par2hip_ctx *ctx, hipStream_t s, const HashEntry *d_list, size_t n, bool aligned4,
+                        bool high_priority) {
+  if (n == 0) return PAR2HIP_OK;
+  ScopedTiming t(ctx, "md5_slices", s);
+  hipLaunchKernelGGL(k_md5_list, dim3((unsigned)((n + 63) / 64)), dim3(64), 0, s, d_list, (long)n, aligned4 ? 1 : 0,
+                     high_priority ? 1 : 0);
+  P2H_CHECK(ctx, hipGetLastError());
+  return PAR2HIP_OK;
 }
 
 int32_t launch_md5(par2hip_ctx *ctx, hipStream_t s, const uint8_t *d_data, size_t n,
@@ -269,44 +298,21 @@ __device__ __forceinline__ uint32_t gf2_mulmod(uint32_t a, uint32_t b) {
 // chunk from a zero register, then a binary tree joins neighbours with
 // left * x^(8 * chunk * 2^k) + right (level constants from the host), and the
 // initial register 0xFFFFFFFF contributes init_term = ~0 * x^(8*slice_size).
-constexpr int kCrcLanes = 256;
 
-struct CrcParams {
-  uint64_t chunk;       // bytes per lane (multiple of 16)
-  uint64_t pad;         // leading virtual zero bytes
-  uint32_t level[8];    // x^(8*chunk*2^k) mod P, k = 0..7
-  uint32_t init_term;   // 0xFFFFFFFF * x^(8*slice_size) mod P
-};
+
 
 __device__ __forceinline__ uint32_t crc_word(uint32_t crc, uint32_t w, const uint32_t (*T)[256]) {
   crc ^= w;
   return T[3][crc & 0xFF] ^ T[2][(crc >> 8) & 0xFF] ^ T[1][(crc >> 16) & 0xFF] ^ T[0][crc >> 24];
 }
 
+// CRC-32 of one slice by the whole 256-lane workgroup (tables T already in
+// LDS): bytes [0, len) of p, zeros up to slice_size; result to *out.
 template <bool kAligned16>
-__global__ __launch_bounds__(kCrcLanes) void k_crc32_slices(const uint8_t *__restrict__ data, uint64_t slice_size,
-                                                            uint64_t stride, const uint64_t *__restrict__ lens,
-                                                            const uint32_t *__restrict__ tab, CrcParams prm,
-                                                            uint32_t *__restrict__ out) {
-  __shared__ uint32_t T[4][256];
-  __shared__ uint32_t part[kCrcLanes / 64];
-  for (int k = threadIdx.x; k < 256; k += kCrcLanes) {
-    uint32_t c = tab[k];
-    T[0][k] = c;
-    // T_j[i]: byte i followed by j zero bytes
-    uint32_t c1 = (c >> 8) ^ tab[c & 0xFF];
-    uint32_t c2 = (c1 >> 8) ^ tab[c1 & 0xFF];
-    uint32_t c3 = (c2 >> 8) ^ tab[c2 & 0xFF];
-    T[1][k] = c1;
-    T[2][k] = c2;
-    T[3][k] = c3;
-  }
-  __syncthreads();
-  const long i = blockIdx.x;
+__device__ __forceinline__ void crc_slice_wg(const uint32_t (*T)[256], uint32_t *part, const uint8_t *__restrict__ p,
+                                             uint64_t len, uint64_t slice_size, const CrcParams &prm,
+                                             uint32_t *__restrict__ out) {
   const int lane = threadIdx.x;
-  const uint8_t *p = data + (uint64_t)i * stride;
-  uint64_t len = lens ? lens[i] : slice_size;
-  if (len > slice_size) len = slice_size;
   // this lane's bytes in slice coordinates: [a, e) real or zero-padding
   const int64_t va = (int64_t)lane * (int64_t)prm.chunk - (int64_t)prm.pad;
   const int64_t ve = va + (int64_t)prm.chunk;
@@ -345,17 +351,56 @@ __global__ __launch_bounds__(kCrcLanes) void k_crc32_slices(const uint8_t *__res
     uint32_t w01 = gf2_mulmod(part[0], prm.level[6]) ^ part[1];
     uint32_t w23 = gf2_mulmod(part[2], prm.level[6]) ^ part[3];
     const uint32_t tot = gf2_mulmod(w01, prm.level[7]) ^ w23;
-    out[i] = ~(prm.init_term ^ tot);
+    *out = ~(prm.init_term ^ tot);
   }
 }
 
+// T_j (j = 0..3, byte followed by j zero bytes) precomputed on the host:
+// one coalesced copy into LDS.
+__device__ __forceinline__ void crc_load_tables(uint32_t (*T)[256], const uint32_t *__restrict__ tab) {
+  for (int k = threadIdx.x; k < 1024; k += blockDim.x) (&T[0][0])[k] = tab[k];
+  __syncthreads();
+}
+
+template <bool kAligned16>
+__global__ __launch_bounds__(kCrcLanes) void k_crc32_slices(const uint8_t *__restrict__ data, uint64_t slice_size,
+                                                            uint64_t stride, const uint64_t *__restrict__ lens,
+                                                            const uint32_t *__restrict__ tab, CrcParams prm,
+                                                            uint32_t *__restrict__ out) {
+  __shared__ uint32_t T[4][256];
+  __shared__ uint32_t part[kCrcLanes / 64];
+  crc_load_tables(T, tab);
+  const long i = blockIdx.x;
+  uint64_t len = lens ? lens[i] : slice_size;
+  if (len > slice_size) len = slice_size;
+  crc_slice_wg<kAligned16>(T, part, data + (uint64_t)i * stride, len, slice_size, prm, out + i);
+}
+
+// Batched form: one workgroup per entry; entry e uses params[e.crc_params].
+__global__ __launch_bounds__(kCrcLanes) void k_crc32_list(const HashEntry *__restrict__ list,
+                                                          const CrcParams *__restrict__ params,
+                                                          const uint32_t *__restrict__ tab) {
+  __shared__ uint32_t T[4][256];
+  __shared__ uint32_t part[kCrcLanes / 64];
+  crc_load_tables(T, tab);
+  const HashEntry e = list[blockIdx.x];
+  const CrcParams prm = params[e.crc_params];
+  const bool a16 = (((uintptr_t)e.data) & 15) == 0 && (prm.pad & 15) == 0;
+  if (a16)
+    crc_slice_wg<true>(T, part, e.data, e.len, e.size, prm, e.crc);
+  else
+    crc_slice_wg<false>(T, part, e.data, e.len, e.size, prm, e.crc);
+}
+
 int32_t crc_tables_init(par2hip_ctx *ctx) {
-  uint32_t h[256];
+  uint32_t h[4][256];
   for (uint32_t k = 0; k < 256; k++) {
     uint32_t c = k;
     for (int b = 0; b < 8; b++) c = (c & 1) ? (c >> 1) ^ 0xEDB88320u : c >> 1;
-    h[k] = c;
+    h[0][k] = c;
   }
+  for (int j = 1; j < 4; j++)
+    for (uint32_t k = 0; k < 256; k++) h[j][k] = (h[j - 1][k] >> 8) ^ h[0][h[j - 1][k] & 0xFF];
   P2H_CHECK(ctx, hipMalloc(&ctx->d_crc_tab, sizeof h));
   P2H_CHECK(ctx, hipMemcpy(ctx->d_crc_tab, h, sizeof h, hipMemcpyHostToDevice));
   return PAR2HIP_OK;
@@ -383,6 +428,20 @@ static uint32_t host_x8n(uint64_t nbytes) {
 int32_t launch_crc32(par2hip_ctx *ctx, hipStream_t s, const uint8_t *d_data, size_t n,
                      size_t slice_size, size_t stride, const uint64_t *d_lens, uint32_t *d_crc) {
   if (n == 0) return PAR2HIP_OK;
+  const CrcParams prm = crc_params_for(slice_size);
+  const bool a16 = ((uintptr_t)d_data % 16 == 0) && (stride % 16 == 0) && (prm.pad % 16 == 0);
+  ScopedTiming t(ctx, "crc32_slices", s);
+  if (a16)
+    hipLaunchKernelGGL(k_crc32_slices<true>, dim3((unsigned)n), dim3(kCrcLanes), 0, s, d_data, (uint64_t)slice_size,
+                       (uint64_t)stride, d_lens, ctx->d_crc_tab, prm, d_crc);
+  else
+    hipLaunchKernelGGL(k_crc32_slices<false>, dim3((unsigned)n), dim3(kCrcLanes), 0, s, d_data, (uint64_t)slice_size,
+                       (uint64_t)stride, d_lens, ctx->d_crc_tab, prm, d_crc);
+  P2H_CHECK(ctx, hipGetLastError());
+  return PAR2HIP_OK;
+}
+
+CrcParams crc_params_for(uint64_t slice_size) {
   CrcParams prm;
   prm.chunk = ((slice_size + kCrcLanes - 1) / kCrcLanes + 15) & ~(uint64_t)15;
   if (prm.chunk == 0) prm.chunk = 16;
@@ -393,14 +452,43 @@ int32_t launch_crc32(par2hip_ctx *ctx, hipStream_t s, const uint8_t *d_data, siz
     lv = host_mulmod(lv, lv);
   }
   prm.init_term = host_mulmod(0xFFFFFFFFu, host_x8n(slice_size));
-  const bool a16 = ((uintptr_t)d_data % 16 == 0) && (stride % 16 == 0) && (prm.pad % 16 == 0);
+  return prm;
+}
+
+// Small slices: one lane per entry, the plain sequential CRC (register
+// starts at ~0, no combine step).
+__global__ __launch_bounds__(256) void k_crc32_list_lane(const HashEntry *__restrict__ list, long n,
+                                                        const uint32_t *__restrict__ tab) {
+  __shared__ uint32_t T[4][256];
+  crc_load_tables(T, tab);
+  const long i = (long)blockIdx.x * 256 + threadIdx.x;
+  if (i >= n) return;
+  const HashEntry e = list[i];
+  const uint8_t *p = e.data;
+  uint32_t crc = 0xFFFFFFFFu;
+  uint64_t o = 0;
+  if ((((uintptr_t)p) & 3) == 0)
+    for (; o + 4 <= e.len; o += 4) crc = crc_word(crc, *reinterpret_cast<const uint32_t *>(p + o), T);
+  for (; o < e.len; o++) crc = T[0][(crc ^ p[o]) & 0xFF] ^ (crc >> 8);
+  for (; o + 4 <= e.size; o += 4) crc = crc_word(crc, 0u, T);
+  for (; o < e.size; o++) crc = T[0][crc & 0xFF] ^ (crc >> 8);
+  *e.crc = ~crc;
+}
+
+int32_t launch_crc32_list(par2hip_ctx *ctx, hipStream_t s, const HashEntry *d_list, size_t n,
+                          const CrcParams *d_params) {
+  if (n == 0) return PAR2HIP_OK;
   ScopedTiming t(ctx, "crc32_slices", s);
-  if (a16)
-    hipLaunchKernelGGL(k_crc32_slices<true>, dim3((unsigned)n), dim3(kCrcLanes), 0, s, d_data, (uint64_t)slice_size,
-                       (uint64_t)stride, d_lens, ctx->d_crc_tab, prm, d_crc);
-  else
-    hipLaunchKernelGGL(k_crc32_slices<false>, dim3((unsigned)n), dim3(kCrcLanes), 0, s, d_data, (uint64_t)slice_size,
-                       (uint64_t)stride, d_lens, ctx->d_crc_tab, prm, d_crc);
+  hipLaunchKernelGGL(k_crc32_list, dim3((unsigned)n), dim3(kCrcLanes), 0, s, d_list, d_params, ctx->d_crc_tab);
+  P2H_CHECK(ctx, hipGetLastError());
+  return PAR2HIP_OK;
+}
+
+int32_t launch_crc32_list_lane(par2hip_ctx *ctx, hipStream_t s, const HashEntry *d_list, size_t n) {
+  if (n == 0) return PAR2HIP_OK;
+  ScopedTiming t(ctx, "crc32_small_slices", s);
+  hipLaunchKernelGGL(k_crc32_list_lane, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, d_list, (long)n,
+                     ctx->d_crc_tab);
   P2H_CHECK(ctx, hipGetLastError());
   return PAR2HIP_OK;
 }

@@ -57,6 +57,7 @@ int32_t scratch(par2hip_ctx *ctx, int role, size_t bytes, void **out) {
       P2H_CHECK(ctx, hipStreamSynchronize(ctx->stream));
       P2H_CHECK(ctx, hipStreamSynchronize(ctx->side));
       P2H_CHECK(ctx, hipStreamSynchronize(ctx->rest));
+      P2H_CHECK(ctx, hipStreamSynchronize(ctx->wide));
       P2H_CHECK(ctx, hipFree(b.ptr));
       b.ptr = nullptr;
       b.bytes = 0;
@@ -77,6 +78,7 @@ int32_t params_begin(par2hip_ctx *ctx, size_t bytes) {
     P2H_CHECK(ctx, hipStreamSynchronize(ctx->stream));
     P2H_CHECK(ctx, hipStreamSynchronize(ctx->side));
     P2H_CHECK(ctx, hipStreamSynchronize(ctx->rest));
+    P2H_CHECK(ctx, hipStreamSynchronize(ctx->wide));
     if (a.host) (void)hipHostFree(a.host);
     if (a.dev) (void)hipFree(a.dev);
     a.host = nullptr;
@@ -460,6 +462,16 @@ int32_t repair_set_impl(par2hip_ctx *ctx, uint8_t *d_slices, size_t n, size_t sl
 
 }  // namespace
 
+namespace par2hip {
+// entry points of the single-set path used by the batch planner (batch.hip)
+int32_t reconstruct_single(par2hip_ctx *ctx, uint8_t *d_slices, size_t n, size_t slice_size, const uint32_t *missing,
+                           size_t m, const uint8_t *d_recovery, const uint32_t *rec_exps, uint8_t *d_out) {
+  return reconstruct_dev_impl(ctx, d_slices, n, slice_size, missing, m, d_recovery, rec_exps, d_out, false);
+}
+int32_t fork_overlap(par2hip_ctx *ctx) { return fork_side(ctx); }
+int32_t join_overlap(par2hip_ctx *ctx) { return join_side(ctx); }
+}  // namespace par2hip
+
 extern "C" {
 
 int32_t par2hip_available(void) {
@@ -484,7 +496,9 @@ int32_t par2hip_ctx_create(int32_t device, par2hip_ctx **out) {
   if (hipGetDeviceProperties(&prop, device) == hipSuccess && prop.multiProcessorCount > 0)
     ctx->num_cus = prop.multiProcessorCount;
   P2H_CHECK(ctx, hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking));
+  P2H_CHECK(ctx, hipStreamCreateWithFlags(&ctx->wide, hipStreamNonBlocking));
   ctx->cur = ctx->stream;
+  if (hipGetDeviceProperties(&prop, device) == hipSuccess) strncpy(ctx->arch, prop.gcnArchName, sizeof ctx->arch - 1);
   {
     // MD5 chains get a few CUs of their own (enough for 64 chains per SIMD-wave,
     // 2048 slices); overlapped work uses the complementary mask.  Without CU
@@ -499,6 +513,7 @@ int32_t par2hip_ctx_create(int32_t device, par2hip_ctx **out) {
       ok = false;
     }
     ctx->masked = ok;
+    ctx->hash_cus = ok ? hash_cus : 0;
     if (!ok) {
       P2H_CHECK(ctx, hipStreamCreateWithFlags(&ctx->side, hipStreamNonBlocking));
       ctx->rest = ctx->stream;
@@ -527,6 +542,7 @@ int32_t par2hip_ctx_destroy(par2hip_ctx *ctx) {
     (void)hipStreamSynchronize(ctx->stream);
     (void)hipStreamSynchronize(ctx->side);
     if (ctx->rest != ctx->stream) (void)hipStreamSynchronize(ctx->rest);
+    (void)hipStreamSynchronize(ctx->wide);
     for (auto &b : ctx->scratch)
       if (b.ptr) (void)hipFree(b.ptr);
     if (ctx->arena.host) (void)hipHostFree(ctx->arena.host);
@@ -544,6 +560,7 @@ int32_t par2hip_ctx_destroy(par2hip_ctx *ctx) {
     (void)hipEventDestroy(ctx->join);
     (void)hipEventDestroy(ctx->join2);
     if (ctx->rest != ctx->stream) (void)hipStreamDestroy(ctx->rest);
+    (void)hipStreamDestroy(ctx->wide);
     (void)hipEventDestroy(ctx->arena.copied);
     (void)hipStreamDestroy(ctx->stream);
     (void)hipStreamDestroy(ctx->side);
@@ -553,6 +570,15 @@ int32_t par2hip_ctx_destroy(par2hip_ctx *ctx) {
 }
 
 const char *par2hip_last_error(par2hip_ctx *ctx) { return ctx ? ctx->err.c_str() : "null context"; }
+
+int32_t par2hip_get_device_info(par2hip_ctx *ctx, par2hip_device_info *out) {
+  if (!ctx || !out) return PAR2HIP_EINVAL;
+  memset(out, 0, sizeof *out);
+  out->compute_units = ctx->num_cus;
+  out->hash_cus = ctx->hash_cus;
+  strncpy(out->arch, ctx->arch, sizeof out->arch - 1);
+  return PAR2HIP_OK;
+}
 
 int32_t par2hip_dev_alloc(par2hip_ctx *ctx, size_t bytes, void **dptr) {
   if (!ctx || !dptr) return PAR2HIP_EINVAL;
@@ -592,6 +618,7 @@ int32_t par2hip_sync(par2hip_ctx *ctx) {
   P2H_CHECK(ctx, hipStreamSynchronize(ctx->stream));
   P2H_CHECK(ctx, hipStreamSynchronize(ctx->side));
   P2H_CHECK(ctx, hipStreamSynchronize(ctx->rest));
+  P2H_CHECK(ctx, hipStreamSynchronize(ctx->wide));
   return PAR2HIP_OK;
 }
 
@@ -893,6 +920,7 @@ int32_t par2hip_profile_read(par2hip_ctx *ctx, par2hip_kernel_stat *out, size_t 
   P2H_CHECK(ctx, hipStreamSynchronize(ctx->stream));
   P2H_CHECK(ctx, hipStreamSynchronize(ctx->side));
   P2H_CHECK(ctx, hipStreamSynchronize(ctx->rest));
+  P2H_CHECK(ctx, hipStreamSynchronize(ctx->wide));
   for (auto &p : ctx->pending) {
     float ms = 0;
     if (hipEventElapsedTime(&ms, p.start, p.stop) == hipSuccess) {

@@ -54,6 +54,20 @@ class DeviceError(Par2Error):
     pass
 
 
+JOB_ENCODE, JOB_REPAIR = 0, 1
+
+
+class Job(C.Structure):
+    """par2hip_job: one PAR2 set to encode or repair in a batch."""
+    _fields_ = [("kind", C.c_int32), ("reserved", C.c_int32), ("d_slices", C.c_void_p), ("n", C.c_uint64),
+                ("slice_size", C.c_uint64), ("exps", C.c_void_p), ("count", C.c_uint64), ("d_recovery", C.c_void_p),
+                ("missing", C.c_void_p), ("d_out", C.c_void_p), ("d_md5", C.c_void_p), ("d_crc", C.c_void_p)]
+
+
+class DeviceInfo(C.Structure):
+    _fields_ = [("compute_units", C.c_int32), ("hash_cus", C.c_int32), ("arch", C.c_char * 32)]
+
+
 class KernelStat(C.Structure):
     _fields_ = [("name", C.c_char * 48), ("launches", C.c_uint64), ("total_ms", C.c_double)]
 
@@ -96,6 +110,8 @@ def lib():
                 "par2hip_reconstruct": ([P, P, S, S, P, S, P, P, P], I32),
                 "par2hip_reconstruct_dev": ([P, P, S, S, P, S, P, P, P, I32], I32),
                 "par2hip_repair_set_dev": ([P, P, S, S, P, P, P, P, S, P, P, C.POINTER(S)], I32),
+                "par2hip_batch_run_dev": ([P, P, S], I32),
+                "par2hip_get_device_info": ([P, P], I32),
                 "par2hip_hash_slices": ([P, P, S, S, S, P, P, P], I32),
                 "par2hip_hash_slices_dev": ([P, P, S, S, S, P, P, P], I32),
                 "par2hip_set_gf_kernel": ([P, I32], I32),
@@ -200,6 +216,11 @@ class Context:
             raise InvalidFormat(msg or "invalid argument")
         raise DeviceError(f"{rc}: {msg}")
 
+    def device_info(self) -> dict:
+        d = DeviceInfo()
+        self._check(lib().par2hip_get_device_info(self.handle, C.byref(d)))
+        return {"compute_units": d.compute_units, "hash_cus": d.hash_cus, "arch": d.arch.decode()}
+
     # ---- memory
     def alloc(self, nbytes: int) -> DeviceBuffer:
         return DeviceBuffer(self, nbytes)
@@ -272,6 +293,30 @@ class Context:
             raise InsufficientRecovery(f"Need {cnt.value} recovery blocks, have {len(exps)}")
         self._check(rc)
         return dmg[:cnt.value].copy()
+
+    def batch_run_dev(self, jobs) -> None:
+        """Run many sets at once.  Each job is a dict: kind ('encode'|'repair'),
+        slices (DeviceBuffer), n, slice_size, exps, recovery (DeviceBuffer),
+        [missing, out (DeviceBuffer) for repair], [md5, crc (DeviceBuffer)]."""
+        arr = (Job * len(jobs))()
+        keep = []
+        for i, j in enumerate(jobs):
+            e = np.ascontiguousarray(j.get("exps", []), np.uint32)
+            miss = np.ascontiguousarray(j.get("missing", []), np.uint32)
+            keep += [e, miss]
+            a = arr[i]
+            a.kind = JOB_REPAIR if j["kind"] == "repair" else JOB_ENCODE
+            a.d_slices = j["slices"].ptr
+            a.n = j["n"]
+            a.slice_size = j["slice_size"]
+            a.exps = e.ctypes.data if len(e) else None
+            a.count = len(miss) if j["kind"] == "repair" else len(e)
+            a.d_recovery = j["recovery"].ptr if j.get("recovery") is not None else None
+            a.missing = miss.ctypes.data if len(miss) else None
+            a.d_out = j["out"].ptr if j.get("out") is not None else None
+            a.d_md5 = j["md5"].ptr if j.get("md5") is not None else None
+            a.d_crc = j["crc"].ptr if j.get("crc") is not None else None
+        self._check(lib().par2hip_batch_run_dev(self.handle, arr, len(jobs)))
 
     def gf_invert(self, a: np.ndarray) -> np.ndarray:
         a = np.ascontiguousarray(a, np.uint16)

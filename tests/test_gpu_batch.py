@@ -1,0 +1,132 @@
+"""Batched runs (many sets of uneven sizes, encode and repair mixed, one
+persistent contraction launch): every job's bytes equal the single-set path
+and the oracle."""
+from __future__ import annotations
+
+import sys
+
+import numpy as np
+import pytest
+
+import oracle
+import par2hip
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print("[case]", *a, flush=True)
+    sys.stdout.flush()
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    if par2hip.available() <= 0:
+        pytest.fail("no gfx950 device visible to libpar2hip.so")
+    c = par2hip.Context(0)
+    yield c
+    c.close()
+
+
+def make_jobs(ctx, rng, specs):
+    """specs: (kind, n, slice_size, r_or_m, consecutive, hashes)"""
+    jobs, host = [], []
+    for kind, n, s, k, consec, hashes in specs:
+        data = rng.integers(0, 256, (n, s), dtype=np.uint8)
+        d_sl = ctx.alloc(n * s)
+        job = {"kind": kind, "slices": d_sl, "n": n, "slice_size": s}
+        h = {"data": data, "kind": kind}
+        if kind == "encode":
+            exps = np.arange(k, dtype=np.uint32) if consec else np.sort(rng.choice(200, k, replace=False)).astype(np.uint32)
+            d_sl.upload(data)
+            job.update(exps=exps, recovery=ctx.alloc(max(k, 1) * s))
+            h["exps"] = exps
+        else:
+            exps = np.arange(k, dtype=np.uint32) + (3 if consec == "shift" else 0)
+            if not consec:
+                exps = np.sort(rng.choice(60, k, replace=False)).astype(np.uint32)
+            rec = oracle.encode(data.view(np.uint16), exps).view(np.uint8).reshape(k, s)
+            missing = np.sort(rng.choice(n, k, replace=False)).astype(np.uint32)
+            damaged = data.copy()
+            damaged[missing] = 0
+            d_sl.upload(damaged)
+            d_rec = ctx.alloc(k * s)
+            d_rec.upload(rec)
+            job.update(exps=exps, recovery=d_rec, missing=missing, out=ctx.alloc(k * s))
+            h.update(exps=exps, missing=missing, damaged=damaged)
+        if hashes:
+            job.update(md5=ctx.alloc(n * 16), crc=ctx.alloc(n * 4))
+        jobs.append(job)
+        host.append(h)
+    return jobs, host
+
+
+def check(ctx, jobs, host):
+    for i, (j, h) in enumerate(zip(jobs, host)):
+        n, s = j["n"], j["slice_size"]
+        if j["kind"] == "encode":
+            r = len(h["exps"])
+            got = j["recovery"].download(r * s).reshape(r, s)
+            want = oracle.encode(h["data"].view(np.uint16), h["exps"]).view(np.uint8).reshape(r, s)
+            assert np.array_equal(got, want), f"job {i} encode"
+            hashed = h["data"]
+        else:
+            m = len(h["missing"])
+            got = j["out"].download(m * s).reshape(m, s)
+            assert np.array_equal(got, h["data"][h["missing"]]), f"job {i} repair"
+            hashed = h["damaged"]
+        if "md5" in j:
+            md5 = j["md5"].download().reshape(n, 16)
+            crc = j["crc"].download().view(np.uint32)
+            wm, wc = oracle.hash_slices(hashed, n, s)
+            assert np.array_equal(md5, wm) and np.array_equal(crc, wc), f"job {i} hashes"
+
+
+def test_batch_mixed_uneven(ctx):
+    rng = np.random.default_rng(2026)
+    specs = [
+        ("encode", 300, 4096, 30, True, True),
+        ("encode", 7, 5376, 3, True, True),
+        ("encode", 1, 2, 1, True, True),        # one word
+        ("encode", 40, 1026, 17, False, True),  # 2-byte aligned rows, arbitrary exponents
+        ("encode", 129, 65540, 9, True, False),  # 4-byte aligned rows
+        ("repair", 200, 2048, 20, True, True),
+        ("repair", 60, 4100, 5, "shift", True),  # consecutive exponents from 3
+        ("repair", 33, 18, 4, False, True),      # gapped exponents: Gauss-Jordan path
+        ("repair", 5, 512, 5, True, False),      # every data slice lost
+        ("encode", 64, 8192, 0, True, True),     # hashes only
+    ]
+    jobs, host = make_jobs(ctx, rng, specs)
+    say("batch mixed", len(jobs))
+    ctx.batch_run_dev(jobs)
+    ctx.sync()
+    check(ctx, jobs, host)
+
+
+def test_batch_matches_single_set_path(ctx):
+    rng = np.random.default_rng(7)
+    specs = [("encode", int(rng.integers(8, 400)), int(rng.integers(2, 3000)) * 4, int(rng.integers(1, 40)), True, True)
+             for _ in range(24)]
+    jobs, host = make_jobs(ctx, rng, specs)
+    say("batch 24 encodes")
+    ctx.batch_run_dev(jobs)
+    ctx.sync()
+    for j, h in zip(jobs, host):
+        n, s, r = j["n"], j["slice_size"], len(h["exps"])
+        single = ctx.encode(h["data"], h["exps"])
+        assert np.array_equal(j["recovery"].download(r * s).reshape(r, s), single)
+
+
+def test_batch_repeat_and_empty(ctx):
+    rng = np.random.default_rng(9)
+    jobs, host = make_jobs(ctx, rng, [("encode", 50, 2048, 6, True, True), ("repair", 50, 2048, 6, True, True)])
+    say("batch repeat")
+    for _ in range(3):
+        ctx.batch_run_dev(jobs)
+    ctx.sync()
+    check(ctx, jobs, host)
+    ctx.batch_run_dev([])
+    with pytest.raises(par2hip.InvalidFormat):
+        bad = dict(jobs[0])
+        bad["slice_size"] = 7
+        ctx.batch_run_dev([bad])
