@@ -306,9 +306,36 @@ __device__ __forceinline__ uint32_t crc_word(uint32_t crc, uint32_t w, const uin
   return T[3][crc & 0xFF] ^ T[2][(crc >> 8) & 0xFF] ^ T[1][(crc >> 16) & 0xFF] ^ T[0][crc >> 24];
 }
 
+// Sequential CRC register update over bytes [o, end) of p, any alignment:
+// single bytes up to a 16-byte boundary, then 64-byte blocks of four 16-byte
+// loads, then words, then the tail bytes.
+__device__ __forceinline__ uint32_t crc_run(uint32_t crc, const uint8_t *__restrict__ p, uint64_t o, uint64_t end,
+                                            const uint32_t (*T)[256]) {
+  uint64_t head = (uint64_t)((16 - ((uintptr_t)(p + o) & 15)) & 15);
+  if (head > end - o) head = end - o;
+  for (const uint64_t h = o + head; o < h; o++) crc = T[0][(crc ^ p[o]) & 0xFF] ^ (crc >> 8);
+  for (; o + 64 <= end; o += 64) {
+    const uint4 *q = reinterpret_cast<const uint4 *>(p + o);
+    const uint4 v0 = q[0], v1 = q[1], v2 = q[2], v3 = q[3];
+    crc = crc_word(crc, v0.x, T); crc = crc_word(crc, v0.y, T); crc = crc_word(crc, v0.z, T); crc = crc_word(crc, v0.w, T);
+    crc = crc_word(crc, v1.x, T); crc = crc_word(crc, v1.y, T); crc = crc_word(crc, v1.z, T); crc = crc_word(crc, v1.w, T);
+    crc = crc_word(crc, v2.x, T); crc = crc_word(crc, v2.y, T); crc = crc_word(crc, v2.z, T); crc = crc_word(crc, v2.w, T);
+    crc = crc_word(crc, v3.x, T); crc = crc_word(crc, v3.y, T); crc = crc_word(crc, v3.z, T); crc = crc_word(crc, v3.w, T);
+  }
+  for (; o + 4 <= end; o += 4) crc = crc_word(crc, *reinterpret_cast<const uint32_t *>(p + o), T);
+  for (; o < end; o++) crc = T[0][(crc ^ p[o]) & 0xFF] ^ (crc >> 8);
+  return crc;
+}
+
+// Register update over n zero bytes.
+__device__ __forceinline__ uint32_t crc_zeros(uint32_t crc, uint64_t n, const uint32_t (*T)[256]) {
+  for (; n >= 4; n -= 4) crc = crc_word(crc, 0u, T);
+  for (; n; n--) crc = T[0][crc & 0xFF] ^ (crc >> 8);
+  return crc;
+}
+
 // CRC-32 of one slice by the whole 256-lane workgroup (tables T already in
 // LDS): bytes [0, len) of p, zeros up to slice_size; result to *out.
-template <bool kAligned16>
 __device__ __forceinline__ void crc_slice_wg(const uint32_t (*T)[256], uint32_t *part, const uint8_t *__restrict__ p,
                                              uint64_t len, uint64_t slice_size, const CrcParams &prm,
                                              uint32_t *__restrict__ out) {
@@ -319,24 +346,10 @@ __device__ __forceinline__ void crc_slice_wg(const uint32_t (*T)[256], uint32_t 
   const uint64_t a = va < 0 ? 0 : (uint64_t)va;
   const uint64_t e = ve < 0 ? 0 : (uint64_t)ve;   // e <= slice_size by construction
   const uint64_t real_end = e < len ? e : len;
-  uint32_t crc = 0;
-  uint64_t o = a;
-  if (kAligned16) {
-    // a is a multiple of 16 here (chunk, pad multiples of 16, aligned base)
-    for (; o + 64 <= real_end; o += 64) {
-      const uint4 *q = reinterpret_cast<const uint4 *>(p + o);
-      const uint4 v0 = q[0], v1 = q[1], v2 = q[2], v3 = q[3];
-      crc = crc_word(crc, v0.x, T); crc = crc_word(crc, v0.y, T); crc = crc_word(crc, v0.z, T); crc = crc_word(crc, v0.w, T);
-      crc = crc_word(crc, v1.x, T); crc = crc_word(crc, v1.y, T); crc = crc_word(crc, v1.z, T); crc = crc_word(crc, v1.w, T);
-      crc = crc_word(crc, v2.x, T); crc = crc_word(crc, v2.y, T); crc = crc_word(crc, v2.z, T); crc = crc_word(crc, v2.w, T);
-      crc = crc_word(crc, v3.x, T); crc = crc_word(crc, v3.y, T); crc = crc_word(crc, v3.z, T); crc = crc_word(crc, v3.w, T);
-    }
-    for (; o + 4 <= real_end; o += 4) crc = crc_word(crc, *reinterpret_cast<const uint32_t *>(p + o), T);
-  }
-  for (; o < real_end; o++) crc = T[0][(crc ^ p[o]) & 0xFF] ^ (crc >> 8);
+  uint32_t crc = a < real_end ? crc_run(0u, p, a, real_end, T) : 0u;
   // zero padding inside this lane's range (only when the slice is short)
-  for (; o + 4 <= e; o += 4) crc = crc_word(crc, 0u, T);
-  for (; o < e; o++) crc = T[0][crc & 0xFF] ^ (crc >> 8);
+  const uint64_t zs = a > real_end ? a : real_end;
+  if (e > zs) crc = crc_zeros(crc, e - zs, T);
 
   // tree join: level k merges block [l, l+2^k) with its right neighbour
   uint32_t v = crc;
@@ -362,7 +375,6 @@ __device__ __forceinline__ void crc_load_tables(uint32_t (*T)[256], const uint32
   __syncthreads();
 }
 
-template <bool kAligned16>
 __global__ __launch_bounds__(kCrcLanes) void k_crc32_slices(const uint8_t *__restrict__ data, uint64_t slice_size,
                                                             uint64_t stride, const uint64_t *__restrict__ lens,
                                                             const uint32_t *__restrict__ tab, CrcParams prm,
@@ -373,7 +385,7 @@ __global__ __launch_bounds__(kCrcLanes) void k_crc32_slices(const uint8_t *__res
   const long i = blockIdx.x;
   uint64_t len = lens ? lens[i] : slice_size;
   if (len > slice_size) len = slice_size;
-  crc_slice_wg<kAligned16>(T, part, data + (uint64_t)i * stride, len, slice_size, prm, out + i);
+  crc_slice_wg(T, part, data + (uint64_t)i * stride, len, slice_size, prm, out + i);
 }
 
 // Batched form: one workgroup per entry; entry e uses params[e.crc_params].
@@ -385,11 +397,7 @@ __global__ __launch_bounds__(kCrcLanes) void k_crc32_list(const HashEntry *__res
   crc_load_tables(T, tab);
   const HashEntry e = list[blockIdx.x];
   const CrcParams prm = params[e.crc_params];
-  const bool a16 = (((uintptr_t)e.data) & 15) == 0 && (prm.pad & 15) == 0;
-  if (a16)
-    crc_slice_wg<true>(T, part, e.data, e.len, e.size, prm, e.crc);
-  else
-    crc_slice_wg<false>(T, part, e.data, e.len, e.size, prm, e.crc);
+  crc_slice_wg(T, part, e.data, e.len, e.size, prm, e.crc);
 }
 
 int32_t crc_tables_init(par2hip_ctx *ctx) {
@@ -429,14 +437,9 @@ int32_t launch_crc32(par2hip_ctx *ctx, hipStream_t s, const uint8_t *d_data, siz
                      size_t slice_size, size_t stride, const uint64_t *d_lens, uint32_t *d_crc) {
   if (n == 0) return PAR2HIP_OK;
   const CrcParams prm = crc_params_for(slice_size);
-  const bool a16 = ((uintptr_t)d_data % 16 == 0) && (stride % 16 == 0) && (prm.pad % 16 == 0);
   ScopedTiming t(ctx, "crc32_slices", s);
-  if (a16)
-    hipLaunchKernelGGL(k_crc32_slices<true>, dim3((unsigned)n), dim3(kCrcLanes), 0, s, d_data, (uint64_t)slice_size,
-                       (uint64_t)stride, d_lens, ctx->d_crc_tab, prm, d_crc);
-  else
-    hipLaunchKernelGGL(k_crc32_slices<false>, dim3((unsigned)n), dim3(kCrcLanes), 0, s, d_data, (uint64_t)slice_size,
-                       (uint64_t)stride, d_lens, ctx->d_crc_tab, prm, d_crc);
+  hipLaunchKernelGGL(k_crc32_slices, dim3((unsigned)n), dim3(kCrcLanes), 0, s, d_data, (uint64_t)slice_size,
+                     (uint64_t)stride, d_lens, ctx->d_crc_tab, prm, d_crc);
   P2H_CHECK(ctx, hipGetLastError());
   return PAR2HIP_OK;
 }
@@ -464,14 +467,9 @@ __global__ __launch_bounds__(256) void k_crc32_list_lane(const HashEntry *__rest
   const long i = (long)blockIdx.x * 256 + threadIdx.x;
   if (i >= n) return;
   const HashEntry e = list[i];
-  const uint8_t *p = e.data;
-  uint32_t crc = 0xFFFFFFFFu;
-  uint64_t o = 0;
-  if ((((uintptr_t)p) & 3) == 0)
-    for (; o + 4 <= e.len; o += 4) crc = crc_word(crc, *reinterpret_cast<const uint32_t *>(p + o), T);
-  for (; o < e.len; o++) crc = T[0][(crc ^ p[o]) & 0xFF] ^ (crc >> 8);
-  for (; o + 4 <= e.size; o += 4) crc = crc_word(crc, 0u, T);
-  for (; o < e.size; o++) crc = T[0][crc & 0xFF] ^ (crc >> 8);
+  const uint64_t len = e.len < e.size ? e.len : e.size;
+  uint32_t crc = crc_run(0xFFFFFFFFu, e.data, 0, len, T);
+  crc = crc_zeros(crc, e.size - len, T);
   *e.crc = ~crc;
 }
 

@@ -164,6 +164,12 @@ class DeviceBuffer:
         self.ctx._check(lib().par2hip_d2h(self.ctx.handle, _p(out), C.c_void_p(self.addr + offset), nbytes))
         return out
 
+    def view(self, offset: int) -> "DeviceView":
+        """A non-owning pointer `offset` bytes into this buffer."""
+        if not 0 <= offset <= self.nbytes:
+            raise InvalidFormat("view beyond buffer")
+        return DeviceView(self, offset)
+
     def free(self) -> None:
         if self.ptr is not None and self.ptr.value:
             lib().par2hip_dev_free(self.ctx.handle, self.ptr)
@@ -175,6 +181,15 @@ class DeviceBuffer:
                 self.free()
         except Exception:
             pass
+
+
+class DeviceView:
+    """Borrowed device pointer into a DeviceBuffer (keeps the owner alive)."""
+
+    def __init__(self, owner: DeviceBuffer, offset: int):
+        self.owner = owner
+        self.nbytes = owner.nbytes - offset
+        self.ptr = C.c_void_p(owner.addr + offset)
 
 
 class Context:

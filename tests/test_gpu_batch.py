@@ -89,7 +89,8 @@ def test_batch_mixed_uneven(ctx):
         ("encode", 7, 5376, 3, True, True),
         ("encode", 1, 2, 1, True, True),        # one word
         ("encode", 40, 1026, 17, False, True),  # 2-byte aligned rows, arbitrary exponents
-        ("encode", 129, 65540, 9, True, False),  # 4-byte aligned rows
+        ("encode", 129, 65540, 9, True, True),   # 4-byte aligned rows, workgroup-per-slice CRC
+        ("repair", 6, (1 << 18) + 20, 3, True, True),
         ("repair", 200, 2048, 20, True, True),
         ("repair", 60, 4100, 5, "shift", True),  # consecutive exponents from 3
         ("repair", 33, 18, 4, False, True),      # gapped exponents: Gauss-Jordan path

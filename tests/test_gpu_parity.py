@@ -155,7 +155,8 @@ def test_hash_golden_vectors(ctx):
         assert md5[i].tobytes().hex() == s["md5"] and int(crc[i]) == s["crc32"]
 
 
-@pytest.mark.parametrize("n,s,stride", [(1, 64, 0), (3, 55, 60), (65, 120, 0), (130, 4096, 4100), (7, 1 << 20, 0)])
+@pytest.mark.parametrize("n,s,stride", [(1, 64, 0), (3, 55, 60), (65, 120, 0), (130, 4096, 4100), (7, 1 << 20, 0),
+                                          (5, 262146, 262150), (4, 99999, 100001)])
 def test_hash_shapes_vs_stdlib(ctx, n, s, stride):
     rng = np.random.default_rng(n + s)
     st = stride or s
@@ -165,6 +166,26 @@ def test_hash_shapes_vs_stdlib(ctx, n, s, stride):
     for i in range(n):
         b = buf[i * st:i * st + s].tobytes()
         assert md5[i].tobytes() == hashlib.md5(b).digest() and int(crc[i]) == zlib.crc32(b)
+
+
+@pytest.mark.parametrize("shift,n,s,stride", [(1, 6, 70000, 70001), (2, 9, 4100, 4102), (3, 3, (1 << 20) + 4, (1 << 20) + 7)])
+def test_hash_dev_unaligned_bases(ctx, shift, n, s, stride):
+    """Slices at any byte address (the device entry point takes them where
+    they lie): both hashes equal the stdlib's."""
+    rng = np.random.default_rng(shift * 1000 + n)
+    buf = rng.integers(0, 256, shift + n * stride, dtype=np.uint8)
+    d = ctx.alloc(len(buf))
+    d.upload(buf)
+    d_md5, d_crc = ctx.alloc(16 * n), ctx.alloc(4 * n)
+    say("hash dev unaligned", shift, n, s, stride)
+    ctx.hash_slices_dev(d.view(shift), n, s, d_md5, d_crc, slice_stride=stride)
+    ctx.sync()
+    md5 = d_md5.download().reshape(n, 16)
+    crc = d_crc.download().view(np.uint32)
+    for i in range(n):
+        b = buf[shift + i * stride:shift + i * stride + s].tobytes()
+        assert md5[i].tobytes() == hashlib.md5(b).digest(), f"md5 slice {i}"
+        assert int(crc[i]) == zlib.crc32(b), f"crc slice {i}"
 
 
 def test_hash_ragged_lengths_zero_padded(ctx):
