@@ -10,6 +10,13 @@
 
 namespace par2hip {
 
+// Largest power of two (at most 16) dividing every address in ptrs.
+static int row_align(const void *const *ptrs, size_t n) {
+  uintptr_t acc = 16;
+  for (size_t i = 0; i < n; i++) acc |= (uintptr_t)ptrs[i];
+  return (int)(acc & (~acc + 1));
+}
+
 namespace {
 
 struct Arena {
@@ -86,7 +93,7 @@ int32_t batch_run(par2hip_ctx *ctx, const par2hip_job *jobs, size_t njobs) {
   // tasks: one fold per repair job (if any present slice), one main per job
   struct TaskPlan { size_t tabs; GfTaskDev t; size_t srcs_at, dsts_at; };
   std::vector<TaskPlan> fold_tasks, main_tasks;
-  auto tab_bytes = [](int ndst, int nsrc) { return (size_t)((ndst + 3) / 4) * nsrc * 64 * 8; };
+  auto tab_bytes = [](int ndst, int nsrc) { return (size_t)gf_tab_entries(ndst, nsrc) * 16; };
   for (size_t k = 0; k < rep.size(); k++) {
     const par2hip_job &J = jobs[rep[k]];
     if (rp[k].np) {
@@ -220,6 +227,7 @@ int32_t batch_run(par2hip_ctx *ctx, const par2hip_job *jobs, size_t njobs) {
     t.ldc = (int)J.n;
     t.tabs = (uint64_t *)(arena + main_tasks[k].tabs);
     t.accumulate = 0;
+    t.align = std::min(row_align((const void *const *)h_srcs, J.n), row_align((const void *const *)h_dsts, J.count));
   }
   std::vector<InvJob> ijobs;
   int max_m = 0;
@@ -278,6 +286,7 @@ int32_t batch_run(par2hip_ctx *ctx, const par2hip_job *jobs, size_t njobs) {
         t.ldc = (int)(2 * m);
         t.tabs = (uint64_t *)(arena + fold_tasks[f].tabs);
         t.accumulate = 0;
+        t.align = std::min(row_align((const void *const *)h_fsrc, m), row_align((const void *const *)h_fdst, m));
         f++;
       }
       GfTaskDev &t = main_tasks[enc.size() + k].t;
@@ -287,6 +296,7 @@ int32_t batch_run(par2hip_ctx *ctx, const par2hip_job *jobs, size_t njobs) {
       t.ldc = (int)rp[k].cs;
       t.tabs = (uint64_t *)(arena + main_tasks[enc.size() + k].tabs);
       t.accumulate = 0;
+      t.align = std::min(row_align((const void *const *)h_srcs, np + m), row_align((const void *const *)h_dsts, m));
     }
   }
   auto put = [&](const void *src, size_t bytes) -> void * {
@@ -299,11 +309,11 @@ int32_t batch_run(par2hip_ctx *ctx, const par2hip_job *jobs, size_t njobs) {
   long max_ft = 0, max_mt = 0;
   for (size_t i = 0; i < ft.size(); i++) {
     ft[i] = fold_tasks[i].t;
-    max_ft = std::max(max_ft, (long)((ft[i].ndst + 3) / 4) * ft[i].nsrc * 64);
+    max_ft = std::max(max_ft, gf_tab_entries(ft[i].ndst, ft[i].nsrc));
   }
   for (size_t i = 0; i < mt.size(); i++) {
     mt[i] = main_tasks[i].t;
-    max_mt = std::max(max_mt, (long)((mt[i].ndst + 3) / 4) * mt[i].nsrc * 64);
+    max_mt = std::max(max_mt, gf_tab_entries(mt[i].ndst, mt[i].nsrc));
   }
   const GfTaskDev *d_ft = (const GfTaskDev *)put(ft.data(), ft.size() * sizeof(GfTaskDev));
   const GfTaskDev *d_mt = (const GfTaskDev *)put(mt.data(), mt.size() * sizeof(GfTaskDev));

@@ -159,20 +159,32 @@ struct GfProblem {
 };
 int32_t run_gf(par2hip_ctx *ctx, hipStream_t s, const GfProblem &p);
 
+// ---- variant T nibble tables.  One 16-byte entry per (group of 8 output
+// rows g, source d, nibble position i, nibble value v), entry index
+// (g * nsrc + d) * 64 + 16 * i + v; it packs the 8 rows' products
+// c_{8g+j,d} * (v << 4i) as u16 j (little-endian, so u32 j/2, half j%2).
+// A workgroup covers kGpw groups = 16 output rows.
+constexpr int kTabRows = 8;
+constexpr int kGpw = 2;
+__host__ __device__ inline int gf_tab_groups(int ndst) { return (ndst + kTabRows - 1) / kTabRows; }
+__host__ __device__ inline long gf_tab_entries(int ndst, int nsrc) { return (long)gf_tab_groups(ndst) * nsrc * 64; }
+__host__ __device__ inline int gf_task_rowblks(int ndst) { return (gf_tab_groups(ndst) + kGpw - 1) / kGpw; }
+
 // ---- batched contraction (many sets in one persistent launch)
 struct GfTaskDev {
   const uint16_t *const *srcs;
   uint16_t *const *dsts;
   const uint16_t *coeff;  // [ndst][ldc]
-  uint64_t *tabs;         // nibble tables of this task, ngroups*nsrc*64 entries
+  uint64_t *tabs;         // nibble tables of this task, gf_tab_entries(ndst, nsrc) 16-byte entries
   long words;
   int nsrc, ndst, ldc, accumulate;
+  int align;  // byte alignment (2..16) shared by every source and destination row
+  int pad_;
 };
 struct GfItem {
   int task, tile, rowblk, pad_;
 };
 constexpr int kBatchWordsPerWg = 2048;  // 256 threads x 8 words
-inline int gf_task_rowblks(int ndst) { return ((ndst + 3) / 4 + 3) / 4; }
 // Nibble tables of every task (grid: elements x tasks), then the persistent
 // kernel works through `items` (sorted by cost) with an atomic cursor.
 int32_t launch_tables_batch(par2hip_ctx *ctx, hipStream_t s, const GfTaskDev *d_tasks, int ntasks,
@@ -214,6 +226,21 @@ struct CrcParams {
   uint32_t init_term;   // 0xFFFFFFFF * x^(8*slice_size) mod P
 };
 CrcParams crc_params_for(uint64_t slice_size);
+// Pointers read from device memory are generic to the compiler, which then
+// emits flat memory operations: their completion count is shared with LDS and
+// scalar loads, so every wait for a table lookup also waits for HBM.  Typing
+// them in the global address space gives global operations with their own
+// counter.
+#define P2H_GLOBAL __attribute__((address_space(1)))
+template <class T>
+__device__ __forceinline__ P2H_GLOBAL T *gptr(T *p) {
+  return (P2H_GLOBAL T *)p;
+}
+// Native vector types (HIP's uint2/uint4 are classes whose operators take
+// generic references) for single vector accesses through global pointers.
+typedef uint32_t u32x2 __attribute__((ext_vector_type(2)));
+typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+
 // One slice of a batched hash pass (device-resident list).
 struct HashEntry {
   const uint8_t *data;  // slice bytes

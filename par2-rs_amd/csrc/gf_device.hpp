@@ -6,55 +6,94 @@
 namespace par2hip {
 
 constexpr int kTpb = 256;  // threads per workgroup
-constexpr int kGpw = 4;    // output groups (of 4 rows) per workgroup -> 16 rows
-constexpr int kDch = 16;   // sources per LDS table chunk
+constexpr int kDch = 16;   // sources per LDS table chunk: 16 x kGpw x 1 KiB = 32 KiB
+
+// c * x for every row of one 8-row group as the table entry laid out above
+// (products via log/exp; zero coefficients and x = 0 give zero).
+__device__ __forceinline__ u32x4 gf_tab_entry(const uint16_t *__restrict__ coeff, int ldc, int ndst, int g, int d,
+                                              int iv, const uint16_t *__restrict__ log_tab,
+                                              const uint16_t *__restrict__ exp_tab) {
+  const uint32_t x = (uint32_t)(iv & 15) << (4 * (iv >> 4));
+  uint32_t w[4] = {0u, 0u, 0u, 0u};
+  if (x) {
+    const uint32_t lx = log_tab[x];
+#pragma unroll
+    for (int j = 0; j < kTabRows; j++) {
+      const int r = kTabRows * g + j;
+      if (r < ndst) {
+        const uint16_t c = coeff[(long)r * ldc + d];
+        if (c) w[j >> 1] |= (uint32_t)exp_tab[lx + log_tab[c]] << (16 * (j & 1));
+      }
+    }
+  }
+  const u32x4 e = {w[0], w[1], w[2], w[3]};
+  return e;
+}
 
 template <int V>
 struct WordVec;
 template <>
-struct WordVec<8> { using T = uint4; };
+struct WordVec<8> { using T = u32x4; };
 template <>
-struct WordVec<4> { using T = uint2; };
+struct WordVec<4> { using T = u32x2; };
 template <>
 struct WordVec<2> { using T = uint32_t; };
 template <>
 struct WordVec<1> { using T = uint16_t; };
 
-template <int V, bool kDynAlign = false>
-__device__ inline void load_words(const uint16_t *__restrict__ p, long w0, long words, uint32_t (&x)[V]) {
-  if (w0 + V <= words && (!kDynAlign || ((uintptr_t)(p + w0) & (2 * V - 1)) == 0)) {
-    typename WordVec<V>::T t = *reinterpret_cast<const typename WordVec<V>::T *>(p + w0);
-    const uint16_t *h = reinterpret_cast<const uint16_t *>(&t);
+// How a thread moves its V words: 0 = one vector access (the whole range is
+// inside the row and aligned to 2V bytes), 1 = dword pairs (inside the row,
+// 4-byte aligned), 2 = word by word with bounds checks (the last, partial
+// tile; rows aligned to 2 bytes only).  Modes 0 and 1 issue unconditional
+// accesses, so the wait before a use covers exactly the load it needs.
+// Words stay packed two per dword (little-endian: word 2i in the low half of
+// r[i]) until they are used.
+template <int V, int kMode>
+__device__ __forceinline__ void load_words(const P2H_GLOBAL uint16_t *__restrict__ p, long w0, long words,
+                                           uint32_t (&r)[(V + 1) / 2]) {
+  using Vec = typename WordVec<V>::T;
+  if constexpr (kMode == 0) {
+    const Vec t = *reinterpret_cast<const P2H_GLOBAL Vec *>(p + w0);
+    if constexpr (V == 1)
+      r[0] = t;
+    else
+      __builtin_memcpy(r, &t, sizeof t);
+  } else if constexpr (kMode == 1) {
 #pragma unroll
-    for (int k = 0; k < V; k++) x[k] = h[k];
-  } else if (kDynAlign && w0 + V <= words && ((uintptr_t)(p + w0) & 3) == 0) {
-#pragma unroll
-    for (int k = 0; k < V; k += 2) {
-      const uint32_t t = *reinterpret_cast<const uint32_t *>(p + w0 + k);
-      x[k] = t & 0xFFFFu;
-      x[k + 1] = t >> 16;
-    }
+    for (int k = 0; k < V; k += 2) r[k / 2] = *reinterpret_cast<const P2H_GLOBAL uint32_t *>(p + w0 + k);
   } else {
 #pragma unroll
-    for (int k = 0; k < V; k++) x[k] = (w0 + k < words) ? p[w0 + k] : 0u;
+    for (int k = 0; k < (V + 1) / 2; k++) r[k] = 0;
+#pragma unroll
+    for (int k = 0; k < V; k++)
+      if (w0 + k < words) r[k / 2] |= (uint32_t)p[w0 + k] << (16 * (k & 1));
   }
 }
 
-template <int V, bool kDynAlign = false>
-__device__ inline void store_words(uint16_t *__restrict__ p, long w0, long words, const uint32_t (&x)[V],
-                                   bool accumulate) {
-  if (w0 + V <= words && (!kDynAlign || ((uintptr_t)(p + w0) & (2 * V - 1)) == 0)) {
-    typename WordVec<V>::T t;
+template <int V, int kMode>
+__device__ __forceinline__ void store_words(P2H_GLOBAL uint16_t *__restrict__ p, long w0, long words,
+                                            const uint32_t (&x)[V], bool accumulate) {
+  using Vec = typename WordVec<V>::T;
+  if constexpr (kMode == 0) {
+    Vec t;
     uint16_t *h = reinterpret_cast<uint16_t *>(&t);
     if (accumulate) {
-      t = *reinterpret_cast<const typename WordVec<V>::T *>(p + w0);
+      t = *reinterpret_cast<const P2H_GLOBAL Vec *>(p + w0);
 #pragma unroll
       for (int k = 0; k < V; k++) h[k] ^= (uint16_t)x[k];
     } else {
 #pragma unroll
       for (int k = 0; k < V; k++) h[k] = (uint16_t)x[k];
     }
-    *reinterpret_cast<typename WordVec<V>::T *>(p + w0) = t;
+    *reinterpret_cast<P2H_GLOBAL Vec *>(p + w0) = t;
+  } else if constexpr (kMode == 1) {
+#pragma unroll
+    for (int k = 0; k < V; k += 2) {
+      P2H_GLOBAL uint32_t *q = reinterpret_cast<P2H_GLOBAL uint32_t *>(p + w0 + k);
+      uint32_t t = (x[k] & 0xFFFFu) | (x[k + 1] << 16);
+      if (accumulate) t ^= *q;
+      *q = t;
+    }
   } else {
 #pragma unroll
     for (int k = 0; k < V; k++)
@@ -62,73 +101,119 @@ __device__ inline void store_words(uint16_t *__restrict__ p, long w0, long words
   }
 }
 
+// acc ^= (tables of source d in lds) x (its V words), the words packed two per
+// dword.  One 16-byte LDS read per nibble and group (8 rows); the four
+// partial products and the accumulator are folded by two three-input XORs.
+template <int V>
+__device__ __forceinline__ void gf_lds_lookup(u32x4 (&acc)[kGpw][V], const u32x4 *__restrict__ t,
+                                              const uint32_t (&r)[(V + 1) / 2]) {
+#pragma unroll
+  for (int k = 0; k < V; k++) {
+    const uint32_t x = r[k / 2] >> (16 * (k & 1));
+    const uint32_t a0 = x & 15u, a1 = 16u + ((x >> 4) & 15u);
+    const uint32_t a2 = 32u + ((x >> 8) & 15u), a3 = 48u + ((x >> 12) & 15u);
+#pragma unroll
+    for (int g = 0; g < kGpw; g++) {
+      const u32x4 *tg = t + g * 64;
+      const u32x4 l0 = tg[a0], l1 = tg[a1], l2 = tg[a2], l3 = tg[a3];
+#pragma unroll
+      for (int q = 0; q < 4; q++) {
+        const uint32_t h = __builtin_amdgcn_bitop3_b32(acc[g][k][q], l0[q], l1[q], 0x96);
+        acc[g][k][q] = __builtin_amdgcn_bitop3_b32(h, l2[q], l3[q], 0x96);
+      }
+    }
+  }
+}
+
+// acc ^= sum over sources d0..d0+dn-1 of (their tables in lds) x (their
+// words).  Memory latency is hidden by the other waves of the CU (software
+// pipelining here only raises register pressure past the occupancy limit).
+template <int V, int kMode>
+__device__ __forceinline__ void gf_lds_sources(u32x4 (&acc)[kGpw][V], const u32x4 *__restrict__ lds,
+                                               const uint16_t *const *__restrict__ srcs, int d0, int dn, long w0,
+                                               long words) {
+  const auto sp = gptr(srcs) + d0;  // the row pointers, read from global memory
+  for (int d = 0; d < dn; d++) {
+    uint32_t r[(V + 1) / 2];
+    load_words<V, kMode>(gptr(sp[d]), w0, words, r);
+    gf_lds_lookup<V>(acc, lds + d * kGpw * 64, r);
+  }
+}
+
+template <int V, int kMode>
+__device__ __forceinline__ void gf_lds_store(const u32x4 (&acc)[kGpw][V], uint16_t *const *__restrict__ dsts,
+                                             int ndst, int g0, long w0, long words, int accumulate) {
+  const auto dp = gptr(dsts);
+#pragma unroll
+  for (int g = 0; g < kGpw; g++) {
+#pragma unroll
+    for (int j = 0; j < kTabRows; j++) {
+      const int r = kTabRows * (g0 + g) + j;
+      if (r < ndst) {
+        uint32_t y[V];
+#pragma unroll
+        for (int k = 0; k < V; k++) y[k] = (acc[g][k][j >> 1] >> (16 * (j & 1))) & 0xFFFFu;
+        store_words<V, kMode>(gptr(dp[r]), w0, words, y, accumulate != 0);
+      }
+    }
+  }
+}
+
 // One workgroup's share of a contraction: words [tile*kTpb*V, +kTpb*V) of
 // output rows [16*rowblk, +16) (variant T, LDS nibble tables).  Must be called
-// by the whole workgroup (it synchronises); `lds` holds 32 KiB.
-// kDynAlign: pick 16-, 4- or 2-byte loads per source row at run time (rows of
-// a set whose slice size is a multiple of 4 bytes but not of 16).
-template <int V, bool kDynAlign>
-__device__ __forceinline__ void gf_lds_tile(uint64_t *__restrict__ lds, const uint16_t *const *__restrict__ srcs,
+// by the whole workgroup (it synchronises); `lds` holds 32 KiB.  align: the
+// byte alignment (2, 4, 8 or 16) shared by every source and destination row;
+// a thread whose V words lie inside the rows uses vector accesses when align
+// >= 2V, dword pairs when align >= 4, and checked word accesses otherwise.
+template <int V>
+__device__ __forceinline__ void gf_lds_tile(u32x4 *__restrict__ lds, const uint16_t *const *__restrict__ srcs,
                                             int nsrc, uint16_t *const *__restrict__ dsts, int ndst,
                                             const uint64_t *__restrict__ tabs, long words, int accumulate,
-                                            int tile, int rowblk) {
+                                            int tile, int rowblk, int align) {
   const int tid = threadIdx.x;
-  const int ngroups = (ndst + 3) / 4;
+  const int ngroups = gf_tab_groups(ndst);
   const int g0 = rowblk * kGpw;
   const long w0 = ((long)tile * kTpb + tid) * V;
+  const bool inside = w0 + V <= words;
+  const int mode = (inside && align >= 2 * V) ? 0 : (inside && align >= 4 && V % 2 == 0) ? 1 : 2;
+  const P2H_GLOBAL u32x4 *gtab = reinterpret_cast<const P2H_GLOBAL u32x4 *>(gptr(tabs));
 
-  uint64_t acc[kGpw][V];
+  u32x4 acc[kGpw][V];
 #pragma unroll
   for (int g = 0; g < kGpw; g++)
 #pragma unroll
-    for (int k = 0; k < V; k++) acc[g][k] = 0;
+    for (int k = 0; k < V; k++) acc[g][k] = (u32x4){0u, 0u, 0u, 0u};
 
   for (int d0 = 0; d0 < nsrc; d0 += kDch) {
     const int dn = min(kDch, nsrc - d0);
     __syncthreads();
-    // stage the tables of sources d0..d0+dn-1 for this workgroup's groups (16-byte units)
-    const int units = kGpw * dn * 32;
+    // stage the tables of sources d0..d0+dn-1 for this workgroup's groups:
+    // lds[(d * kGpw + g) * 64 + e] = entry e of (group g0+g, source d0+d)
+    const int units = kGpw * dn * 64;
     for (int u = tid; u < units; u += kTpb) {
-      const int g = u / (dn * 32), rem = u - g * dn * 32, d = rem >> 5, k = rem & 31;
-      uint4 val = make_uint4(0, 0, 0, 0);
-      if (g0 + g < ngroups)
-        val = reinterpret_cast<const uint4 *>(tabs + ((long)(g0 + g) * nsrc + d0 + d) * 64)[k];
-      reinterpret_cast<uint4 *>(lds + (d * kGpw + g) * 64)[k] = val;
+      const int g = u / (dn * 64), rem = u - g * dn * 64, d = rem >> 6, e = rem & 63;
+      u32x4 val = {0u, 0u, 0u, 0u};
+      if (g0 + g < ngroups) val = gtab[((long)(g0 + g) * nsrc + d0 + d) * 64 + e];
+      lds[(d * kGpw + g) * 64 + e] = val;
     }
     __syncthreads();
     if (w0 < words) {
-      for (int d = 0; d < dn; d++) {
-        uint32_t x[V];
-        load_words<V, kDynAlign>(srcs[d0 + d], w0, words, x);
-        const uint64_t *t = lds + d * kGpw * 64;
-#pragma unroll
-        for (int k = 0; k < V; k++) {
-          const uint32_t a0 = x[k] & 15u, a1 = 16u + ((x[k] >> 4) & 15u);
-          const uint32_t a2 = 32u + ((x[k] >> 8) & 15u), a3 = 48u + (x[k] >> 12);
-#pragma unroll
-          for (int g = 0; g < kGpw; g++) {
-            const uint64_t *tg = t + g * 64;
-            acc[g][k] ^= tg[a0] ^ tg[a1] ^ tg[a2] ^ tg[a3];
-          }
-        }
-      }
+      if (mode == 0)
+        gf_lds_sources<V, 0>(acc, lds, srcs, d0, dn, w0, words);
+      else if (V % 2 == 0 && mode == 1)
+        gf_lds_sources<V, (V % 2 == 0) ? 1 : 2>(acc, lds, srcs, d0, dn, w0, words);
+      else
+        gf_lds_sources<V, 2>(acc, lds, srcs, d0, dn, w0, words);
     }
   }
 
   if (w0 >= words) return;
-#pragma unroll
-  for (int g = 0; g < kGpw; g++) {
-#pragma unroll
-    for (int j = 0; j < 4; j++) {
-      const int r = 4 * (g0 + g) + j;
-      if (r < ndst) {
-        uint32_t y[V];
-#pragma unroll
-        for (int k = 0; k < V; k++) y[k] = (uint32_t)(acc[g][k] >> (16 * j)) & 0xFFFFu;
-        store_words<V, kDynAlign>(dsts[r], w0, words, y, accumulate != 0);
-      }
-    }
-  }
+  if (mode == 0)
+    gf_lds_store<V, 0>(acc, dsts, ndst, g0, w0, words, accumulate);
+  else if (V % 2 == 0 && mode == 1)
+    gf_lds_store<V, (V % 2 == 0) ? 1 : 2>(acc, dsts, ndst, g0, w0, words, accumulate);
+  else
+    gf_lds_store<V, 2>(acc, dsts, ndst, g0, w0, words, accumulate);
 }
 
 }  // namespace par2hip

@@ -47,33 +47,15 @@ int32_t launch_coeff_from_logs(par2hip_ctx *ctx, hipStream_t s, uint16_t *coeff,
 }
 
 // ------------------------------------------------- variant T: nibble tables --
-// Table layout: tabs[g][d][i*16 + v], g = output group (rows 4g..4g+3),
-// d = source, i = nibble position, v = nibble value; entry packs the four
-// rows' products c_{4g+j,d} * (v << 4i) at bits [16j, 16j+16).
+// One thread per 16-byte table entry (layout in common.hpp).
 __global__ void k_nibble_tables(uint64_t *__restrict__ tabs, const uint16_t *__restrict__ coeff,
                                 int nrow, int nsrc, int ldc, const uint16_t *__restrict__ log_tab,
                                 const uint16_t *__restrict__ exp_tab) {
   const long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  const int ngroups = (nrow + 3) / 4;
-  if (idx >= (long)ngroups * nsrc * 64) return;
-  const int iv = (int)(idx & 63);
+  if (idx >= gf_tab_entries(nrow, nsrc)) return;
   const long gd = idx >> 6;
   const int d = (int)(gd % nsrc), g = (int)(gd / nsrc);
-  const int i = iv >> 4, v = iv & 15;
-  const uint32_t x = (uint32_t)v << (4 * i);
-  uint64_t e = 0;
-  if (x) {
-    const uint32_t lx = log_tab[x];
-#pragma unroll
-    for (int j = 0; j < 4; j++) {
-      const int r = 4 * g + j;
-      if (r < nrow) {
-        const uint16_t c = coeff[(long)r * ldc + d];
-        if (c) e |= (uint64_t)exp_tab[lx + log_tab[c]] << (16 * j);
-      }
-    }
-  }
-  tabs[idx] = e;
+  reinterpret_cast<u32x4 *>(tabs)[idx] = gf_tab_entry(coeff, ldc, nrow, g, d, (int)(idx & 63), log_tab, exp_tab);
 }
 
 // Workgroup w covers words [tile*kTpb*V, +kTpb*V) of output rows
@@ -85,7 +67,7 @@ __global__ __launch_bounds__(kTpb, 2) void k_gf_lds(const uint16_t *const *__res
                                                    uint16_t *const *__restrict__ dsts, int ndst,
                                                    const uint64_t *__restrict__ tabs, long words,
                                                    int accumulate, int ntiles, int nrowblk) {
-  __shared__ uint64_t lds[kDch * kGpw * 64];  // [d][g][i*16+v], 32 KiB
+  __shared__ u32x4 lds[kDch * kGpw * 64];  // [d][g][i*16+v], 32 KiB
   const int L = blockIdx.x;
   const int xcd = L & 7, q = L >> 3;
   const int tiles_per_xcd = (ntiles + 7) >> 3;
@@ -93,7 +75,7 @@ __global__ __launch_bounds__(kTpb, 2) void k_gf_lds(const uint16_t *const *__res
   const int local_tile = q / nrowblk, rowblk = q % nrowblk;
   const int tile = local_tile * 8 + xcd;
   if (local_tile >= tiles_per_xcd || tile >= ntiles) return;
-  gf_lds_tile<V, false>(lds, srcs, nsrc, dsts, ndst, tabs, words, accumulate, tile, rowblk);
+  gf_lds_tile<V>(lds, srcs, nsrc, dsts, ndst, tabs, words, accumulate, tile, rowblk, 2 * V);
 }
 
 template <int V>
@@ -101,8 +83,7 @@ static int32_t launch_lds_variant(par2hip_ctx *ctx, hipStream_t s, const GfProbl
                                   const uint64_t *tabs) {
   const long per_wg = (long)kTpb * V;
   const int ntiles = (int)((p.words + per_wg - 1) / per_wg);
-  const int ngroups = (p.ndst + 3) / 4;
-  const int nrowblk = (ngroups + kGpw - 1) / kGpw;
+  const int nrowblk = gf_task_rowblks(p.ndst);
   const int tiles_per_xcd = (ntiles + 7) / 8;
   const long nwg = 8L * tiles_per_xcd * nrowblk;
   if (nwg > 0x7fffffffL) return fail(ctx, PAR2HIP_EINVAL, "GF problem too large for one launch");
@@ -114,13 +95,12 @@ static int32_t launch_lds_variant(par2hip_ctx *ctx, hipStream_t s, const GfProbl
 }
 
 int32_t run_gf_lds(par2hip_ctx *ctx, hipStream_t s, const GfProblem &p) {
-  const int ngroups = (p.ndst + 3) / 4;
-  const size_t tab_bytes = (size_t)ngroups * p.nsrc * 64 * sizeof(uint64_t);
+  const size_t tab_bytes = (size_t)gf_tab_entries(p.ndst, p.nsrc) * 16;
   void *tabs = nullptr;
   int32_t rc = scratch(ctx, kScratchTables, tab_bytes, &tabs);
   if (rc) return rc;
   {
-    const long total = (long)ngroups * p.nsrc * 64;
+    const long total = gf_tab_entries(p.ndst, p.nsrc);
     ScopedTiming t(ctx, "nibble_tables", s);
     hipLaunchKernelGGL(k_nibble_tables, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, s,
                        (uint64_t *)tabs, p.coeff, p.ndst, p.nsrc, p.ldc ? p.ldc : p.nsrc, ctx->d_log, ctx->d_exp);
@@ -129,7 +109,7 @@ int32_t run_gf_lds(par2hip_ctx *ctx, hipStream_t s, const GfProblem &p) {
   // words per thread: the widest the alignment allows (per-workgroup table
   // staging is amortised over more words); narrower only when the grid would
   // not even give every resident workgroup slot (2 per CU) one workgroup
-  const int nrowblk = (ngroups + kGpw - 1) / kGpw;
+  const int nrowblk = gf_task_rowblks(p.ndst);
   const long slots = 2L * ctx->num_cus;
   auto nwg = [&](int v) { return (long)((p.words + (size_t)kTpb * v - 1) / ((size_t)kTpb * v)) * nrowblk; };
   int v = p.align_words % 8 == 0 ? 8 : p.align_words % 4 == 0 ? 4 : p.align_words % 2 == 0 ? 2 : 1;
@@ -178,27 +158,12 @@ int32_t launch_coeff_batch(par2hip_ctx *ctx, hipStream_t s, const CoeffJob *d_jo
 __global__ void k_tables_batch(const GfTaskDev *__restrict__ tasks, const uint16_t *__restrict__ log_tab,
                                const uint16_t *__restrict__ exp_tab) {
   const GfTaskDev t = tasks[blockIdx.y];
-  const int ngroups = (t.ndst + 3) / 4;
   const long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (idx >= (long)ngroups * t.nsrc * 64) return;
-  const int iv = (int)(idx & 63);
+  if (idx >= gf_tab_entries(t.ndst, t.nsrc)) return;
   const long gd = idx >> 6;
   const int d = (int)(gd % t.nsrc), g = (int)(gd / t.nsrc);
-  const int i = iv >> 4, v = iv & 15;
-  const uint32_t x = (uint32_t)v << (4 * i);
-  uint64_t e = 0;
-  if (x) {
-    const uint32_t lx = log_tab[x];
-#pragma unroll
-    for (int jj = 0; jj < 4; jj++) {
-      const int r = 4 * g + jj;
-      if (r < t.ndst) {
-        const uint16_t c = t.coeff[(long)r * t.ldc + d];
-        if (c) e |= (uint64_t)exp_tab[lx + log_tab[c]] << (16 * jj);
-      }
-    }
-  }
-  t.tabs[idx] = e;
+  reinterpret_cast<P2H_GLOBAL u32x4 *>(gptr(t.tabs))[idx] =
+      gf_tab_entry(t.coeff, t.ldc, t.ndst, g, d, (int)(idx & 63), log_tab, exp_tab);
 }
 
 int32_t launch_tables_batch(par2hip_ctx *ctx, hipStream_t s, const GfTaskDev *d_tasks, int ntasks,
@@ -219,7 +184,7 @@ __global__ void k_reset_cursor(unsigned *cursor) { *cursor = 0u; }
 __global__ __launch_bounds__(kTpb, 2) void k_gf_lds_batch(const GfTaskDev *__restrict__ tasks,
                                                          const GfItem *__restrict__ items, int nitems,
                                                          unsigned *__restrict__ cursor) {
-  __shared__ uint64_t lds[kDch * kGpw * 64];
+  __shared__ u32x4 lds[kDch * kGpw * 64];
   __shared__ int s_item;
   for (;;) {
     if (threadIdx.x == 0) s_item = (int)atomicAdd(cursor, 1u);
@@ -229,7 +194,7 @@ __global__ __launch_bounds__(kTpb, 2) void k_gf_lds_batch(const GfTaskDev *__res
     if (it >= nitems) break;
     const GfItem w = items[it];
     const GfTaskDev t = tasks[w.task];
-    gf_lds_tile<8, true>(lds, t.srcs, t.nsrc, t.dsts, t.ndst, t.tabs, t.words, t.accumulate, w.tile, w.rowblk);
+    gf_lds_tile<8>(lds, t.srcs, t.nsrc, t.dsts, t.ndst, t.tabs, t.words, t.accumulate, w.tile, w.rowblk, t.align);
   }
 }
 

@@ -8,9 +8,10 @@
 //
 // MD5 is a serial chain per slice, so one lane owns one slice (64 slices per
 // wavefront); its latency per 64-byte block bounds the kernel.  CRC-32 is
-// linear, so one wavefront owns one slice: each lane folds a contiguous
-// chunk with slice-by-4 tables held in LDS, and the 64 partial remainders are
-// combined by multiplying with x^(8*bytes_after) mod P.
+// linear, so a large slice is split over a 256-lane workgroup: each lane folds
+// a contiguous chunk with slice-by-4 tables held in LDS, and the partial
+// remainders are joined by multiplying with x^(8*bytes_after) mod P.  Slices
+// under 64 KiB take one lane each (the plain sequential CRC).
 #include "common.hpp"
 
 namespace par2hip {
@@ -118,7 +119,7 @@ __device__ __forceinline__ void md5_compress(uint32_t st[4], const uint32_t M[16
 }
 
 // Bytes [pos, pos+64) of a slice whose real length is len, zeros beyond.
-__device__ __forceinline__ void md5_load_partial(const uint8_t *__restrict__ p, uint64_t pos, uint64_t len,
+__device__ __forceinline__ void md5_load_partial(const P2H_GLOBAL uint8_t *__restrict__ p, uint64_t pos, uint64_t len,
                                                  uint32_t M[16]) {
 #pragma unroll
   for (int k = 0; k < 16; k++) {
@@ -135,43 +136,54 @@ __device__ __forceinline__ void md5_load_partial(const uint8_t *__restrict__ p, 
 // (the IFSC zero padding), digest to out[0..3].  kAlign: 16 or 4 = p has that
 // byte alignment; 1 = unaligned (byte loads).
 template <int kAlign>
-__device__ __forceinline__ void md5_slice(const uint8_t *__restrict__ p, uint64_t len, uint64_t total,
-                                          uint32_t *__restrict__ out) {
+__device__ __forceinline__ void md5_slice(const P2H_GLOBAL uint8_t *__restrict__ p, uint64_t len, uint64_t total,
+                                          P2H_GLOBAL uint32_t *__restrict__ out) {
   uint32_t st[4] = {0x67452301u, 0xefcdab89u, 0x98badcfeu, 0x10325476u};
   uint32_t M[16];
   const uint64_t full = len & ~(uint64_t)63;  // blocks made entirely of real bytes
   uint64_t pos = 0;
-  if (kAlign == 16) {
-    // Two register sets alternate: while one block is compressed the next
-    // one's loads are in flight, with no copies between the sets.
-    const uint4 *q = reinterpret_cast<const uint4 *>(p);
+  if (kAlign >= 4) {
+    // Three register sets rotate with a prefetch distance of two blocks: the
+    // loads of block k+2 are issued before block k is compressed.  Every
+    // fetch is unconditional (indexes past the end re-read the last block),
+    // so each wait covers exactly the set about to be compressed.
     const uint64_t nblk = full >> 6;
-    uint32_t A[16], B[16];
-    auto fetch = [&](uint32_t (&D)[16], uint64_t blk) {
-      const uint4 *r = q + 4 * blk;
-      const uint4 t0 = r[0], t1 = r[1], t2 = r[2], t3 = r[3];
-      D[0] = t0.x; D[1] = t0.y; D[2] = t0.z; D[3] = t0.w;
-      D[4] = t1.x; D[5] = t1.y; D[6] = t1.z; D[7] = t1.w;
-      D[8] = t2.x; D[9] = t2.y; D[10] = t2.z; D[11] = t2.w;
-      D[12] = t3.x; D[13] = t3.y; D[14] = t3.z; D[15] = t3.w;
-    };
-    uint64_t blk = 0;
-    if (nblk) fetch(A, 0);
-    for (; blk + 2 <= nblk; blk += 2) {
-      fetch(B, blk + 1);
-      md5_compress(st, A);
-      if (blk + 2 < nblk) fetch(A, blk + 2);
-      md5_compress(st, B);
-    }
-    if (blk < nblk) md5_compress(st, A);
-    pos = full;
-  } else if (kAlign == 4) {
-    for (pos = 0; pos < full; pos += 64) {
-      const uint32_t *q = reinterpret_cast<const uint32_t *>(p + pos);
+    if (nblk) {
+      const uint64_t last = nblk - 1;
+      uint32_t A[16], B[16], C[16];
+      auto fetch = [&](uint32_t (&D)[16], uint64_t blk) {
+        blk = blk < last ? blk : last;
+        if (kAlign == 16) {
+          const P2H_GLOBAL u32x4 *r = reinterpret_cast<const P2H_GLOBAL u32x4 *>(p) + 4 * blk;
+          const u32x4 t0 = r[0], t1 = r[1], t2 = r[2], t3 = r[3];
+          D[0] = t0.x; D[1] = t0.y; D[2] = t0.z; D[3] = t0.w;
+          D[4] = t1.x; D[5] = t1.y; D[6] = t1.z; D[7] = t1.w;
+          D[8] = t2.x; D[9] = t2.y; D[10] = t2.z; D[11] = t2.w;
+          D[12] = t3.x; D[13] = t3.y; D[14] = t3.z; D[15] = t3.w;
+        } else {
+          const P2H_GLOBAL uint32_t *r = reinterpret_cast<const P2H_GLOBAL uint32_t *>(p) + 16 * blk;
 #pragma unroll
-      for (int k = 0; k < 16; k++) M[k] = q[k];
-      md5_compress(st, M);
+          for (int k = 0; k < 16; k++) D[k] = r[k];
+        }
+        // keep the loads here: the scheduler would otherwise sink them to
+        // their first use and expose the full memory latency per block
+        __builtin_amdgcn_sched_barrier(0);
+      };
+      fetch(A, 0);
+      fetch(B, 1);
+      uint64_t blk = 0;
+      for (; blk + 3 <= nblk; blk += 3) {
+        fetch(C, blk + 2);
+        md5_compress(st, A);
+        fetch(A, blk + 3);
+        md5_compress(st, B);
+        fetch(B, blk + 4);
+        md5_compress(st, C);
+      }
+      if (blk < nblk) md5_compress(st, A);
+      if (blk + 1 < nblk) md5_compress(st, B);
     }
+    pos = full;
   } else {
     for (pos = 0; pos < full; pos += 64) {
 #pragma unroll
@@ -214,7 +226,8 @@ __device__ __forceinline__ void md5_slice(const uint8_t *__restrict__ p, uint64_
     M[15] = (uint32_t)(bits >> 32);
     md5_compress(st, M);
   }
-  *reinterpret_cast<uint4 *>(out) = make_uint4(st[0], st[1], st[2], st[3]);
+  const u32x4 digest = {st[0], st[1], st[2], st[3]};
+  *reinterpret_cast<P2H_GLOBAL u32x4 *>(out) = digest;
 }
 
 // One lane per slice (64 slices per wavefront).
@@ -230,7 +243,7 @@ __global__ __launch_bounds__(64) void k_md5_slices(const uint8_t *__restrict__ d
   if (i >= n) return;
   uint64_t len = lens ? lens[i] : slice_size;
   if (len > slice_size) len = slice_size;
-  md5_slice<kAlign>(data + (uint64_t)i * stride, len, slice_size, out + 4 * i);
+  md5_slice<kAlign>(gptr(data) + (uint64_t)i * stride, len, slice_size, gptr(out) + 4 * i);
 }
 
 // Batched form: one lane per entry of a slice list (slices of many sets,
@@ -242,9 +255,9 @@ __global__ __launch_bounds__(64) void k_md5_list(const HashEntry *__restrict__ l
   if (i >= n) return;
   const HashEntry e = list[i];
   if (aligned4)
-    md5_slice<4>(e.data, e.len, e.size, e.md5);
+    md5_slice<4>(gptr(e.data), e.len, e.size, gptr(e.md5));
   else
-    md5_slice<1>(e.data, e.len, e.size, e.md5);
+    md5_slice<1>(gptr(e.data), e.len, e.size, gptr(e.md5));
 }
 
 int32_t launch_md5_list(par2hip_ctx *ctx, hipStream_t s, const HashEntry *d_list, size_t n, bool aligned4,
@@ -309,20 +322,20 @@ __device__ __forceinline__ uint32_t crc_word(uint32_t crc, uint32_t w, const uin
 // Sequential CRC register update over bytes [o, end) of p, any alignment:
 // single bytes up to a 16-byte boundary, then 64-byte blocks of four 16-byte
 // loads, then words, then the tail bytes.
-__device__ __forceinline__ uint32_t crc_run(uint32_t crc, const uint8_t *__restrict__ p, uint64_t o, uint64_t end,
+__device__ __forceinline__ uint32_t crc_run(uint32_t crc, const P2H_GLOBAL uint8_t *__restrict__ p, uint64_t o, uint64_t end,
                                             const uint32_t (*T)[256]) {
   uint64_t head = (uint64_t)((16 - ((uintptr_t)(p + o) & 15)) & 15);
   if (head > end - o) head = end - o;
   for (const uint64_t h = o + head; o < h; o++) crc = T[0][(crc ^ p[o]) & 0xFF] ^ (crc >> 8);
   for (; o + 64 <= end; o += 64) {
-    const uint4 *q = reinterpret_cast<const uint4 *>(p + o);
-    const uint4 v0 = q[0], v1 = q[1], v2 = q[2], v3 = q[3];
+    const P2H_GLOBAL u32x4 *q = reinterpret_cast<const P2H_GLOBAL u32x4 *>(p + o);
+    const u32x4 v0 = q[0], v1 = q[1], v2 = q[2], v3 = q[3];
     crc = crc_word(crc, v0.x, T); crc = crc_word(crc, v0.y, T); crc = crc_word(crc, v0.z, T); crc = crc_word(crc, v0.w, T);
     crc = crc_word(crc, v1.x, T); crc = crc_word(crc, v1.y, T); crc = crc_word(crc, v1.z, T); crc = crc_word(crc, v1.w, T);
     crc = crc_word(crc, v2.x, T); crc = crc_word(crc, v2.y, T); crc = crc_word(crc, v2.z, T); crc = crc_word(crc, v2.w, T);
     crc = crc_word(crc, v3.x, T); crc = crc_word(crc, v3.y, T); crc = crc_word(crc, v3.z, T); crc = crc_word(crc, v3.w, T);
   }
-  for (; o + 4 <= end; o += 4) crc = crc_word(crc, *reinterpret_cast<const uint32_t *>(p + o), T);
+  for (; o + 4 <= end; o += 4) crc = crc_word(crc, *reinterpret_cast<const P2H_GLOBAL uint32_t *>(p + o), T);
   for (; o < end; o++) crc = T[0][(crc ^ p[o]) & 0xFF] ^ (crc >> 8);
   return crc;
 }
@@ -336,9 +349,10 @@ __device__ __forceinline__ uint32_t crc_zeros(uint32_t crc, uint64_t n, const ui
 
 // CRC-32 of one slice by the whole 256-lane workgroup (tables T already in
 // LDS): bytes [0, len) of p, zeros up to slice_size; result to *out.
-__device__ __forceinline__ void crc_slice_wg(const uint32_t (*T)[256], uint32_t *part, const uint8_t *__restrict__ p,
-                                             uint64_t len, uint64_t slice_size, const CrcParams &prm,
-                                             uint32_t *__restrict__ out) {
+__device__ __forceinline__ void crc_slice_wg(const uint32_t (*T)[256], uint32_t *part,
+                                             const P2H_GLOBAL uint8_t *__restrict__ p, uint64_t len,
+                                             uint64_t slice_size, const CrcParams &prm,
+                                             P2H_GLOBAL uint32_t *__restrict__ out) {
   const int lane = threadIdx.x;
   // this lane's bytes in slice coordinates: [a, e) real or zero-padding
   const int64_t va = (int64_t)lane * (int64_t)prm.chunk - (int64_t)prm.pad;
@@ -385,7 +399,7 @@ __global__ __launch_bounds__(kCrcLanes) void k_crc32_slices(const uint8_t *__res
   const long i = blockIdx.x;
   uint64_t len = lens ? lens[i] : slice_size;
   if (len > slice_size) len = slice_size;
-  crc_slice_wg(T, part, data + (uint64_t)i * stride, len, slice_size, prm, out + i);
+  crc_slice_wg(T, part, gptr(data) + (uint64_t)i * stride, len, slice_size, prm, gptr(out) + i);
 }
 
 // Batched form: one workgroup per entry; entry e uses params[e.crc_params].
@@ -397,7 +411,7 @@ __global__ __launch_bounds__(kCrcLanes) void k_crc32_list(const HashEntry *__res
   crc_load_tables(T, tab);
   const HashEntry e = list[blockIdx.x];
   const CrcParams prm = params[e.crc_params];
-  crc_slice_wg(T, part, e.data, e.len, e.size, prm, e.crc);
+  crc_slice_wg(T, part, gptr(e.data), e.len, e.size, prm, gptr(e.crc));
 }
 
 int32_t crc_tables_init(par2hip_ctx *ctx) {
@@ -468,9 +482,9 @@ __global__ __launch_bounds__(256) void k_crc32_list_lane(const HashEntry *__rest
   if (i >= n) return;
   const HashEntry e = list[i];
   const uint64_t len = e.len < e.size ? e.len : e.size;
-  uint32_t crc = crc_run(0xFFFFFFFFu, e.data, 0, len, T);
+  uint32_t crc = crc_run(0xFFFFFFFFu, gptr(e.data), 0, len, T);
   crc = crc_zeros(crc, e.size - len, T);
-  *e.crc = ~crc;
+  *gptr(e.crc) = ~crc;
 }
 
 int32_t launch_crc32_list(par2hip_ctx *ctx, hipStream_t s, const HashEntry *d_list, size_t n,
