@@ -352,12 +352,191 @@ def bench_batched(args, ctx):
     }
 
 
+def draw_requests(seed, count, sigma=1.0):
+    """Served workload: request sizes lognormal (median 64 MiB, sigma given),
+    capped at 8 GiB; slice size uniform over 256 KiB..1 MiB in 4-byte steps;
+    30% repairs with ceil(U[1,10]% * N) erasures (recovery rows 0..m-1),
+    creates with redundancy U[1,20]% (R by the creator's rule,
+    creator.rs:324-331).  Arrival gaps are Exp(1), scaled by 1/rate later."""
+    rng = np.random.default_rng(seed)
+    reqs, t = [], 0.0
+    for _ in range(count):
+        t += float(rng.exponential(1.0))
+        size = min(float(np.exp(np.log(64 * 2**20) + sigma * rng.standard_normal())), 8.0 * 2**30)
+        s = int(rng.integers(65536, 262145)) * 4
+        n = int(min(32768, max(1, -(-int(size) // s))))
+        if rng.random() < 0.3:
+            m = max(1, int(np.ceil(rng.uniform(0.01, 0.10) * n)))
+            reqs.append({"t": t, "kind": "repair", "n": n, "s": s, "k": m})
+        else:
+            r = max(1, -(-int(n * s * rng.uniform(1, 20) / 100.0) // s))
+            reqs.append({"t": t, "kind": "encode", "n": n, "s": s, "k": r})
+    return reqs
+
+
+def bench_served(args, ctx):
+    """Served workload (Poisson arrivals, continuous batching through the
+    serving loop) against the plain per-request path (one request at a time,
+    single-set entry points) on the same request stream."""
+    import threading
+
+    rng = np.random.default_rng(args.seed + 100)
+    template = draw_requests(args.seed, 3000)
+    pool_bytes = 16 << 30
+    d_pool = ctx.alloc(pool_bytes)
+    block = random_bytes(21, 1 << 30)
+    for o in range(0, pool_bytes, len(block)):
+        d_pool.upload(block, offset=o)
+    del block
+    ctx.sync()
+    # every request reads its slices (and, for repairs, its recovery rows) as
+    # windows of the device pool; erasure positions are drawn per request
+    for q in template:
+        nb = q["n"] * q["s"]
+        q["off"] = int(rng.integers(0, (pool_bytes - nb) // 256)) * 256
+        q["exps"] = np.arange(q["k"], dtype=np.uint32)
+        if q["kind"] == "repair":
+            q["rec_off"] = int(rng.integers(0, (pool_bytes - q["k"] * q["s"]) // 256)) * 256
+            q["missing"] = np.sort(rng.choice(q["n"], q["k"], replace=False)).astype(np.uint32)
+    max_out = max(q["k"] * q["s"] for q in template)
+    max_n = max(q["n"] for q in template)
+    d_out, d_md5, d_crc = ctx.alloc(max_out), ctx.alloc(max_n * 16), ctx.alloc(max_n * 4)
+
+    def view(off):
+        return d_pool.view(off)
+
+    def job_of(q):
+        j = {"kind": q["kind"], "slices": view(q["off"]), "n": q["n"], "slice_size": q["s"], "exps": q["exps"]}
+        if q["kind"] == "repair":
+            j.update(recovery=view(q["rec_off"]), missing=q["missing"])
+        return j
+
+    def plain_one(q):
+        n, s = q["n"], q["s"]
+        if q["kind"] == "encode":
+            ctx.encode_dev(view(q["off"]), n, s, q["exps"], d_out, d_md5, d_crc)
+        else:
+            ctx.hash_slices_dev(view(q["off"]), n, s, d_md5, d_crc)
+            ctx.reconstruct_dev(view(q["off"]), n, s, q["missing"], view(q["rec_off"]), q["exps"], d_out)
+        ctx.sync()
+
+    def run_plain(reqs, rate):
+        """Real-time arrivals at `rate`/s, served one at a time in order."""
+        lat, t0 = [], time.perf_counter()
+        for q in reqs:
+            arr = t0 + q["t"] / rate
+            now = time.perf_counter()
+            if now < arr:
+                time.sleep(arr - now)
+            plain_one(q)
+            lat.append(time.perf_counter() - arr)
+        span = time.perf_counter() - t0
+        return np.array(lat) * 1e3, span
+
+    srv = par2hip.Server(0, hbm_budget=args.hbm_budget, lanes=args.lanes, max_batch=args.max_batch)
+
+    def run_served(reqs, rate):
+        tickets = [None] * len(reqs)
+        base = srv.now_ms()
+        t0 = time.perf_counter()
+
+        def submitter():
+            for i, q in enumerate(reqs):
+                arr = t0 + q["t"] / rate
+                now = time.perf_counter()
+                if now < arr:
+                    time.sleep(arr - now)
+                tickets[i] = srv.submit(job_of(q), hashes=True, arrival_ms=base + q["t"] / rate * 1e3)
+
+        th = threading.Thread(target=submitter)
+        th.start()
+        lat, sizes, last = [], [], 0.0
+        for i in range(len(reqs)):
+            while tickets[i] is None:
+                time.sleep(0.0002)
+            res = srv.wait(tickets[i])
+            lat.append(res["t_done_ms"] - res["t_arrival_ms"])
+            sizes.append(res["batch_size"])
+            last = max(last, res["t_done_ms"])
+            srv.release(tickets[i])
+        th.join()
+        span = (last - base) / 1e3
+        return np.array(lat), span, float(np.mean(sizes))
+
+    # bit-exactness: served outputs equal the plain path's for a sample
+    sample = template[:24]
+    exact = True
+    for q in sample:
+        plain_one(q)
+        rows = q["k"] * q["s"]
+        want = (d_out.download(rows), d_md5.download(q["n"] * 16), d_crc.download(q["n"] * 4))
+        t = srv.submit(job_of(q), hashes=True)
+        res = srv.wait(t)
+        got = (par2hip.d2h(res["d_out"], rows, ctx), par2hip.d2h(res["d_md5"], q["n"] * 16, ctx),
+               par2hip.d2h(res["d_crc"], q["n"] * 4, ctx))
+        exact &= all(np.array_equal(a, b) for a, b in zip(want, got))
+        srv.release(t)
+
+    # plain capacity: requests back to back
+    cap_reqs = template[:150]
+    t0 = time.perf_counter()
+    for q in cap_reqs:
+        plain_one(q)
+    t_service = (time.perf_counter() - t0) / len(cap_reqs)
+    rate_p = 0.7 / t_service
+    n_p = args.served_requests
+    lat_p, span_p = run_plain(template[:n_p], rate_p)
+    bytes_p = sum(q["n"] * q["s"] for q in template[:n_p])
+    p99_p = float(np.percentile(lat_p, 99))
+    log(f"plain: service {t_service * 1e3:.2f} ms/request, rate {rate_p:.1f}/s, p50 "
+        f"{np.percentile(lat_p, 50):.1f} ms p99 {p99_p:.1f} ms, {bytes_p / span_p / 1e9:.2f} GB/s")
+    runs = []
+    for k in args.served_loads:
+        rate = k * rate_p
+        n_s = min(len(template), int(args.served_requests * max(1.0, k / 2)))
+        lat_s, span_s, mean_batch = run_served(template[:n_s], rate)
+        bytes_s = sum(q["n"] * q["s"] for q in template[:n_s])
+        r = {"load_x_plain": k, "rate_per_s": round(rate, 1), "requests": n_s,
+             "p50_ms": round(float(np.percentile(lat_s, 50)), 2), "p99_ms": round(float(np.percentile(lat_s, 99)), 2),
+             "GBps": round(bytes_s / span_s / 1e9, 3), "mean_batch": round(mean_batch, 2)}
+        log("served", r)
+        runs.append(r)
+        if r["p99_ms"] > p99_p:
+            break
+    ok = [r for r in runs if r["p99_ms"] <= p99_p]
+    best = max(ok, key=lambda r: r["GBps"]) if ok else runs[0]
+    st = srv.stats()
+    srv.close()
+    mean_req = float(np.mean([q["n"] * q["s"] for q in template]))
+    return {
+        "workload": "served",
+        "config": {"workload": "served_poisson", "request_size": "lognormal median 64 MiB sigma 1.0 cap 8 GiB",
+                   "mean_request_MiB": round(mean_req / 2**20, 1), "slice_bytes": "U{256 KiB..1 MiB} step 4 B",
+                   "repairs": "30%, erasures ceil(U[1,10]% N)", "creates": "redundancy U[1,20]%",
+                   "hbm_budget_GB": round(args.hbm_budget / 1e9, 1), "lanes": args.lanes,
+                   "max_batch": args.max_batch, "seed": args.seed, "hashes": "md5+crc32 of every input slice",
+                   "resident": "hbm (device input pool)"},
+        "value": best["GBps"], "p50_ms": best["p50_ms"], "p99_ms": best["p99_ms"], "exact": bool(exact),
+        "served": runs, "server_stats": st,
+        "plain_path": {"rate_per_s": round(rate_p, 1), "service_ms": round(t_service * 1e3, 3),
+                       "p50_ms": round(float(np.percentile(lat_p, 50)), 2), "p99_ms": round(p99_p, 2),
+                       "GBps": round(bytes_p / span_p / 1e9, 3),
+                       "speedup_at_equal_p99": round(best["GBps"] / (bytes_p / span_p / 1e9), 3)},
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--workload", choices=["encode", "repair", "micro", "batched"], default="encode")
+    ap.add_argument("--workload", choices=["encode", "repair", "micro", "batched", "served"], default="encode")
+    ap.add_argument("--hbm-budget", type=int, default=200 * 10**9, help="served: server HBM budget in bytes")
+    ap.add_argument("--lanes", type=int, default=2, help="served: concurrent batches")
+    ap.add_argument("--max-batch", type=int, default=512, help="served: requests per batch at most")
+    ap.add_argument("--served-requests", type=int, default=400, help="served: requests per plain run")
+    ap.add_argument("--served-loads", type=float, nargs="+", default=[1, 2, 3, 4, 5, 6],
+                    help="served: offered loads as multiples of the plain run's")
     ap.add_argument("--seed", type=int, default=4)
     ap.add_argument("--kernel", type=int, default=0, help="0 auto, 1 LDS tables, 2 MFMA")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
@@ -375,6 +554,19 @@ def main():
     log("workload", args.workload, "steps", args.steps, "warmup", args.warmup, "device", ctx.device_info())
     if args.workload == "micro":
         print(json.dumps({"micro": bench_micro(args, ctx)}), flush=True)
+        ctx.close()
+        return
+    if args.workload == "served":
+        res = bench_served(args, ctx)
+        out = {
+            "metric": METRIC, "value": res["value"], "unit": "GB/s", "n_gpus": 1, "steps": len(res["served"]),
+            "warmup": 0, "ms_per_step": None, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+            "dtype": "u16", "data": "synthetic (seeded uniform random bytes)", "config": res["config"],
+            "p50_ms": res["p50_ms"], "p99_ms": res["p99_ms"], "bit_exact_sample": res["exact"],
+            "device": ctx.device_info(), "served": res["served"], "server_stats": res["server_stats"],
+            "plain_path": res["plain_path"],
+        }
+        print(json.dumps(out), flush=True)
         ctx.close()
         return
     if args.workload == "batched":

@@ -201,6 +201,54 @@ typedef struct {
  * non-consecutive exponents take the Gauss-Jordan path after the batch. */
 int32_t par2hip_batch_run_dev(par2hip_ctx *ctx, const par2hip_job *jobs, size_t njobs);
 
+/* ------------------------------------------------------------- serving -- */
+
+/* A request loop with continuous batching (no counterpart in the reference,
+ * which runs one Par2Creator::create / Par2Repairer::repair per call:
+ * src/creator.rs:261-531, src/repair.rs:736-1125).  Requests are admitted in
+ * arrival order while their HBM footprint (input slices + outputs + working
+ * memory) fits hbm_budget; `lanes` worker threads, each with its own context,
+ * each take everything admitted so far (at most max_batch requests) as one
+ * batch, so requests that arrive while batches run join the next one and the
+ * batches of different lanes overlap on the device.  Input slices are
+ * device-resident and caller-owned (they must stay valid until the request is
+ * done); the server allocates the outputs from a pool of hbm_budget bytes and
+ * keeps them until par2hip_server_release. */
+typedef struct par2hip_server par2hip_server;
+typedef struct {
+    par2hip_job job;       /* inputs; d_recovery (encode), d_out, d_md5 and d_crc
+                              are ignored and allocated by the server */
+    int32_t hashes;        /* 1: also MD5/CRC32 of every input slice */
+    int32_t reserved;
+    double arrival_ms;     /* server clock (par2hip_server_now_ms); 0 = submit time */
+} par2hip_request;
+typedef struct {
+    int32_t status;        /* PAR2HIP_OK or the error of this request alone */
+    int32_t batch_size;    /* requests in the batch that ran this one */
+    void *d_out;           /* encode: count recovery slices; repair: count rebuilt slices */
+    void *d_md5;           /* n * 16 bytes, or NULL */
+    void *d_crc;           /* n u32, or NULL */
+    double t_arrival_ms, t_submit_ms, t_admit_ms, t_start_ms, t_done_ms;
+} par2hip_result;
+typedef struct {
+    uint64_t submitted, completed, failed, batches, max_batch_seen;
+    uint64_t charged, peak_charged;  /* bytes of the budget held now / at most */
+    uint64_t pending, admitted;      /* requests waiting for memory / for a lane */
+    uint64_t arena_fallbacks;        /* batches whose working memory did not come from the pool */
+} par2hip_server_stats;
+int32_t par2hip_server_create(int32_t device, uint64_t hbm_budget, int32_t lanes, int32_t max_batch,
+                              par2hip_server **out);
+/* Finishes the admitted requests, drops the never-admitted ones, frees all. */
+int32_t par2hip_server_destroy(par2hip_server *srv);
+double par2hip_server_now_ms(par2hip_server *srv);
+/* Non-blocking.  PAR2HIP_EINVAL for a malformed job, PAR2HIP_ENOMEM for one
+ * whose footprint exceeds the whole budget. */
+int32_t par2hip_server_submit(par2hip_server *srv, const par2hip_request *req, uint64_t *ticket);
+/* Blocks until the request is done; its outputs stay valid until release. */
+int32_t par2hip_server_wait(par2hip_server *srv, uint64_t ticket, par2hip_result *res);
+int32_t par2hip_server_release(par2hip_server *srv, uint64_t ticket);
+int32_t par2hip_server_get_stats(par2hip_server *srv, par2hip_server_stats *st);
+
 /* -------------------------------------------------------------- hashes -- */
 
 /* IFSC MD5 + CRC32 of n slices, each zero-padded to slice_size

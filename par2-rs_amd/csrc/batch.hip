@@ -43,7 +43,20 @@ int32_t reconstruct_single(par2hip_ctx *ctx, uint8_t *d_slices, size_t n, size_t
 int32_t fork_overlap(par2hip_ctx *ctx);
 int32_t join_overlap(par2hip_ctx *ctx);
 
-int32_t batch_run(par2hip_ctx *ctx, const par2hip_job *jobs, size_t njobs) {
+size_t batch_job_arena_bytes(const par2hip_job &J) {
+  auto r = [](size_t b) { return (b + 255) & ~(size_t)255; };
+  const size_t n = J.n, c = J.count;
+  if (c == 0) return 0;
+  if (J.kind == PAR2HIP_JOB_ENCODE) return r(c * n * 2) + r((size_t)gf_tab_entries((int)c, (int)n) * 16);
+  if (!J.exps || !consecutive_exps(J.exps, c)) return 0;  // single-set path after the batch
+  const size_t m = c, np = n - m;
+  const size_t vs = (std::max<size_t>(np, 1) + 7) & ~(size_t)7, cs = (np + m + 7) & ~(size_t)7;
+  size_t t = r(m * 2 * m * 2) + r(m * vs * 2) + r(m * cs * 2);
+  if (np) t += r((size_t)gf_tab_entries((int)m, (int)m) * 16);
+  return t + r((size_t)gf_tab_entries((int)m, (int)n) * 16);
+}
+
+int32_t batch_run(par2hip_ctx *ctx, const par2hip_job *jobs, size_t njobs, uint8_t *ext_arena, size_t ext_bytes) {
   // ---- validate and classify
   std::vector<size_t> enc, rep, rep_slow;
   for (size_t j = 0; j < njobs; j++) {
@@ -121,9 +134,13 @@ int32_t batch_run(par2hip_ctx *ctx, const par2hip_job *jobs, size_t njobs) {
     main_tasks.push_back(tp);
   }
   const size_t cursor_off = ar.take(512);
-  void *arena_v = nullptr;
+  void *arena_v = ext_arena;
   int32_t rc;
-  if ((rc = scratch(ctx, kScratchBatch, std::max<size_t>(ar.used, 4096), &arena_v))) return rc;
+  if (ext_arena) {
+    if (ar.used > ext_bytes) return fail(ctx, PAR2HIP_EINVAL, "batch working memory larger than the arena given");
+  } else if ((rc = scratch(ctx, kScratchBatch, std::max<size_t>(ar.used, 4096), &arena_v))) {
+    return rc;
+  }
   uint8_t *arena = (uint8_t *)arena_v;
 
   // ---- work items (sorted by cost: sources per task, descending)
@@ -378,5 +395,5 @@ extern "C" int32_t par2hip_batch_run_dev(par2hip_ctx *ctx, const par2hip_job *jo
   if (njobs == 0) return PAR2HIP_OK;
   std::lock_guard<std::recursive_mutex> lk(ctx->mu);
   (void)hipSetDevice(ctx->device);
-  return par2hip::batch_run(ctx, jobs, njobs);
+  return par2hip::batch_run(ctx, jobs, njobs, nullptr, 0);
 }

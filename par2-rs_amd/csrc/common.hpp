@@ -185,6 +185,12 @@ struct GfItem {
   int task, tile, rowblk, pad_;
 };
 constexpr int kBatchWordsPerWg = 2048;  // 256 threads x 8 words
+// Many jobs as one batch.  arena: device working memory for the batch (at
+// least the sum of batch_job_arena_bytes over the jobs plus
+// kBatchArenaExtra), or nullptr for the context's own scratch.
+int32_t batch_run(par2hip_ctx *ctx, const par2hip_job *jobs, size_t njobs, uint8_t *arena, size_t arena_bytes);
+size_t batch_job_arena_bytes(const par2hip_job &job);
+constexpr size_t kBatchArenaExtra = 512;
 // Nibble tables of every task (grid: elements x tasks), then the persistent
 // kernel works through `items` (sorted by cost) with an atomic cursor.
 int32_t launch_tables_batch(par2hip_ctx *ctx, hipStream_t s, const GfTaskDev *d_tasks, int ntasks,

@@ -64,6 +64,23 @@ class Job(C.Structure):
                 ("missing", C.c_void_p), ("d_out", C.c_void_p), ("d_md5", C.c_void_p), ("d_crc", C.c_void_p)]
 
 
+class Request(C.Structure):
+    """par2hip_request: one request to the serving loop."""
+    _fields_ = [("job", Job), ("hashes", C.c_int32), ("reserved", C.c_int32), ("arrival_ms", C.c_double)]
+
+
+class Result(C.Structure):
+    """par2hip_result: a finished request (outputs stay until release)."""
+    _fields_ = [("status", C.c_int32), ("batch_size", C.c_int32), ("d_out", C.c_void_p), ("d_md5", C.c_void_p),
+                ("d_crc", C.c_void_p), ("t_arrival_ms", C.c_double), ("t_submit_ms", C.c_double),
+                ("t_admit_ms", C.c_double), ("t_start_ms", C.c_double), ("t_done_ms", C.c_double)]
+
+
+class ServerStats(C.Structure):
+    _fields_ = [(k, C.c_uint64) for k in ("submitted", "completed", "failed", "batches", "max_batch_seen",
+                                           "charged", "peak_charged", "pending", "admitted", "arena_fallbacks")]
+
+
 class DeviceInfo(C.Structure):
     _fields_ = [("compute_units", C.c_int32), ("hash_cus", C.c_int32), ("arch", C.c_char * 32)]
 
@@ -117,6 +134,13 @@ def lib():
                 "par2hip_set_gf_kernel": ([P, I32], I32),
                 "par2hip_profile_enable": ([P, I32], I32),
                 "par2hip_profile_read": ([P, P, S, C.POINTER(S)], I32),
+                "par2hip_server_create": ([I32, C.c_uint64, I32, I32, C.POINTER(P)], I32),
+                "par2hip_server_destroy": ([P], I32),
+                "par2hip_server_now_ms": ([P], C.c_double),
+                "par2hip_server_submit": ([P, P, C.POINTER(C.c_uint64)], I32),
+                "par2hip_server_wait": ([P, C.c_uint64, P], I32),
+                "par2hip_server_release": ([P, C.c_uint64], I32),
+                "par2hip_server_get_stats": ([P, P], I32),
             }
             for name, (args, res) in sig.items():
                 f = getattr(L, name)
@@ -370,6 +394,90 @@ class Context:
         self._check(lib().par2hip_profile_read(self.handle, arr, 64, C.byref(cnt)))
         return {arr[i].name.decode(): (int(arr[i].launches), float(arr[i].total_ms))
                 for i in range(min(cnt.value, 64))}
+
+
+def _status(rc: int, what: str) -> None:
+    if rc == OK:
+        return
+    if rc == ESINGULAR:
+        raise RepairFailed(what)
+    if rc == ERECOVERY:
+        raise InsufficientRecovery(what)
+    if rc == EINVAL:
+        raise InvalidFormat(what)
+    raise DeviceError(f"{rc}: {what}")
+
+
+class Server:
+    """The serving loop (continuous batching, HBM-budget admission).
+
+    submit() takes a request dict like Context.batch_run_dev's jobs (slices,
+    n, slice_size, exps, and for repairs recovery + missing) without output
+    buffers; the server allocates them.  wait() returns a dict with the
+    device outputs (valid until release()) and the server-clock timestamps."""
+
+    def __init__(self, device: int = 0, hbm_budget: int = 1 << 30, lanes: int = 2, max_batch: int = 256):
+        h = C.c_void_p()
+        _status(lib().par2hip_server_create(device, int(hbm_budget), lanes, max_batch, C.byref(h)), "server_create")
+        self.handle = h
+        self._keep: dict[int, tuple] = {}
+
+    def close(self) -> None:
+        if self.handle is not None:
+            lib().par2hip_server_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def now_ms(self) -> float:
+        return float(lib().par2hip_server_now_ms(self.handle))
+
+    def submit(self, job: dict, hashes: bool = True, arrival_ms: float = 0.0) -> int:
+        e = np.ascontiguousarray(job.get("exps", []), np.uint32)
+        miss = np.ascontiguousarray(job.get("missing", []), np.uint32)
+        r = Request()
+        a = r.job
+        a.kind = JOB_REPAIR if job["kind"] == "repair" else JOB_ENCODE
+        a.d_slices = job["slices"].ptr
+        a.n = job["n"]
+        a.slice_size = job["slice_size"]
+        a.exps = e.ctypes.data if len(e) else None
+        a.count = len(miss) if job["kind"] == "repair" else len(e)
+        a.d_recovery = job["recovery"].ptr if job["kind"] == "repair" and job.get("recovery") is not None else None
+        a.missing = miss.ctypes.data if len(miss) else None
+        r.hashes = 1 if hashes else 0
+        r.arrival_ms = arrival_ms
+        t = C.c_uint64(0)
+        _status(lib().par2hip_server_submit(self.handle, C.byref(r), C.byref(t)), "server_submit")
+        self._keep[t.value] = (job,)  # the input buffers must outlive the request
+        return t.value
+
+    def wait(self, ticket: int) -> dict:
+        res = Result()
+        _status(lib().par2hip_server_wait(self.handle, ticket, C.byref(res)), "server_wait")
+        out = {k: getattr(res, k) for k, _ in Result._fields_}
+        _status(res.status, f"request {ticket}")
+        return out
+
+    def release(self, ticket: int) -> None:
+        _status(lib().par2hip_server_release(self.handle, ticket), "server_release")
+        self._keep.pop(ticket, None)
+
+    def stats(self) -> dict:
+        st = ServerStats()
+        _status(lib().par2hip_server_get_stats(self.handle, C.byref(st)), "server_stats")
+        return {k: int(getattr(st, k)) for k, _ in ServerStats._fields_}
+
+
+def d2h(ptr: int, nbytes: int, ctx: "Context") -> np.ndarray:
+    """Copy nbytes from a device address (e.g. a server output) to the host."""
+    out = np.empty(nbytes, np.uint8)
+    ctx._check(lib().par2hip_d2h(ctx.handle, _p(out), C.c_void_p(ptr), nbytes))
+    return out
 
 
 class HipStrategy:

@@ -47,4 +47,8 @@ def test_no_device_is_reported_not_raised():
     h = C.c_void_p()
     assert lib.par2hip_ctx_create(0, C.byref(h)) == par2hip.ENODEV
     assert lib.par2hip_encode(None, None, 0, 0, None, 0, None, None, None) == par2hip.EINVAL
+    srv = C.c_void_p()
+    assert lib.par2hip_server_create(0, 1 << 30, 2, 64, C.byref(srv)) == par2hip.ENODEV
+    assert lib.par2hip_server_create(0, 0, 2, 64, C.byref(srv)) == par2hip.EINVAL
+    assert lib.par2hip_server_create(0, 1 << 30, 0, 64, C.byref(srv)) == par2hip.EINVAL
     assert not par2hip.HipStrategy().is_available()
