@@ -201,6 +201,19 @@ typedef struct {
  * non-consecutive exponents take the Gauss-Jordan path after the batch. */
 int32_t par2hip_batch_run_dev(par2hip_ctx *ctx, const par2hip_job *jobs, size_t njobs);
 
+/* A planned batch (no counterpart in the reference): the host planning,
+ * parameter upload and working memory of par2hip_batch_run_dev done once, so
+ * the same jobs (same device buffers; their contents may change) can be run
+ * many times.  With use_graph the launches are captured into one HIP graph
+ * and each launch replays it.  Every repair job needs consecutive recovery
+ * exponents (PAR2HIP_EINVAL otherwise).  Launches are asynchronous on the ctx
+ * stream; destroy waits for them. */
+typedef struct par2hip_batch_plan par2hip_batch_plan;
+int32_t par2hip_batch_plan_create(par2hip_ctx *ctx, const par2hip_job *jobs, size_t njobs,
+                                  int32_t use_graph, par2hip_batch_plan **out);
+int32_t par2hip_batch_plan_launch(par2hip_batch_plan *plan);
+int32_t par2hip_batch_plan_destroy(par2hip_batch_plan *plan);
+
 /* ------------------------------------------------------------- serving -- */
 
 /* A request loop with continuous batching (no counterpart in the reference,

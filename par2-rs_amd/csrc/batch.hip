@@ -56,7 +56,27 @@ size_t batch_job_arena_bytes(const par2hip_job &J) {
   return t + r((size_t)gf_tab_entries((int)m, (int)n) * 16);
 }
 
-int32_t batch_run(par2hip_ctx *ctx, const par2hip_job *jobs, size_t njobs, uint8_t *ext_arena, size_t ext_bytes) {
+// Everything a batch's launches need, with the parameter block already
+// uploaded: built once, launched once (batch_run) or many times (a plan, whose
+// parameters and working memory it owns, optionally replayed as a graph).
+struct BatchPlan {
+  uint8_t *own_params = nullptr, *own_arena = nullptr;
+  const GfTaskDev *d_ft = nullptr, *d_mt = nullptr;
+  const GfItem *d_fi = nullptr, *d_mi = nullptr;
+  const CoeffJob *d_cj = nullptr;
+  const InvJob *d_ij = nullptr;
+  const HashEntry *d_md5l = nullptr, *d_crcl = nullptr, *d_crcs = nullptr;
+  const CrcParams *d_cp = nullptr;
+  size_t n_ft = 0, n_mt = 0, n_fi = 0, n_mi = 0, n_cj = 0, n_ij = 0, n_md5 = 0, n_crcl = 0, n_crcs = 0;
+  long max_coeff = 0, max_ft = 0, max_mt = 0;
+  int max_m = 0;
+  bool aligned4 = true;
+  unsigned *cursors = nullptr;
+  std::vector<size_t> slow;  // repair jobs for the single-set path (gapped exponents)
+};
+
+static int32_t plan_build(par2hip_ctx *ctx, const par2hip_job *jobs, size_t njobs, uint8_t *ext_arena,
+                          size_t ext_bytes, bool own, BatchPlan &P) {
   // ---- validate and classify
   std::vector<size_t> enc, rep, rep_slow;
   for (size_t j = 0; j < njobs; j++) {
@@ -136,12 +156,16 @@ int32_t batch_run(par2hip_ctx *ctx, const par2hip_job *jobs, size_t njobs, uint8
   const size_t cursor_off = ar.take(512);
   void *arena_v = ext_arena;
   int32_t rc;
-  if (ext_arena) {
+  if (own) {
+    P2H_CHECK(ctx, hipMalloc(&arena_v, std::max<size_t>(ar.used, 4096)));
+    P.own_arena = (uint8_t *)arena_v;
+  } else if (ext_arena) {
     if (ar.used > ext_bytes) return fail(ctx, PAR2HIP_EINVAL, "batch working memory larger than the arena given");
   } else if ((rc = scratch(ctx, kScratchBatch, std::max<size_t>(ar.used, 4096), &arena_v))) {
     return rc;
   }
   uint8_t *arena = (uint8_t *)arena_v;
+  P.slow = rep_slow;
 
   // ---- work items (sorted by cost: sources per task, descending)
   auto make_items = [&](std::vector<TaskPlan> &tasks, std::vector<GfItem> &items, const std::vector<long> &words) {
@@ -219,7 +243,8 @@ int32_t batch_run(par2hip_ctx *ctx, const par2hip_job *jobs, size_t njobs, uint8
                         (md5_list.size() + crc_list.size() + crc_small.size()) * sizeof(HashEntry) +
                         crc_params.size() * sizeof(CrcParams) +
                         64 * 256;
-  if ((rc = params_begin(ctx, pbytes))) return rc;
+  if (own) P2H_CHECK(ctx, hipMalloc((void **)&P.own_params, pbytes));
+  if ((rc = params_begin(ctx, pbytes, P.own_params))) return rc;
   const std::vector<uint32_t> logs_all = constant_logs(32768);
   std::vector<CoeffJob> cjobs;
   long max_coeff = 0;
@@ -343,43 +368,80 @@ int32_t batch_run(par2hip_ctx *ctx, const par2hip_job *jobs, size_t njobs, uint8
   const HashEntry *d_crcs = (const HashEntry *)put(crc_small.data(), crc_small.size() * sizeof(HashEntry));
   const CrcParams *d_cp = (const CrcParams *)put(crc_params.data(), crc_params.size() * sizeof(CrcParams));
   if ((rc = params_commit(ctx))) return rc;
+  P.d_ft = d_ft;
+  P.d_mt = d_mt;
+  P.d_fi = d_fi;
+  P.d_mi = d_mi;
+  P.d_cj = d_cj;
+  P.d_ij = d_ij;
+  P.d_md5l = d_md5l;
+  P.d_crcl = d_crcl;
+  P.d_crcs = d_crcs;
+  P.d_cp = d_cp;
+  P.n_ft = ft.size();
+  P.n_mt = mt.size();
+  P.n_fi = fold_items.size();
+  P.n_mi = main_items.size();
+  P.n_cj = cjobs.size();
+  P.n_ij = ijobs.size();
+  P.n_md5 = md5_list.size();
+  P.n_crcl = crc_list.size();
+  P.n_crcs = crc_small.size();
+  P.max_coeff = max_coeff;
+  P.max_ft = max_ft;
+  P.max_mt = max_mt;
+  P.max_m = max_m;
+  P.aligned4 = aligned4;
+  P.cursors = (unsigned *)(arena + cursor_off);
+  return PAR2HIP_OK;
+}
 
-  // ---- launches.  MD5 chains overlap everything else: on the reserved side
-  // CUs when they fit there (one wave per SIMD), otherwise -- a throughput
-  // load of many chains -- on an unmasked stream over the whole device at
-  // normal issue priority (boosted, they would starve the CRC and contraction
-  // waves that share their SIMDs); the longest chains are listed first.
-  const size_t md5_waves = (md5_list.size() + 63) / 64;
+// The launches of a planned batch, queued on the ctx streams (the ctx stream
+// is the one to wait on; everything forked from it is joined back).  MD5
+// chains overlap everything else: on the reserved side CUs when they fit
+// there (one wave per SIMD), otherwise -- a throughput load of many chains --
+// on an unmasked stream over the whole device at normal issue priority
+// (boosted, they would starve the CRC and contraction waves that share their
+// SIMDs); the longest chains are listed first.
+static int32_t plan_launch(par2hip_ctx *ctx, const BatchPlan &P) {
+  int32_t rc;
+  const size_t md5_waves = (P.n_md5 + 63) / 64;
   const bool side_fits = ctx->masked && md5_waves <= (size_t)(4 * ctx->hash_cus);
-  const bool overlap = !md5_list.empty() && side_fits;
+  const bool overlap = P.n_md5 && side_fits;
   hipEvent_t wide_done = nullptr;
   if (overlap) {
     if ((rc = fork_overlap(ctx))) return rc;
-    if ((rc = launch_md5_list(ctx, ctx->side, d_md5l, md5_list.size(), aligned4, false))) return rc;
-  } else if (!md5_list.empty()) {
+    if ((rc = launch_md5_list(ctx, ctx->side, P.d_md5l, P.n_md5, P.aligned4, false))) return rc;
+  } else if (P.n_md5) {
     P2H_CHECK(ctx, hipEventRecord(ctx->fork, ctx->stream));
     P2H_CHECK(ctx, hipStreamWaitEvent(ctx->wide, ctx->fork, 0));
-    if ((rc = launch_md5_list(ctx, ctx->wide, d_md5l, md5_list.size(), aligned4, false))) return rc;
+    if ((rc = launch_md5_list(ctx, ctx->wide, P.d_md5l, P.n_md5, P.aligned4, false))) return rc;
     wide_done = ctx->join;
   }
   hipStream_t s = ctx->cur;
-  unsigned *cursors = (unsigned *)(arena + cursor_off);
-  if ((rc = launch_crc32_list(ctx, s, d_crcl, crc_list.size(), d_cp))) return rc;
-  if ((rc = launch_crc32_list_lane(ctx, s, d_crcs, crc_small.size()))) return rc;
-  if ((rc = launch_coeff_batch(ctx, s, d_cj, (int)cjobs.size(), max_coeff))) return rc;
-  if ((rc = launch_inverse_batch(ctx, s, d_ij, (int)ijobs.size(), max_m, 0))) return rc;
-  if ((rc = launch_tables_batch(ctx, s, d_ft, (int)ft.size(), max_ft))) return rc;
-  if ((rc = launch_gf_batch(ctx, s, d_ft, d_fi, (int)fold_items.size(), cursors, "gf_fold"))) return rc;
-  if ((rc = launch_tables_batch(ctx, s, d_mt, (int)mt.size(), max_mt))) return rc;
-  if ((rc = launch_gf_batch(ctx, s, d_mt, d_mi, (int)main_items.size(), cursors + 64, "gf_batch"))) return rc;
+  if ((rc = launch_crc32_list(ctx, s, P.d_crcl, P.n_crcl, P.d_cp))) return rc;
+  if ((rc = launch_crc32_list_lane(ctx, s, P.d_crcs, P.n_crcs))) return rc;
+  if ((rc = launch_coeff_batch(ctx, s, P.d_cj, (int)P.n_cj, P.max_coeff))) return rc;
+  if ((rc = launch_inverse_batch(ctx, s, P.d_ij, (int)P.n_ij, P.max_m, 0))) return rc;
+  if ((rc = launch_tables_batch(ctx, s, P.d_ft, (int)P.n_ft, P.max_ft))) return rc;
+  if ((rc = launch_gf_batch(ctx, s, P.d_ft, P.d_fi, (int)P.n_fi, P.cursors, "gf_fold"))) return rc;
+  if ((rc = launch_tables_batch(ctx, s, P.d_mt, (int)P.n_mt, P.max_mt))) return rc;
+  if ((rc = launch_gf_batch(ctx, s, P.d_mt, P.d_mi, (int)P.n_mi, P.cursors + 64, "gf_batch"))) return rc;
   if (overlap && (rc = join_overlap(ctx))) return rc;
   if (wide_done) {
     P2H_CHECK(ctx, hipEventRecord(wide_done, ctx->wide));
     P2H_CHECK(ctx, hipStreamWaitEvent(ctx->stream, wide_done, 0));
   }
+  return PAR2HIP_OK;
+}
 
+int32_t batch_run(par2hip_ctx *ctx, const par2hip_job *jobs, size_t njobs, uint8_t *ext_arena, size_t ext_bytes) {
+  BatchPlan P;
+  int32_t rc;
+  if ((rc = plan_build(ctx, jobs, njobs, ext_arena, ext_bytes, false, P))) return rc;
+  if ((rc = plan_launch(ctx, P))) return rc;
   // repair jobs whose exponents are not consecutive take the Gauss-Jordan path
-  for (size_t j : rep_slow) {
+  for (size_t j : P.slow) {
     const par2hip_job &J = jobs[j];
     if ((rc = reconstruct_single(ctx, (uint8_t *)J.d_slices, J.n, J.slice_size, J.missing, J.count,
                                  (const uint8_t *)J.d_recovery, J.exps, (uint8_t *)J.d_out)))
@@ -389,6 +451,85 @@ int32_t batch_run(par2hip_ctx *ctx, const par2hip_job *jobs, size_t njobs, uint8
 }
 
 }  // namespace par2hip
+
+struct par2hip_batch_plan {
+  par2hip_ctx *ctx = nullptr;
+  par2hip::BatchPlan plan;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+};
+
+static void plan_free(par2hip_batch_plan *p) {
+  if (p->exec) (void)hipGraphExecDestroy(p->exec);
+  if (p->graph) (void)hipGraphDestroy(p->graph);
+  if (p->plan.own_params) (void)hipFree(p->plan.own_params);
+  if (p->plan.own_arena) (void)hipFree(p->plan.own_arena);
+  delete p;
+}
+
+extern "C" int32_t par2hip_batch_plan_create(par2hip_ctx *ctx, const par2hip_job *jobs, size_t njobs,
+                                             int32_t use_graph, par2hip_batch_plan **out) {
+  if (!ctx || !out || njobs == 0 || !jobs) return PAR2HIP_EINVAL;
+  *out = nullptr;
+  std::lock_guard<std::recursive_mutex> lk(ctx->mu);
+  (void)hipSetDevice(ctx->device);
+  auto *p = new par2hip_batch_plan();
+  p->ctx = ctx;
+  int32_t rc = par2hip::plan_build(ctx, jobs, njobs, nullptr, 0, true, p->plan);
+  if (rc == PAR2HIP_OK && !p->plan.slow.empty())
+    rc = par2hip::fail(ctx, PAR2HIP_EINVAL, "a planned batch needs consecutive recovery exponents in every repair job");
+  if (rc == PAR2HIP_OK && use_graph) {
+    // capture the launches (the parameter upload above is not part of the
+    // graph: replays read the plan's own block); timing events stay out
+    const bool prof = ctx->profile;
+    ctx->profile = false;
+    hipError_t e = hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeRelaxed);
+    if (e == hipSuccess) {
+      rc = par2hip::plan_launch(ctx, p->plan);
+      hipGraph_t g = nullptr;
+      const hipError_t e2 = hipStreamEndCapture(ctx->stream, &g);
+      p->graph = g;
+      if (rc == PAR2HIP_OK && e2 != hipSuccess) rc = par2hip::hip_fail(ctx, e2, "hipStreamEndCapture");
+      if (rc == PAR2HIP_OK) {
+        e = hipGraphInstantiate(&p->exec, p->graph, nullptr, nullptr, 0);
+        if (e != hipSuccess) rc = par2hip::hip_fail(ctx, e, "hipGraphInstantiate");
+      }
+    } else {
+      rc = par2hip::hip_fail(ctx, e, "hipStreamBeginCapture");
+    }
+    ctx->cur = ctx->stream;
+    ctx->profile = prof;
+  }
+  if (rc != PAR2HIP_OK) {
+    (void)hipStreamSynchronize(ctx->stream);
+    plan_free(p);
+    return rc;
+  }
+  *out = p;
+  return PAR2HIP_OK;
+}
+
+extern "C" int32_t par2hip_batch_plan_launch(par2hip_batch_plan *p) {
+  if (!p) return PAR2HIP_EINVAL;
+  par2hip_ctx *ctx = p->ctx;
+  std::lock_guard<std::recursive_mutex> lk(ctx->mu);
+  (void)hipSetDevice(ctx->device);
+  if (p->exec) {
+    P2H_CHECK(ctx, hipGraphLaunch(p->exec, ctx->stream));
+    return PAR2HIP_OK;
+  }
+  return par2hip::plan_launch(ctx, p->plan);
+}
+
+extern "C" int32_t par2hip_batch_plan_destroy(par2hip_batch_plan *p) {
+  if (!p) return PAR2HIP_EINVAL;
+  {
+    std::lock_guard<std::recursive_mutex> lk(p->ctx->mu);
+    (void)par2hip_sync(p->ctx);
+  }
+  plan_free(p);
+  return PAR2HIP_OK;
+}
 
 extern "C" int32_t par2hip_batch_run_dev(par2hip_ctx *ctx, const par2hip_job *jobs, size_t njobs) {
   if (!ctx || (njobs && !jobs)) return PAR2HIP_EINVAL;

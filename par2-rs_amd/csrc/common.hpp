@@ -53,6 +53,7 @@ struct ParamArena {
   size_t cap = 0;
   size_t used = 0;
   hipEvent_t copied = nullptr;  // the last H2D out of `host` has finished
+  uint8_t *target = nullptr;    // device block of the current call (dev unless a plan owns one)
 };
 
 struct DevBuf {
@@ -119,7 +120,9 @@ int32_t scratch(par2hip_ctx *ctx, int role, size_t bytes, void **out);
 // Parameter blocks: begin() waits for the previous upload, alloc() carves
 // 256-byte aligned pieces (returns host and device addresses), commit()
 // uploads everything on the ctx stream.
-int32_t params_begin(par2hip_ctx *ctx, size_t bytes);
+// One parameter block per API call: host staging in `arena.host`, device
+// copy in `target` (a plan-owned block of at least `bytes`) or the ctx's own.
+int32_t params_begin(par2hip_ctx *ctx, size_t bytes, uint8_t *target = nullptr);
 void *params_alloc(par2hip_ctx *ctx, size_t bytes, void **dev);
 int32_t params_commit(par2hip_ctx *ctx);
 

@@ -71,7 +71,7 @@ int32_t scratch(par2hip_ctx *ctx, int role, size_t bytes, void **out) {
   return PAR2HIP_OK;
 }
 
-int32_t params_begin(par2hip_ctx *ctx, size_t bytes) {
+int32_t params_begin(par2hip_ctx *ctx, size_t bytes, uint8_t *target) {
   ParamArena &a = ctx->arena;
   if (a.copied) P2H_CHECK(ctx, hipEventSynchronize(a.copied));
   if (a.cap < bytes) {
@@ -89,6 +89,7 @@ int32_t params_begin(par2hip_ctx *ctx, size_t bytes) {
     a.cap = cap;
   }
   a.used = 0;
+  a.target = target ? target : a.dev;
   return PAR2HIP_OK;
 }
 
@@ -96,14 +97,14 @@ void *params_alloc(par2hip_ctx *ctx, size_t bytes, void **dev) {
   ParamArena &a = ctx->arena;
   size_t off = (a.used + 255) & ~(size_t)255;
   a.used = off + bytes;
-  *dev = a.dev + off;
+  *dev = a.target + off;
   return a.host + off;
 }
 
 int32_t params_commit(par2hip_ctx *ctx) {
   ParamArena &a = ctx->arena;
   if (a.used) {
-    P2H_CHECK(ctx, hipMemcpyAsync(a.dev, a.host, a.used, hipMemcpyHostToDevice, ctx->cur));
+    P2H_CHECK(ctx, hipMemcpyAsync(a.target, a.host, a.used, hipMemcpyHostToDevice, ctx->cur));
     P2H_CHECK(ctx, hipEventRecord(a.copied, ctx->cur));
   }
   return PAR2HIP_OK;

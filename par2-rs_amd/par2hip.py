@@ -134,6 +134,9 @@ def lib():
                 "par2hip_set_gf_kernel": ([P, I32], I32),
                 "par2hip_profile_enable": ([P, I32], I32),
                 "par2hip_profile_read": ([P, P, S, C.POINTER(S)], I32),
+                "par2hip_batch_plan_create": ([P, P, S, I32, C.POINTER(P)], I32),
+                "par2hip_batch_plan_launch": ([P], I32),
+                "par2hip_batch_plan_destroy": ([P], I32),
                 "par2hip_server_create": ([I32, C.c_uint64, I32, I32, C.POINTER(P)], I32),
                 "par2hip_server_destroy": ([P], I32),
                 "par2hip_server_now_ms": ([P], C.c_double),
@@ -337,6 +340,18 @@ class Context:
         """Run many sets at once.  Each job is a dict: kind ('encode'|'repair'),
         slices (DeviceBuffer), n, slice_size, exps, recovery (DeviceBuffer),
         [missing, out (DeviceBuffer) for repair], [md5, crc (DeviceBuffer)]."""
+        arr, keep = self._jobs(jobs)
+        self._check(lib().par2hip_batch_run_dev(self.handle, arr, len(jobs)))
+
+    def batch_plan(self, jobs, graph: bool = True) -> "BatchPlan":
+        """Plan the jobs once (see batch_run_dev) for repeated launches,
+        replayed as one HIP graph when graph is True."""
+        arr, keep = self._jobs(jobs)
+        h = C.c_void_p()
+        self._check(lib().par2hip_batch_plan_create(self.handle, arr, len(jobs), 1 if graph else 0, C.byref(h)))
+        return BatchPlan(self, h, jobs)
+
+    def _jobs(self, jobs):
         arr = (Job * len(jobs))()
         keep = []
         for i, j in enumerate(jobs):
@@ -355,7 +370,7 @@ class Context:
             a.d_out = j["out"].ptr if j.get("out") is not None else None
             a.d_md5 = j["md5"].ptr if j.get("md5") is not None else None
             a.d_crc = j["crc"].ptr if j.get("crc") is not None else None
-        self._check(lib().par2hip_batch_run_dev(self.handle, arr, len(jobs)))
+        return arr, keep
 
     def gf_invert(self, a: np.ndarray) -> np.ndarray:
         a = np.ascontiguousarray(a, np.uint16)
@@ -394,6 +409,27 @@ class Context:
         self._check(lib().par2hip_profile_read(self.handle, arr, 64, C.byref(cnt)))
         return {arr[i].name.decode(): (int(arr[i].launches), float(arr[i].total_ms))
                 for i in range(min(cnt.value, 64))}
+
+
+class BatchPlan:
+    """A planned batch (Context.batch_plan); launch() queues one run."""
+
+    def __init__(self, ctx: "Context", handle, jobs):
+        self.ctx, self.handle, self.jobs = ctx, handle, jobs
+
+    def launch(self) -> None:
+        self.ctx._check(lib().par2hip_batch_plan_launch(self.handle))
+
+    def close(self) -> None:
+        if self.handle is not None:
+            lib().par2hip_batch_plan_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 def _status(rc: int, what: str) -> None:

@@ -131,3 +131,61 @@ def test_batch_repeat_and_empty(ctx):
         bad = dict(jobs[0])
         bad["slice_size"] = 7
         ctx.batch_run_dev([bad])
+
+
+def clear_outputs(jobs):
+    for j in jobs:
+        buf = j["recovery"] if j["kind"] == "encode" else j["out"]
+        buf.upload(np.zeros(buf.nbytes, np.uint8))
+        for k in ("md5", "crc"):
+            if k in j:
+                j[k].upload(np.zeros(j[k].nbytes, np.uint8))
+
+
+@pytest.mark.parametrize("graph", [False, True])
+def test_batch_plan_replays(ctx, graph):
+    """A planned batch launched three times (outputs cleared and unrelated
+    work on the same context between launches) gives the oracle's bytes
+    every time, and follows new input contents in the same buffers."""
+    rng = np.random.default_rng(41 + int(graph))
+    specs = [
+        ("encode", 120, 4096, 12, True, True),
+        ("encode", 9, 5376, 3, False, True),
+        ("repair", 80, 2052, 6, True, True),
+        ("repair", 40, 4100, 4, "shift", True),
+        ("encode", 70, 65540, 5, True, True),
+        ("encode", 16, 8192, 0, True, True),  # hashes only
+    ]
+    jobs, host = make_jobs(ctx, rng, specs)
+    say("plan", "graph" if graph else "eager")
+    plan = ctx.batch_plan(jobs, graph=graph)
+    try:
+        side = rng.integers(0, 256, (20, 2048), dtype=np.uint8)
+        for rep in range(3):
+            clear_outputs(jobs)
+            other = ctx.encode(side, np.arange(3, dtype=np.uint32))  # eager work between replays
+            assert np.array_equal(other, oracle.encode(side.view(np.uint16), np.arange(3)).view(np.uint8).reshape(3, -1))
+            plan.launch()
+            ctx.sync()
+            check(ctx, jobs, host)
+        # new contents in the encode jobs' input buffers
+        for j, h in zip(jobs, host):
+            if j["kind"] == "encode":
+                h["data"] = rng.integers(0, 256, h["data"].shape, dtype=np.uint8)
+                j["slices"].upload(h["data"])
+        clear_outputs(jobs)
+        plan.launch()
+        ctx.sync()
+        check(ctx, jobs, host)
+    finally:
+        plan.close()
+
+
+def test_batch_plan_rejects_gapped_repair(ctx):
+    rng = np.random.default_rng(43)
+    jobs, _ = make_jobs(ctx, rng, [("encode", 20, 2048, 4, True, False), ("repair", 33, 18, 4, False, True)])
+    with pytest.raises(par2hip.InvalidFormat):
+        ctx.batch_plan(jobs, graph=True)
+    # the context still works afterwards
+    ctx.batch_run_dev(jobs)
+    ctx.sync()
