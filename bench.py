@@ -557,6 +557,8 @@ def main():
                     help="served: offered loads as multiples of the plain run's")
     ap.add_argument("--seed", type=int, default=4)
     ap.add_argument("--kernel", type=int, default=0, help="0 auto, 1 LDS tables, 2 MFMA")
+    ap.add_argument("--autotune", type=int, default=1,
+                    help="1: tile shapes tuned on the first (warm-up) call of each shape class; 0: fixed heuristic")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--traffic", type=float, default=None,
                     help="HBM bytes per GF launch measured with rocprofv3 --pmc (FETCH_SIZE+WRITE_SIZE)")
@@ -569,6 +571,7 @@ def main():
         print("bench.py: no gfx950 device visible to libpar2hip.so", file=sys.stderr)
         sys.exit(1)
     ctx = par2hip.Context(0)
+    ctx.set_autotune(bool(args.autotune))
     log("workload", args.workload, "steps", args.steps, "warmup", args.warmup, "device", ctx.device_info())
     if args.workload == "micro":
         print(json.dumps({"micro": bench_micro(args, ctx)}), flush=True)

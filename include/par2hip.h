@@ -281,6 +281,25 @@ int32_t par2hip_hash_slices_dev(par2hip_ctx *ctx, const void *d_slices, size_t n
  * 1 = LDS nibble tables, 2 = bit-sliced MFMA.  Returns the previous value. */
 int32_t par2hip_set_gf_kernel(par2hip_ctx *ctx, int32_t which);
 
+/* Tile-shape autotuning of the LDS contraction (no counterpart in the
+ * reference).  mode 0: a fixed heuristic picks the tile shape (words per
+ * thread, output rows per workgroup, sources per LDS chunk); mode 1: the first
+ * overwriting contraction of each shape class (rows, sources, words,
+ * alignment) on this ctx times every candidate shape on that problem and the
+ * fastest is kept for the process.  Returns the previous mode. */
+int32_t par2hip_set_autotune(par2hip_ctx *ctx, int32_t mode);
+/* Use tile shape (v words per thread, g groups of 8 rows per workgroup, d
+ * sources per LDS chunk) for every LDS contraction on this ctx whose
+ * alignment allows v (for tests and manual tuning); (0, 0, 0) clears it.
+ * PAR2HIP_EINVAL for a shape that is not among the candidates. */
+int32_t par2hip_force_tile_shape(par2hip_ctx *ctx, int32_t v, int32_t g, int32_t d);
+/* The process-wide table of tuned shapes, as text (one "key v g d" line
+ * each): saved to a file, or loaded from one (entries add to and replace
+ * those in the table; *loaded receives the number read; a malformed line or
+ * an unknown shape gives PAR2HIP_EINVAL, keeping the lines before it). */
+int32_t par2hip_autotune_save(const char *path);
+int32_t par2hip_autotune_load(const char *path, size_t *loaded);
+
 /* Per-kernel device timings, recorded with HIP events on the stream each
  * kernel runs on, while enabled. */
 typedef struct {

@@ -175,7 +175,7 @@ static int32_t plan_build(par2hip_ctx *ctx, const par2hip_job *jobs, size_t njob
                      [&](int a, int b) { return (long)tasks[a].t.nsrc * words[a] > (long)tasks[b].t.nsrc * words[b]; });
     for (int ti : order) {
       const int ntiles = (int)((words[ti] + kBatchWordsPerWg - 1) / kBatchWordsPerWg);
-      const int nrb = gf_task_rowblks(tasks[ti].t.ndst);
+      const int nrb = gf_rowblks(tasks[ti].t.ndst, gf_batch_groups(tasks[ti].t.ndst));
       for (int tile = 0; tile < ntiles; tile++)
         for (int rb = 0; rb < nrb; rb++) items.push_back(GfItem{ti, tile, rb, 0});
     }
@@ -270,6 +270,7 @@ static int32_t plan_build(par2hip_ctx *ctx, const par2hip_job *jobs, size_t njob
     t.tabs = (uint64_t *)(arena + main_tasks[k].tabs);
     t.accumulate = 0;
     t.align = std::min(row_align((const void *const *)h_srcs, J.n), row_align((const void *const *)h_dsts, J.count));
+    t.groups = gf_batch_groups(t.ndst);
   }
   std::vector<InvJob> ijobs;
   int max_m = 0;
@@ -329,6 +330,7 @@ static int32_t plan_build(par2hip_ctx *ctx, const par2hip_job *jobs, size_t njob
         t.tabs = (uint64_t *)(arena + fold_tasks[f].tabs);
         t.accumulate = 0;
         t.align = std::min(row_align((const void *const *)h_fsrc, m), row_align((const void *const *)h_fdst, m));
+        t.groups = gf_batch_groups(t.ndst);
         f++;
       }
       GfTaskDev &t = main_tasks[enc.size() + k].t;
@@ -339,6 +341,7 @@ static int32_t plan_build(par2hip_ctx *ctx, const par2hip_job *jobs, size_t njob
       t.tabs = (uint64_t *)(arena + main_tasks[enc.size() + k].tabs);
       t.accumulate = 0;
       t.align = std::min(row_align((const void *const *)h_srcs, np + m), row_align((const void *const *)h_dsts, m));
+      t.groups = gf_batch_groups(t.ndst);
     }
   }
   auto put = [&](const void *src, size_t bytes) -> void * {

@@ -85,6 +85,8 @@ struct par2hip_ctx {
   int gf_kernel = 0;                  // 0 auto, 1 LDS tables, 2 MFMA
   int num_cus = 256;                  // compute units of the device
   bool profile = false;
+  bool autotune = false;  // time candidate tile shapes on first use of a shape class
+  int forced_shape = -1;  // tile shape used by every LDS contraction (-1: none)
   std::vector<par2hip::KernelStat> stats;
   std::vector<par2hip::PendingTiming> pending;
   std::vector<hipEvent_t> event_pool;
@@ -161,17 +163,24 @@ struct GfProblem {
   const char *tag = nullptr;    // kernel name for timing (default: gf_muladd_<variant>)
 };
 int32_t run_gf(par2hip_ctx *ctx, hipStream_t s, const GfProblem &p);
+// The process-wide table of tuned tile shapes as text (one "key v g d" line each).
+int32_t tune_table_save(const char *path);
+// Index of tile shape (v, g, d) among the candidates, or -1.
+int tile_shape_index(int v, int g, int d);
+int32_t tune_table_load(const char *path, size_t *loaded);
 
 // ---- variant T nibble tables.  One 16-byte entry per (group of 8 output
 // rows g, source d, nibble position i, nibble value v), entry index
 // (g * nsrc + d) * 64 + 16 * i + v; it packs the 8 rows' products
 // c_{8g+j,d} * (v << 4i) as u16 j (little-endian, so u32 j/2, half j%2).
-// A workgroup covers kGpw groups = 16 output rows.
+// A workgroup covers G groups = 8G output rows (G = 1, 2 or 4).
 constexpr int kTabRows = 8;
-constexpr int kGpw = 2;
 __host__ __device__ inline int gf_tab_groups(int ndst) { return (ndst + kTabRows - 1) / kTabRows; }
 __host__ __device__ inline long gf_tab_entries(int ndst, int nsrc) { return (long)gf_tab_groups(ndst) * nsrc * 64; }
-__host__ __device__ inline int gf_task_rowblks(int ndst) { return (gf_tab_groups(ndst) + kGpw - 1) / kGpw; }
+__host__ __device__ inline int gf_rowblks(int ndst, int groups) { return (gf_tab_groups(ndst) + groups - 1) / groups; }
+// Groups per workgroup of a batched task: one for at most 8 rows (no padded
+// group), two otherwise.
+__host__ __device__ inline int gf_batch_groups(int ndst) { return ndst <= kTabRows ? 1 : 2; }
 
 // ---- batched contraction (many sets in one persistent launch)
 struct GfTaskDev {
@@ -181,8 +190,8 @@ struct GfTaskDev {
   uint64_t *tabs;         // nibble tables of this task, gf_tab_entries(ndst, nsrc) 16-byte entries
   long words;
   int nsrc, ndst, ldc, accumulate;
-  int align;  // byte alignment (2..16) shared by every source and destination row
-  int pad_;
+  int align;   // byte alignment (2..16) shared by every source and destination row
+  int groups;  // 8-row groups per workgroup (gf_batch_groups)
 };
 struct GfItem {
   int task, tile, rowblk, pad_;

@@ -6,7 +6,10 @@
 namespace par2hip {
 
 constexpr int kTpb = 256;  // threads per workgroup
-constexpr int kDch = 16;   // sources per LDS table chunk: 16 x kGpw x 1 KiB = 32 KiB
+
+// Tile shape of the contraction: V words per thread, G groups of 8 output
+// rows per workgroup (accumulators: G*V 16-byte registers), D sources per
+// LDS table chunk (D*G KiB of LDS).
 
 // c * x for every row of one 8-row group as the table entry laid out above
 // (products via log/exp; zero coefficients and x = 0 give zero).
@@ -104,8 +107,8 @@ __device__ __forceinline__ void store_words(P2H_GLOBAL uint16_t *__restrict__ p,
 // acc ^= (tables of source d in lds) x (its V words), the words packed two per
 // dword.  One 16-byte LDS read per nibble and group (8 rows); the four
 // partial products and the accumulator are folded by two three-input XORs.
-template <int V>
-__device__ __forceinline__ void gf_lds_lookup(u32x4 (&acc)[kGpw][V], const u32x4 *__restrict__ t,
+template <int V, int G>
+__device__ __forceinline__ void gf_lds_lookup(u32x4 (&acc)[G][V], const u32x4 *__restrict__ t,
                                               const uint32_t (&r)[(V + 1) / 2]) {
 #pragma unroll
   for (int k = 0; k < V; k++) {
@@ -113,7 +116,7 @@ __device__ __forceinline__ void gf_lds_lookup(u32x4 (&acc)[kGpw][V], const u32x4
     const uint32_t a0 = x & 15u, a1 = 16u + ((x >> 4) & 15u);
     const uint32_t a2 = 32u + ((x >> 8) & 15u), a3 = 48u + ((x >> 12) & 15u);
 #pragma unroll
-    for (int g = 0; g < kGpw; g++) {
+    for (int g = 0; g < G; g++) {
       const u32x4 *tg = t + g * 64;
       const u32x4 l0 = tg[a0], l1 = tg[a1], l2 = tg[a2], l3 = tg[a3];
 #pragma unroll
@@ -128,24 +131,24 @@ __device__ __forceinline__ void gf_lds_lookup(u32x4 (&acc)[kGpw][V], const u32x4
 // acc ^= sum over sources d0..d0+dn-1 of (their tables in lds) x (their
 // words).  Memory latency is hidden by the other waves of the CU (software
 // pipelining here only raises register pressure past the occupancy limit).
-template <int V, int kMode>
-__device__ __forceinline__ void gf_lds_sources(u32x4 (&acc)[kGpw][V], const u32x4 *__restrict__ lds,
+template <int V, int G, int kMode>
+__device__ __forceinline__ void gf_lds_sources(u32x4 (&acc)[G][V], const u32x4 *__restrict__ lds,
                                                const uint16_t *const *__restrict__ srcs, int d0, int dn, long w0,
                                                long words) {
   const auto sp = gptr(srcs) + d0;  // the row pointers, read from global memory
   for (int d = 0; d < dn; d++) {
     uint32_t r[(V + 1) / 2];
     load_words<V, kMode>(gptr(sp[d]), w0, words, r);
-    gf_lds_lookup<V>(acc, lds + d * kGpw * 64, r);
+    gf_lds_lookup<V, G>(acc, lds + d * G * 64, r);
   }
 }
 
-template <int V, int kMode>
-__device__ __forceinline__ void gf_lds_store(const u32x4 (&acc)[kGpw][V], uint16_t *const *__restrict__ dsts,
+template <int V, int G, int kMode>
+__device__ __forceinline__ void gf_lds_store(const u32x4 (&acc)[G][V], uint16_t *const *__restrict__ dsts,
                                              int ndst, int g0, long w0, long words, int accumulate) {
   const auto dp = gptr(dsts);
 #pragma unroll
-  for (int g = 0; g < kGpw; g++) {
+  for (int g = 0; g < G; g++) {
 #pragma unroll
     for (int j = 0; j < kTabRows; j++) {
       const int r = kTabRows * (g0 + g) + j;
@@ -160,60 +163,62 @@ __device__ __forceinline__ void gf_lds_store(const u32x4 (&acc)[kGpw][V], uint16
 }
 
 // One workgroup's share of a contraction: words [tile*kTpb*V, +kTpb*V) of
-// output rows [16*rowblk, +16) (variant T, LDS nibble tables).  Must be called
-// by the whole workgroup (it synchronises); `lds` holds 32 KiB.  align: the
-// byte alignment (2, 4, 8 or 16) shared by every source and destination row;
-// a thread whose V words lie inside the rows uses vector accesses when align
-// >= 2V, dword pairs when align >= 4, and checked word accesses otherwise.
-template <int V>
+// output rows [8*G*rowblk, +8*G) (variant T, LDS nibble tables).  Must be
+// called by the whole workgroup (it synchronises); `lds` holds D*G KiB.
+// align: the byte alignment (2, 4, 8 or 16) shared by every source and
+// destination row; a thread whose V words lie inside the rows uses vector
+// accesses when align >= 2V, dword pairs when align >= 4, and checked word
+// accesses otherwise.
+template <int V, int G, int D>
 __device__ __forceinline__ void gf_lds_tile(u32x4 *__restrict__ lds, const uint16_t *const *__restrict__ srcs,
                                             int nsrc, uint16_t *const *__restrict__ dsts, int ndst,
                                             const uint64_t *__restrict__ tabs, long words, int accumulate,
                                             int tile, int rowblk, int align) {
   const int tid = threadIdx.x;
   const int ngroups = gf_tab_groups(ndst);
-  const int g0 = rowblk * kGpw;
+  const int g0 = rowblk * G;
   const long w0 = ((long)tile * kTpb + tid) * V;
   const bool inside = w0 + V <= words;
   const int mode = (inside && align >= 2 * V) ? 0 : (inside && align >= 4 && V % 2 == 0) ? 1 : 2;
   const P2H_GLOBAL u32x4 *gtab = reinterpret_cast<const P2H_GLOBAL u32x4 *>(gptr(tabs));
+  constexpr int kMode1 = (V % 2 == 0) ? 1 : 2;
 
-  u32x4 acc[kGpw][V];
+  u32x4 acc[G][V];
 #pragma unroll
-  for (int g = 0; g < kGpw; g++)
+  for (int g = 0; g < G; g++)
 #pragma unroll
     for (int k = 0; k < V; k++) acc[g][k] = (u32x4){0u, 0u, 0u, 0u};
 
-  for (int d0 = 0; d0 < nsrc; d0 += kDch) {
-    const int dn = min(kDch, nsrc - d0);
+  for (int d0 = 0; d0 < nsrc; d0 += D) {
+    const int dn = min(D, nsrc - d0);
     __syncthreads();
     // stage the tables of sources d0..d0+dn-1 for this workgroup's groups:
-    // lds[(d * kGpw + g) * 64 + e] = entry e of (group g0+g, source d0+d)
-    const int units = kGpw * dn * 64;
+    // lds[(d * G + g) * 64 + e] = entry e of (group g0+g, source d0+d)
+    const int units = G * dn * 64;
     for (int u = tid; u < units; u += kTpb) {
       const int g = u / (dn * 64), rem = u - g * dn * 64, d = rem >> 6, e = rem & 63;
       u32x4 val = {0u, 0u, 0u, 0u};
       if (g0 + g < ngroups) val = gtab[((long)(g0 + g) * nsrc + d0 + d) * 64 + e];
-      lds[(d * kGpw + g) * 64 + e] = val;
+      lds[(d * G + g) * 64 + e] = val;
     }
     __syncthreads();
     if (w0 < words) {
       if (mode == 0)
-        gf_lds_sources<V, 0>(acc, lds, srcs, d0, dn, w0, words);
-      else if (V % 2 == 0 && mode == 1)
-        gf_lds_sources<V, (V % 2 == 0) ? 1 : 2>(acc, lds, srcs, d0, dn, w0, words);
+        gf_lds_sources<V, G, 0>(acc, lds, srcs, d0, dn, w0, words);
+      else if (mode == 1)
+        gf_lds_sources<V, G, kMode1>(acc, lds, srcs, d0, dn, w0, words);
       else
-        gf_lds_sources<V, 2>(acc, lds, srcs, d0, dn, w0, words);
+        gf_lds_sources<V, G, 2>(acc, lds, srcs, d0, dn, w0, words);
     }
   }
 
   if (w0 >= words) return;
   if (mode == 0)
-    gf_lds_store<V, 0>(acc, dsts, ndst, g0, w0, words, accumulate);
-  else if (V % 2 == 0 && mode == 1)
-    gf_lds_store<V, (V % 2 == 0) ? 1 : 2>(acc, dsts, ndst, g0, w0, words, accumulate);
+    gf_lds_store<V, G, 0>(acc, dsts, ndst, g0, w0, words, accumulate);
+  else if (mode == 1)
+    gf_lds_store<V, G, kMode1>(acc, dsts, ndst, g0, w0, words, accumulate);
   else
-    gf_lds_store<V, 2>(acc, dsts, ndst, g0, w0, words, accumulate);
+    gf_lds_store<V, G, 2>(acc, dsts, ndst, g0, w0, words, accumulate);
 }
 
 }  // namespace par2hip

@@ -14,6 +14,9 @@
 // table of 8-byte entries covers all 32 LDS banks exactly once, so any lane
 // pattern is conflict-free (equal addresses broadcast).
 #include <algorithm>
+#include <cstdio>
+#include <map>
+#include <mutex>
 
 #include "gf_device.hpp"
 
@@ -59,15 +62,15 @@ __global__ void k_nibble_tables(uint64_t *__restrict__ tabs, const uint16_t *__r
 }
 
 // Workgroup w covers words [tile*kTpb*V, +kTpb*V) of output rows
-// [16*rowblk, +16).  The 1-D grid is remapped so that the row blocks of one
+// [8G*rowblk, +8G).  The 1-D grid is remapped so that the row blocks of one
 // word tile run on the same XCD back to back (workgroups are dealt to the 8
 // XCDs round-robin), letting them share the tile's input lines in that L2.
-template <int V>
+template <int V, int G, int D>
 __global__ __launch_bounds__(kTpb, 2) void k_gf_lds(const uint16_t *const *__restrict__ srcs, int nsrc,
                                                    uint16_t *const *__restrict__ dsts, int ndst,
                                                    const uint64_t *__restrict__ tabs, long words,
                                                    int accumulate, int ntiles, int nrowblk) {
-  __shared__ u32x4 lds[kDch * kGpw * 64];  // [d][g][i*16+v], 32 KiB
+  __shared__ u32x4 lds[D * G * 64];  // [d][g][i*16+v]
   const int L = blockIdx.x;
   const int xcd = L & 7, q = L >> 3;
   const int tiles_per_xcd = (ntiles + 7) >> 3;
@@ -75,23 +78,118 @@ __global__ __launch_bounds__(kTpb, 2) void k_gf_lds(const uint16_t *const *__res
   const int local_tile = q / nrowblk, rowblk = q % nrowblk;
   const int tile = local_tile * 8 + xcd;
   if (local_tile >= tiles_per_xcd || tile >= ntiles) return;
-  gf_lds_tile<V>(lds, srcs, nsrc, dsts, ndst, tabs, words, accumulate, tile, rowblk, 2 * V);
+  gf_lds_tile<V, G, D>(lds, srcs, nsrc, dsts, ndst, tabs, words, accumulate, tile, rowblk, 2 * V);
 }
 
-template <int V>
-static int32_t launch_lds_variant(par2hip_ctx *ctx, hipStream_t s, const GfProblem &p,
-                                  const uint64_t *tabs) {
+template <int V, int G, int D>
+static int32_t launch_lds_variant(par2hip_ctx *ctx, hipStream_t s, const GfProblem &p, const uint64_t *tabs) {
   const long per_wg = (long)kTpb * V;
   const int ntiles = (int)((p.words + per_wg - 1) / per_wg);
-  const int nrowblk = gf_task_rowblks(p.ndst);
+  const int nrowblk = gf_rowblks(p.ndst, G);
   const int tiles_per_xcd = (ntiles + 7) / 8;
   const long nwg = 8L * tiles_per_xcd * nrowblk;
   if (nwg > 0x7fffffffL) return fail(ctx, PAR2HIP_EINVAL, "GF problem too large for one launch");
   ScopedTiming t(ctx, p.tag ? p.tag : "gf_muladd_lds", s);
-  hipLaunchKernelGGL(k_gf_lds<V>, dim3((unsigned)nwg), dim3(kTpb), 0, s, p.srcs, p.nsrc, p.dsts,
-                     p.ndst, tabs, (long)p.words, p.accumulate ? 1 : 0, ntiles, nrowblk);
+  hipLaunchKernelGGL((k_gf_lds<V, G, D>), dim3((unsigned)nwg), dim3(kTpb), 0, s, p.srcs, p.nsrc, p.dsts, p.ndst, tabs,
+                     (long)p.words, p.accumulate ? 1 : 0, ntiles, nrowblk);
   P2H_CHECK(ctx, hipGetLastError());
   return PAR2HIP_OK;
+}
+
+// ---- tile shapes and their tuning
+struct TileShape {
+  int v, g, d;
+};
+// Candidates (words per thread, 8-row groups per workgroup, sources per LDS
+// chunk); every V has a G = 1 and a G = 2 shape with D = 16 for the default.
+constexpr TileShape kShapes[] = {{8, 2, 16}, {8, 1, 16}, {8, 2, 32}, {4, 4, 16}, {4, 2, 16}, {4, 1, 16}, {4, 2, 32},
+                                 {2, 4, 16}, {2, 2, 16}, {2, 1, 16}, {1, 4, 16}, {1, 2, 16}, {1, 1, 16}};
+constexpr int kNumShapes = (int)(sizeof kShapes / sizeof kShapes[0]);
+
+static int32_t launch_shape(par2hip_ctx *ctx, hipStream_t s, const GfProblem &p, const uint64_t *tabs, int i) {
+  switch (i) {
+    case 0: return launch_lds_variant<8, 2, 16>(ctx, s, p, tabs);
+    case 1: return launch_lds_variant<8, 1, 16>(ctx, s, p, tabs);
+    case 2: return launch_lds_variant<8, 2, 32>(ctx, s, p, tabs);
+    case 3: return launch_lds_variant<4, 4, 16>(ctx, s, p, tabs);
+    case 4: return launch_lds_variant<4, 2, 16>(ctx, s, p, tabs);
+    case 5: return launch_lds_variant<4, 1, 16>(ctx, s, p, tabs);
+    case 6: return launch_lds_variant<4, 2, 32>(ctx, s, p, tabs);
+    case 7: return launch_lds_variant<2, 4, 16>(ctx, s, p, tabs);
+    case 8: return launch_lds_variant<2, 2, 16>(ctx, s, p, tabs);
+    case 9: return launch_lds_variant<2, 1, 16>(ctx, s, p, tabs);
+    case 10: return launch_lds_variant<1, 4, 16>(ctx, s, p, tabs);
+    case 11: return launch_lds_variant<1, 2, 16>(ctx, s, p, tabs);
+    default: return launch_lds_variant<1, 1, 16>(ctx, s, p, tabs);
+  }
+}
+
+static int shape_index(int v, int g, int d) {
+  for (int i = 0; i < kNumShapes; i++)
+    if (kShapes[i].v == v && kShapes[i].g == g && kShapes[i].d == d) return i;
+  return -1;
+}
+
+// Fixed heuristic: the widest V the alignment allows (table staging is
+// amortised over more words), narrower only when the grid would not give
+// every resident workgroup slot (2 per CU) one workgroup; one 8-row group per
+// workgroup when there are at most 8 rows, two otherwise; 16-source chunks.
+static int default_shape(const par2hip_ctx *ctx, const GfProblem &p, int vmax) {
+  const int g = p.ndst <= kTabRows ? 1 : 2;
+  const long slots = 2L * ctx->num_cus;
+  const long nrowblk = gf_rowblks(p.ndst, g);
+  int v = vmax;
+  while (v > 1 && (long)((p.words + (size_t)kTpb * v - 1) / ((size_t)kTpb * v)) * nrowblk < slots) v /= 2;
+  return shape_index(v, g, 16);
+}
+
+int tile_shape_index(int v, int g, int d) { return shape_index(v, g, d); }
+
+// Process-wide table of tuned shapes, keyed by the problem's class.
+std::mutex g_tune_mu;
+std::map<uint64_t, int> g_tuned;
+
+static int bit_length(uint64_t x) { return x ? 64 - __builtin_clzll(x) : 0; }
+static uint64_t shape_key(const GfProblem &p, int vmax) {
+  const uint64_t rows = (uint64_t)std::min(gf_tab_groups(p.ndst), 255);
+  return (uint64_t)vmax | rows << 8 | (uint64_t)bit_length((uint64_t)p.nsrc) << 16 |
+         (uint64_t)bit_length(p.words) << 24;
+}
+
+// Times every candidate the alignment allows on the problem itself (two
+// launches each after a warm-up; the outputs are overwritten with the same
+// values each time) and returns the fastest.
+static int32_t tune_shape(par2hip_ctx *ctx, hipStream_t s, const GfProblem &p, const uint64_t *tabs, int vmax,
+                          int *best) {
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  P2H_CHECK(ctx, hipEventCreate(&e0));
+  P2H_CHECK(ctx, hipEventCreate(&e1));
+  const bool prof = ctx->profile;
+  ctx->profile = false;
+  float best_ms = 1e30f;
+  int32_t rc = PAR2HIP_OK;
+  *best = default_shape(ctx, p, vmax);
+  for (int i = 0; i < kNumShapes && rc == PAR2HIP_OK; i++) {
+    if (kShapes[i].v > vmax) continue;
+    if ((rc = launch_shape(ctx, s, p, tabs, i))) break;
+    (void)hipEventRecord(e0, s);
+    for (int k = 0; k < 2 && rc == PAR2HIP_OK; k++) rc = launch_shape(ctx, s, p, tabs, i);
+    (void)hipEventRecord(e1, s);
+    if (hipEventSynchronize(e1) != hipSuccess) {
+      rc = fail(ctx, PAR2HIP_EHIP, "tuning timing");
+      break;
+    }
+    float ms = 0;
+    (void)hipEventElapsedTime(&ms, e0, e1);
+    if (ms < best_ms) {
+      best_ms = ms;
+      *best = i;
+    }
+  }
+  ctx->profile = prof;
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  return rc;
 }
 
 int32_t run_gf_lds(par2hip_ctx *ctx, hipStream_t s, const GfProblem &p) {
@@ -106,18 +204,57 @@ int32_t run_gf_lds(par2hip_ctx *ctx, hipStream_t s, const GfProblem &p) {
                        (uint64_t *)tabs, p.coeff, p.ndst, p.nsrc, p.ldc ? p.ldc : p.nsrc, ctx->d_log, ctx->d_exp);
     P2H_CHECK(ctx, hipGetLastError());
   }
-  // words per thread: the widest the alignment allows (per-workgroup table
-  // staging is amortised over more words); narrower only when the grid would
-  // not even give every resident workgroup slot (2 per CU) one workgroup
-  const int nrowblk = gf_task_rowblks(p.ndst);
-  const long slots = 2L * ctx->num_cus;
-  auto nwg = [&](int v) { return (long)((p.words + (size_t)kTpb * v - 1) / ((size_t)kTpb * v)) * nrowblk; };
-  int v = p.align_words % 8 == 0 ? 8 : p.align_words % 4 == 0 ? 4 : p.align_words % 2 == 0 ? 2 : 1;
-  while (v > 1 && nwg(v) < slots) v /= 2;
-  if (v == 8) return launch_lds_variant<8>(ctx, s, p, (const uint64_t *)tabs);
-  if (v == 4) return launch_lds_variant<4>(ctx, s, p, (const uint64_t *)tabs);
-  if (v == 2) return launch_lds_variant<2>(ctx, s, p, (const uint64_t *)tabs);
-  return launch_lds_variant<1>(ctx, s, p, (const uint64_t *)tabs);
+  const int vmax = p.align_words % 8 == 0 ? 8 : p.align_words % 4 == 0 ? 4 : p.align_words % 2 == 0 ? 2 : 1;
+  int shape = ctx->forced_shape >= 0 && kShapes[ctx->forced_shape].v <= vmax ? ctx->forced_shape : -1;
+  if (shape < 0 && ctx->autotune) {
+    const uint64_t key = shape_key(p, vmax);
+    {
+      std::lock_guard<std::mutex> g(g_tune_mu);
+      auto it = g_tuned.find(key);
+      if (it != g_tuned.end()) shape = it->second;
+    }
+    if (shape < 0 && !p.accumulate) {  // an accumulating contraction cannot be re-run
+      if ((rc = tune_shape(ctx, s, p, (const uint64_t *)tabs, vmax, &shape))) return rc;
+      std::lock_guard<std::mutex> g(g_tune_mu);
+      g_tuned[key] = shape;
+    }
+  }
+  if (shape < 0) shape = default_shape(ctx, p, vmax);
+  return launch_shape(ctx, s, p, (const uint64_t *)tabs, shape);
+}
+
+int32_t tune_table_save(const char *path) {
+  FILE *f = fopen(path, "w");
+  if (!f) return PAR2HIP_EINVAL;
+  std::lock_guard<std::mutex> g(g_tune_mu);
+  fprintf(f, "# par2hip contraction tile shapes: key v g d\n");
+  for (const auto &kv : g_tuned)
+    fprintf(f, "%llx %d %d %d\n", (unsigned long long)kv.first, kShapes[kv.second].v, kShapes[kv.second].g,
+            kShapes[kv.second].d);
+  return fclose(f) == 0 ? PAR2HIP_OK : PAR2HIP_EINVAL;
+}
+
+int32_t tune_table_load(const char *path, size_t *loaded) {
+  FILE *f = fopen(path, "r");
+  if (!f) return PAR2HIP_EINVAL;
+  char line[256];
+  size_t n = 0;
+  int32_t rc = PAR2HIP_OK;
+  std::lock_guard<std::mutex> g(g_tune_mu);
+  while (fgets(line, sizeof line, f)) {
+    if (line[0] == '#' || line[0] == '\n') continue;
+    unsigned long long key;
+    int v, gg, d;
+    if (sscanf(line, "%llx %d %d %d", &key, &v, &gg, &d) != 4 || shape_index(v, gg, d) < 0 || v > (int)(key & 0xff)) {
+      rc = PAR2HIP_EINVAL;  // malformed line or an unknown shape: stop, keep what was read
+      break;
+    }
+    g_tuned[(uint64_t)key] = shape_index(v, gg, d);
+    n++;
+  }
+  fclose(f);
+  if (loaded) *loaded = n;
+  return rc;
 }
 
 int32_t run_gf(par2hip_ctx *ctx, hipStream_t s, const GfProblem &p) {
@@ -184,7 +321,7 @@ __global__ void k_reset_cursor(unsigned *cursor) { *cursor = 0u; }
 __global__ __launch_bounds__(kTpb, 2) void k_gf_lds_batch(const GfTaskDev *__restrict__ tasks,
                                                          const GfItem *__restrict__ items, int nitems,
                                                          unsigned *__restrict__ cursor) {
-  __shared__ u32x4 lds[kDch * kGpw * 64];
+  __shared__ u32x4 lds[16 * 2 * 64];  // D = 16 sources x up to 2 groups
   __shared__ int s_item;
   for (;;) {
     if (threadIdx.x == 0) s_item = (int)atomicAdd(cursor, 1u);
@@ -194,7 +331,12 @@ __global__ __launch_bounds__(kTpb, 2) void k_gf_lds_batch(const GfTaskDev *__res
     if (it >= nitems) break;
     const GfItem w = items[it];
     const GfTaskDev t = tasks[w.task];
-    gf_lds_tile<8>(lds, t.srcs, t.nsrc, t.dsts, t.ndst, t.tabs, t.words, t.accumulate, w.tile, w.rowblk, t.align);
+    if (t.groups == 1)
+      gf_lds_tile<8, 1, 16>(lds, t.srcs, t.nsrc, t.dsts, t.ndst, t.tabs, t.words, t.accumulate, w.tile, w.rowblk,
+                            t.align);
+    else
+      gf_lds_tile<8, 2, 16>(lds, t.srcs, t.nsrc, t.dsts, t.ndst, t.tabs, t.words, t.accumulate, w.tile, w.rowblk,
+                            t.align);
   }
 }
 

@@ -908,6 +908,34 @@ int32_t par2hip_set_gf_kernel(par2hip_ctx *ctx, int32_t which) {
   return prev;
 }
 
+int32_t par2hip_set_autotune(par2hip_ctx *ctx, int32_t mode) {
+  if (!ctx || mode < 0 || mode > 1) return PAR2HIP_EINVAL;
+  Guard g(ctx);
+  const int32_t prev = ctx->autotune ? 1 : 0;
+  ctx->autotune = mode == 1;
+  return prev;
+}
+
+int32_t par2hip_force_tile_shape(par2hip_ctx *ctx, int32_t v, int32_t g, int32_t d) {
+  if (!ctx) return PAR2HIP_EINVAL;
+  Guard gd(ctx);
+  if (v == 0 && g == 0 && d == 0) {
+    ctx->forced_shape = -1;
+    return PAR2HIP_OK;
+  }
+  const int i = par2hip::tile_shape_index(v, g, d);
+  if (i < 0) return par2hip::fail(ctx, PAR2HIP_EINVAL, "no such tile shape");
+  ctx->forced_shape = i;
+  return PAR2HIP_OK;
+}
+
+int32_t par2hip_autotune_save(const char *path) { return path ? par2hip::tune_table_save(path) : PAR2HIP_EINVAL; }
+
+int32_t par2hip_autotune_load(const char *path, size_t *loaded) {
+  if (loaded) *loaded = 0;
+  return path ? par2hip::tune_table_load(path, loaded) : PAR2HIP_EINVAL;
+}
+
 int32_t par2hip_profile_enable(par2hip_ctx *ctx, int32_t on) {
   if (!ctx) return PAR2HIP_EINVAL;
   Guard g(ctx);

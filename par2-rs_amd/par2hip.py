@@ -134,6 +134,10 @@ def lib():
                 "par2hip_set_gf_kernel": ([P, I32], I32),
                 "par2hip_profile_enable": ([P, I32], I32),
                 "par2hip_profile_read": ([P, P, S, C.POINTER(S)], I32),
+                "par2hip_set_autotune": ([P, I32], I32),
+                "par2hip_force_tile_shape": ([P, I32, I32, I32], I32),
+                "par2hip_autotune_save": ([C.c_char_p], I32),
+                "par2hip_autotune_load": ([C.c_char_p, C.POINTER(S)], I32),
                 "par2hip_batch_plan_create": ([P, P, S, I32, C.POINTER(P)], I32),
                 "par2hip_batch_plan_launch": ([P], I32),
                 "par2hip_batch_plan_destroy": ([P], I32),
@@ -400,6 +404,21 @@ class Context:
             self._check(rc)
         return rc
 
+    def set_autotune(self, on: bool) -> bool:
+        """Tile-shape autotuning on first use of each shape class; returns the previous setting."""
+        rc = lib().par2hip_set_autotune(self.handle, 1 if on else 0)
+        if rc < 0:
+            self._check(rc)
+        return bool(rc)
+
+    TILE_SHAPES = [(8, 2, 16), (8, 1, 16), (8, 2, 32), (4, 4, 16), (4, 2, 16), (4, 1, 16), (4, 2, 32),
+                   (2, 4, 16), (2, 2, 16), (2, 1, 16), (1, 4, 16), (1, 2, 16), (1, 1, 16)]
+
+    def force_tile_shape(self, shape=None) -> None:
+        """Use tile shape (v, g, d) for every LDS contraction; None clears it."""
+        v, g, d = shape if shape else (0, 0, 0)
+        self._check(lib().par2hip_force_tile_shape(self.handle, v, g, d))
+
     def profile(self, on: bool = True) -> None:
         self._check(lib().par2hip_profile_enable(self.handle, 1 if on else 0))
 
@@ -430,6 +449,16 @@ class BatchPlan:
             self.close()
         except Exception:
             pass
+
+
+def autotune_save(path: str) -> None:
+    _status(lib().par2hip_autotune_save(path.encode()), f"autotune_save {path}")
+
+
+def autotune_load(path: str) -> int:
+    n = C.c_size_t(0)
+    _status(lib().par2hip_autotune_load(path.encode(), C.byref(n)), f"autotune_load {path}")
+    return n.value
 
 
 def _status(rc: int, what: str) -> None:

@@ -52,3 +52,25 @@ def test_no_device_is_reported_not_raised():
     assert lib.par2hip_server_create(0, 0, 2, 64, C.byref(srv)) == par2hip.EINVAL
     assert lib.par2hip_server_create(0, 1 << 30, 0, 64, C.byref(srv)) == par2hip.EINVAL
     assert not par2hip.HipStrategy().is_available()
+
+
+def test_autotune_table_text_roundtrip(tmp_path):
+    """The tuned-shape table loads from and saves to text without a device;
+    unknown shapes and malformed lines are refused."""
+    if not os.path.exists(par2hip.LIB_PATH):
+        pytest.skip("libpar2hip.so not built")
+    good = tmp_path / "tune.txt"
+    good.write_text("# key v g d\n8 8 2 16\n10208 4 1 16\n")
+    assert par2hip.autotune_load(str(good)) == 2
+    out = tmp_path / "saved.txt"
+    par2hip.autotune_save(str(out))
+    lines = [l for l in out.read_text().splitlines() if not l.startswith("#")]
+    assert "8 8 2 16" in lines and "10208 4 1 16" in lines
+    bad = tmp_path / "bad.txt"
+    bad.write_text("9 8 3 16\n")  # no such shape
+    with pytest.raises(par2hip.InvalidFormat):
+        par2hip.autotune_load(str(bad))
+    wide = tmp_path / "wide.txt"
+    wide.write_text("4 8 2 16\n")  # wider than the key's alignment allows
+    with pytest.raises(par2hip.InvalidFormat):
+        par2hip.autotune_load(str(wide))
