@@ -31,6 +31,13 @@ sys.path.insert(0, os.path.join(ROOT, "par2-rs_amd"))
 import par2hip  # noqa: E402
 
 HBM_PEAK_GBS = 8000.0
+# HBM bytes per launch of the plain-encode contraction (1024 x 1 MiB, R=102)
+# with the fixed-heuristic tile shape (8 words per thread, 2 groups of 8 rows
+# per workgroup, 16 sources per LDS chunk): rocprofv3 --pmc FETCH_SIZE and
+# --pmc WRITE_SIZE runs of this workload, each scaled by the same counter on a
+# 1 GiB device-to-device copy (--workload copy): FETCH_SIZE reads half the
+# bytes the copy reads (factor 2.0), WRITE_SIZE all it writes (factor 1.0).
+C2_GF_TRAFFIC_BYTES = 2.475e9
 METRIC = "encode/repair GB/s source data + p50/p99 latency vs plain GPU path, bit-exact"
 
 
@@ -117,6 +124,9 @@ def bench_encode(args, ctx):
     md5_avg = stats.get("md5_slices", (1, 0.0))[1] / max(stats.get("md5_slices", (1, 0))[0], 1)
     crc_avg = stats.get("crc32_slices", (1, 0.0))[1] / max(stats.get("crc32_slices", (1, 0))[0], 1)
     achieved = algo_bytes / (gf_avg * 1e-3) / 1e9 if gf_avg > 0 else 0.0
+    traffic = args.traffic
+    if traffic is None and not args.autotune and gf_name == "gf_muladd_lds" and (n, s, r) == (1024, 1 << 20, 102):
+        traffic = C2_GF_TRAFFIC_BYTES
     return {
         "workload": "plain_encode",
         "config": {"workload": "plain_encode_1GiB", "n_slices": n, "slice_bytes": s, "recovery_slices": r,
@@ -125,7 +135,8 @@ def bench_encode(args, ctx):
         "src_bytes": src_bytes, "total_s": total, "lat": lat, "exact": exact,
         "kernel_ms": {k: v[1] / max(v[0], 1) for k, v in stats.items()},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": args.traffic,
+                     "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": traffic,
+                     "traffic_x_algorithmic": round(traffic / algo_bytes, 3) if traffic else None,
                      "kernel": gf_name, "kernel_ms": round(gf_avg, 4), "algorithmic_bytes": algo_bytes,
                      "gf_muladds": n * r * (s // 2), "md5_ms": round(md5_avg, 4), "crc32_ms": round(crc_avg, 4)},
         "cpu": (n, s, r),
@@ -159,9 +170,14 @@ def bench_micro(args, ctx):
         out[name] = {"wall_ms": round(wall, 4), **{k: round(v[1] / max(v[0], 1), 4) for k, v in st.items()}}
         log(name, out[name])
 
+    tuned = ctx.set_autotune(False)
     for k, nm in ((1, "gf_lds"), (2, "gf_mfma")):
         ctx.set_gf_kernel(k)
         timed(nm, lambda: ctx.encode_dev(d_sl, n, s, exps, d_rec), args.steps)
+    ctx.set_gf_kernel(1)
+    ctx.set_autotune(True)  # tuned on the warm-up call
+    timed("gf_lds_tuned", lambda: ctx.encode_dev(d_sl, n, s, exps, d_rec), args.steps)
+    ctx.set_autotune(False)
     ctx.set_gf_kernel(0)
     timed("md5_only", lambda: ctx.hash_slices_dev(d_sl, n, s, d_md5, None), max(2, args.steps // 4))
     # the repair-shaped contraction: 2000 sources x 100 outputs x 256 Ki words
@@ -173,6 +189,10 @@ def bench_micro(args, ctx):
     for k, nm in ((1, "gf_lds_2000x100"), (2, "gf_mfma_2000x100")):
         ctx.set_gf_kernel(k)
         timed(nm, lambda: ctx.encode_dev(d_sl3, n3, s3, exps3, d_rec3), args.steps)
+    ctx.set_gf_kernel(1)
+    ctx.set_autotune(True)
+    timed("gf_lds_2000x100_tuned", lambda: ctx.encode_dev(d_sl3, n3, s3, exps3, d_rec3), args.steps)
+    ctx.set_autotune(tuned)
     ctx.set_gf_kernel(0)
     d_sl3.free()
     d_rec3.free()
@@ -548,7 +568,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--workload", choices=["encode", "repair", "micro", "batched", "served"], default="encode")
+    ap.add_argument("--workload", choices=["encode", "repair", "micro", "batched", "served", "copy"], default="encode")
     ap.add_argument("--hbm-budget", type=int, default=200 * 10**9, help="served: server HBM budget in bytes")
     ap.add_argument("--lanes", type=int, default=2, help="served: concurrent batches")
     ap.add_argument("--max-batch", type=int, default=512, help="served: requests per batch at most")
@@ -557,8 +577,10 @@ def main():
                     help="served: offered loads as multiples of the plain run's")
     ap.add_argument("--seed", type=int, default=4)
     ap.add_argument("--kernel", type=int, default=0, help="0 auto, 1 LDS tables, 2 MFMA")
-    ap.add_argument("--autotune", type=int, default=1,
-                    help="1: tile shapes tuned on the first (warm-up) call of each shape class; 0: fixed heuristic")
+    ap.add_argument("--autotune", type=int, default=None,
+                    help="1: tile shapes tuned on the first (warm-up) call of each shape class; 0: fixed heuristic "
+                         "(default: 0 for encode/repair, whose steps the MD5 chains bound, so the contraction's "
+                         "shape and its measured HBM traffic stay fixed; 1 otherwise)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--traffic", type=float, default=None,
                     help="HBM bytes per GF launch measured with rocprofv3 --pmc (FETCH_SIZE+WRITE_SIZE)")
@@ -570,9 +592,28 @@ def main():
     if par2hip.available() <= 0:
         print("bench.py: no gfx950 device visible to libpar2hip.so", file=sys.stderr)
         sys.exit(1)
+    if args.autotune is None:
+        args.autotune = 0 if args.workload in ("encode", "repair") else 1
     ctx = par2hip.Context(0)
     ctx.set_autotune(bool(args.autotune))
     log("workload", args.workload, "steps", args.steps, "warmup", args.warmup, "device", ctx.device_info())
+    if args.workload == "copy":
+        # calibration for the HBM byte counters: device-to-device copies of a
+        # known size (1 GiB read + 1 GiB written per copy)
+        nb = 1 << 30
+        a, b = ctx.alloc(nb), ctx.alloc(nb)
+        a.upload(random_bytes(3, nb))
+        ctx.copy_dev(b, a, nb)
+        ctx.sync()
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            ctx.copy_dev(b, a, nb)
+        ctx.sync()
+        dt = (time.perf_counter() - t0) / args.steps
+        print(json.dumps({"copy": {"bytes_read": nb, "bytes_written": nb, "ms": round(dt * 1e3, 4),
+                                   "GBps_rw": round(2 * nb / dt / 1e9, 1)}}), flush=True)
+        ctx.close()
+        return
     if args.workload == "micro":
         print(json.dumps({"micro": bench_micro(args, ctx)}), flush=True)
         ctx.close()

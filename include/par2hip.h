@@ -81,6 +81,8 @@ int32_t par2hip_dev_alloc(par2hip_ctx *ctx, size_t bytes, void **dptr);
 int32_t par2hip_dev_free(par2hip_ctx *ctx, void *dptr);
 int32_t par2hip_h2d(par2hip_ctx *ctx, void *dst, const void *src, size_t bytes);
 int32_t par2hip_d2h(par2hip_ctx *ctx, void *dst, const void *src, size_t bytes);
+/* Device-to-device copy, asynchronous on the ctx stream. */
+int32_t par2hip_d2d(par2hip_ctx *ctx, void *dst, const void *src, size_t bytes);
 int32_t par2hip_sync(par2hip_ctx *ctx);
 
 /* --------------------------------------------- strategy-shaped (host) -- */

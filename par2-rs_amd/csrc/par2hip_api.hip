@@ -613,6 +613,13 @@ int32_t par2hip_d2h(par2hip_ctx *ctx, void *dst, const void *src, size_t bytes) 
   return PAR2HIP_OK;
 }
 
+int32_t par2hip_d2d(par2hip_ctx *ctx, void *dst, const void *src, size_t bytes) {
+  if (!ctx || (bytes && (!dst || !src))) return PAR2HIP_EINVAL;
+  Guard g(ctx);
+  P2H_CHECK(ctx, hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, ctx->stream));
+  return PAR2HIP_OK;
+}
+
 int32_t par2hip_sync(par2hip_ctx *ctx) {
   if (!ctx) return PAR2HIP_EINVAL;
   Guard g(ctx);

@@ -117,6 +117,7 @@ def lib():
                 "par2hip_dev_free": ([P, P], I32),
                 "par2hip_h2d": ([P, P, P, S], I32),
                 "par2hip_d2h": ([P, P, P, S], I32),
+                "par2hip_d2d": ([P, P, P, S], I32),
                 "par2hip_sync": ([P], I32),
                 "par2hip_gf_mul_slice": ([P, C.c_uint16, P, S], I32),
                 "par2hip_gf_muladd": ([P, P, P, S, C.c_uint16], I32),
@@ -403,6 +404,12 @@ class Context:
         if rc < 0:
             self._check(rc)
         return rc
+
+    def copy_dev(self, dst: DeviceBuffer, src: DeviceBuffer, nbytes: int) -> None:
+        """Device-to-device copy on the ctx stream."""
+        if nbytes > min(dst.nbytes, src.nbytes):
+            raise InvalidFormat("copy beyond buffer")
+        self._check(lib().par2hip_d2d(self.handle, dst.ptr, src.ptr, nbytes))
 
     def set_autotune(self, on: bool) -> bool:
         """Tile-shape autotuning on first use of each shape class; returns the previous setting."""
