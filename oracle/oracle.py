@@ -21,6 +21,14 @@ LIB_PATH = os.path.join(HERE, "liboracle.so")
 _lib = None
 
 
+class _FileIn(C.Structure):
+    _fields_ = [("name", C.c_char_p), ("data", C.c_void_p), ("len", C.c_uint64)]
+
+
+class _VolumeOut(C.Structure):
+    _fields_ = [("name", C.c_char * 1024), ("bytes", C.c_void_p), ("len", C.c_uint64)]
+
+
 def build() -> None:
     subprocess.run(["make", "-s", "-C", HERE], check=True)
 
@@ -63,6 +71,17 @@ def lib():
         L.po_hash_slices_mt.argtypes = [P, S, S, P, P, C.c_int]
         L.po_have_avx2.restype = C.c_int
         L.po_init()
+        L.po_par2_block_size.argtypes = [C.c_uint64]
+        L.po_par2_block_size.restype = C.c_uint64
+        L.po_par2_file_id.argtypes = [P, C.c_uint64, C.c_char_p, P]
+        L.po_par2_padding_width.argtypes = [C.c_uint32]
+        L.po_par2_padding_width.restype = C.c_int
+        L.po_par2_volume_name.argtypes = [C.c_char_p, C.c_uint32, C.c_uint32, C.c_int, C.c_char_p, S]
+        L.po_par2_create.argtypes = [P, S, C.c_uint64, C.c_float, C.c_int, C.c_char_p,
+                                     C.POINTER(C.POINTER(_VolumeOut)), C.POINTER(S)]
+        L.po_par2_create.restype = C.c_int
+        L.po_par2_free.argtypes = [C.POINTER(_VolumeOut), S]
+        L.po_par2_free.restype = None
         _lib = L
     return _lib
 
@@ -225,3 +244,51 @@ def hash_slices_mt(data: np.ndarray, n: int, slice_size: int, threads: int = 0):
     crcs = np.zeros(n, np.uint32)
     lib().po_hash_slices_mt(_p(data), n, slice_size, _p(md5s), _p(crcs), threads)
     return md5s, crcs
+
+
+# ---- PAR2 creation (src/creator.rs:261-531, src/writer.rs, src/volumes.rs) ----
+SCHEME_SINGLE, SCHEME_EXPONENTIAL = 0, 1
+
+
+def par2_block_size(total: int) -> int:
+    """calculate_block_size (creator.rs:29-67), po_par2_block_size."""
+    return int(lib().po_par2_block_size(total))
+
+
+def par2_file_id(hash16k: bytes, length: int, name: str) -> bytes:
+    h = np.frombuffer(hash16k, np.uint8).copy()
+    out = np.zeros(16, np.uint8)
+    lib().po_par2_file_id(_p(h), length, name.encode(), _p(out))
+    return out.tobytes()
+
+
+def par2_volume_name(stem: str, start: int, end: int, width: int) -> str:
+    buf = C.create_string_buffer(1024)
+    lib().po_par2_volume_name(stem.encode(), start, end, width, buf, 1024)
+    return buf.value.decode()
+
+
+def par2_padding_width(max_exponent: int) -> int:
+    return int(lib().po_par2_padding_width(max_exponent))
+
+
+def par2_create(files, block_size: int = 0, redundancy: float = 5.0, scheme: int = SCHEME_EXPONENTIAL,
+                stem: str = "out"):
+    """Par2Creator::create on in-memory files [(name, bytes)]: list of
+    (output path, bytes), the main .par2 first (po_par2_create)."""
+    bufs = [np.frombuffer(d, np.uint8) if len(d) else np.zeros(1, np.uint8) for _, d in files]
+    arr = (_FileIn * len(files))()
+    names = [n.encode() for n, _ in files]
+    for i, ((_, d), b) in enumerate(zip(files, bufs)):
+        arr[i].name = names[i]
+        arr[i].data = b.ctypes.data
+        arr[i].len = len(d)
+    vols = C.POINTER(_VolumeOut)()
+    nv = C.c_size_t(0)
+    rc = lib().po_par2_create(arr, len(files), block_size, redundancy, scheme, stem.encode(), C.byref(vols),
+                              C.byref(nv))
+    if rc != 0:
+        raise ValueError(f"po_par2_create: {rc}")
+    out = [(vols[i].name.decode(), C.string_at(vols[i].bytes, vols[i].len)) for i in range(nv.value)]
+    lib().po_par2_free(vols, nv.value)
+    return out

@@ -65,10 +65,17 @@ def read_packets(buf: bytes):
 
 
 def load(paths) -> Par2Set:
-    s = Par2Set()
+    bufs = []
     for p in paths:
         with open(p, "rb") as f:
-            buf = f.read()
+            bufs.append(f.read())
+    return load_buffers(bufs)
+
+
+def load_buffers(bufs) -> Par2Set:
+    """Same as load() on the bytes of the .par2 files."""
+    s = Par2Set()
+    for buf in bufs:
         for pkt in read_packets(buf):
             if pkt is None:
                 s.bad_packets += 1
