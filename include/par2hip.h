@@ -264,6 +264,34 @@ int32_t par2hip_server_wait(par2hip_server *srv, uint64_t ticket, par2hip_result
 int32_t par2hip_server_release(par2hip_server *srv, uint64_t ticket);
 int32_t par2hip_server_get_stats(par2hip_server *srv, par2hip_server_stats *st);
 
+/* ---------------------------------------------------------- .par2 files -- */
+
+/* Par2Creator::create (src/creator.rs:261-531) with the slice hashes,
+ * recovery slices and recovery packet digests on the device: writes
+ * <out_stem>.par2 and, for the exponential scheme, the recovery volumes
+ * <out_stem>.volS+C.par2 (src/volumes.rs:90-208), byte for byte as the
+ * reference lays them out (packets: src/writer.rs:83-264; creator text
+ * src/creator.rs:496).  files: the input files' names (as stored in the
+ * FileDesc packets) and contents; block_size 0 = the reference's automatic
+ * size (creator.rs:29-67), else >= 2048 and a multiple of 4;
+ * redundancy_percent as the reference's f32.  outs receives the written
+ * files (main .par2 first); *nouts the number needed (PAR2HIP_EINVAL when
+ * max_outs is smaller, before anything is written). */
+typedef struct {
+    const char *name;
+    const void *data;
+    uint64_t len;
+} par2hip_input_file;
+typedef struct {
+    char path[1024];
+    uint64_t bytes;
+} par2hip_output_file;
+enum { PAR2HIP_VOLUMES_SINGLE = 0, PAR2HIP_VOLUMES_EXPONENTIAL = 1 };
+int32_t par2hip_create_par2(par2hip_ctx *ctx, const par2hip_input_file *files, size_t nfiles,
+                            uint64_t block_size, float redundancy_percent, int32_t scheme,
+                            const char *out_stem, par2hip_output_file *outs, size_t max_outs,
+                            size_t *nouts);
+
 /* -------------------------------------------------------------- hashes -- */
 
 /* IFSC MD5 + CRC32 of n slices, each zero-padded to slice_size

@@ -155,8 +155,9 @@ int po_par2_create(const po_file_in *files, size_t nfiles, uint64_t block_size, 
     total += files[f].len;
   }
   qsort(meta, nfiles, sizeof *meta, cmp_meta);  /* IDs are distinct MD5s in practice */
-  if (block_size == 0) block_size = po_par2_block_size(total);
-  if (block_size % 2) {
+  if (block_size == 0) {
+    block_size = po_par2_block_size(total);
+  } else if (block_size < 2048 || block_size % 4) { /* with_block_size, creator.rs:185-211 */
     free(meta);
     return -1;
   }

@@ -33,7 +33,7 @@ int po_par2_padding_width(uint32_t max_exponent);
 void po_par2_volume_name(const char *stem, uint32_t start, uint32_t end, int width, char *out, size_t cap);
 
 /* Par2Creator::create (creator.rs:261-531) on in-memory files.  block_size 0
- * = automatic; redundancy is the f32 percentage; out_stem is the output path
+ * = automatic, else at least 2048 and a multiple of 4 (creator.rs:185-211); redundancy is the f32 percentage; out_stem is the output path
  * without ".par2".  Returns 0 and the output files (first the main .par2), or
  * -1 bad argument, -2 too many blocks (data + recovery > 65535, or more than
  * 32768 data blocks), -3 single scheme without recovery blocks. */

@@ -279,6 +279,11 @@ int32_t launch_crc32_list_lane(par2hip_ctx *ctx, hipStream_t s, const HashEntry 
 
 // high_priority: the MD5 chains are the critical path of the caller, so their
 // waves issue ahead of co-resident waves of other kernels.
+// Whole-set encode on device buffers (par2hip_api.hip), queued on ctx->stream;
+// recovery row i at d_rec + i * rec_stride (0: slice_size).
+int32_t encode_dev_impl(par2hip_ctx *ctx, const uint8_t *d_slices, size_t n, size_t slice_size,
+                        const uint32_t *exps, size_t r, uint8_t *d_rec, uint32_t *d_md5, uint32_t *d_crc,
+                        size_t rec_stride);
 int32_t launch_md5(par2hip_ctx *ctx, hipStream_t s, const uint8_t *d_data, size_t n,
                    size_t slice_size, size_t stride, const uint64_t *d_lens, uint32_t *d_md5,
                    bool high_priority = false);

@@ -1,0 +1,312 @@
+// creator.hip -- PAR2 creation over the GPU entry points: what
+// Par2Creator::create (src/creator.rs:261-531) does with the reference's
+// encoder and hashers, with the slice hashes, the recovery slices and the
+// recovery packets' digests computed on the device.
+//
+// Host: whole-file and first-16 KiB MD5s (one serial chain per file, threads
+// across files), File IDs and the canonical order, the small packets and the
+// files.  Device: IFSC MD5/CRC32 of every zero-padded block, the recovery
+// slices, and the MD5 of every recovery packet: the contraction writes each
+// recovery slice into a packet image row that already holds the set ID, the
+// packet type and the exponent, so the packet digest (MD5 over set ID, type
+// and body, src/writer.rs:97-105) is one more slice hash.
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <memory>
+#include <string>
+#include <thread>
+#include <vector>
+
+#include "common.hpp"
+#include "md5_host.hpp"
+
+namespace par2hip {
+namespace {
+
+const uint8_t kMagic[8] = {'P', 'A', 'R', '2', 0, 'P', 'K', 'T'};
+// packet types of the PAR 2.0 specification (src/writer.rs:26-51)
+const uint8_t kMain[16] = {'P', 'A', 'R', ' ', '2', '.', '0', 0, 'M', 'a', 'i', 'n', 0, 0, 0, 0};
+const uint8_t kFileDesc[16] = {'P', 'A', 'R', ' ', '2', '.', '0', 0, 'F', 'i', 'l', 'e', 'D', 'e', 's', 'c'};
+const uint8_t kIfsc[16] = {'P', 'A', 'R', ' ', '2', '.', '0', 0, 'I', 'F', 'S', 'C', 0, 0, 0, 0};
+const uint8_t kRecvSlic[16] = {'P', 'A', 'R', ' ', '2', '.', '0', 0, 'R', 'e', 'c', 'v', 'S', 'l', 'i', 'c'};
+const uint8_t kCreator[16] = {'P', 'A', 'R', ' ', '2', '.', '0', 0, 'C', 'r', 'e', 'a', 't', 'o', 'r', 0};
+// creator text written by the reference (src/creator.rs:496)
+const char kCreatorText[] = "par2-rs v0.1.0 https://github.com/zephleggett/par2-rs";
+
+struct Bytes {
+  std::vector<uint8_t> v;
+  void put(const void *p, size_t n) {
+    const uint8_t *b = static_cast<const uint8_t *>(p);
+    v.insert(v.end(), b, b + n);
+  }
+  void u32(uint32_t x) {
+    for (int i = 0; i < 4; i++) v.push_back((uint8_t)(x >> (8 * i)));
+  }
+  void u64(uint64_t x) {
+    for (int i = 0; i < 8; i++) v.push_back((uint8_t)(x >> (8 * i)));
+  }
+};
+
+// One packet: header (magic, length, MD5 over set ID + type + body, set ID,
+// type) and the body zero-padded to a multiple of 4 (src/writer.rs:83-124).
+void append_packet(Bytes &out, const uint8_t type[16], const uint8_t set_id[16], const std::vector<uint8_t> &body) {
+  std::vector<uint8_t> padded(body);
+  padded.resize((body.size() + 3) & ~(size_t)3, 0);
+  Md5Host h;
+  h.update(set_id, 16);
+  h.update(type, 16);
+  h.update(padded.data(), padded.size());
+  uint8_t digest[16];
+  h.finish(digest);
+  out.put(kMagic, 8);
+  out.u64(64 + padded.size());
+  out.put(digest, 16);
+  out.put(set_id, 16);
+  out.put(type, 16);
+  out.put(padded.data(), padded.size());
+}
+
+// calculate_block_size (src/creator.rs:29-67): about 2000 blocks, 2 KiB to
+// 4 MiB, multiple of 4, never more than 65535 blocks.
+uint64_t auto_block_size(uint64_t total) {
+  if (total == 0) return 2048;
+  uint64_t bs = (total / 2000 + 3) & ~(uint64_t)3;
+  const uint64_t kMax = (uint64_t)4 << 20;
+  uint64_t floor_bs = std::max<uint64_t>((total + 65534) / 65535, 2048);
+  floor_bs = (floor_bs + 3) & ~(uint64_t)3;
+  if (floor_bs > kMax) return floor_bs;
+  return std::max<uint64_t>(std::min<uint64_t>(std::max<uint64_t>(bs, 2048), kMax), floor_bs);
+}
+
+struct Input {
+  const par2hip_input_file *f;
+  uint8_t full[16], h16k[16], id[16];
+  uint64_t blocks = 0, first = 0;
+};
+
+// File ID order of par2cmdline (src/parser.rs:556-564): byte 15 first.
+bool id_less(const Input &a, const Input &b) {
+  for (int i = 15; i >= 0; i--)
+    if (a.id[i] != b.id[i]) return a.id[i] < b.id[i];
+  return false;
+}
+
+int digits(uint64_t x) {  // src/volumes.rs:47-59
+  int w = 1;
+  while (x >= 10) {
+    x /= 10;
+    w++;
+  }
+  return w;
+}
+
+struct DevMem {
+  void *p = nullptr;
+  ~DevMem() {
+    if (p) (void)hipFree(p);
+  }
+};
+
+}  // namespace
+
+int32_t create_par2(par2hip_ctx *ctx, const par2hip_input_file *files, size_t nfiles, uint64_t block_size,
+                    float redundancy, int32_t scheme, const char *stem, par2hip_output_file *outs, size_t max_outs,
+                    size_t *nouts) {
+  *nouts = 0;
+  if (nfiles == 0 || !stem || !std::isfinite(redundancy) || redundancy < 0.0f ||
+      (scheme != PAR2HIP_VOLUMES_SINGLE && scheme != PAR2HIP_VOLUMES_EXPONENTIAL))
+    return fail(ctx, PAR2HIP_EINVAL, "create: need files, a stem, a finite redundancy >= 0 and a known scheme");
+  for (size_t i = 0; i < nfiles; i++)
+    if (!files[i].name || (files[i].len && !files[i].data))
+      return fail(ctx, PAR2HIP_EINVAL, "create: every file needs a name and its bytes");
+  if (block_size && (block_size < 2048 || block_size % 4))  // with_block_size (creator.rs:185-211)
+    return fail(ctx, PAR2HIP_EINVAL, "create: block size must be at least 2048 and a multiple of 4");
+
+  // ---- file digests and File IDs (creator.rs:80-135, writer.rs:56-63), threads across files
+  std::vector<Input> in(nfiles);
+  uint64_t total = 0;
+  for (size_t i = 0; i < nfiles; i++) {
+    in[i].f = &files[i];
+    total += files[i].len;
+  }
+  {
+    const size_t nt = std::max<size_t>(1, std::min<size_t>(nfiles, std::thread::hardware_concurrency()));
+    std::vector<std::thread> th;
+    for (size_t t = 0; t < nt; t++)
+      th.emplace_back([&, t] {
+        for (size_t i = t; i < nfiles; i += nt) {
+          Input &x = in[i];
+          const uint8_t *d = static_cast<const uint8_t *>(x.f->data);
+          Md5Host::digest(d, x.f->len, x.full);
+          Md5Host::digest(d, std::min<uint64_t>(x.f->len, 16384), x.h16k);
+          Md5Host h;
+          h.update(x.h16k, 16);
+          uint8_t len8[8];
+          for (int k = 0; k < 8; k++) len8[k] = (uint8_t)(x.f->len >> (8 * k));
+          h.update(len8, 8);
+          h.update(x.f->name, strlen(x.f->name));
+          h.finish(x.id);
+        }
+      });
+    for (auto &t : th) t.join();
+  }
+  std::stable_sort(in.begin(), in.end(), id_less);
+
+  // ---- block and recovery counts (creator.rs:291-347)
+  if (block_size == 0) block_size = auto_block_size(total);
+  uint64_t n = 0;
+  for (Input &x : in) {
+    x.first = n;
+    x.blocks = (x.f->len + block_size - 1) / block_size;
+    n += x.blocks;
+  }
+  const uint64_t recovery_bytes = (uint64_t)((double)(n * block_size) * ((double)redundancy / 100.0));
+  const uint64_t r = (recovery_bytes + block_size - 1) / block_size;
+  if (n > 32768) return fail(ctx, PAR2HIP_EINVAL, "create: more than 32768 input blocks repeat PAR2 constants");
+  if (r > 65535 || n + r > 65535) return fail(ctx, PAR2HIP_EINVAL, "create: data + recovery blocks exceed 65535");
+  if (scheme == PAR2HIP_VOLUMES_SINGLE && r == 0)  // creator.rs:455-457 needs one volume
+    return fail(ctx, PAR2HIP_EINVAL, "create: the single-file scheme needs at least one recovery block");
+
+  // ---- output files and their exponent ranges (volumes.rs:90-208, creator.rs:453-469)
+  struct Vol {
+    std::string path;
+    uint64_t start, end;  // [start, end)
+  };
+  std::vector<Vol> vols;
+  vols.push_back({std::string(stem) + ".par2", 0, scheme == PAR2HIP_VOLUMES_SINGLE ? r : 0});
+  if (scheme == PAR2HIP_VOLUMES_EXPONENTIAL) {
+    const int w = digits(r ? r - 1 : 0);
+    for (uint64_t e = 0, want = 1; e < r; want = std::min<uint64_t>(want * 2, 1024)) {
+      const uint64_t cnt = std::min(want, r - e);
+      char name[64];
+      snprintf(name, sizeof name, ".vol%0*llu+%0*llu.par2", w, (unsigned long long)e, w, (unsigned long long)cnt);
+      vols.push_back({std::string(stem) + name, e, e + cnt});
+      e += cnt;
+    }
+  }
+  *nouts = vols.size();
+  if (vols.size() > max_outs) return fail(ctx, PAR2HIP_EINVAL, "create: output array too small");
+
+  // ---- Main packet body and recovery set ID (writer.rs:137-173, :66-71)
+  Bytes mainb;
+  mainb.u64(block_size);
+  mainb.u32((uint32_t)nfiles);
+  for (const Input &x : in) mainb.put(x.id, 16);
+  uint8_t set_id[16];
+  Md5Host::digest(mainb.v.data(), mainb.v.size(), set_id);
+
+  // ---- device: blocks in canonical order, IFSC hashes, recovery packet image
+  const uint64_t row = 36 + block_size;  // set ID, type, exponent, slice
+  std::vector<uint8_t> md5s(n * 16), image(r * row), pkt_md5(r * 16);
+  std::vector<uint32_t> crcs(n);
+  if (n) {
+    DevMem d_blocks, d_hash, d_img, d_pkt;
+    P2H_CHECK(ctx, hipMalloc(&d_blocks.p, n * block_size));
+    P2H_CHECK(ctx, hipMalloc(&d_hash.p, n * 20));
+    P2H_CHECK(ctx, hipMemsetAsync(d_blocks.p, 0, n * block_size, ctx->stream));
+    for (const Input &x : in)
+      if (x.f->len)
+        P2H_CHECK(ctx, hipMemcpyAsync((uint8_t *)d_blocks.p + x.first * block_size, x.f->data, x.f->len,
+                                      hipMemcpyHostToDevice, ctx->stream));
+    if (r) {
+      P2H_CHECK(ctx, hipMalloc(&d_img.p, r * row));
+      P2H_CHECK(ctx, hipMalloc(&d_pkt.p, r * 16));
+      std::vector<uint8_t> prefix(r * 36);
+      for (uint64_t e = 0; e < r; e++) {
+        memcpy(&prefix[e * 36], set_id, 16);
+        memcpy(&prefix[e * 36 + 16], kRecvSlic, 16);
+        for (int k = 0; k < 4; k++) prefix[e * 36 + 32 + k] = (uint8_t)(e >> (8 * k));
+      }
+      P2H_CHECK(ctx, hipMemcpy2DAsync(d_img.p, row, prefix.data(), 36, 36, r, hipMemcpyHostToDevice, ctx->stream));
+      // the prefix buffer must stay alive until the copy has run
+      P2H_CHECK(ctx, hipStreamSynchronize(ctx->stream));
+    }
+    std::vector<uint32_t> exps(r);
+    for (uint64_t e = 0; e < r; e++) exps[e] = (uint32_t)e;
+    int32_t rc = encode_dev_impl(ctx, (const uint8_t *)d_blocks.p, n, block_size, exps.data(), r,
+                                 r ? (uint8_t *)d_img.p + 36 : nullptr, (uint32_t *)d_hash.p,
+                                 (uint32_t *)((uint8_t *)d_hash.p + n * 16), row);
+    if (rc) return rc;
+    if (r && (rc = launch_md5(ctx, ctx->stream, (const uint8_t *)d_img.p, r, row, row, nullptr,
+                              (uint32_t *)d_pkt.p, false)))
+      return rc;
+    P2H_CHECK(ctx, hipMemcpyAsync(md5s.data(), d_hash.p, n * 16, hipMemcpyDeviceToHost, ctx->stream));
+    P2H_CHECK(ctx, hipMemcpyAsync(crcs.data(), (uint8_t *)d_hash.p + n * 16, n * 4, hipMemcpyDeviceToHost,
+                                  ctx->stream));
+    if (r) {
+      P2H_CHECK(ctx, hipMemcpyAsync(image.data(), d_img.p, r * row, hipMemcpyDeviceToHost, ctx->stream));
+      P2H_CHECK(ctx, hipMemcpyAsync(pkt_md5.data(), d_pkt.p, r * 16, hipMemcpyDeviceToHost, ctx->stream));
+    }
+    P2H_CHECK(ctx, hipStreamSynchronize(ctx->stream));
+  } else if (r) {
+    // no data blocks: every recovery slice is zero
+    for (uint64_t e = 0; e < r; e++) {
+      uint8_t *p = &image[e * row];
+      memcpy(p, set_id, 16);
+      memcpy(p + 16, kRecvSlic, 16);
+      for (int k = 0; k < 4; k++) p[32 + k] = (uint8_t)(e >> (8 * k));
+      Md5Host::digest(p, row, &pkt_md5[e * 16]);
+    }
+  }
+
+  // ---- the packets every file starts with (creator.rs:483-516)
+  Bytes head;
+  append_packet(head, kCreator, set_id, std::vector<uint8_t>(kCreatorText, kCreatorText + strlen(kCreatorText)));
+  append_packet(head, kMain, set_id, mainb.v);
+  for (const Input &x : in) {
+    Bytes b;
+    b.put(x.id, 16);
+    b.put(x.full, 16);
+    b.put(x.h16k, 16);
+    b.u64(x.f->len);
+    b.put(x.f->name, strlen(x.f->name) + 1);
+    append_packet(head, kFileDesc, set_id, b.v);
+  }
+  for (const Input &x : in) {
+    Bytes b;
+    b.put(x.id, 16);
+    for (uint64_t k = 0; k < x.blocks; k++) {
+      b.put(&md5s[(x.first + k) * 16], 16);
+      b.u32(crcs[x.first + k]);
+    }
+    append_packet(head, kIfsc, set_id, b.v);
+  }
+
+  // ---- write the files: head, then the volume's recovery packets
+  for (size_t i = 0; i < vols.size(); i++) {
+    const Vol &v = vols[i];
+    FILE *f = fopen(v.path.c_str(), "wb");
+    if (!f) return fail(ctx, PAR2HIP_EINVAL, "create: cannot write " + v.path);
+    bool ok = fwrite(head.v.data(), 1, head.v.size(), f) == head.v.size();
+    uint64_t bytes = head.v.size();
+    for (uint64_t e = v.start; e < v.end && ok; e++) {
+      Bytes hdr;
+      hdr.put(kMagic, 8);
+      hdr.u64(64 + 4 + block_size);
+      hdr.put(&pkt_md5[e * 16], 16);
+      ok = fwrite(hdr.v.data(), 1, hdr.v.size(), f) == hdr.v.size() &&
+           fwrite(&image[e * row], 1, row, f) == row;
+      bytes += hdr.v.size() + row;
+    }
+    ok = (fclose(f) == 0) && ok;
+    if (!ok) return fail(ctx, PAR2HIP_EINVAL, "create: write failed for " + v.path);
+    snprintf(outs[i].path, sizeof outs[i].path, "%s", v.path.c_str());
+    outs[i].bytes = bytes;
+  }
+  return PAR2HIP_OK;
+}
+
+}  // namespace par2hip
+
+extern "C" int32_t par2hip_create_par2(par2hip_ctx *ctx, const par2hip_input_file *files, size_t nfiles,
+                                       uint64_t block_size, float redundancy_percent, int32_t scheme,
+                                       const char *out_stem, par2hip_output_file *outs, size_t max_outs,
+                                       size_t *nouts) {
+  if (!ctx || !nouts || (max_outs && !outs) || (nfiles && !files)) return PAR2HIP_EINVAL;
+  std::lock_guard<std::recursive_mutex> lk(ctx->mu);
+  (void)hipSetDevice(ctx->device);
+  return par2hip::create_par2(ctx, files, nfiles, block_size, redundancy_percent, scheme, out_stem, outs, max_outs,
+                              nouts);
+}

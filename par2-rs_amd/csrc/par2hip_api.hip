@@ -212,10 +212,12 @@ int align_words_of(uintptr_t p) {
   return a;
 }
 
-// Common encode body: device buffers in, queued on ctx->stream.
-int32_t encode_dev_impl(par2hip_ctx *ctx, const uint8_t *d_slices, size_t n, size_t slice_size,
-                        const uint32_t *exps, size_t r, uint8_t *d_rec, uint32_t *d_md5,
-                        uint32_t *d_crc) {
+// Common encode body: device buffers in, queued on ctx->stream.  Recovery
+// row i goes to d_rec + i * rec_stride (rec_stride 0: slice_size).
+int32_t encode_body(par2hip_ctx *ctx, const uint8_t *d_slices, size_t n, size_t slice_size,
+                    const uint32_t *exps, size_t r, uint8_t *d_rec, uint32_t *d_md5,
+                    uint32_t *d_crc, size_t rec_stride) {
+  if (rec_stride == 0) rec_stride = slice_size;
   if (n == 0 || slice_size == 0 || (slice_size & 1))
     return fail(ctx, PAR2HIP_EINVAL, "need n >= 1 and an even, non-zero slice size");
   if (n > 32768) return fail(ctx, PAR2HIP_EINVAL, "more than 32768 input slices repeat PAR2 constants");
@@ -234,7 +236,7 @@ int32_t encode_dev_impl(par2hip_ctx *ctx, const uint8_t *d_slices, size_t n, siz
   memcpy(h_logs, logs.data(), n * 4);
   for (size_t i = 0; i < r; i++) h_exps[i] = exps[i] % kOrder;  // c^e = c^(e mod 65535)
   for (size_t d = 0; d < n; d++) h_srcs[d] = (const uint16_t *)(d_slices + d * slice_size);
-  for (size_t i = 0; i < r; i++) h_dsts[i] = (uint16_t *)(d_rec + i * slice_size);
+  for (size_t i = 0; i < r; i++) h_dsts[i] = (uint16_t *)(d_rec + i * rec_stride);
   if ((rc = params_commit(ctx))) return rc;
 
   // MD5 is a long serial chain per slice: it runs on the side stream's CUs
@@ -261,7 +263,7 @@ int32_t encode_dev_impl(par2hip_ctx *ctx, const uint8_t *d_slices, size_t n, siz
     p.words = words;
     p.accumulate = false;
     p.align_words = std::min({align_words_of((uintptr_t)d_slices), align_words_of((uintptr_t)d_rec),
-                              align_words_of(slice_size)});
+                              align_words_of(slice_size), align_words_of(rec_stride)});
     if ((rc = run_gf(ctx, ctx->cur, p))) return rc;
   }
   return PAR2HIP_OK;
@@ -468,6 +470,11 @@ namespace par2hip {
 int32_t reconstruct_single(par2hip_ctx *ctx, uint8_t *d_slices, size_t n, size_t slice_size, const uint32_t *missing,
                            size_t m, const uint8_t *d_recovery, const uint32_t *rec_exps, uint8_t *d_out) {
   return reconstruct_dev_impl(ctx, d_slices, n, slice_size, missing, m, d_recovery, rec_exps, d_out, false);
+}
+int32_t encode_dev_impl(par2hip_ctx *ctx, const uint8_t *d_slices, size_t n, size_t slice_size,
+                        const uint32_t *exps, size_t r, uint8_t *d_rec, uint32_t *d_md5, uint32_t *d_crc,
+                        size_t rec_stride) {
+  return encode_body(ctx, d_slices, n, slice_size, exps, r, d_rec, d_md5, d_crc, rec_stride);
 }
 int32_t fork_overlap(par2hip_ctx *ctx) { return fork_side(ctx); }
 int32_t join_overlap(par2hip_ctx *ctx) { return join_side(ctx); }
@@ -735,8 +742,8 @@ int32_t par2hip_encode_dev(par2hip_ctx *ctx, const void *d_slices, size_t n, siz
                            const uint32_t *exps, size_t r, void *d_recovery, void *d_md5, void *d_crc) {
   if (!ctx || !d_slices || (r && !d_recovery)) return PAR2HIP_EINVAL;
   Guard g(ctx);
-  return encode_dev_impl(ctx, (const uint8_t *)d_slices, n, slice_size, exps, r, (uint8_t *)d_recovery,
-                         (uint32_t *)d_md5, (uint32_t *)d_crc);
+  return encode_body(ctx, (const uint8_t *)d_slices, n, slice_size, exps, r, (uint8_t *)d_recovery,
+                     (uint32_t *)d_md5, (uint32_t *)d_crc, 0);
 }
 
 int32_t par2hip_encode(par2hip_ctx *ctx, const uint8_t *slices, size_t n, size_t slice_size,
@@ -754,7 +761,7 @@ int32_t par2hip_encode(par2hip_ctx *ctx, const uint8_t *slices, size_t n, size_t
   P2H_CHECK(ctx, hipMemcpyAsync(din, slices, n * slice_size, hipMemcpyHostToDevice, ctx->stream));
   uint32_t *dmd5 = md5_out ? (uint32_t *)dh : nullptr;
   uint32_t *dcrc = crc_out ? (uint32_t *)dh + 4 * n : nullptr;
-  if ((rc = encode_dev_impl(ctx, (const uint8_t *)din, n, slice_size, exps, r, (uint8_t *)dout, dmd5, dcrc)))
+  if ((rc = encode_body(ctx, (const uint8_t *)din, n, slice_size, exps, r, (uint8_t *)dout, dmd5, dcrc, 0)))
     return rc;
   if (r) P2H_CHECK(ctx, hipMemcpyAsync(recovery, dout, r * slice_size, hipMemcpyDeviceToHost, ctx->stream));
   if (md5_out) P2H_CHECK(ctx, hipMemcpyAsync(md5_out, dmd5, n * 16, hipMemcpyDeviceToHost, ctx->stream));

@@ -81,6 +81,17 @@ class ServerStats(C.Structure):
                                            "charged", "peak_charged", "pending", "admitted", "arena_fallbacks")]
 
 
+class InputFile(C.Structure):
+    _fields_ = [("name", C.c_char_p), ("data", C.c_void_p), ("len", C.c_uint64)]
+
+
+class OutputFile(C.Structure):
+    _fields_ = [("path", C.c_char * 1024), ("bytes", C.c_uint64)]
+
+
+VOLUMES_SINGLE, VOLUMES_EXPONENTIAL = 0, 1
+
+
 class DeviceInfo(C.Structure):
     _fields_ = [("compute_units", C.c_int32), ("hash_cus", C.c_int32), ("arch", C.c_char * 32)]
 
@@ -135,6 +146,7 @@ def lib():
                 "par2hip_set_gf_kernel": ([P, I32], I32),
                 "par2hip_profile_enable": ([P, I32], I32),
                 "par2hip_profile_read": ([P, P, S, C.POINTER(S)], I32),
+                "par2hip_create_par2": ([P, P, S, C.c_uint64, C.c_float, I32, C.c_char_p, P, S, C.POINTER(S)], I32),
                 "par2hip_set_autotune": ([P, I32], I32),
                 "par2hip_force_tile_shape": ([P, I32, I32, I32], I32),
                 "par2hip_autotune_save": ([C.c_char_p], I32),
@@ -404,6 +416,30 @@ class Context:
         if rc < 0:
             self._check(rc)
         return rc
+
+    def create_par2(self, files, out_stem: str, block_size: int = 0, redundancy: float = 5.0,
+                    scheme: int = VOLUMES_EXPONENTIAL) -> list:
+        """Par2Creator::create: files [(name, bytes)] -> written [(path, size)],
+        the main .par2 first (par2hip_create_par2)."""
+        arr = (InputFile * len(files))()
+        keep = []
+        for i, (name, data) in enumerate(files):
+            b = np.frombuffer(data, np.uint8) if len(data) else np.zeros(1, np.uint8)
+            keep.append((name.encode(), b))
+            arr[i].name = keep[-1][0]
+            arr[i].data = b.ctypes.data
+            arr[i].len = len(data)
+        cap = 64
+        while True:
+            outs = (OutputFile * cap)()
+            n = C.c_size_t(0)
+            rc = lib().par2hip_create_par2(self.handle, arr, len(files), block_size, redundancy, scheme,
+                                           out_stem.encode(), outs, cap, C.byref(n))
+            if rc == EINVAL and n.value > cap:
+                cap = n.value
+                continue
+            self._check(rc)
+            return [(outs[i].path.decode(), int(outs[i].bytes)) for i in range(n.value)]
 
     def copy_dev(self, dst: DeviceBuffer, src: DeviceBuffer, nbytes: int) -> None:
         """Device-to-device copy on the ctx stream."""
