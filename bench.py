@@ -563,12 +563,101 @@ def bench_served(args, ctx):
     }
 
 
+def bench_create(args, ctx):
+    """End-to-end PAR2 creation (Par2Creator::create, creator.rs:261-531):
+    16 files of 64 MiB (1 GiB) from host memory to written .par2 files, the
+    reference's automatic block size, 10% redundancy, exponential volumes.
+    One step = one par2hip_create_par2 call (file digests on host threads
+    overlapped with H2D, IFSC hashes, recovery slices and recovery packet
+    digests on the device, file writes)."""
+    import tempfile
+
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import oracle
+    import par2fmt
+
+    nfiles, fbytes = 16, 64 << 20
+    pool = random_bytes(31, nfiles * fbytes)
+    files = [(f"part{i:02d}.bin", pool[i * fbytes:(i + 1) * fbytes].tobytes()) for i in range(nfiles)]
+    out_dir = tempfile.mkdtemp(prefix="par2create_")
+    stem = os.path.join(out_dir, "set")
+    # exactness: the C1 configuration against the oracle's creator, byte for byte
+    c1 = random_bytes(41, 4 << 20).tobytes()
+    got = ctx.create_par2([("c1.bin", c1)], os.path.join(out_dir, "c1"), block_size=65536, redundancy=5.0)
+    want = oracle.par2_create([("c1.bin", c1)], block_size=65536, redundancy=5.0, stem=os.path.join(out_dir, "c1"))
+    exact = [p for p, _ in got] == [n for n, _ in want] and all(open(p, "rb").read() == b for (p, _), (_, b) in
+                                                                 zip(got, want))
+    written = ctx.create_par2(files, stem, block_size=0, redundancy=10.0)  # warm-up
+    lat = []
+    for _ in range(args.steps):
+        t0 = time.perf_counter()
+        written = ctx.create_par2(files, stem, block_size=0, redundancy=10.0)
+        lat.append(time.perf_counter() - t0)
+    # structure and a column window of the recovery slices of the 1 GiB set
+    st = par2fmt.load([p for p, _ in written])
+    bs = st.slice_size
+    blocks_per_file = -(-fbytes // bs)
+    order = [st.files[f].name for f in st.file_ids]
+    exact &= st.bad_packets == 0 and sorted(order) == sorted(n for n, _ in files)
+    byname = dict(files)
+    import hashlib
+    import zlib
+    for fid in st.file_ids[:2]:
+        d = st.files[fid]
+        exact &= d.md5 == hashlib.md5(byname[d.name]).digest()
+        blk = byname[d.name][:bs]
+        exact &= st.ifsc[fid][0] == (hashlib.md5(blk).digest(), zlib.crc32(blk))
+    w0, nw = 1024, 32
+    cols = np.stack([np.frombuffer(byname[name][k * bs:k * bs + bs].ljust(bs, b"\0"), np.uint16)[w0:w0 + nw]
+                     for name in order for k in range(blocks_per_file)])
+    r = len(st.recovery)
+    ref = oracle.encode(np.ascontiguousarray(cols), np.arange(r))
+    exact &= all(np.array_equal(np.frombuffer(st.recovery[e], np.uint16)[w0:w0 + nw], ref[e]) for e in range(r))
+    total_out = sum(sz for _, sz in written)
+    n_blocks = nfiles * blocks_per_file
+    for p, _ in written + got:
+        os.remove(p)
+    os.rmdir(out_dir)
+    src = nfiles * fbytes
+    res = {
+        "workload": "create",
+        "config": {"workload": "create_16x64MiB", "files": nfiles, "file_bytes": fbytes, "block_size": bs,
+                   "input_slices": n_blocks, "recovery_slices": r, "redundancy_percent": 10.0,
+                   "volumes": "exponential", "output_bytes": total_out, "source": "host memory -> .par2 files"},
+        "src_bytes": src, "total_s": float(np.sum(lat)), "lat": lat, "exact": bool(exact), "kernel_ms": {},
+    }
+    if not args.no_cpu:
+        # the reference's method on the host: AVX2 encode with the
+        # StreamingEncoder batching, IFSC hashing and the whole-file MD5s,
+        # all multithreaded; packet assembly and writes not included
+        threads = min(16, os.cpu_count() or 1)
+        data = np.concatenate([np.frombuffer(b, np.uint8) for _, b in files])
+        nb = n_blocks
+        padded = np.zeros(nb * bs, np.uint8)
+        for i in range(nfiles):
+            padded[i * blocks_per_file * bs:i * blocks_per_file * bs + fbytes] = data[i * fbytes:(i + 1) * fbytes]
+        from concurrent.futures import ThreadPoolExecutor
+        t0 = time.perf_counter()
+        with ThreadPoolExecutor(threads) as ex:
+            digests = ex.map(lambda f: hashlib.md5(f[1]).digest(), files)
+            oracle.encode_avx2(padded.reshape(nb, bs).view(np.uint16), np.arange(r), threads=threads)
+            oracle.hash_slices_mt(padded.reshape(nb, bs), nb, bs, threads=threads)
+            list(digests)
+        dt = time.perf_counter() - t0
+        res["cpu_baseline"] = {"value": round(src / dt / 1e9, 4), "unit": "GB/s", "cores": threads, "kind": "port",
+                               "sample": f"the same 1 GiB set: AVX2 encode of {nb} x {bs} B into {r} recovery "
+                                         f"slices + IFSC MD5/CRC32 + whole-file MD5s, {dt:.2f} s wall"}
+    return res
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--workload", choices=["encode", "repair", "micro", "batched", "served", "copy"], default="encode")
+    ap.add_argument("--workload", choices=["encode", "repair", "micro", "batched", "served", "copy", "create"],
+                    default="encode")
     ap.add_argument("--hbm-budget", type=int, default=200 * 10**9, help="served: server HBM budget in bytes")
     ap.add_argument("--lanes", type=int, default=2, help="served: concurrent batches")
     ap.add_argument("--max-batch", type=int, default=512, help="served: requests per batch at most")
@@ -633,6 +722,8 @@ def main():
         return
     if args.workload == "batched":
         res = bench_batched(args, ctx)
+    elif args.workload == "create":
+        res = bench_create(args, ctx)
     else:
         res = bench_encode(args, ctx) if args.workload == "encode" else bench_repair(args, ctx)
     lat = np.array(res["lat"]) * 1e3
@@ -656,6 +747,8 @@ def main():
     if "plain_ms" in res:
         out["plain_path"] = {"ms_per_batch": round(res["plain_ms"], 3), "value": round(res["plain_gbs"], 3),
                              "unit": "GB/s", "speedup_batched": round(value / res["plain_gbs"], 3)}
+    if "cpu_baseline" in res:
+        out["cpu_baseline"] = res["cpu_baseline"]
     if not args.no_cpu and "cpu" in res:
         threads = min(16, os.cpu_count() or 1)
         n, ss, r = res["cpu"]

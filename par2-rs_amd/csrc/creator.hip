@@ -124,33 +124,24 @@ int32_t create_par2(par2hip_ctx *ctx, const par2hip_input_file *files, size_t nf
   if (block_size && (block_size < 2048 || block_size % 4))  // with_block_size (creator.rs:185-211)
     return fail(ctx, PAR2HIP_EINVAL, "create: block size must be at least 2048 and a multiple of 4");
 
-  // ---- file digests and File IDs (creator.rs:80-135, writer.rs:56-63), threads across files
+  // ---- File IDs (writer.rs:56-63): they need only the first 16 KiB of each
+  // file, and fix the block order, the Main packet and the set ID; the
+  // whole-file MD5s (creator.rs:80-135, one serial chain each) run on host
+  // threads while the device works and are needed only for FileDesc.
   std::vector<Input> in(nfiles);
   uint64_t total = 0;
   for (size_t i = 0; i < nfiles; i++) {
-    in[i].f = &files[i];
+    Input &x = in[i];
+    x.f = &files[i];
     total += files[i].len;
-  }
-  {
-    const size_t nt = std::max<size_t>(1, std::min<size_t>(nfiles, std::thread::hardware_concurrency()));
-    std::vector<std::thread> th;
-    for (size_t t = 0; t < nt; t++)
-      th.emplace_back([&, t] {
-        for (size_t i = t; i < nfiles; i += nt) {
-          Input &x = in[i];
-          const uint8_t *d = static_cast<const uint8_t *>(x.f->data);
-          Md5Host::digest(d, x.f->len, x.full);
-          Md5Host::digest(d, std::min<uint64_t>(x.f->len, 16384), x.h16k);
-          Md5Host h;
-          h.update(x.h16k, 16);
-          uint8_t len8[8];
-          for (int k = 0; k < 8; k++) len8[k] = (uint8_t)(x.f->len >> (8 * k));
-          h.update(len8, 8);
-          h.update(x.f->name, strlen(x.f->name));
-          h.finish(x.id);
-        }
-      });
-    for (auto &t : th) t.join();
+    Md5Host::digest(x.f->data, std::min<uint64_t>(x.f->len, 16384), x.h16k);
+    Md5Host h;
+    h.update(x.h16k, 16);
+    uint8_t len8[8];
+    for (int k = 0; k < 8; k++) len8[k] = (uint8_t)(x.f->len >> (8 * k));
+    h.update(len8, 8);
+    h.update(x.f->name, strlen(x.f->name));
+    h.finish(x.id);
   }
   std::stable_sort(in.begin(), in.end(), id_less);
 
@@ -188,6 +179,23 @@ int32_t create_par2(par2hip_ctx *ctx, const par2hip_input_file *files, size_t nf
   }
   *nouts = vols.size();
   if (vols.size() > max_outs) return fail(ctx, PAR2HIP_EINVAL, "create: output array too small");
+
+  // whole-file digests, in the background (threads across files)
+  std::vector<std::thread> hashers;
+  {
+    const size_t nt = std::max<size_t>(1, std::min<size_t>(nfiles, std::thread::hardware_concurrency()));
+    for (size_t t = 0; t < nt; t++)
+      hashers.emplace_back([&in, nt, t] {
+        for (size_t i = t; i < in.size(); i += nt) Md5Host::digest(in[i].f->data, in[i].f->len, in[i].full);
+      });
+  }
+  struct Joiner {
+    std::vector<std::thread> &th;
+    ~Joiner() {
+      for (auto &t : th)
+        if (t.joinable()) t.join();
+    }
+  } joiner{hashers};
 
   // ---- Main packet body and recovery set ID (writer.rs:137-173, :66-71)
   Bytes mainb;
@@ -250,6 +258,8 @@ int32_t create_par2(par2hip_ctx *ctx, const par2hip_input_file *files, size_t nf
       Md5Host::digest(p, row, &pkt_md5[e * 16]);
     }
   }
+
+  for (auto &t : hashers) t.join();
 
   // ---- the packets every file starts with (creator.rs:483-516)
   Bytes head;
