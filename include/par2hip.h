@@ -45,7 +45,9 @@ enum {
     PAR2HIP_ENOMEM = -3,    /* device allocation failed */
     PAR2HIP_ESINGULAR = -4, /* erasure matrix singular ("Singular matrix", par2_rs.rs:369) */
     PAR2HIP_EHIP = -5,      /* HIP runtime error; see par2hip_last_error */
-    PAR2HIP_ERECOVERY = -6  /* fewer recovery slices than erasures (par2_rs.rs:332-338) */
+    PAR2HIP_ERECOVERY = -6, /* fewer recovery slices than erasures (par2_rs.rs:332-338) */
+    PAR2HIP_EREPAIR = -7    /* rebuilt files failed verification; nothing was committed
+                               (RepairFailed, src/repair.rs:1050-1080) */
 };
 
 /* Strategy priority, same scale as `Priority` (src/galois/simd/mod.rs:15-21). */
@@ -291,6 +293,38 @@ int32_t par2hip_create_par2(par2hip_ctx *ctx, const par2hip_input_file *files, s
                             uint64_t block_size, float redundancy_percent, int32_t scheme,
                             const char *out_stem, par2hip_output_file *outs, size_t max_outs,
                             size_t *nouts);
+
+/* What a .par2 index and the volumes beside it hold (Par2Info::load,
+ * src/info.rs; parser src/parser.rs:181-547).  Needs no device. */
+typedef struct {
+    uint64_t block_size;
+    uint64_t blocks;            /* input blocks of the described files */
+    uint32_t files;             /* described files */
+    uint32_t recovery_slices;   /* distinct recovery exponents found */
+    uint32_t min_exponent, max_exponent;
+    uint8_t set_id[16];
+} par2hip_set_info;
+int32_t par2hip_par2_info(const char *par2_path, par2hip_set_info *info);
+
+/* Par2Repairer: load par2_path and every .par2 beside it with the same
+ * recovery set ID (src/parser.rs:181-547), verify the protected files in
+ * base_dir (NULL: the index's directory) with IFSC block checks on the device
+ * (src/verify.rs:87-330; a missing file or a size mismatch makes all of its
+ * blocks damaged), rebuild the damaged blocks on the device from the lowest
+ * exponents, write each repaired file to <file>.par2tmp seeded with the
+ * original, verify every temp on the device and rename them over the
+ * originals only when all verify (src/repair.rs:445-1125).  verify_only: stop
+ * after verification.  Errors: PAR2HIP_ERECOVERY (nothing written),
+ * PAR2HIP_ESINGULAR, PAR2HIP_EREPAIR (temps removed, originals untouched). */
+typedef struct {
+    uint64_t block_size, blocks, blocks_damaged;
+    uint32_t files, files_damaged, files_missing;
+    uint32_t recovery_slices;
+    int32_t repaired;           /* 1 when damaged files were rewritten */
+    int32_t reserved;
+} par2hip_repair_report;
+int32_t par2hip_repair_files(par2hip_ctx *ctx, const char *par2_path, const char *base_dir,
+                             int32_t verify_only, par2hip_repair_report *report);
 
 /* -------------------------------------------------------------- hashes -- */
 

@@ -1,0 +1,544 @@
+// repairer.hip -- loading a PAR2 set and repairing its files over the GPU
+// entry points: what Par2Repairer does with the reference's parser, verifier
+// and repair driver, with the slice hashes (verify and pre-commit check) and
+// the rebuild on the device.
+//
+//   parse   src/parser.rs:181-547 (packets, resync, MD5 check of every packet
+//           but the recovery slices, which are located and read at repair
+//           time; volumes = every .par2 beside the index with the same
+//           recovery set ID), canonical order src/parser.rs:556-596
+//   verify  src/verify.rs:87-330, :662-760 (size first; IFSC block checks, or
+//           the 16 KiB and whole-file MD5 without IFSC; a missing file or a
+//           size mismatch makes every block of the file damaged)
+//   repair  src/repair.rs:445-1125 (lowest exponents first, <final>.par2tmp
+//           seeded with the original, rebuilt blocks written, every temp
+//           verified, renamed only when all verify, temps removed otherwise)
+#include <dirent.h>
+#include <sys/stat.h>
+#include <unistd.h>
+
+#include <algorithm>
+#include <array>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <string>
+#include <vector>
+
+#include "common.hpp"
+#include "md5_host.hpp"
+
+namespace par2hip {
+namespace {
+
+const uint8_t kMagic[8] = {'P', 'A', 'R', '2', 0, 'P', 'K', 'T'};
+const uint8_t kMain[16] = {'P', 'A', 'R', ' ', '2', '.', '0', 0, 'M', 'a', 'i', 'n', 0, 0, 0, 0};
+const uint8_t kFileDesc[16] = {'P', 'A', 'R', ' ', '2', '.', '0', 0, 'F', 'i', 'l', 'e', 'D', 'e', 's', 'c'};
+const uint8_t kIfsc[16] = {'P', 'A', 'R', ' ', '2', '.', '0', 0, 'I', 'F', 'S', 'C', 0, 0, 0, 0};
+const uint8_t kRecvSlic[16] = {'P', 'A', 'R', ' ', '2', '.', '0', 0, 'R', 'e', 'c', 'v', 'S', 'l', 'i', 'c'};
+const char kTmpSuffix[] = ".par2tmp";  // src/repair.rs:63-67
+
+using Id = std::array<uint8_t, 16>;
+
+uint64_t le64(const uint8_t *p) {
+  uint64_t v = 0;
+  for (int i = 7; i >= 0; i--) v = v << 8 | p[i];
+  return v;
+}
+uint32_t le32(const uint8_t *p) { return (uint32_t)p[0] | p[1] << 8 | p[2] << 16 | (uint32_t)p[3] << 24; }
+Id id_at(const uint8_t *p) {
+  Id x;
+  memcpy(x.data(), p, 16);
+  return x;
+}
+
+struct FileRec {
+  Id id, md5, md5k;
+  uint64_t length = 0;
+  std::string name;
+};
+struct RecSlice {
+  uint32_t exponent;
+  std::string path;
+  uint64_t data_offset;  // of the recovery bytes (after the exponent)
+};
+struct Par2Set {
+  bool have_set_id = false;
+  Id set_id{};
+  uint64_t block_size = 0;
+  std::vector<Id> main_order;
+  std::vector<FileRec> files_in_order;  // FileDesc order of first appearance
+  std::map<Id, FileRec> files;
+  std::map<Id, std::vector<std::pair<Id, uint32_t>>> ifsc;  // md5, crc32 per slice
+  std::vector<RecSlice> recs;
+};
+
+bool file_size(const std::string &p, uint64_t *sz) {
+  struct stat st;
+  if (stat(p.c_str(), &st) != 0 || !S_ISREG(st.st_mode)) return false;
+  *sz = (uint64_t)st.st_size;
+  return true;
+}
+
+// One .par2 file (parser.rs:181-400).  recovery_only: a volume beside the
+// index (only its recovery slices are taken, parser.rs:443-520).
+int32_t parse_file(const std::string &path, Par2Set &s, bool recovery_only, std::string *err) {
+  FILE *f = fopen(path.c_str(), "rb");
+  if (!f) {
+    *err = "cannot open " + path;
+    return PAR2HIP_EINVAL;
+  }
+  uint64_t size = 0;
+  file_size(path, &size);
+  std::vector<std::pair<uint64_t, uint64_t>> deferred;  // recovery packets seen before the block size
+  std::vector<uint8_t> body;
+  uint64_t pos = 0;
+  uint8_t h[64];
+  while (pos < size) {
+    if (fseeko(f, (off_t)pos, SEEK_SET) != 0 || fread(h, 1, 64, f) < 64) break;
+    if (memcmp(h, kMagic, 8) != 0) {
+      pos++;
+      continue;
+    }
+    if (!s.have_set_id && !recovery_only) {
+      s.set_id = id_at(h + 32);
+      s.have_set_id = true;
+    }
+    const uint64_t len = le64(h + 8);
+    if (len < 64 || len > size - pos) {
+      pos++;
+      continue;
+    }
+    const uint64_t next = pos + len + ((len % 4) ? 4 - len % 4 : 0);
+    if (memcmp(h + 48, kRecvSlic, 16) == 0) {
+      if (len >= 68) {
+        if (s.block_size == 0) {
+          deferred.push_back({pos, len});
+        } else {
+          uint8_t e[4];
+          if (fread(e, 1, 4, f) == 4 && len >= 68 + s.block_size) s.recs.push_back({le32(e), path, pos + 68});
+        }
+      }
+      pos = next;
+      continue;
+    }
+    body.resize(len - 64);
+    if (fread(body.data(), 1, body.size(), f) < body.size()) break;
+    Md5Host md;
+    md.update(h + 32, 32);
+    md.update(body.data(), body.size());
+    uint8_t digest[16];
+    md.finish(digest);
+    if (memcmp(digest, h + 16, 16) != 0) {  // a damaged packet is skipped
+      pos = next;
+      continue;
+    }
+    if (!recovery_only) {
+      if (memcmp(h + 48, kMain, 16) == 0 && body.size() >= 12) {
+        const uint64_t bs = le64(body.data());
+        const uint32_t nrec = le32(body.data() + 8);
+        if (body.size() >= 12 + 16ull * nrec) {
+          s.block_size = bs;
+          if (s.main_order.empty())
+            for (uint32_t k = 0; k < nrec; k++) s.main_order.push_back(id_at(body.data() + 12 + 16 * k));
+        }
+      } else if (memcmp(h + 48, kFileDesc, 16) == 0 && body.size() >= 56) {
+        FileRec r;
+        r.id = id_at(body.data());
+        r.md5 = id_at(body.data() + 16);
+        r.md5k = id_at(body.data() + 32);
+        r.length = le64(body.data() + 48);
+        size_t nl = 0;
+        while (56 + nl < body.size() && body[56 + nl]) nl++;
+        r.name.assign((const char *)body.data() + 56, nl);
+        if (s.files.emplace(r.id, r).second) s.files_in_order.push_back(r);
+      } else if (memcmp(h + 48, kIfsc, 16) == 0 && body.size() >= 16) {
+        std::vector<std::pair<Id, uint32_t>> ent;
+        for (size_t k = 16; k + 20 <= body.size(); k += 20) ent.push_back({id_at(body.data() + k), le32(&body[k + 16])});
+        s.ifsc[id_at(body.data())] = ent;
+      }
+    }
+    pos = next;
+  }
+  for (auto &d : deferred) {  // parser.rs:415-440
+    uint8_t e[4];
+    if (fseeko(f, (off_t)(d.first + 64), SEEK_SET) == 0 && fread(e, 1, 4, f) == 4 && d.second >= 68 + s.block_size)
+      s.recs.push_back({le32(e), path, d.first + 68});
+  }
+  fclose(f);
+  return PAR2HIP_OK;
+}
+
+// The recovery set ID of a .par2 file: that of its first packet.
+bool first_set_id(const std::string &path, Id *out) {
+  FILE *f = fopen(path.c_str(), "rb");
+  if (!f) return false;
+  uint8_t h[64];
+  uint64_t size = 0, pos = 0;
+  file_size(path, &size);
+  bool ok = false;
+  while (pos + 64 <= size) {
+    if (fseeko(f, (off_t)pos, SEEK_SET) != 0 || fread(h, 1, 64, f) < 64) break;
+    if (memcmp(h, kMagic, 8) == 0) {
+      *out = id_at(h + 32);
+      ok = true;
+      break;
+    }
+    pos++;
+  }
+  fclose(f);
+  return ok;
+}
+
+std::string dir_of(const std::string &p) {
+  const size_t k = p.find_last_of('/');
+  return k == std::string::npos ? "." : p.substr(0, k);
+}
+
+int32_t load_set(const std::string &index, Par2Set &s, std::string *err) {
+  int32_t rc = parse_file(index, s, false, err);
+  if (rc) return rc;
+  if (!s.have_set_id) {
+    *err = "no packet in " + index;
+    return PAR2HIP_EINVAL;
+  }
+  if (s.block_size == 0 || s.block_size % 4) {  // parser.rs:364-380
+    *err = "block size missing, zero or not a multiple of 4";
+    return PAR2HIP_EINVAL;
+  }
+  // the other .par2 files beside the index with the same set ID
+  std::vector<std::string> names;
+  {
+    const std::string dir = dir_of(index);
+    DIR *dp = opendir(dir.c_str());
+    if (dp) {
+      while (dirent *e = readdir(dp)) {
+        std::string n = e->d_name;
+        if (n.size() < 5) continue;
+        std::string ext = n.substr(n.size() - 5);
+        std::transform(ext.begin(), ext.end(), ext.begin(), ::tolower);
+        if (ext != ".par2") continue;
+        const std::string full = dir + "/" + n;
+        const size_t slash = index.find_last_of('/');
+        if (n == (slash == std::string::npos ? index : index.substr(slash + 1))) continue;  // the index itself
+        names.push_back(full);
+      }
+      closedir(dp);
+    }
+  }
+  std::sort(names.begin(), names.end());
+  for (const std::string &v : names) {
+    Id sid;
+    if (first_set_id(v, &sid) && sid == s.set_id) (void)parse_file(v, s, true, err);
+  }
+  return PAR2HIP_OK;
+}
+
+// Files in the canonical block order (parser.rs:566-596): the Main packet's
+// list first, then any other described file by File ID (byte 15 first).
+std::vector<FileRec> canonical_files(const Par2Set &s) {
+  std::vector<FileRec> out;
+  std::vector<Id> seen;
+  for (const Id &id : s.main_order) {
+    auto it = s.files.find(id);
+    if (it != s.files.end() && std::find(seen.begin(), seen.end(), id) == seen.end()) {
+      out.push_back(it->second);
+      seen.push_back(id);
+    }
+  }
+  std::vector<FileRec> rest;
+  for (const FileRec &r : s.files_in_order)
+    if (std::find(seen.begin(), seen.end(), r.id) == seen.end()) rest.push_back(r);
+  std::sort(rest.begin(), rest.end(), [](const FileRec &a, const FileRec &b) {
+    for (int i = 15; i >= 0; i--)
+      if (a.id[i] != b.id[i]) return a.id[i] < b.id[i];
+    return false;
+  });
+  out.insert(out.end(), rest.begin(), rest.end());
+  return out;
+}
+
+bool read_range(const std::string &path, uint64_t off, uint64_t n, uint8_t *dst) {
+  FILE *f = fopen(path.c_str(), "rb");
+  if (!f) return false;
+  const bool ok = fseeko(f, (off_t)off, SEEK_SET) == 0 && fread(dst, 1, n, f) == n;
+  fclose(f);
+  return ok;
+}
+
+struct DevMem {
+  void *p = nullptr;
+  ~DevMem() {
+    if (p) (void)hipFree(p);
+  }
+};
+
+
+// Sorted by exponent, one slice per exponent (the first found).
+std::vector<RecSlice> recovery_by_exponent(const Par2Set &s) {
+  std::vector<RecSlice> r = s.recs;
+  std::stable_sort(r.begin(), r.end(), [](const RecSlice &a, const RecSlice &b) { return a.exponent < b.exponent; });
+  r.erase(std::unique(r.begin(), r.end(), [](const RecSlice &a, const RecSlice &b) { return a.exponent == b.exponent; }),
+          r.end());
+  return r;
+}
+
+bool md5_file_prefix(const std::string &path, uint64_t n, uint8_t out[16]) {
+  FILE *f = fopen(path.c_str(), "rb");
+  if (!f) return false;
+  Md5Host h;
+  std::vector<uint8_t> buf(1 << 20);
+  uint64_t left = n;
+  bool ok = true;
+  while (left) {
+    const size_t want = (size_t)std::min<uint64_t>(left, buf.size());
+    if (fread(buf.data(), 1, want, f) != want) {
+      ok = false;
+      break;
+    }
+    h.update(buf.data(), want);
+    left -= want;
+  }
+  fclose(f);
+  h.finish(out);
+  return ok;
+}
+
+}  // namespace
+
+int32_t set_info(const char *index, par2hip_set_info *info) {
+  Par2Set s;
+  std::string err;
+  int32_t rc = load_set(index, s, &err);
+  if (rc) return rc;
+  memset(info, 0, sizeof *info);
+  info->block_size = s.block_size;
+  info->files = (uint32_t)s.files.size();
+  memcpy(info->set_id, s.set_id.data(), 16);
+  for (const FileRec &r : s.files_in_order) info->blocks += (r.length + s.block_size - 1) / s.block_size;
+  const std::vector<RecSlice> recs = recovery_by_exponent(s);
+  info->recovery_slices = (uint32_t)recs.size();
+  if (!recs.empty()) {
+    info->min_exponent = recs.front().exponent;
+    info->max_exponent = recs.back().exponent;
+  }
+  return PAR2HIP_OK;
+}
+
+int32_t repair_files(par2hip_ctx *ctx, const char *index, const char *base_dir, int verify_only,
+                     par2hip_repair_report *rep) {
+  memset(rep, 0, sizeof *rep);
+  Par2Set s;
+  std::string err;
+  int32_t rc = load_set(index, s, &err);
+  if (rc) return fail(ctx, rc, err);
+  const std::vector<FileRec> files = canonical_files(s);
+  const uint64_t bs = s.block_size;
+  const std::string base = base_dir && *base_dir ? base_dir : dir_of(index);
+
+  // ---- block layout of the files in canonical order (repair.rs:499-545)
+  std::vector<uint64_t> first(files.size()), nblk(files.size());
+  uint64_t n = 0;
+  for (size_t i = 0; i < files.size(); i++) {
+    first[i] = n;
+    nblk[i] = (files[i].length + bs - 1) / bs;
+    n += nblk[i];
+  }
+  const std::vector<RecSlice> recs = recovery_by_exponent(s);
+  rep->files = (uint32_t)files.size();
+  rep->blocks = n;
+  rep->block_size = bs;
+  rep->recovery_slices = (uint32_t)recs.size();
+  if (n > 32768) return fail(ctx, PAR2HIP_EINVAL, "more than 32768 input blocks repeat PAR2 constants");
+  if (n + recs.size() > 65535) return fail(ctx, PAR2HIP_EINVAL, "data + recovery blocks exceed 65535");
+  if (n == 0) return PAR2HIP_OK;
+
+  // ---- verify (verify.rs:87-330): every file's blocks on the device
+  enum { kOk, kDamaged, kMissing, kSize };
+  std::vector<int> status(files.size(), kOk);
+  std::vector<uint64_t> present_size(files.size(), 0);
+  DevMem d_slices, d_hash;
+  P2H_CHECK(ctx, hipMalloc(&d_slices.p, n * bs));
+  P2H_CHECK(ctx, hipMalloc(&d_hash.p, n * 20));
+  P2H_CHECK(ctx, hipMemsetAsync(d_slices.p, 0, n * bs, ctx->stream));
+  std::vector<uint8_t> host;
+  for (size_t i = 0; i < files.size(); i++) {
+    const std::string path = base + "/" + files[i].name;
+    if (!file_size(path, &present_size[i])) {
+      status[i] = kMissing;
+      continue;
+    }
+    if (present_size[i] != files[i].length) {  // verify_file_smart: size first
+      status[i] = kSize;
+      continue;
+    }
+    host.resize(files[i].length);
+    if (files[i].length && !read_range(path, 0, files[i].length, host.data()))
+      return fail(ctx, PAR2HIP_EINVAL, "cannot read " + path);
+    if (files[i].length) {
+      P2H_CHECK(ctx, hipMemcpyAsync((uint8_t *)d_slices.p + first[i] * bs, host.data(), files[i].length,
+                                    hipMemcpyHostToDevice, ctx->stream));
+      P2H_CHECK(ctx, hipStreamSynchronize(ctx->stream));
+    }
+  }
+  if ((rc = par2hip_hash_slices_dev(ctx, d_slices.p, n, bs, bs, nullptr, d_hash.p, (uint8_t *)d_hash.p + n * 16)))
+    return rc;
+  std::vector<uint8_t> md5s(n * 16);
+  std::vector<uint32_t> crcs(n);
+  P2H_CHECK(ctx, hipMemcpyAsync(md5s.data(), d_hash.p, n * 16, hipMemcpyDeviceToHost, ctx->stream));
+  P2H_CHECK(ctx, hipStreamSynchronize(ctx->stream));
+  P2H_CHECK(ctx, hipMemcpyAsync(crcs.data(), (uint8_t *)d_hash.p + n * 16, n * 4, hipMemcpyDeviceToHost,
+                                ctx->stream));
+  P2H_CHECK(ctx, hipStreamSynchronize(ctx->stream));
+  std::vector<uint32_t> missing;  // global block indices, canonical order
+  for (size_t i = 0; i < files.size(); i++) {
+    bool all = status[i] != kOk;
+    std::vector<uint32_t> bad;
+    if (!all) {
+      auto it = s.ifsc.find(files[i].id);
+      if (it != s.ifsc.end()) {  // IFSC block checks (verify.rs:548-654)
+        for (uint64_t k = 0; k < nblk[i]; k++) {
+          const uint64_t g = first[i] + k;
+          const bool known = k < it->second.size();
+          if (!known || crcs[g] != it->second[k].second || memcmp(&md5s[g * 16], it->second[k].first.data(), 16) != 0)
+            bad.push_back((uint32_t)g);
+        }
+      } else {  // 16 KiB then whole-file MD5 (verify.rs:700-745)
+        uint8_t d[16];
+        const std::string path = base + "/" + files[i].name;
+        const bool ok16 = md5_file_prefix(path, std::min<uint64_t>(files[i].length, 16384), d) &&
+                          (files[i].length < 16384 || memcmp(d, files[i].md5k.data(), 16) == 0);
+        all = !(ok16 && md5_file_prefix(path, files[i].length, d) && memcmp(d, files[i].md5.data(), 16) == 0);
+      }
+    }
+    if (all)
+      for (uint64_t k = 0; k < nblk[i]; k++) bad.push_back((uint32_t)(first[i] + k));
+    if (!bad.empty() && status[i] == kOk) status[i] = kDamaged;
+    missing.insert(missing.end(), bad.begin(), bad.end());
+    if (status[i] == kMissing) rep->files_missing++;
+    else if (status[i] != kOk) rep->files_damaged++;
+  }
+  rep->blocks_damaged = missing.size();
+  if (missing.empty() || verify_only) return PAR2HIP_OK;
+  const size_t m = missing.size();
+  if (m > recs.size())
+    return fail(ctx, PAR2HIP_ERECOVERY,
+                "need " + std::to_string(m) + " recovery blocks, have " + std::to_string(recs.size()));
+
+  // ---- rebuild on the device from the m lowest exponents (repair.rs:575-604)
+  std::vector<uint8_t> rec(m * bs);
+  std::vector<uint32_t> exps(m);
+  for (size_t k = 0; k < m; k++) {
+    exps[k] = recs[k].exponent;
+    if (!read_range(recs[k].path, recs[k].data_offset, bs, &rec[k * bs]))
+      return fail(ctx, PAR2HIP_EINVAL, "cannot read recovery slice from " + recs[k].path);
+  }
+  DevMem d_rec, d_out;
+  P2H_CHECK(ctx, hipMalloc(&d_rec.p, m * bs));
+  P2H_CHECK(ctx, hipMalloc(&d_out.p, m * bs));
+  P2H_CHECK(ctx, hipMemcpyAsync(d_rec.p, rec.data(), m * bs, hipMemcpyHostToDevice, ctx->stream));
+  P2H_CHECK(ctx, hipStreamSynchronize(ctx->stream));
+  if ((rc = par2hip_reconstruct_dev(ctx, d_slices.p, n, bs, missing.data(), m, d_rec.p, exps.data(), d_out.p, 0)))
+    return rc;
+  std::vector<uint8_t> out(m * bs);
+  P2H_CHECK(ctx, hipMemcpyAsync(out.data(), d_out.p, m * bs, hipMemcpyDeviceToHost, ctx->stream));
+  P2H_CHECK(ctx, hipStreamSynchronize(ctx->stream));
+
+  // ---- temp files (repair.rs:736-1040): originals untouched until commit
+  std::vector<std::pair<std::string, std::string>> temps;  // tmp, final
+  auto cleanup = [&] {
+    for (auto &t : temps) (void)remove(t.first.c_str());
+  };
+  std::vector<size_t> touched;
+  for (size_t i = 0; i < files.size(); i++) {
+    if (status[i] == kOk) continue;
+    const std::string final_path = base + "/" + files[i].name, tmp = final_path + kTmpSuffix;
+    FILE *f = fopen(tmp.c_str(), "wb+");
+    if (!f) {
+      cleanup();
+      return fail(ctx, PAR2HIP_EINVAL, "cannot create " + tmp);
+    }
+    temps.push_back({tmp, final_path});
+    touched.push_back(i);
+    bool ok = ftruncate(fileno(f), (off_t)files[i].length) == 0;
+    if (ok && status[i] != kMissing && present_size[i]) {  // seed with the original's bytes
+      const uint64_t copy = std::min(present_size[i], files[i].length);
+      std::vector<uint8_t> orig(copy);
+      ok = read_range(final_path, 0, copy, orig.data()) && fseeko(f, 0, SEEK_SET) == 0 &&
+           fwrite(orig.data(), 1, copy, f) == copy;
+    }
+    for (size_t k = 0; k < m && ok; k++) {
+      const uint64_t g = missing[k];
+      if (g < first[i] || g >= first[i] + nblk[i]) continue;
+      const uint64_t off = (g - first[i]) * bs;
+      const uint64_t len = std::min<uint64_t>(bs, files[i].length - off);
+      ok = fseeko(f, (off_t)off, SEEK_SET) == 0 && fwrite(&out[k * bs], 1, len, f) == len;
+    }
+    ok = (fclose(f) == 0) && ok;
+    if (!ok) {
+      cleanup();
+      return fail(ctx, PAR2HIP_EINVAL, "cannot write " + tmp);
+    }
+  }
+
+  // ---- every temp verified before anything is renamed (repair.rs:1050-1080)
+  for (size_t t = 0; t < touched.size(); t++) {
+    const size_t i = touched[t];
+    const std::string &tmp = temps[t].first;
+    bool ok = true;
+    auto it = s.ifsc.find(files[i].id);
+    if (it != s.ifsc.end() && nblk[i]) {
+      host.assign(nblk[i] * bs, 0);
+      ok = read_range(tmp, 0, files[i].length, host.data());
+      if (ok) {
+        P2H_CHECK(ctx,
+                  hipMemcpyAsync(d_slices.p, host.data(), nblk[i] * bs, hipMemcpyHostToDevice, ctx->stream));
+        P2H_CHECK(ctx, hipStreamSynchronize(ctx->stream));
+        if ((rc = par2hip_hash_slices_dev(ctx, d_slices.p, nblk[i], bs, bs, nullptr, d_hash.p,
+                                          (uint8_t *)d_hash.p + nblk[i] * 16))) {
+          cleanup();
+          return rc;
+        }
+        P2H_CHECK(ctx,
+                  hipMemcpyAsync(md5s.data(), d_hash.p, nblk[i] * 16, hipMemcpyDeviceToHost, ctx->stream));
+        P2H_CHECK(ctx, hipStreamSynchronize(ctx->stream));
+        P2H_CHECK(ctx, hipMemcpyAsync(crcs.data(), (uint8_t *)d_hash.p + nblk[i] * 16, nblk[i] * 4,
+                                      hipMemcpyDeviceToHost, ctx->stream));
+        P2H_CHECK(ctx, hipStreamSynchronize(ctx->stream));
+        for (uint64_t k = 0; k < nblk[i] && ok; k++)
+          ok = k < it->second.size() && crcs[k] == it->second[k].second &&
+               memcmp(&md5s[k * 16], it->second[k].first.data(), 16) == 0;
+      }
+    } else {
+      uint8_t d[16];
+      ok = md5_file_prefix(tmp, files[i].length, d) && memcmp(d, files[i].md5.data(), 16) == 0;
+    }
+    if (!ok) {
+      cleanup();
+      return fail(ctx, PAR2HIP_EREPAIR, "repair completed but verification failed - files may still be damaged");
+    }
+  }
+  for (size_t t = 0; t < temps.size(); t++) {
+    if (rename(temps[t].first.c_str(), temps[t].second.c_str()) != 0) {
+      for (size_t u = t; u < temps.size(); u++) (void)remove(temps[u].first.c_str());
+      return fail(ctx, PAR2HIP_EINVAL, "cannot rename " + temps[t].first);
+    }
+  }
+  rep->repaired = 1;
+  return PAR2HIP_OK;
+}
+
+}  // namespace par2hip
+
+extern "C" int32_t par2hip_par2_info(const char *par2_path, par2hip_set_info *info) {
+  if (!par2_path || !info) return PAR2HIP_EINVAL;
+  return par2hip::set_info(par2_path, info);
+}
+
+extern "C" int32_t par2hip_repair_files(par2hip_ctx *ctx, const char *par2_path, const char *base_dir,
+                                        int32_t verify_only, par2hip_repair_report *report) {
+  if (!ctx || !par2_path || !report) return PAR2HIP_EINVAL;
+  std::lock_guard<std::recursive_mutex> lk(ctx->mu);
+  (void)hipSetDevice(ctx->device);
+  return par2hip::repair_files(ctx, par2_path, base_dir, verify_only, report);
+}

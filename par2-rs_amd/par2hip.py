@@ -30,7 +30,7 @@ import numpy as np
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libpar2hip.so")
 
-OK, EINVAL, ENODEV, ENOMEM, ESINGULAR, EHIP, ERECOVERY = 0, -1, -2, -3, -4, -5, -6
+OK, EINVAL, ENODEV, ENOMEM, ESINGULAR, EHIP, ERECOVERY, EREPAIR = 0, -1, -2, -3, -4, -5, -6, -7
 PRIORITY_OPTIMAL = 40  # Priority::Optimal (src/galois/simd/mod.rs:20)
 
 
@@ -92,6 +92,18 @@ class OutputFile(C.Structure):
 VOLUMES_SINGLE, VOLUMES_EXPONENTIAL = 0, 1
 
 
+class SetInfo(C.Structure):
+    _fields_ = [("block_size", C.c_uint64), ("blocks", C.c_uint64), ("files", C.c_uint32),
+                ("recovery_slices", C.c_uint32), ("min_exponent", C.c_uint32), ("max_exponent", C.c_uint32),
+                ("set_id", C.c_uint8 * 16)]
+
+
+class RepairReport(C.Structure):
+    _fields_ = [("block_size", C.c_uint64), ("blocks", C.c_uint64), ("blocks_damaged", C.c_uint64),
+                ("files", C.c_uint32), ("files_damaged", C.c_uint32), ("files_missing", C.c_uint32),
+                ("recovery_slices", C.c_uint32), ("repaired", C.c_int32), ("reserved", C.c_int32)]
+
+
 class DeviceInfo(C.Structure):
     _fields_ = [("compute_units", C.c_int32), ("hash_cus", C.c_int32), ("arch", C.c_char * 32)]
 
@@ -147,6 +159,8 @@ def lib():
                 "par2hip_profile_enable": ([P, I32], I32),
                 "par2hip_profile_read": ([P, P, S, C.POINTER(S)], I32),
                 "par2hip_create_par2": ([P, P, S, C.c_uint64, C.c_float, I32, C.c_char_p, P, S, C.POINTER(S)], I32),
+                "par2hip_par2_info": ([C.c_char_p, P], I32),
+                "par2hip_repair_files": ([P, C.c_char_p, C.c_char_p, I32, P], I32),
                 "par2hip_set_autotune": ([P, I32], I32),
                 "par2hip_force_tile_shape": ([P, I32, I32, I32], I32),
                 "par2hip_autotune_save": ([C.c_char_p], I32),
@@ -441,6 +455,17 @@ class Context:
             self._check(rc)
             return [(outs[i].path.decode(), int(outs[i].bytes)) for i in range(n.value)]
 
+    def repair_files(self, par2_path: str, base_dir: str | None = None, verify_only: bool = False) -> dict:
+        """Par2Repairer: verify the set's files and repair them in place through
+        temp files (par2hip_repair_files); returns the report."""
+        rep = RepairReport()
+        rc = lib().par2hip_repair_files(self.handle, par2_path.encode(), base_dir.encode() if base_dir else None,
+                                        1 if verify_only else 0, C.byref(rep))
+        if rc == EREPAIR:
+            raise RepairFailed((lib().par2hip_last_error(self.handle) or b"").decode())
+        self._check(rc)
+        return {k: int(getattr(rep, k)) for k, _ in RepairReport._fields_ if k != "reserved"}
+
     def copy_dev(self, dst: DeviceBuffer, src: DeviceBuffer, nbytes: int) -> None:
         """Device-to-device copy on the ctx stream."""
         if nbytes > min(dst.nbytes, src.nbytes):
@@ -494,6 +519,15 @@ class BatchPlan:
             pass
 
 
+def par2_info(path: str) -> dict:
+    """What a .par2 index and the volumes beside it hold (no device needed)."""
+    info = SetInfo()
+    _status(lib().par2hip_par2_info(path.encode(), C.byref(info)), f"par2_info {path}")
+    d = {k: int(getattr(info, k)) for k, _ in SetInfo._fields_ if k != "set_id"}
+    d["set_id"] = bytes(info.set_id)
+    return d
+
+
 def autotune_save(path: str) -> None:
     _status(lib().par2hip_autotune_save(path.encode()), f"autotune_save {path}")
 
@@ -511,6 +545,8 @@ def _status(rc: int, what: str) -> None:
         raise RepairFailed(what)
     if rc == ERECOVERY:
         raise InsufficientRecovery(what)
+    if rc == EREPAIR:
+        raise RepairFailed(what)
     if rc == EINVAL:
         raise InvalidFormat(what)
     raise DeviceError(f"{rc}: {what}")
