@@ -682,7 +682,10 @@ def main():
         print("bench.py: no gfx950 device visible to libpar2hip.so", file=sys.stderr)
         sys.exit(1)
     if args.autotune is None:
-        args.autotune = 0 if args.workload in ("encode", "repair") else 1
+        # encode/repair: the MD5 chains bound the step, the contraction keeps
+        # the heuristic's shape (and its measured traffic); served: every
+        # request is a new shape class, so tuning would be paid per request
+        args.autotune = 0 if args.workload in ("encode", "repair", "served") else 1
     ctx = par2hip.Context(0)
     ctx.set_autotune(bool(args.autotune))
     log("workload", args.workload, "steps", args.steps, "warmup", args.warmup, "device", ctx.device_info())
