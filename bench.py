@@ -662,7 +662,7 @@ def main():
     ap.add_argument("--lanes", type=int, default=2, help="served: concurrent batches")
     ap.add_argument("--max-batch", type=int, default=512, help="served: requests per batch at most")
     ap.add_argument("--served-requests", type=int, default=400, help="served: requests per plain run")
-    ap.add_argument("--served-loads", type=float, nargs="+", default=[1, 2, 3, 4, 5, 6],
+    ap.add_argument("--served-loads", type=float, nargs="+", default=[6, 8, 10, 12, 14, 16],
                     help="served: offered loads as multiples of the plain run's")
     ap.add_argument("--seed", type=int, default=4)
     ap.add_argument("--kernel", type=int, default=0, help="0 auto, 1 LDS tables, 2 MFMA")
@@ -675,8 +675,8 @@ def main():
                     help="HBM bytes per GF launch measured with rocprofv3 --pmc (FETCH_SIZE+WRITE_SIZE)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
-        print("bench.py: this build targets one GPU; multi-GPU runs are not part of it "
-              "(the column split that would shard it is described in DESIGN.md)", file=sys.stderr)
+        print("bench.py: measures one GPU only; a multi-GPU run would split every slice's word "
+              "range across GPUs with no collective, which this bench does not launch", file=sys.stderr)
         sys.exit(2)
     if par2hip.available() <= 0:
         print("bench.py: no gfx950 device visible to libpar2hip.so", file=sys.stderr)
