@@ -68,7 +68,7 @@ struct BatchPlan {
   const HashEntry *d_md5l = nullptr, *d_crcl = nullptr, *d_crcs = nullptr;
   const CrcParams *d_cp = nullptr;
   size_t n_ft = 0, n_mt = 0, n_fi = 0, n_mi = 0, n_cj = 0, n_ij = 0, n_md5 = 0, n_crcl = 0, n_crcs = 0;
-  long max_coeff = 0, max_ft = 0, max_mt = 0;
+  long max_coeff = 0, pairs_ft = 0, pairs_mt = 0;
   int max_m = 0;
   bool aligned4 = true;
   unsigned *cursors = nullptr;
@@ -350,15 +350,18 @@ static int32_t plan_build(par2hip_ctx *ctx, const par2hip_job *jobs, size_t njob
     if (bytes) memcpy(h, src, bytes);
     return dev;
   };
+  // tasks' (8-row group, source) pairs numbered consecutively for the table build
   std::vector<GfTaskDev> ft(fold_tasks.size()), mt(main_tasks.size());
-  long max_ft = 0, max_mt = 0;
+  long pairs_ft = 0, pairs_mt = 0;
   for (size_t i = 0; i < ft.size(); i++) {
     ft[i] = fold_tasks[i].t;
-    max_ft = std::max(max_ft, gf_tab_entries(ft[i].ndst, ft[i].nsrc));
+    ft[i].pair0 = pairs_ft;
+    pairs_ft += (long)gf_tab_groups(ft[i].ndst) * ft[i].nsrc;
   }
   for (size_t i = 0; i < mt.size(); i++) {
     mt[i] = main_tasks[i].t;
-    max_mt = std::max(max_mt, gf_tab_entries(mt[i].ndst, mt[i].nsrc));
+    mt[i].pair0 = pairs_mt;
+    pairs_mt += (long)gf_tab_groups(mt[i].ndst) * mt[i].nsrc;
   }
   const GfTaskDev *d_ft = (const GfTaskDev *)put(ft.data(), ft.size() * sizeof(GfTaskDev));
   const GfTaskDev *d_mt = (const GfTaskDev *)put(mt.data(), mt.size() * sizeof(GfTaskDev));
@@ -391,8 +394,8 @@ static int32_t plan_build(par2hip_ctx *ctx, const par2hip_job *jobs, size_t njob
   P.n_crcl = crc_list.size();
   P.n_crcs = crc_small.size();
   P.max_coeff = max_coeff;
-  P.max_ft = max_ft;
-  P.max_mt = max_mt;
+  P.pairs_ft = pairs_ft;
+  P.pairs_mt = pairs_mt;
   P.max_m = max_m;
   P.aligned4 = aligned4;
   P.cursors = (unsigned *)(arena + cursor_off);
@@ -426,9 +429,9 @@ static int32_t plan_launch(par2hip_ctx *ctx, const BatchPlan &P) {
   if ((rc = launch_crc32_list_lane(ctx, s, P.d_crcs, P.n_crcs))) return rc;
   if ((rc = launch_coeff_batch(ctx, s, P.d_cj, (int)P.n_cj, P.max_coeff))) return rc;
   if ((rc = launch_inverse_batch(ctx, s, P.d_ij, (int)P.n_ij, P.max_m, 0))) return rc;
-  if ((rc = launch_tables_batch(ctx, s, P.d_ft, (int)P.n_ft, P.max_ft))) return rc;
+  if ((rc = launch_tables_batch(ctx, s, P.d_ft, (int)P.n_ft, P.pairs_ft))) return rc;
   if ((rc = launch_gf_batch(ctx, s, P.d_ft, P.d_fi, (int)P.n_fi, P.cursors, "gf_fold"))) return rc;
-  if ((rc = launch_tables_batch(ctx, s, P.d_mt, (int)P.n_mt, P.max_mt))) return rc;
+  if ((rc = launch_tables_batch(ctx, s, P.d_mt, (int)P.n_mt, P.pairs_mt))) return rc;
   if ((rc = launch_gf_batch(ctx, s, P.d_mt, P.d_mi, (int)P.n_mi, P.cursors + 64, "gf_batch"))) return rc;
   if (overlap && (rc = join_overlap(ctx))) return rc;
   if (wide_done) {

@@ -192,6 +192,7 @@ struct GfTaskDev {
   int nsrc, ndst, ldc, accumulate;
   int align;   // byte alignment (2..16) shared by every source and destination row
   int groups;  // 8-row groups per workgroup (gf_batch_groups)
+  long pair0;  // first (8-row group, source) pair of this task in the batch's table numbering
 };
 struct GfItem {
   int task, tile, rowblk, pad_;
@@ -203,10 +204,10 @@ constexpr int kBatchWordsPerWg = 2048;  // 256 threads x 8 words
 int32_t batch_run(par2hip_ctx *ctx, const par2hip_job *jobs, size_t njobs, uint8_t *arena, size_t arena_bytes);
 size_t batch_job_arena_bytes(const par2hip_job &job);
 constexpr size_t kBatchArenaExtra = 512;
-// Nibble tables of every task (grid: elements x tasks), then the persistent
-// kernel works through `items` (sorted by cost) with an atomic cursor.
-int32_t launch_tables_batch(par2hip_ctx *ctx, hipStream_t s, const GfTaskDev *d_tasks, int ntasks,
-                            long max_entries);
+// Nibble tables of every task (npairs = all tasks' (8-row group, source)
+// pairs, numbered by GfTaskDev::pair0), then the persistent kernel works
+// through `items` (sorted by cost) with an atomic cursor.
+int32_t launch_tables_batch(par2hip_ctx *ctx, hipStream_t s, const GfTaskDev *d_tasks, int ntasks, long npairs);
 int32_t launch_gf_batch(par2hip_ctx *ctx, hipStream_t s, const GfTaskDev *d_tasks, const GfItem *d_items,
                         int nitems, unsigned *d_cursor, const char *tag);
 // coefficient matrices of many encode jobs: coeff[r][d] = exp[(log_d * e_r) mod 65535]

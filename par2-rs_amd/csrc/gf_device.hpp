@@ -11,26 +11,45 @@ constexpr int kTpb = 256;  // threads per workgroup
 // rows per workgroup (accumulators: G*V 16-byte registers), D sources per
 // LDS table chunk (D*G KiB of LDS).
 
-// c * x for every row of one 8-row group as the table entry laid out above
-// (products via log/exp; zero coefficients and x = 0 give zero).
-__device__ __forceinline__ u32x4 gf_tab_entry(const uint16_t *__restrict__ coeff, int ldc, int ndst, int g, int d,
-                                              int iv, const uint16_t *__restrict__ log_tab,
-                                              const uint16_t *__restrict__ exp_tab) {
-  const uint32_t x = (uint32_t)(iv & 15) << (4 * (iv >> 4));
-  uint32_t w[4] = {0u, 0u, 0u, 0u};
-  if (x) {
-    const uint32_t lx = log_tab[x];
+// c * x and c * x^4 in GF(2^16) mod 0x1100B (x^16 = x^12 + x^3 + x + 1, the
+// low 16 bits of the polynomial): the bits shifted out at the top come back
+// multiplied by 0x100B, which for at most 4 bits needs no further reduction.
+__device__ __forceinline__ uint32_t gf_mul_x(uint32_t c) {
+  return ((c << 1) & 0xFFFFu) ^ ((c >> 15) ? (kPoly & 0xFFFFu) : 0u);
+}
+__device__ __forceinline__ uint32_t gf_mul_x4(uint32_t c) {
+  const uint32_t t = c >> 12;
+  return ((c << 4) & 0xFFFFu) ^ (t << 12) ^ (t << 3) ^ (t << 1) ^ t;
+}
+
+// The 64 table entries of one (8-row group, source) pair, written by one
+// wavefront without table lookups.  Lane (j, i, h) = (row j, nibble
+// position i, high bit h of the nibble value) forms c_j * x^(4i), its
+// multiples by x, x^2, x^3, and from those the products for the nibble
+// values v = 8h..8h+7 in Gray-code order (one XOR each); it stores them as
+// row j of entries 16i + v.  Rows past ndst get zero products.
+__device__ __forceinline__ void gf_tab_pair(P2H_GLOBAL uint16_t *__restrict__ tab, const uint16_t *__restrict__ coeff,
+                                            int ldc, int ndst, int g, int d, int lane) {
+  const int j = lane >> 3, i = (lane >> 1) & 3, h = lane & 1;
+  const int r = kTabRows * g + j;
+  uint32_t c = r < ndst ? coeff[(long)r * ldc + d] : 0u;
 #pragma unroll
-    for (int j = 0; j < kTabRows; j++) {
-      const int r = kTabRows * g + j;
-      if (r < ndst) {
-        const uint16_t c = coeff[(long)r * ldc + d];
-        if (c) w[j >> 1] |= (uint32_t)exp_tab[lx + log_tab[c]] << (16 * (j & 1));
-      }
-    }
+  for (int k = 0; k < 3; k++) {
+    const uint32_t t = gf_mul_x4(c);
+    c = k < i ? t : c;
   }
-  const u32x4 e = {w[0], w[1], w[2], w[3]};
-  return e;
+  const uint32_t b0 = c, b1 = gf_mul_x(b0), b2 = gf_mul_x(b1), b3 = gf_mul_x(b2);
+  // u16 index of row j in entry 16i + 8h + u is 8 * (16i + 8h + u) + j
+  P2H_GLOBAL uint16_t *q = tab + 8 * (16 * i + 8 * h) + j;
+  uint32_t v = h ? b3 : 0u;            // u = 0
+  q[8 * 0] = (uint16_t)v;
+  v ^= b0; q[8 * 1] = (uint16_t)v;     // u = 1
+  v ^= b1; q[8 * 3] = (uint16_t)v;     // u = 3
+  v ^= b0; q[8 * 2] = (uint16_t)v;     // u = 2
+  v ^= b2; q[8 * 6] = (uint16_t)v;     // u = 6
+  v ^= b0; q[8 * 7] = (uint16_t)v;     // u = 7
+  v ^= b1; q[8 * 5] = (uint16_t)v;     // u = 5
+  v ^= b0; q[8 * 4] = (uint16_t)v;     // u = 4
 }
 
 template <int V>

@@ -50,15 +50,14 @@ int32_t launch_coeff_from_logs(par2hip_ctx *ctx, hipStream_t s, uint16_t *coeff,
 }
 
 // ------------------------------------------------- variant T: nibble tables --
-// One thread per 16-byte table entry (layout in common.hpp).
-__global__ void k_nibble_tables(uint64_t *__restrict__ tabs, const uint16_t *__restrict__ coeff,
-                                int nrow, int nsrc, int ldc, const uint16_t *__restrict__ log_tab,
-                                const uint16_t *__restrict__ exp_tab) {
-  const long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (idx >= gf_tab_entries(nrow, nsrc)) return;
-  const long gd = idx >> 6;
-  const int d = (int)(gd % nsrc), g = (int)(gd / nsrc);
-  reinterpret_cast<u32x4 *>(tabs)[idx] = gf_tab_entry(coeff, ldc, nrow, g, d, (int)(idx & 63), log_tab, exp_tab);
+// One wavefront per (8-row group, source) pair, four pairs per workgroup.
+__global__ __launch_bounds__(256) void k_nibble_tables(uint64_t *__restrict__ tabs, const uint16_t *__restrict__ coeff,
+                                                       int nrow, int nsrc, int ldc) {
+  const long pair = (long)blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (pair >= (long)gf_tab_groups(nrow) * nsrc) return;
+  const int d = (int)(pair % nsrc), g = (int)(pair / nsrc);
+  gf_tab_pair(reinterpret_cast<P2H_GLOBAL uint16_t *>(gptr(tabs)) + pair * 512, coeff, ldc, nrow, g, d,
+              threadIdx.x & 63);
 }
 
 // Workgroup w covers words [tile*kTpb*V, +kTpb*V) of output rows
@@ -209,10 +208,10 @@ int32_t run_gf_lds(par2hip_ctx *ctx, hipStream_t s, const GfProblem &p) {
   int32_t rc = scratch(ctx, kScratchTables, tab_bytes, &tabs);
   if (rc) return rc;
   {
-    const long total = gf_tab_entries(p.ndst, p.nsrc);
+    const long pairs = (long)gf_tab_groups(p.ndst) * p.nsrc;
     ScopedTiming t(ctx, "nibble_tables", s);
-    hipLaunchKernelGGL(k_nibble_tables, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, s,
-                       (uint64_t *)tabs, p.coeff, p.ndst, p.nsrc, p.ldc ? p.ldc : p.nsrc, ctx->d_log, ctx->d_exp);
+    hipLaunchKernelGGL(k_nibble_tables, dim3((unsigned)((pairs + 3) / 4)), dim3(256), 0, s, (uint64_t *)tabs, p.coeff,
+                       p.ndst, p.nsrc, p.ldc ? p.ldc : p.nsrc);
     P2H_CHECK(ctx, hipGetLastError());
   }
   const int vmax = p.align_words % 8 == 0 ? 8 : p.align_words % 4 == 0 ? 4 : p.align_words % 2 == 0 ? 2 : 1;
@@ -303,23 +302,33 @@ int32_t launch_coeff_batch(par2hip_ctx *ctx, hipStream_t s, const CoeffJob *d_jo
   return PAR2HIP_OK;
 }
 
-__global__ void k_tables_batch(const GfTaskDev *__restrict__ tasks, const uint16_t *__restrict__ log_tab,
-                               const uint16_t *__restrict__ exp_tab) {
-  const GfTaskDev t = tasks[blockIdx.y];
-  const long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (idx >= gf_tab_entries(t.ndst, t.nsrc)) return;
-  const long gd = idx >> 6;
-  const int d = (int)(gd % t.nsrc), g = (int)(gd / t.nsrc);
-  reinterpret_cast<P2H_GLOBAL u32x4 *>(gptr(t.tabs))[idx] =
-      gf_tab_entry(t.coeff, t.ldc, t.ndst, g, d, (int)(idx & 63), log_tab, exp_tab);
+// Every task's tables: pairs (8-row group, source) of all tasks numbered
+// consecutively (task t owns [pair0, pair0 + groups*nsrc)), one wavefront
+// per pair, grid-stride over the numbering; the owning task is found by
+// binary search on pair0 (uniform per wavefront).
+__global__ __launch_bounds__(256) void k_tables_batch(const GfTaskDev *__restrict__ tasks, int ntasks, long npairs) {
+  const int lane = threadIdx.x & 63;
+  const long step = (long)gridDim.x * 4;
+  for (long pair = (long)blockIdx.x * 4 + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6); pair < npairs;
+       pair += step) {
+    int lo = 0, hi = ntasks - 1;
+    while (lo < hi) {
+      const int mid = (lo + hi + 1) >> 1;
+      if (tasks[mid].pair0 <= pair) lo = mid;
+      else hi = mid - 1;
+    }
+    const GfTaskDev &t = tasks[lo];
+    const long q = pair - t.pair0;
+    const int d = (int)(q % t.nsrc), g = (int)(q / t.nsrc);
+    gf_tab_pair(reinterpret_cast<P2H_GLOBAL uint16_t *>(gptr(t.tabs)) + q * 512, t.coeff, t.ldc, t.ndst, g, d, lane);
+  }
 }
 
-int32_t launch_tables_batch(par2hip_ctx *ctx, hipStream_t s, const GfTaskDev *d_tasks, int ntasks,
-                            long max_entries) {
-  if (ntasks == 0 || max_entries == 0) return PAR2HIP_OK;
+int32_t launch_tables_batch(par2hip_ctx *ctx, hipStream_t s, const GfTaskDev *d_tasks, int ntasks, long npairs) {
+  if (ntasks == 0 || npairs == 0) return PAR2HIP_OK;
   ScopedTiming t(ctx, "nibble_tables", s);
-  hipLaunchKernelGGL(k_tables_batch, dim3((unsigned)((max_entries + 255) / 256), (unsigned)ntasks), dim3(256), 0, s,
-                     d_tasks, ctx->d_log, ctx->d_exp);
+  const long grid = std::min<long>((npairs + 3) / 4, 16L * ctx->num_cus);
+  hipLaunchKernelGGL(k_tables_batch, dim3((unsigned)grid), dim3(256), 0, s, d_tasks, ntasks, npairs);
   P2H_CHECK(ctx, hipGetLastError());
   return PAR2HIP_OK;
 }
@@ -329,7 +338,7 @@ __global__ void k_reset_cursor(unsigned *cursor) { *cursor = 0u; }
 // Persistent: every workgroup takes the next work item from the cursor until
 // none is left (every wave reaches the exit because the cursor only grows).
 // Items are sorted by cost (sources per task, descending) on the host.
-__global__ __launch_bounds__(kTpb, 2) void k_gf_lds_batch(const GfTaskDev *__restrict__ tasks,
+__global__ __launch_bounds__(kTpb, 4) void k_gf_lds_batch(const GfTaskDev *__restrict__ tasks,
                                                          const GfItem *__restrict__ items, int nitems,
                                                          unsigned *__restrict__ cursor) {
   __shared__ u32x4 lds[16 * 2 * 64];  // D = 16 sources x up to 2 groups
@@ -355,8 +364,18 @@ int32_t launch_gf_batch(par2hip_ctx *ctx, hipStream_t s, const GfTaskDev *d_task
                         int nitems, unsigned *d_cursor, const char *tag) {
   if (nitems == 0) return PAR2HIP_OK;
   hipLaunchKernelGGL(k_reset_cursor, dim3(1), dim3(1), 0, s, d_cursor);
+  // as many persistent workgroups as fit on the device at once (registers and
+  // LDS decide; the work list absorbs any that start late)
+  static const int per_cu = [] {
+    int nb = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, reinterpret_cast<const void *>(k_gf_lds_batch), kTpb, 0) !=
+            hipSuccess ||
+        nb < 1)
+      nb = 2;
+    return nb;
+  }();
   ScopedTiming t(ctx, tag, s);
-  const int grid = std::min(nitems, 2 * ctx->num_cus);
+  const int grid = std::min(nitems, per_cu * ctx->num_cus);
   hipLaunchKernelGGL(k_gf_lds_batch, dim3(grid), dim3(kTpb), 0, s, d_tasks, d_items, nitems, d_cursor);
   P2H_CHECK(ctx, hipGetLastError());
   return PAR2HIP_OK;

@@ -28,6 +28,14 @@ __constant__ uint32_t kMd5T[64] = {
     0xf4292244, 0x432aff97, 0xab9423a7, 0xfc93a039, 0x655b59c3, 0x8f0ccc92, 0xffeff47d, 0x85845dd1,
     0x6fa87e4f, 0xfe2ce6e0, 0xa3014314, 0x4e0811a1, 0xf7537e82, 0xbd3af235, 0x2ad7d2bb, 0xeb86d391};
 
+// Register sets of message words in flight per MD5 lane (prefetch distance
+// kMd5Sets-1 blocks of 64 bytes).
+#ifndef P2H_MD5_SETS
+#define P2H_MD5_SETS 3
+#endif
+constexpr int kMd5Sets = P2H_MD5_SETS;
+static_assert(kMd5Sets >= 2, "at least one block must be in flight");
+
 __device__ __forceinline__ uint32_t rotl(uint32_t x, int s) { return __builtin_amdgcn_alignbit(x, x, 32 - s); }
 
 #define MD5_STEP(f, a, b, c, d, m, t, s) \
@@ -143,14 +151,15 @@ __device__ __forceinline__ void md5_slice(const P2H_GLOBAL uint8_t *__restrict__
   const uint64_t full = len & ~(uint64_t)63;  // blocks made entirely of real bytes
   uint64_t pos = 0;
   if (kAlign >= 4) {
-    // Three register sets rotate with a prefetch distance of two blocks: the
-    // loads of block k+2 are issued before block k is compressed.  Every
-    // fetch is unconditional (indexes past the end re-read the last block),
-    // so each wait covers exactly the set about to be compressed.
+    // kMd5Sets register sets rotate with a prefetch distance of kMd5Sets-1
+    // blocks: the loads of block k+kMd5Sets-1 are issued before block k is
+    // compressed.  Every fetch is unconditional (indexes past the end re-read
+    // the last block), so each wait covers exactly the set about to be
+    // compressed.
     const uint64_t nblk = full >> 6;
     if (nblk) {
       const uint64_t last = nblk - 1;
-      uint32_t A[16], B[16], C[16];
+      uint32_t S[kMd5Sets][16];
       auto fetch = [&](uint32_t (&D)[16], uint64_t blk) {
         blk = blk < last ? blk : last;
         if (kAlign == 16) {
@@ -169,19 +178,20 @@ __device__ __forceinline__ void md5_slice(const P2H_GLOBAL uint8_t *__restrict__
         // their first use and expose the full memory latency per block
         __builtin_amdgcn_sched_barrier(0);
       };
-      fetch(A, 0);
-      fetch(B, 1);
+#pragma unroll
+      for (int j = 0; j < kMd5Sets - 1; j++) fetch(S[j], j);
       uint64_t blk = 0;
-      for (; blk + 3 <= nblk; blk += 3) {
-        fetch(C, blk + 2);
-        md5_compress(st, A);
-        fetch(A, blk + 3);
-        md5_compress(st, B);
-        fetch(B, blk + 4);
-        md5_compress(st, C);
+      for (; blk + kMd5Sets <= nblk; blk += kMd5Sets) {
+#pragma unroll
+        for (int j = 0; j < kMd5Sets; j++) {
+          fetch(S[(j + kMd5Sets - 1) % kMd5Sets], blk + j + kMd5Sets - 1);
+          md5_compress(st, S[j]);
+        }
       }
-      if (blk < nblk) md5_compress(st, A);
-      if (blk + 1 < nblk) md5_compress(st, B);
+      // fewer than kMd5Sets blocks remain; they sit in sets 0..kMd5Sets-2
+#pragma unroll
+      for (int j = 0; j < kMd5Sets - 1; j++)
+        if (blk + j < nblk) md5_compress(st, S[j]);
     }
     pos = full;
   } else {
