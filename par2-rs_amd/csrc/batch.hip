@@ -403,40 +403,46 @@ static int32_t plan_build(par2hip_ctx *ctx, const par2hip_job *jobs, size_t njob
 }
 
 // The launches of a planned batch, queued on the ctx streams (the ctx stream
-// is the one to wait on; everything forked from it is joined back).  MD5
-// chains overlap everything else: on the reserved side CUs when they fit
-// there (one wave per SIMD), otherwise -- a throughput load of many chains --
-// on an unmasked stream over the whole device at normal issue priority
-// (boosted, they would starve the CRC and contraction waves that share their
-// SIMDs); the longest chains are listed first.
+// is the one to wait on; everything forked from it is joined back).  The
+// hashes do not feed the contraction, so they overlap it: MD5 chains on the
+// reserved side CUs when they fit there (one wave per SIMD) and from the
+// start; otherwise -- a throughput load of many chains -- MD5 and the CRC-32
+// lists go to an unmasked stream over the whole device.  That stream is
+// forked just before the main contraction, so the persistent contraction
+// workgroups and the hash workgroups become ready together and share the
+// device, instead of the many short hash workgroups taking every slot first.
+// Waves of the hash lists find room only beside fewer than four contraction
+// workgroups per CU, so most MD5 chains run as the contraction drains; the
+// longest chains are listed first.
 static int32_t plan_launch(par2hip_ctx *ctx, const BatchPlan &P) {
   int32_t rc;
   const size_t md5_waves = (P.n_md5 + 63) / 64;
   const bool side_fits = ctx->masked && md5_waves <= (size_t)(4 * ctx->hash_cus);
-  const bool overlap = P.n_md5 && side_fits;
-  hipEvent_t wide_done = nullptr;
-  if (overlap) {
+  const bool md5_side = P.n_md5 && side_fits;
+  const bool md5_wide = P.n_md5 && !side_fits;
+  const bool wide = md5_wide || P.n_crcl || P.n_crcs;
+  if (md5_side) {
     if ((rc = fork_overlap(ctx))) return rc;
     if ((rc = launch_md5_list(ctx, ctx->side, P.d_md5l, P.n_md5, P.aligned4, false))) return rc;
-  } else if (P.n_md5) {
-    P2H_CHECK(ctx, hipEventRecord(ctx->fork, ctx->stream));
-    P2H_CHECK(ctx, hipStreamWaitEvent(ctx->wide, ctx->fork, 0));
-    if ((rc = launch_md5_list(ctx, ctx->wide, P.d_md5l, P.n_md5, P.aligned4, false))) return rc;
-    wide_done = ctx->join;
   }
   hipStream_t s = ctx->cur;
-  if ((rc = launch_crc32_list(ctx, s, P.d_crcl, P.n_crcl, P.d_cp))) return rc;
-  if ((rc = launch_crc32_list_lane(ctx, s, P.d_crcs, P.n_crcs))) return rc;
   if ((rc = launch_coeff_batch(ctx, s, P.d_cj, (int)P.n_cj, P.max_coeff))) return rc;
   if ((rc = launch_inverse_batch(ctx, s, P.d_ij, (int)P.n_ij, P.max_m, 0))) return rc;
   if ((rc = launch_tables_batch(ctx, s, P.d_ft, (int)P.n_ft, P.pairs_ft))) return rc;
   if ((rc = launch_gf_batch(ctx, s, P.d_ft, P.d_fi, (int)P.n_fi, P.cursors, "gf_fold"))) return rc;
   if ((rc = launch_tables_batch(ctx, s, P.d_mt, (int)P.n_mt, P.pairs_mt))) return rc;
+  if (wide) {
+    P2H_CHECK(ctx, hipEventRecord(ctx->fork, s));
+    P2H_CHECK(ctx, hipStreamWaitEvent(ctx->wide, ctx->fork, 0));
+    if (md5_wide && (rc = launch_md5_list(ctx, ctx->wide, P.d_md5l, P.n_md5, P.aligned4, false))) return rc;
+    if ((rc = launch_crc32_list(ctx, ctx->wide, P.d_crcl, P.n_crcl, P.d_cp))) return rc;
+    if ((rc = launch_crc32_list_lane(ctx, ctx->wide, P.d_crcs, P.n_crcs))) return rc;
+  }
   if ((rc = launch_gf_batch(ctx, s, P.d_mt, P.d_mi, (int)P.n_mi, P.cursors + 64, "gf_batch"))) return rc;
-  if (overlap && (rc = join_overlap(ctx))) return rc;
-  if (wide_done) {
-    P2H_CHECK(ctx, hipEventRecord(wide_done, ctx->wide));
-    P2H_CHECK(ctx, hipStreamWaitEvent(ctx->stream, wide_done, 0));
+  if (md5_side && (rc = join_overlap(ctx))) return rc;
+  if (wide) {
+    P2H_CHECK(ctx, hipEventRecord(ctx->join, ctx->wide));
+    P2H_CHECK(ctx, hipStreamWaitEvent(ctx->stream, ctx->join, 0));
   }
   return PAR2HIP_OK;
 }
