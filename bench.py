@@ -118,7 +118,7 @@ def bench_encode(args, ctx):
     exact = bool(np.array_equal(rec[:, w0:w0 + 32], oracle.encode(cols, exps)))
     src_bytes = n * s
     algo_bytes = n * s + r * s  # read every input once, write every recovery slice once
-    gf_name = "gf_muladd_mfma" if "gf_muladd_mfma" in stats else "gf_muladd_lds"
+    gf_name = next((k for k in ("gf_muladd_mfma", "gf_muladd_mfma_i8") if k in stats), "gf_muladd_lds")
     launches, gf_ms = stats.get(gf_name, (0, 0.0))
     gf_avg = gf_ms / max(launches, 1)
     md5_avg = stats.get("md5_slices", (1, 0.0))[1] / max(stats.get("md5_slices", (1, 0))[0], 1)
@@ -171,7 +171,7 @@ def bench_micro(args, ctx):
         log(name, out[name])
 
     tuned = ctx.set_autotune(False)
-    for k, nm in ((1, "gf_lds"), (2, "gf_mfma")):
+    for k, nm in ((1, "gf_lds"), (2, "gf_mfma"), (3, "gf_mfma_i8")):
         ctx.set_gf_kernel(k)
         timed(nm, lambda: ctx.encode_dev(d_sl, n, s, exps, d_rec), args.steps)
     ctx.set_gf_kernel(1)
@@ -186,7 +186,7 @@ def bench_micro(args, ctx):
     d_rec3 = ctx.alloc(r3 * s3)
     d_sl3.upload(random_bytes(99, n3 * s3))
     exps3 = np.arange(r3, dtype=np.uint32)
-    for k, nm in ((1, "gf_lds_2000x100"), (2, "gf_mfma_2000x100")):
+    for k, nm in ((1, "gf_lds_2000x100"), (2, "gf_mfma_2000x100"), (3, "gf_mfma_i8_2000x100")):
         ctx.set_gf_kernel(k)
         timed(nm, lambda: ctx.encode_dev(d_sl3, n3, s3, exps3, d_rec3), args.steps)
     ctx.set_gf_kernel(1)
@@ -665,7 +665,7 @@ def main():
     ap.add_argument("--served-loads", type=float, nargs="+", default=[6, 8, 10, 12, 14, 16],
                     help="served: offered loads as multiples of the plain run's")
     ap.add_argument("--seed", type=int, default=4)
-    ap.add_argument("--kernel", type=int, default=0, help="0 auto, 1 LDS tables, 2 MFMA")
+    ap.add_argument("--kernel", type=int, default=0, help="0 auto, 1 LDS tables, 2 MFMA FP4, 3 MFMA int8")
     ap.add_argument("--autotune", type=int, default=None,
                     help="1: tile shapes tuned on the first (warm-up) call of each shape class; 0: fixed heuristic "
                          "(default: 0 for encode/repair, whose steps the MD5 chains bound, so the contraction's "

@@ -82,7 +82,7 @@ struct par2hip_ctx {
   int *d_status = nullptr;            // device status word (singular flag)
   par2hip::ParamArena arena;
   par2hip::DevBuf scratch[11];        // grow-only device scratch, by role
-  int gf_kernel = 0;                  // 0 auto, 1 LDS tables, 2 MFMA
+  int gf_kernel = 0;                  // 0 auto, 1 LDS tables, 2 MFMA FP4, 3 MFMA int8
   int num_cus = 256;                  // compute units of the device
   bool profile = false;
   bool autotune = false;  // time candidate tile shapes on first use of a shape class
@@ -231,7 +231,7 @@ struct InvJob {
 int32_t launch_inverse_batch(par2hip_ctx *ctx, hipStream_t s, const InvJob *d_jobs, int njobs, int max_m,
                              long max_vand);
 int32_t run_gf_lds(par2hip_ctx *ctx, hipStream_t s, const GfProblem &p);
-int32_t run_gf_mfma(par2hip_ctx *ctx, hipStream_t s, const GfProblem &p);
+int32_t run_gf_mfma(par2hip_ctx *ctx, hipStream_t s, const GfProblem &p, bool int8);
 // Kernel used when the caller leaves the choice to the library.
 constexpr int kAutoGfKernel = 1;
 

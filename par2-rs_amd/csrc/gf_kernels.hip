@@ -273,11 +273,12 @@ int32_t run_gf(par2hip_ctx *ctx, hipStream_t s, const GfProblem &p) {
     if (p.accumulate) return PAR2HIP_OK;
     return fail(ctx, PAR2HIP_EINVAL, "contraction with no sources and overwrite semantics");
   }
-  // 1 = LDS tables, 2 = bit-sliced MFMA (needs 16-byte aligned operands), 0 = automatic
+  // 1 = LDS tables, 2 / 3 = bit-sliced MFMA with FP4 / int8 operands (needs
+  // 16-byte aligned operands), 0 = automatic
   const bool mfma_ok = p.align_words >= 8;
   int which = ctx->gf_kernel;
   if (which == 0) which = kAutoGfKernel;
-  if (which == 2 && mfma_ok) return run_gf_mfma(ctx, s, p);
+  if ((which == 2 || which == 3) && mfma_ok) return run_gf_mfma(ctx, s, p, which == 3);
   return run_gf_lds(ctx, s, p);
 }
 

@@ -25,11 +25,25 @@
 // registers are ((y << 1), y, (y >> 1), (y >> 2)) & 0x22222222, i.e. nibble p
 // of register t carries bit 4*(p%4)+t of source s = p/4.  A is pre-packed in
 // the same shape (one u32 per lane per fragment per step, expanded on chip).
+//
+// The int8 form runs the same contraction on v_mfma_i32_32x32x32_i8 (0/1
+// bytes, int32 accumulators, parity of the integer sum): a 64-wide step is
+// two MFMAs, half hh covering sources 4ks+2hh and 4ks+2hh+1.  With y the
+// packed pair (bit 16s+i = bit i of source s), lane half g's operand register
+// t is (y >> (4g+t)) & 0x01010101: byte b carries bit 8b+4g+t of y, so the 32
+// K values of one MFMA are the 32 bits of y.  The packed A of half hh is the
+// FP4 form's lane m + 32hh, so both forms share k_pack_a.  At the dense rates
+// (int8 5 POP/s, FP4 10 PFLOP/s) the int8 form needs twice the matrix-core
+// time of the FP4 one; both are kept and measured.
+#include <type_traits>
+
 #include "common.hpp"
 
 namespace par2hip {
 
+typedef int v4i __attribute__((ext_vector_type(4)));
 typedef int v8i __attribute__((ext_vector_type(8)));
+typedef int v16i __attribute__((ext_vector_type(16)));
 typedef float v16f __attribute__((ext_vector_type(16)));
 
 constexpr int kMfWaves = 8;              // waves per workgroup (two per SIMD)
@@ -74,6 +88,18 @@ __global__ void k_pack_a(uint32_t *__restrict__ apk, const uint16_t *__restrict_
   apk[idx] = out;
 }
 
+// int8 operand of lane half g from a packed pair y (bit 16s+i = bit i of
+// source s): register t holds bits 8b+4g+t of y as bytes b = 0..3.
+__device__ __forceinline__ v4i expand_i8(uint32_t y, int g) {
+  const uint32_t z = y >> (4 * g);
+  v4i o;
+  o[0] = (int)(z & 0x01010101u);
+  o[1] = (int)((z >> 1) & 0x01010101u);
+  o[2] = (int)((z >> 2) & 0x01010101u);
+  o[3] = (int)((z >> 3) & 0x01010101u);
+  return o;
+}
+
 __device__ __forceinline__ v8i expand_fp4(uint32_t y) {
   v8i o = {0, 0, 0, 0, 0, 0, 0, 0};
   o[0] = (int)((y << 1) & kNib);
@@ -91,12 +117,17 @@ __device__ __forceinline__ v8i expand_fp4(uint32_t y) {
 // wave expands only its own B columns.  Two waves per SIMD let one wave's LDS
 // reads and expansion overlap the other's MFMAs.
 // Lane (n, g) of wave v owns words tile + 64v + 2n + f for B fragment f.
+// kI8: int8 operands (A kept packed in LDS and expanded by each wave) instead
+// of FP4 (A expanded once by the staging threads).
+template <bool kI8>
 __global__ __launch_bounds__(kMfThreads, 1) void k_gf_mfma(const uint16_t *const *__restrict__ srcs, int nsrc,
                                                            uint16_t *const *__restrict__ dsts, int ndst,
                                                            const uint32_t *__restrict__ apk, int nmf, int nks,
                                                            long words, int accumulate, int ntiles, int nrowblk) {
+  using AUnit = typename std::conditional<kI8, uint32_t, uint4>::type;
+  using Acc = typename std::conditional<kI8, v16i, v16f>::type;
   __shared__ uint4 sB[2][4 * kMfKc][kMfWgWords / 8];   // [buf][source][16-byte unit], 64 KiB
-  __shared__ uint4 sA[2][kMfKc][kMfRows][64];         // [buf][ks][fragment][lane], FP4 operands, 64 KiB
+  __shared__ AUnit sA[2][kMfKc][kMfRows][64];         // [buf][ks][fragment][lane]: FP4 64 KiB, packed 8 KiB
 
   const int L = blockIdx.x;
   const int xcd = L & 7, q = L >> 3;
@@ -150,17 +181,22 @@ __global__ __launch_bounds__(kMfThreads, 1) void k_gf_mfma(const uint16_t *const
     const int frag = tid >> 7, ksl = (tid >> 4) & 7, l4 = 4 * (tid & 15);
     const uint32_t y[4] = {ra.x, ra.y, ra.z, ra.w};
 #pragma unroll
-    for (int e = 0; e < 4; e++)
-      sA[buf][ksl][frag][l4 + e] = make_uint4((y[e] << 1) & kNib, y[e] & kNib, (y[e] >> 1) & kNib, (y[e] >> 2) & kNib);
+    for (int e = 0; e < 4; e++) {
+      if constexpr (kI8)
+        sA[buf][ksl][frag][l4 + e] = y[e];
+      else
+        sA[buf][ksl][frag][l4 + e] =
+            make_uint4((y[e] << 1) & kNib, y[e] & kNib, (y[e] >> 1) & kNib, (y[e] >> 2) & kNib);
+    }
   };
 
-  v16f acc[kMfRows][kMfCols];
+  Acc acc[kMfRows][kMfCols];
 #pragma unroll
   for (int i = 0; i < kMfRows; i++)
 #pragma unroll
     for (int f = 0; f < kMfCols; f++)
 #pragma unroll
-      for (int k = 0; k < 16; k++) acc[i][f][k] = 0.f;
+      for (int k = 0; k < 16; k++) acc[i][f][k] = 0;
 
   load_stage(0);
   store_stage(0);
@@ -174,18 +210,36 @@ __global__ __launch_bounds__(kMfThreads, 1) void k_gf_mfma(const uint16_t *const
     // Kept rolled: unrolled, the operands of 8 steps spill at 2 waves/SIMD.
 #pragma unroll 1
     for (int ksl = 0; ksl < kMfKc; ksl++) {
-      const uint32_t x0 = reinterpret_cast<const uint32_t *>(&sB[buf][4 * ksl + 2 * g][0])[bcol >> 1];
-      const uint32_t x1 = reinterpret_cast<const uint32_t *>(&sB[buf][4 * ksl + 2 * g + 1][0])[bcol >> 1];
-      v8i b[kMfCols];
-      b[0] = expand_fp4(__builtin_amdgcn_perm(x1, x0, 0x05040100u));  // word 0: src 2g low, 2g+1 high
-      b[1] = expand_fp4(__builtin_amdgcn_perm(x1, x0, 0x07060302u));  // word 1
+      if constexpr (kI8) {
 #pragma unroll
-      for (int i = 0; i < kMfRows; i++) {
-        const uint4 t = sA[buf][ksl][i][lane];
-        const v8i a = v8i{(int)t.x, (int)t.y, (int)t.z, (int)t.w, 0, 0, 0, 0};
+        for (int hh = 0; hh < 2; hh++) {
+          const uint32_t x0 = reinterpret_cast<const uint32_t *>(&sB[buf][4 * ksl + 2 * hh][0])[bcol >> 1];
+          const uint32_t x1 = reinterpret_cast<const uint32_t *>(&sB[buf][4 * ksl + 2 * hh + 1][0])[bcol >> 1];
+          v4i b[kMfCols];
+          b[0] = expand_i8(__builtin_amdgcn_perm(x1, x0, 0x05040100u), g);  // word 0: src 2hh low, 2hh+1 high
+          b[1] = expand_i8(__builtin_amdgcn_perm(x1, x0, 0x07060302u), g);  // word 1
 #pragma unroll
-        for (int f = 0; f < kMfCols; f++)
-          acc[i][f] = __builtin_amdgcn_mfma_scale_f32_32x32x64_f8f6f4(a, b[f], acc[i][f], 4, 4, 0, 127, 0, 127);
+          for (int i = 0; i < kMfRows; i++) {
+            const v4i a = expand_i8(sA[buf][ksl][i][n + 32 * hh], g);
+#pragma unroll
+            for (int f = 0; f < kMfCols; f++)
+              acc[i][f] = __builtin_amdgcn_mfma_i32_32x32x32_i8(a, b[f], acc[i][f], 0, 0, 0);
+          }
+        }
+      } else {
+        const uint32_t x0 = reinterpret_cast<const uint32_t *>(&sB[buf][4 * ksl + 2 * g][0])[bcol >> 1];
+        const uint32_t x1 = reinterpret_cast<const uint32_t *>(&sB[buf][4 * ksl + 2 * g + 1][0])[bcol >> 1];
+        v8i b[kMfCols];
+        b[0] = expand_fp4(__builtin_amdgcn_perm(x1, x0, 0x05040100u));  // word 0: src 2g low, 2g+1 high
+        b[1] = expand_fp4(__builtin_amdgcn_perm(x1, x0, 0x07060302u));  // word 1
+#pragma unroll
+        for (int i = 0; i < kMfRows; i++) {
+          const uint4 t = sA[buf][ksl][i][lane];
+          const v8i a = v8i{(int)t.x, (int)t.y, (int)t.z, (int)t.w, 0, 0, 0, 0};
+#pragma unroll
+          for (int f = 0; f < kMfCols; f++)
+            acc[i][f] = __builtin_amdgcn_mfma_scale_f32_32x32x64_f8f6f4(a, b[f], acc[i][f], 4, 4, 0, 127, 0, 127);
+        }
       }
     }
     // the other buffer was last read in stage c-1, which every wave finished
@@ -206,7 +260,7 @@ __global__ __launch_bounds__(kMfThreads, 1) void k_gf_mfma(const uint16_t *const
     for (int f = 0; f < kMfCols; f++) {
       uint32_t word = 0;
 #pragma unroll
-      for (int k = 0; k < 16; k++) word |= ((uint32_t)acc[i][f][k] & 1u) << k;
+      for (int k = 0; k < 16; k++) word |= ((uint32_t)acc[i][f][k] & 1u) << k;  // exact integer sums
       v[f] = word;
     }
     uint16_t *p = dsts[r];
@@ -220,7 +274,7 @@ __global__ __launch_bounds__(kMfThreads, 1) void k_gf_mfma(const uint16_t *const
   }
 }
 
-int32_t run_gf_mfma(par2hip_ctx *ctx, hipStream_t s, const GfProblem &p) {
+int32_t run_gf_mfma(par2hip_ctx *ctx, hipStream_t s, const GfProblem &p, bool int8) {
   const int nmf = (p.ndst + 1) / 2;
   const int nks = (p.nsrc + 3) / 4;
   const size_t apk_bytes = (size_t)nmf * nks * 64 * sizeof(uint32_t);
@@ -238,9 +292,13 @@ int32_t run_gf_mfma(par2hip_ctx *ctx, hipStream_t s, const GfProblem &p) {
   const int nrowblk = (nmf + kMfRows - 1) / kMfRows;
   const long nwg = 8L * ((ntiles + 7) / 8) * nrowblk;
   if (nwg > 0x7fffffffL) return fail(ctx, PAR2HIP_EINVAL, "GF problem too large for one launch");
-  ScopedTiming t(ctx, p.tag ? p.tag : "gf_muladd_mfma", s);
-  hipLaunchKernelGGL(k_gf_mfma, dim3((unsigned)nwg), dim3(kMfThreads), 0, s, p.srcs, p.nsrc, p.dsts, p.ndst,
-                     (const uint32_t *)apk, nmf, nks, (long)p.words, p.accumulate ? 1 : 0, ntiles, nrowblk);
+  ScopedTiming t(ctx, p.tag ? p.tag : (int8 ? "gf_muladd_mfma_i8" : "gf_muladd_mfma"), s);
+  if (int8)
+    hipLaunchKernelGGL(k_gf_mfma<true>, dim3((unsigned)nwg), dim3(kMfThreads), 0, s, p.srcs, p.nsrc, p.dsts, p.ndst,
+                       (const uint32_t *)apk, nmf, nks, (long)p.words, p.accumulate ? 1 : 0, ntiles, nrowblk);
+  else
+    hipLaunchKernelGGL(k_gf_mfma<false>, dim3((unsigned)nwg), dim3(kMfThreads), 0, s, p.srcs, p.nsrc, p.dsts, p.ndst,
+                       (const uint32_t *)apk, nmf, nks, (long)p.words, p.accumulate ? 1 : 0, ntiles, nrowblk);
   P2H_CHECK(ctx, hipGetLastError());
   return PAR2HIP_OK;
 }

@@ -915,7 +915,7 @@ int32_t par2hip_hash_slices(par2hip_ctx *ctx, const uint8_t *slices, size_t n, s
 
 // ------------------------------------------------------------- tuning --
 int32_t par2hip_set_gf_kernel(par2hip_ctx *ctx, int32_t which) {
-  if (!ctx || which < 0 || which > 2) return PAR2HIP_EINVAL;
+  if (!ctx || which < 0 || which > 3) return PAR2HIP_EINVAL;
   Guard g(ctx);
   int32_t prev = ctx->gf_kernel;
   ctx->gf_kernel = which;

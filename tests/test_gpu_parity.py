@@ -35,12 +35,13 @@ def ctx():
     c.close()
 
 
-KERNELS = {"lds": 1, "mfma": 2}
+KERNELS = {"lds": 1, "mfma_fp4": 2, "mfma_i8": 3}
 
 
 @pytest.fixture(params=list(KERNELS))
 def kctx(request, ctx):
-    """The context with one contraction kernel forced (LDS tables / bit-sliced MFMA)."""
+    """The context with one contraction kernel forced (LDS tables / bit-sliced
+    MFMA with FP4 or int8 operands)."""
     prev = ctx.set_gf_kernel(KERNELS[request.param])
     yield ctx
     ctx.set_gf_kernel(prev)
