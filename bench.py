@@ -38,6 +38,12 @@ HBM_PEAK_GBS = 8000.0
 # 1 GiB device-to-device copy (--workload copy): FETCH_SIZE reads half the
 # bytes the copy reads (factor 2.0), WRITE_SIZE all it writes (factor 1.0).
 C2_GF_TRAFFIC_BYTES = 2.475e9
+# Unit utilisation of the same contraction (default tile shape, alone on the
+# device) from rocprofv3 --pmc SQ_LDS_IDX_ACTIVE / SQ_ACTIVE_INST_VALU /
+# SQ_LDS_BANK_CONFLICT / GRBM_GUI_ACTIVE runs of this workload: the binding
+# units are the LDS lookups and their XOR folds, not HBM.
+C2_GF_UNITS = {"lds_util": 0.76, "valu_busy": 0.665, "lds_bank_conflict": 0.0, "sclk_ghz": 2.0,
+               "source": "rocprofv3 --pmc, profiles/gf_units_utilisation_pmc.md"}
 METRIC = "encode/repair GB/s source data + p50/p99 latency vs plain GPU path, bit-exact"
 
 
@@ -125,7 +131,8 @@ def bench_encode(args, ctx):
     crc_avg = stats.get("crc32_slices", (1, 0.0))[1] / max(stats.get("crc32_slices", (1, 0))[0], 1)
     achieved = algo_bytes / (gf_avg * 1e-3) / 1e9 if gf_avg > 0 else 0.0
     traffic = args.traffic
-    if traffic is None and not args.autotune and gf_name == "gf_muladd_lds" and (n, s, r) == (1024, 1 << 20, 102):
+    default_shape = not args.autotune and gf_name == "gf_muladd_lds" and (n, s, r) == (1024, 1 << 20, 102)
+    if traffic is None and default_shape:
         traffic = C2_GF_TRAFFIC_BYTES
     return {
         "workload": "plain_encode",
@@ -138,7 +145,8 @@ def bench_encode(args, ctx):
                      "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": traffic,
                      "traffic_x_algorithmic": round(traffic / algo_bytes, 3) if traffic else None,
                      "kernel": gf_name, "kernel_ms": round(gf_avg, 4), "algorithmic_bytes": algo_bytes,
-                     "gf_muladds": n * r * (s // 2), "md5_ms": round(md5_avg, 4), "crc32_ms": round(crc_avg, 4)},
+                     "gf_muladds": n * r * (s // 2), "md5_ms": round(md5_avg, 4), "crc32_ms": round(crc_avg, 4),
+                     "units": C2_GF_UNITS if default_shape else None},
         "cpu": (n, s, r),
     }
 
@@ -246,7 +254,7 @@ def bench_repair(args, ctx):
     ctx.profile(False)
     out = d_out.download().reshape(m, s)
     exact = bool(np.array_equal(got, missing) and np.array_equal(out, data[missing]))
-    gf_name = "gf_muladd_mfma" if "gf_muladd_mfma" in stats else "gf_muladd_lds"
+    gf_name = next((k for k in ("gf_muladd_mfma", "gf_muladd_mfma_i8") if k in stats), "gf_muladd_lds")
     launches, gf_ms = stats.get(gf_name, (0, 0.0))
     gf_avg = gf_ms / max(launches, 1)
     algo_bytes = n * s + m * s + m * s
