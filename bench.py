@@ -572,12 +572,16 @@ def bench_served(args, ctx):
 
 
 def bench_create(args, ctx):
-    """End-to-end PAR2 creation (Par2Creator::create, creator.rs:261-531):
-    16 files of 64 MiB (1 GiB) from host memory to written .par2 files, the
-    reference's automatic block size, 10% redundancy, exponential volumes.
-    One step = one par2hip_create_par2 call (file digests on host threads
-    overlapped with H2D, IFSC hashes, recovery slices and recovery packet
-    digests on the device, file writes)."""
+    """End-to-end PAR2 creation (Par2Creator::create, creator.rs:261-531)
+    from host memory to written .par2 files, exponential volumes.  Default:
+    16 files of 64 MiB (1 GiB), the reference's automatic block size, 10%
+    redundancy.  --create-files 1 --create-file-mib 1024 --create-block
+    1048576 is the plain-encode set as the creator makes it: 1024 blocks of
+    1 MiB, and -r 10 gives R = 103 by the creator's rounding (creator.rs:
+    320-335), one more than the 102 of the kernel workload.  One step = one
+    par2hip_create_par2 call (file digests on host threads overlapped with
+    H2D, IFSC hashes, recovery slices and recovery packet digests on the
+    device, file writes)."""
     import tempfile
 
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
@@ -585,7 +589,8 @@ def bench_create(args, ctx):
     import oracle
     import par2fmt
 
-    nfiles, fbytes = 16, 64 << 20
+    nfiles, fbytes = args.create_files, args.create_file_mib << 20
+    block, pct = args.create_block, args.create_redundancy
     pool = random_bytes(31, nfiles * fbytes)
     files = [(f"part{i:02d}.bin", pool[i * fbytes:(i + 1) * fbytes].tobytes()) for i in range(nfiles)]
     out_dir = tempfile.mkdtemp(prefix="par2create_")
@@ -596,11 +601,11 @@ def bench_create(args, ctx):
     want = oracle.par2_create([("c1.bin", c1)], block_size=65536, redundancy=5.0, stem=os.path.join(out_dir, "c1"))
     exact = [p for p, _ in got] == [n for n, _ in want] and all(open(p, "rb").read() == b for (p, _), (_, b) in
                                                                  zip(got, want))
-    written = ctx.create_par2(files, stem, block_size=0, redundancy=10.0)  # warm-up
+    written = ctx.create_par2(files, stem, block_size=block, redundancy=pct)  # warm-up
     lat = []
     for _ in range(args.steps):
         t0 = time.perf_counter()
-        written = ctx.create_par2(files, stem, block_size=0, redundancy=10.0)
+        written = ctx.create_par2(files, stem, block_size=block, redundancy=pct)
         lat.append(time.perf_counter() - t0)
     # structure and a column window of the recovery slices of the 1 GiB set
     st = par2fmt.load([p for p, _ in written])
@@ -630,8 +635,9 @@ def bench_create(args, ctx):
     src = nfiles * fbytes
     res = {
         "workload": "create",
-        "config": {"workload": "create_16x64MiB", "files": nfiles, "file_bytes": fbytes, "block_size": bs,
-                   "input_slices": n_blocks, "recovery_slices": r, "redundancy_percent": 10.0,
+        "config": {"workload": f"create_{nfiles}x{fbytes >> 20}MiB", "files": nfiles, "file_bytes": fbytes,
+                   "block_size": bs, "block_rule": "given" if block else "automatic (creator.rs:29-67)",
+                   "input_slices": n_blocks, "recovery_slices": r, "redundancy_percent": pct,
                    "volumes": "exponential", "output_bytes": total_out, "source": "host memory -> .par2 files"},
         "src_bytes": src, "total_s": float(np.sum(lat)), "lat": lat, "exact": bool(exact), "kernel_ms": {},
     }
@@ -654,7 +660,7 @@ def bench_create(args, ctx):
             list(digests)
         dt = time.perf_counter() - t0
         res["cpu_baseline"] = {"value": round(src / dt / 1e9, 4), "unit": "GB/s", "cores": threads, "kind": "port",
-                               "sample": f"the same 1 GiB set: AVX2 encode of {nb} x {bs} B into {r} recovery "
+                               "sample": f"the same {src / 2**30:.3g} GiB set: AVX2 encode of {nb} x {bs} B into {r} recovery "
                                          f"slices + IFSC MD5/CRC32 + whole-file MD5s, {dt:.2f} s wall"}
     return res
 
@@ -672,6 +678,10 @@ def main():
     ap.add_argument("--served-requests", type=int, default=400, help="served: requests per plain run")
     ap.add_argument("--served-loads", type=float, nargs="+", default=[6, 8, 10, 12, 14, 16],
                     help="served: offered loads as multiples of the plain run's")
+    ap.add_argument("--create-files", type=int, default=16, help="create: input files")
+    ap.add_argument("--create-file-mib", type=int, default=64, help="create: MiB per input file")
+    ap.add_argument("--create-block", type=int, default=0, help="create: block size in bytes (0: automatic)")
+    ap.add_argument("--create-redundancy", type=float, default=10.0, help="create: redundancy percent")
     ap.add_argument("--seed", type=int, default=4)
     ap.add_argument("--kernel", type=int, default=0, help="0 auto, 1 LDS tables, 2 MFMA FP4, 3 MFMA int8")
     ap.add_argument("--autotune", type=int, default=None,
