@@ -38,12 +38,21 @@ HBM_PEAK_GBS = 8000.0
 # 1 GiB device-to-device copy (--workload copy): FETCH_SIZE reads half the
 # bytes the copy reads (factor 2.0), WRITE_SIZE all it writes (factor 1.0).
 C2_GF_TRAFFIC_BYTES = 2.475e9
+# The same measurement for the bit-sliced FP4 contraction that the automatic
+# choice runs in the encode step (beside the MD5 chains): FETCH_SIZE 1,522,064
+# KiB x 2.0 + WRITE_SIZE 104,448 KiB per launch.
+C2_GF_MFMA_TRAFFIC_BYTES = 3.224e9
 # Unit utilisation of the same contraction (default tile shape, alone on the
 # device) from rocprofv3 --pmc SQ_LDS_IDX_ACTIVE / SQ_ACTIVE_INST_VALU /
 # SQ_LDS_BANK_CONFLICT / GRBM_GUI_ACTIVE runs of this workload: the binding
 # units are the LDS lookups and their XOR folds, not HBM.
 C2_GF_UNITS = {"lds_util": 0.76, "valu_busy": 0.665, "lds_bank_conflict": 0.0, "sclk_ghz": 2.0,
                "source": "rocprofv3 --pmc, profiles/gf_units_utilisation_pmc.md"}
+# The same for the bit-sliced FP4 matrix-core contraction (SQ_VALU_MFMA_BUSY_CYCLES etc.).
+C2_GF_MFMA_UNITS = {"mfma_util": 0.349, "lds_util": 0.229, "valu_busy": 0.26, "lds_bank_conflict": 0.065,
+                    "sclk_ghz": 2.37, "source": "rocprofv3 --pmc, profiles/gf_units_utilisation_pmc.md"}
+FP4_DENSE_PEAK_TFLOPS = 10100.0
+INT8_DENSE_PEAK_TOPS = 5000.0
 METRIC = "encode/repair GB/s source data + p50/p99 latency vs plain GPU path, bit-exact"
 
 
@@ -131,9 +140,29 @@ def bench_encode(args, ctx):
     crc_avg = stats.get("crc32_slices", (1, 0.0))[1] / max(stats.get("crc32_slices", (1, 0))[0], 1)
     achieved = algo_bytes / (gf_avg * 1e-3) / 1e9 if gf_avg > 0 else 0.0
     traffic = args.traffic
-    default_shape = not args.autotune and gf_name == "gf_muladd_lds" and (n, s, r) == (1024, 1 << 20, 102)
+    c2 = (n, s, r) == (1024, 1 << 20, 102)
+    default_shape = not args.autotune and gf_name == "gf_muladd_lds" and c2
     if traffic is None and default_shape:
         traffic = C2_GF_TRAFFIC_BYTES
+    if traffic is None and gf_name == "gf_muladd_mfma" and c2:
+        traffic = C2_GF_MFMA_TRAFFIC_BYTES
+    muladds = n * r * (s // 2)
+    hbm = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+           "frac": round(achieved / HBM_PEAK_GBS, 5)}
+    if gf_name.startswith("gf_muladd_mfma"):
+        # the bit-sliced contraction is one (16R x 16N) x (16N x W) GF(2) GEMM on
+        # the matrix cores: 256 binary multiply-adds, 512 operations, per GF
+        # multiply-add; its roof is the dense FP4 (int8) rate
+        i8 = gf_name.endswith("_i8")
+        ops = 512.0 * muladds / (gf_avg * 1e-3) / 1e12 if gf_avg > 0 else 0.0
+        peak = INT8_DENSE_PEAK_TOPS if i8 else FP4_DENSE_PEAK_TFLOPS
+        roof = {"bound": "mfma", "achieved": round(ops, 1), "peak": peak, "unit": "TOP/s" if i8 else "TFLOP/s",
+                "frac": round(ops / peak, 4), "hbm_achieved_GBps": round(achieved, 2),
+                "hbm_frac": round(achieved / HBM_PEAK_GBS, 5)}
+        units = C2_GF_MFMA_UNITS if (c2 and not i8) else None
+    else:
+        roof = dict(hbm)
+        units = C2_GF_UNITS if default_shape else None
     return {
         "workload": "plain_encode",
         "config": {"workload": "plain_encode_1GiB", "n_slices": n, "slice_bytes": s, "recovery_slices": r,
@@ -141,12 +170,11 @@ def bench_encode(args, ctx):
                    "resident": "hbm"},
         "src_bytes": src_bytes, "total_s": total, "lat": lat, "exact": exact,
         "kernel_ms": {k: v[1] / max(v[0], 1) for k, v in stats.items()},
-        "roofline": {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": traffic,
+        "roofline": {**roof, "traffic": traffic,
                      "traffic_x_algorithmic": round(traffic / algo_bytes, 3) if traffic else None,
                      "kernel": gf_name, "kernel_ms": round(gf_avg, 4), "algorithmic_bytes": algo_bytes,
-                     "gf_muladds": n * r * (s // 2), "md5_ms": round(md5_avg, 4), "crc32_ms": round(crc_avg, 4),
-                     "units": C2_GF_UNITS if default_shape else None},
+                     "gf_muladds": muladds, "md5_ms": round(md5_avg, 4), "crc32_ms": round(crc_avg, 4),
+                     "units": units},
         "cpu": (n, s, r),
     }
 

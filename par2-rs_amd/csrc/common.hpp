@@ -161,7 +161,15 @@ struct GfProblem {
   bool accumulate;
   int align_words;              // every pointer and `words` stride aligned to this many words
   const char *tag = nullptr;    // kernel name for timing (default: gf_muladd_<variant>)
+  // > 0: the contraction runs beside serial work (the MD5 chains of the same
+  // step) expected to take at least this long; the automatic kernel choice
+  // then prefers the form that fits under it while drawing the least power
+  double hidden_ms = 0;
 };
+// Lower bound on the time of n MD5 chains of `bytes` each (zero padding and
+// length block included) on `simds` SIMDs, one wave of 64 chains per SIMD at a
+// time, at the 2.4 GHz peak shader clock.
+double md5_chains_ms(size_t n, uint64_t bytes, int simds);
 int32_t run_gf(par2hip_ctx *ctx, hipStream_t s, const GfProblem &p);
 // The process-wide table of tuned tile shapes as text (one "key v g d" line each).
 int32_t tune_table_save(const char *path);

@@ -12,6 +12,9 @@
 // a contiguous chunk with slice-by-4 tables held in LDS, and the partial
 // remainders are joined by multiplying with x^(8*bytes_after) mod P.  Slices
 // under 64 KiB take one lane each (the plain sequential CRC).
+#include <algorithm>
+#include <cmath>
+
 #include "common.hpp"
 
 namespace par2hip {
@@ -268,6 +271,15 @@ __global__ __launch_bounds__(64) void k_md5_list(const HashEntry *__restrict__ l
     md5_slice<4>(gptr(e.data), e.len, e.size, gptr(e.md5));
   else
     md5_slice<1>(gptr(e.data), e.len, e.size, gptr(e.md5));
+}
+
+double md5_chains_ms(size_t n, uint64_t bytes, int simds) {
+  // 337 instructions per 64-byte block in md5_compress's loop, one issue per
+  // 4 clocks per wave (a wave64 VALU instruction occupies its SIMD 4 clocks)
+  const double blocks = (double)((bytes + 8) / 64 + 1);
+  const double waves = (double)((n + 63) / 64);
+  const double rounds = std::ceil(waves / std::max(simds, 1));
+  return rounds * blocks * 337.0 * 4.0 / 2.4e9 * 1e3;
 }
 
 int32_t launch_md5_list(par2hip_ctx *ctx, hipStream_t s, const HashEntry *d_list, size_t n, bool aligned4,

@@ -264,6 +264,9 @@ int32_t encode_body(par2hip_ctx *ctx, const uint8_t *d_slices, size_t n, size_t 
     p.accumulate = false;
     p.align_words = std::min({align_words_of((uintptr_t)d_slices), align_words_of((uintptr_t)d_rec),
                               align_words_of(slice_size), align_words_of(rec_stride)});
+    // with MD5 on the reserved CUs the contraction only has to finish before
+    // the chains do
+    if (d_md5 && ctx->masked) p.hidden_ms = md5_chains_ms(n, slice_size, 4 * ctx->hash_cus);
     if ((rc = run_gf(ctx, ctx->cur, p))) return rc;
   }
   return PAR2HIP_OK;

@@ -439,3 +439,54 @@ def test_reconstruct_many_erasures(kctx):
     say("reconstruct m=300")
     out = ctx.reconstruct(data, missing, rec, np.arange(m))
     assert np.array_equal(out, data[missing])
+
+
+# ------------------------------------------------- automatic kernel choice --
+def _encode_dev_profiled(ctx, data, r):
+    n, s = data.shape
+    d_sl, d_rec, d_md5, d_crc = ctx.alloc(n * s), ctx.alloc(r * s), ctx.alloc(n * 16), ctx.alloc(n * 4)
+    d_sl.upload(data.reshape(-1))
+    prev = ctx.set_gf_kernel(0)
+    ctx.profile(True)
+    ctx.profile_read()
+    try:
+        ctx.encode_dev(d_sl, n, s, np.arange(r, dtype=np.uint32), d_rec, d_md5, d_crc)
+        ctx.sync()
+        stats = ctx.profile_read()
+    finally:
+        ctx.profile(False)
+        ctx.set_gf_kernel(prev)
+    rec = d_rec.download().reshape(r, s)
+    md5 = d_md5.download().reshape(n, 16)
+    for b in (d_sl, d_rec, d_md5, d_crc):
+        b.free()
+    return rec, md5, stats
+
+
+def test_auto_choice_under_md5_chains(ctx):
+    """1024 x 1 MiB, R=102 with the IFSC hashes: the matrix-core contraction
+    fits under the MD5 chains, so the automatic choice takes it (on a device
+    with reserved hash CUs), and the step stays bit-exact."""
+    n, s, r = 1024, 1 << 20, 102
+    data = np.random.default_rng(21).integers(0, 256, (n, s), dtype=np.uint8)
+    say("auto choice, encode", n, s, r)
+    rec, md5, stats = _encode_dev_profiled(ctx, data, r)
+    if ctx.device_info()["hash_cus"] > 0:
+        assert "gf_muladd_mfma" in stats and "gf_muladd_lds" not in stats, stats
+    for w0 in (0, 123457):
+        cols = np.ascontiguousarray(data.view(np.uint16)[:, w0:w0 + 64])
+        assert np.array_equal(rec.view(np.uint16)[:, w0:w0 + 64], oracle.encode(cols, np.arange(r))), w0
+    for i in (0, 517, n - 1):
+        assert md5[i].tobytes() == hashlib.md5(data[i].tobytes()).digest(), i
+
+
+def test_auto_choice_short_chains_keep_lds(ctx):
+    """2000 x 8 KiB, R=200: the chains are far shorter than a matrix-core
+    contraction of this size, so the LDS-table kernel runs."""
+    n, s, r = 2000, 8192, 200
+    data = np.random.default_rng(22).integers(0, 256, (n, s), dtype=np.uint8)
+    say("auto choice, short chains", n, s, r)
+    rec, _, stats = _encode_dev_profiled(ctx, data, r)
+    assert "gf_muladd_lds" in stats and "gf_muladd_mfma" not in stats, stats
+    cols = np.ascontiguousarray(data.view(np.uint16)[:, 1000:1064])
+    assert np.array_equal(rec.view(np.uint16)[:, 1000:1064], oracle.encode(cols, np.arange(r)))
