@@ -72,6 +72,12 @@ struct par2hip_ctx {
   bool masked = false;                // side/rest carry complementary CU masks
   int hash_cus = 0;                   // CUs of the side stream's mask
   hipStream_t wide = nullptr;         // unmasked helper stream (hash loads too big for the side CUs)
+  // narrower complements of the side stream (narrow_cus[i] CUs each), for a
+  // contraction that only has to finish under the MD5 chains and should draw
+  // as little power as that allows; rest_cur is the one in use until join
+  hipStream_t narrow[3] = {nullptr, nullptr, nullptr};
+  int narrow_cus[3] = {0, 0, 0};
+  hipStream_t rest_cur = nullptr;
   char arch[32] = {0};
   hipEvent_t fork = nullptr, join = nullptr, join2 = nullptr;
   std::recursive_mutex mu;
@@ -161,11 +167,12 @@ struct GfProblem {
   bool accumulate;
   int align_words;              // every pointer and `words` stride aligned to this many words
   const char *tag = nullptr;    // kernel name for timing (default: gf_muladd_<variant>)
-  // > 0: the contraction runs beside serial work (the MD5 chains of the same
-  // step) expected to take at least this long; the automatic kernel choice
-  // then prefers the form that fits under it while drawing the least power
-  double hidden_ms = 0;
+  int kernel = 0;               // 1 LDS tables, 2 / 3 MFMA FP4 / int8; 0: the context's choice
 };
+// Rate of the bit-sliced FP4 contraction per CU (binary multiply-adds per
+// second) measured on the 1 GiB / R=102 encode: 1.40e13 over 5.02 ms on 256
+// CUs alone and over 8.04 ms on 160 CUs beside the MD5 chains.
+constexpr double kFp4MacPerCuSec = 1.09e13;
 // Lower bound on the time of n MD5 chains of `bytes` each (zero padding and
 // length block included) on `simds` SIMDs, one wave of 64 chains per SIMD at a
 // time, at the 2.4 GHz peak shader clock.

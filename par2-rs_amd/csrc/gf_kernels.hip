@@ -276,23 +276,8 @@ int32_t run_gf(par2hip_ctx *ctx, hipStream_t s, const GfProblem &p) {
   // 1 = LDS tables, 2 / 3 = bit-sliced MFMA with FP4 / int8 operands (needs
   // 16-byte aligned operands), 0 = automatic
   const bool mfma_ok = p.align_words >= 8;
-  int which = ctx->gf_kernel;
-  if (which == 0) {
-    which = kAutoGfKernel;
-    // Alone, the LDS-table contraction is the faster one.  Beside serial MD5
-    // chains the chip is power-bound: the LDS form pulls the shader clock to
-    // ~2.0-2.2 GHz for all CUs, the FP4 matrix-core form leaves it at ~2.37
-    // GHz, so the chains (the step's critical path) finish ~0.7 ms sooner on
-    // the 1 GiB / R=102 encode.  Choose it when its estimated time fits under
-    // the chains: 256 binary MACs per GF multiply-add at the rate measured
-    // for this kernel, 6.7e12 MAC/s per CU, with a 5% margin.
-    if (p.hidden_ms > 0 && mfma_ok) {
-      const int cus = ctx->masked ? ctx->num_cus - ctx->hash_cus : ctx->num_cus;
-      const double macs = 256.0 * p.ndst * p.nsrc * (double)p.words;
-      const double est_ms = macs / (6.7e12 * std::max(cus, 1)) * 1e3;
-      if (est_ms * 1.05 <= p.hidden_ms) which = 2;
-    }
-  }
+  int which = p.kernel ? p.kernel : ctx->gf_kernel;
+  if (which == 0) which = kAutoGfKernel;
   if ((which == 2 || which == 3) && mfma_ok) return run_gf_mfma(ctx, s, p, which == 3);
   return run_gf_lds(ctx, s, p);
 }

@@ -118,12 +118,16 @@ __device__ __forceinline__ v8i expand_fp4(uint32_t y) {
 // reads and expansion overlap the other's MFMAs.
 // Lane (n, g) of wave v owns words tile + 64v + 2n + f for B fragment f.
 // kI8: int8 operands (A kept packed in LDS and expanded by each wave) instead
-// of FP4 (A expanded once by the staging threads).
-template <bool kI8>
+// of FP4 (A expanded once by the staging threads).  kRagged: the launch covers
+// tiles tile0.. whose words may run past the end of the rows (only the last
+// tile column), loaded word by word with checks; the main launch has no such
+// code, so its loads carry no branches.
+template <bool kI8, bool kRagged>
 __global__ __launch_bounds__(kMfThreads, 1) void k_gf_mfma(const uint16_t *const *__restrict__ srcs, int nsrc,
                                                            uint16_t *const *__restrict__ dsts, int ndst,
                                                            const uint32_t *__restrict__ apk, int nmf, int nks,
-                                                           long words, int accumulate, int ntiles, int nrowblk) {
+                                                           long words, int accumulate, int ntiles, int nrowblk,
+                                                           int tile0) {
   using AUnit = typename std::conditional<kI8, uint32_t, uint4>::type;
   using Acc = typename std::conditional<kI8, v16i, v16f>::type;
   __shared__ uint4 sB[2][4 * kMfKc][kMfWgWords / 8];   // [buf][source][16-byte unit], 64 KiB
@@ -133,8 +137,9 @@ __global__ __launch_bounds__(kMfThreads, 1) void k_gf_mfma(const uint16_t *const
   const int xcd = L & 7, q = L >> 3;
   const int tiles_per_xcd = (ntiles + 7) >> 3;
   const int local_tile = q / nrowblk, rowblk = q % nrowblk;
-  const int tile = local_tile * 8 + xcd;
-  if (local_tile >= tiles_per_xcd || tile >= ntiles) return;
+  const int tile_local = local_tile * 8 + xcd;
+  if (local_tile >= tiles_per_xcd || tile_local >= ntiles) return;
+  const int tile = tile0 + tile_local;
 
   const int tid = threadIdx.x;
   const int wave = tid >> 6, lane = tid & 63;
@@ -146,40 +151,62 @@ __global__ __launch_bounds__(kMfThreads, 1) void k_gf_mfma(const uint16_t *const
   // staging roles.  B: unit tid + 512k -> source (tid>>6) + 8k, 16-byte column tid&63.
   // A: thread tid owns packed words 4*tid..4*tid+3 of the stage: fragment tid>>7,
   // K-step (tid>>4)&7, lanes 4*(tid&15)..+3.
-  uint4 rb[4], ra;
-  auto load_stage = [&](int c) {
+  //
+  // The loads of stage c+1 are issued before stage c's MFMAs and consumed only
+  // by store_stage after them, so they must not be waited for in between: no
+  // load sits behind a branch or feeds a select before store_stage.  The four
+  // source pointers of a stage are wave-uniform and fetched one stage ahead;
+  // sources past the last one and A words past the last fragment or K-step
+  // are loaded from clamped (valid) addresses and zeroed in store_stage.  Only
+  // a ragged last word tile takes the checked per-word path.
+  const int wv = __builtin_amdgcn_readfirstlane(wave);
+  const P2H_GLOBAL uint16_t *pn[4];  // source rows of the next stage to load
+  auto fetch_ptrs = [&](int c) {
+    pn[0] = gptr(gptr(srcs)[min(32 * c + wv, nsrc - 1)]);
+    pn[1] = gptr(gptr(srcs)[min(32 * c + wv + 8, nsrc - 1)]);
+    pn[2] = gptr(gptr(srcs)[min(32 * c + wv + 16, nsrc - 1)]);
+    pn[3] = gptr(gptr(srcs)[min(32 * c + wv + 24, nsrc - 1)]);
+  };
+  u32x4 rb0, rb1, rb2, rb3, ra;
+  uint32_t ok_mask;  // bit k: source row k of the stage exists; bit 4: the A words exist
+  auto load_b = [&](int c, int k, const P2H_GLOBAL uint16_t *src) -> u32x4 {
+    const long w = tile_w0 + 8 * (tid & 63);
+    if (32 * c + wv + 8 * k < nsrc) ok_mask |= 1u << k;
+    if constexpr (!kRagged) {
+      return *reinterpret_cast<const P2H_GLOBAL u32x4 *>(src + w);
+    } else {
+      uint16_t t[8];
 #pragma unroll
-    for (int k = 0; k < 4; k++) {
-      const int d = 32 * c + (tid >> 6) + 8 * k;
-      const long w = tile_w0 + 8 * (tid & 63);
-      uint4 v = make_uint4(0, 0, 0, 0);
-      if (d < nsrc) {
-        const uint16_t *p = srcs[d];
-        if (w + 8 <= words) {
-          v = *reinterpret_cast<const uint4 *>(p + w);
-        } else if (w < words) {
-          uint16_t t[8];
-#pragma unroll
-          for (int e = 0; e < 8; e++) t[e] = (w + e < words) ? p[w + e] : 0;
-          v = make_uint4(t[0] | (uint32_t)t[1] << 16, t[2] | (uint32_t)t[3] << 16, t[4] | (uint32_t)t[5] << 16,
-                         t[6] | (uint32_t)t[7] << 16);
-        }
-      }
-      rb[k] = v;
+      for (int e = 0; e < 8; e++) t[e] = (w + e < words) ? src[w + e] : 0;
+      const u32x4 v = {t[0] | (uint32_t)t[1] << 16, t[2] | (uint32_t)t[3] << 16, t[4] | (uint32_t)t[5] << 16,
+                       t[6] | (uint32_t)t[7] << 16};
+      return v;
     }
+  };
+  auto load_stage = [&](int c) {
+    ok_mask = 0;
+    rb0 = load_b(c, 0, pn[0]);
+    rb1 = load_b(c, 1, pn[1]);
+    rb2 = load_b(c, 2, pn[2]);
+    rb3 = load_b(c, 3, pn[3]);
     {
       const int frag = tid >> 7, ksl = (tid >> 4) & 7, l4 = 4 * (tid & 15);
       const int ks = kMfKc * c + ksl;
-      ra = make_uint4(0, 0, 0, 0);
-      if (mf0 + frag < nmf && ks < nks)
-        ra = *reinterpret_cast<const uint4 *>(apk + ((long)(mf0 + frag) * nks + ks) * 64 + l4);
+      if (mf0 + frag < nmf && ks < nks) ok_mask |= 1u << 4;
+      const long idx = ((long)min(mf0 + frag, nmf - 1) * nks + min(ks, nks - 1)) * 64 + l4;
+      ra = *reinterpret_cast<const P2H_GLOBAL u32x4 *>(gptr(apk) + idx);
     }
+    fetch_ptrs(c + 1);
   };
   auto store_stage = [&](int buf) {
+    const u32x4 zero = {0u, 0u, 0u, 0u};
+    const u32x4 b[4] = {(ok_mask & 1u) ? rb0 : zero, (ok_mask & 2u) ? rb1 : zero, (ok_mask & 4u) ? rb2 : zero,
+                        (ok_mask & 8u) ? rb3 : zero};
 #pragma unroll
-    for (int k = 0; k < 4; k++) sB[buf][(tid >> 6) + 8 * k][tid & 63] = rb[k];
+    for (int k = 0; k < 4; k++) sB[buf][(tid >> 6) + 8 * k][tid & 63] = make_uint4(b[k].x, b[k].y, b[k].z, b[k].w);
     const int frag = tid >> 7, ksl = (tid >> 4) & 7, l4 = 4 * (tid & 15);
-    const uint32_t y[4] = {ra.x, ra.y, ra.z, ra.w};
+    const u32x4 a = (ok_mask & 16u) ? ra : zero;
+    const uint32_t y[4] = {a.x, a.y, a.z, a.w};
 #pragma unroll
     for (int e = 0; e < 4; e++) {
       if constexpr (kI8)
@@ -198,6 +225,7 @@ __global__ __launch_bounds__(kMfThreads, 1) void k_gf_mfma(const uint16_t *const
 #pragma unroll
       for (int k = 0; k < 16; k++) acc[i][f][k] = 0;
 
+  fetch_ptrs(0);
   load_stage(0);
   store_stage(0);
   __syncthreads();
@@ -263,11 +291,11 @@ __global__ __launch_bounds__(kMfThreads, 1) void k_gf_mfma(const uint16_t *const
       for (int k = 0; k < 16; k++) word |= ((uint32_t)acc[i][f][k] & 1u) << k;  // exact integer sums
       v[f] = word;
     }
-    uint16_t *p = dsts[r];
+    P2H_GLOBAL uint16_t *p = gptr(gptr(dsts)[r]);
     if (w + 2 <= words) {
       uint32_t out = v[0] | (v[1] << 16);
-      if (accumulate) out ^= *reinterpret_cast<const uint32_t *>(p + w);
-      *reinterpret_cast<uint32_t *>(p + w) = out;
+      if (accumulate) out ^= *reinterpret_cast<const P2H_GLOBAL uint32_t *>(p + w);
+      *reinterpret_cast<P2H_GLOBAL uint32_t *>(p + w) = out;
     } else {
       p[w] = accumulate ? (uint16_t)(p[w] ^ v[0]) : (uint16_t)v[0];
     }
@@ -289,16 +317,23 @@ int32_t run_gf_mfma(par2hip_ctx *ctx, hipStream_t s, const GfProblem &p, bool in
     P2H_CHECK(ctx, hipGetLastError());
   }
   const int ntiles = (int)((p.words + kMfWgWords - 1) / kMfWgWords);
+  const int nfull = (int)(p.words / kMfWgWords);  // tiles whose words all lie inside the rows
   const int nrowblk = (nmf + kMfRows - 1) / kMfRows;
   const long nwg = 8L * ((ntiles + 7) / 8) * nrowblk;
   if (nwg > 0x7fffffffL) return fail(ctx, PAR2HIP_EINVAL, "GF problem too large for one launch");
   ScopedTiming t(ctx, p.tag ? p.tag : (int8 ? "gf_muladd_mfma_i8" : "gf_muladd_mfma"), s);
-  if (int8)
-    hipLaunchKernelGGL(k_gf_mfma<true>, dim3((unsigned)nwg), dim3(kMfThreads), 0, s, p.srcs, p.nsrc, p.dsts, p.ndst,
-                       (const uint32_t *)apk, nmf, nks, (long)p.words, p.accumulate ? 1 : 0, ntiles, nrowblk);
-  else
-    hipLaunchKernelGGL(k_gf_mfma<false>, dim3((unsigned)nwg), dim3(kMfThreads), 0, s, p.srcs, p.nsrc, p.dsts, p.ndst,
-                       (const uint32_t *)apk, nmf, nks, (long)p.words, p.accumulate ? 1 : 0, ntiles, nrowblk);
+  auto launch = [&](auto kernel, int tiles, int tile0) {
+    const long grid = 8L * ((tiles + 7) / 8) * nrowblk;
+    hipLaunchKernelGGL(kernel, dim3((unsigned)grid), dim3(kMfThreads), 0, s, p.srcs, p.nsrc, p.dsts, p.ndst,
+                       (const uint32_t *)apk, nmf, nks, (long)p.words, p.accumulate ? 1 : 0, tiles, nrowblk, tile0);
+  };
+  if (int8) {
+    if (nfull) launch(k_gf_mfma<true, false>, nfull, 0);
+    if (ntiles > nfull) launch(k_gf_mfma<true, true>, ntiles - nfull, nfull);
+  } else {
+    if (nfull) launch(k_gf_mfma<false, false>, nfull, 0);
+    if (ntiles > nfull) launch(k_gf_mfma<false, true>, ntiles - nfull, nfull);
+  }
   P2H_CHECK(ctx, hipGetLastError());
   return PAR2HIP_OK;
 }

@@ -58,6 +58,8 @@ int32_t scratch(par2hip_ctx *ctx, int role, size_t bytes, void **out) {
       P2H_CHECK(ctx, hipStreamSynchronize(ctx->side));
       P2H_CHECK(ctx, hipStreamSynchronize(ctx->rest));
       P2H_CHECK(ctx, hipStreamSynchronize(ctx->wide));
+      for (hipStream_t q : ctx->narrow)
+        if (q) P2H_CHECK(ctx, hipStreamSynchronize(q));
       P2H_CHECK(ctx, hipFree(b.ptr));
       b.ptr = nullptr;
       b.bytes = 0;
@@ -79,6 +81,8 @@ int32_t params_begin(par2hip_ctx *ctx, size_t bytes, uint8_t *target) {
     P2H_CHECK(ctx, hipStreamSynchronize(ctx->side));
     P2H_CHECK(ctx, hipStreamSynchronize(ctx->rest));
     P2H_CHECK(ctx, hipStreamSynchronize(ctx->wide));
+    for (hipStream_t q : ctx->narrow)
+      if (q) P2H_CHECK(ctx, hipStreamSynchronize(q));
     if (a.host) (void)hipHostFree(a.host);
     if (a.dev) (void)hipFree(a.dev);
     a.host = nullptr;
@@ -179,22 +183,27 @@ void probe_devices() {
 // Overlapped section: the side stream (MD5 chains, a few reserved CUs) and
 // the rest stream (all other CUs) both start after everything queued on the
 // main stream; the call's main work then goes to `rest` until join.
-int32_t fork_side(par2hip_ctx *ctx) {
+// work: the stream for the call's main work (default: rest, every CU outside
+// the side stream's mask).
+int32_t fork_side(par2hip_ctx *ctx, hipStream_t work = nullptr) {
+  if (!work) work = ctx->rest;
   P2H_CHECK(ctx, hipEventRecord(ctx->fork, ctx->stream));
   P2H_CHECK(ctx, hipStreamWaitEvent(ctx->side, ctx->fork, 0));
-  if (ctx->rest != ctx->stream) P2H_CHECK(ctx, hipStreamWaitEvent(ctx->rest, ctx->fork, 0));
-  ctx->cur = ctx->rest;
+  if (work != ctx->stream) P2H_CHECK(ctx, hipStreamWaitEvent(work, ctx->fork, 0));
+  ctx->cur = ctx->rest_cur = work;
   return PAR2HIP_OK;
 }
 
 int32_t join_side(par2hip_ctx *ctx) {
   P2H_CHECK(ctx, hipEventRecord(ctx->join, ctx->side));
   P2H_CHECK(ctx, hipStreamWaitEvent(ctx->stream, ctx->join, 0));
-  if (ctx->rest != ctx->stream) {
-    P2H_CHECK(ctx, hipEventRecord(ctx->join2, ctx->rest));
+  hipStream_t work = ctx->rest_cur ? ctx->rest_cur : ctx->rest;
+  if (work != ctx->stream) {
+    P2H_CHECK(ctx, hipEventRecord(ctx->join2, work));
     P2H_CHECK(ctx, hipStreamWaitEvent(ctx->stream, ctx->join2, 0));
   }
   ctx->cur = ctx->stream;
+  ctx->rest_cur = nullptr;
   return PAR2HIP_OK;
 }
 
@@ -202,9 +211,43 @@ int32_t join_side(par2hip_ctx *ctx) {
 struct Overlap {
   par2hip_ctx *ctx;
   int32_t rc;
-  explicit Overlap(par2hip_ctx *c) : ctx(c) { rc = fork_side(c); }
+  explicit Overlap(par2hip_ctx *c, hipStream_t work = nullptr) : ctx(c) { rc = fork_side(c, work); }
   ~Overlap() { (void)join_side(ctx); }
 };
+
+// Kernel and CUs of an encode contraction that overlaps the MD5 chains of
+// the same call on the side CUs.  Alone, the LDS-table contraction is the
+// faster one; beside the chains the chip is power-bound: the LDS form holds
+// every CU busy on lookups and pulls the shader clock to ~2.0-2.2 GHz, and
+// the chains, the step's critical path, slow with it.  The bit-sliced FP4
+// form on only as many CUs as it needs to finish under the chains (here:
+// the narrowest of the narrow streams and the full complement that brings
+// its estimate under 88% of the chains' lower-bound time) leaves them the
+// clock (1 GiB / R=102: 10.7 ms with the LDS form on 248 CUs, 10.75 ms with
+// the FP4 form on 248, 9.98 ms on 160).  If no width fits, the contraction
+// is the critical path and the LDS form on every other CU runs.
+struct GfPlan {
+  int kernel = 0;             // 0: the context's choice
+  hipStream_t work = nullptr;  // nullptr: rest
+};
+static GfPlan plan_overlapped_gf(const par2hip_ctx *ctx, size_t n, size_t slice_size, size_t r, int align_words) {
+  GfPlan g;
+  if (!ctx->masked || ctx->gf_kernel != 0 || r == 0) return g;
+  g.kernel = 1;
+  if (align_words < 8) return g;
+  const double md5_ms = md5_chains_ms(n, slice_size, 4 * ctx->hash_cus);
+  const double macs = 256.0 * (double)r * (double)n * (double)(slice_size / 2);
+  for (int i = 0; i <= 3; i++) {
+    const int cus = i < 3 ? ctx->narrow_cus[i] : ctx->num_cus - ctx->hash_cus;
+    if (cus <= 0 || (i < 3 && !ctx->narrow[i])) continue;
+    if (macs / (kFp4MacPerCuSec * cus) * 1e3 <= 0.88 * md5_ms) {
+      g.kernel = 2;
+      g.work = i < 3 ? ctx->narrow[i] : nullptr;
+      break;
+    }
+  }
+  return g;
+}
 
 int align_words_of(uintptr_t p) {
   int a = 8;
@@ -241,9 +284,12 @@ int32_t encode_body(par2hip_ctx *ctx, const uint8_t *d_slices, size_t n, size_t 
 
   // MD5 is a long serial chain per slice: it runs on the side stream's CUs
   // from the start while CRC-32 and the contraction use the other CUs.
+  const int align = std::min({align_words_of((uintptr_t)d_slices), align_words_of((uintptr_t)d_rec),
+                              align_words_of(slice_size), align_words_of(rec_stride)});
+  const GfPlan plan = d_md5 ? plan_overlapped_gf(ctx, n, slice_size, r, align) : GfPlan{};
   std::unique_ptr<Overlap> ov;
   if (d_md5) {
-    ov.reset(new Overlap(ctx));
+    ov.reset(new Overlap(ctx, plan.work));
     if (ov->rc) return ov->rc;
     if ((rc = launch_md5(ctx, ctx->side, d_slices, n, slice_size, slice_size, nullptr, d_md5, !ctx->masked)))
       return rc;
@@ -262,11 +308,8 @@ int32_t encode_body(par2hip_ctx *ctx, const uint8_t *d_slices, size_t n, size_t 
     p.ndst = (int)r;
     p.words = words;
     p.accumulate = false;
-    p.align_words = std::min({align_words_of((uintptr_t)d_slices), align_words_of((uintptr_t)d_rec),
-                              align_words_of(slice_size), align_words_of(rec_stride)});
-    // with MD5 on the reserved CUs the contraction only has to finish before
-    // the chains do
-    if (d_md5 && ctx->masked) p.hidden_ms = md5_chains_ms(n, slice_size, 4 * ctx->hash_cus);
+    p.align_words = align;
+    p.kernel = plan.kernel;
     if ((rc = run_gf(ctx, ctx->cur, p))) return rc;
   }
   return PAR2HIP_OK;
@@ -525,6 +568,16 @@ int32_t par2hip_ctx_create(int32_t device, par2hip_ctx **out) {
     }
     ctx->masked = ok;
     ctx->hash_cus = ok ? hash_cus : 0;
+    for (int i = 0; ok && i < 3; i++) {
+      const int k = 128 + 32 * i;  // 128, 160, 192 CUs after the side stream's
+      if (hash_cus + k >= ncu) break;
+      std::vector<uint32_t> m((ncu + 31) / 32, 0u);
+      for (int c = hash_cus; c < hash_cus + k; c++) m[c / 32] |= 1u << (c % 32);
+      if (hipExtStreamCreateWithCUMask(&ctx->narrow[i], (uint32_t)m.size(), m.data()) == hipSuccess)
+        ctx->narrow_cus[i] = k;
+      else
+        ctx->narrow[i] = nullptr;
+    }
     if (!ok) {
       P2H_CHECK(ctx, hipStreamCreateWithFlags(&ctx->side, hipStreamNonBlocking));
       ctx->rest = ctx->stream;
@@ -554,6 +607,8 @@ int32_t par2hip_ctx_destroy(par2hip_ctx *ctx) {
     (void)hipStreamSynchronize(ctx->side);
     if (ctx->rest != ctx->stream) (void)hipStreamSynchronize(ctx->rest);
     (void)hipStreamSynchronize(ctx->wide);
+    for (hipStream_t q : ctx->narrow)
+      if (q) (void)hipStreamSynchronize(q);
     for (auto &b : ctx->scratch)
       if (b.ptr) (void)hipFree(b.ptr);
     if (ctx->arena.host) (void)hipHostFree(ctx->arena.host);
@@ -572,6 +627,8 @@ int32_t par2hip_ctx_destroy(par2hip_ctx *ctx) {
     (void)hipEventDestroy(ctx->join2);
     if (ctx->rest != ctx->stream) (void)hipStreamDestroy(ctx->rest);
     (void)hipStreamDestroy(ctx->wide);
+    for (hipStream_t q : ctx->narrow)
+      if (q) (void)hipStreamDestroy(q);
     (void)hipEventDestroy(ctx->arena.copied);
     (void)hipStreamDestroy(ctx->stream);
     (void)hipStreamDestroy(ctx->side);
@@ -637,6 +694,8 @@ int32_t par2hip_sync(par2hip_ctx *ctx) {
   P2H_CHECK(ctx, hipStreamSynchronize(ctx->side));
   P2H_CHECK(ctx, hipStreamSynchronize(ctx->rest));
   P2H_CHECK(ctx, hipStreamSynchronize(ctx->wide));
+  for (hipStream_t q : ctx->narrow)
+    if (q) P2H_CHECK(ctx, hipStreamSynchronize(q));
   return PAR2HIP_OK;
 }
 
@@ -967,6 +1026,8 @@ int32_t par2hip_profile_read(par2hip_ctx *ctx, par2hip_kernel_stat *out, size_t 
   P2H_CHECK(ctx, hipStreamSynchronize(ctx->side));
   P2H_CHECK(ctx, hipStreamSynchronize(ctx->rest));
   P2H_CHECK(ctx, hipStreamSynchronize(ctx->wide));
+  for (hipStream_t q : ctx->narrow)
+    if (q) P2H_CHECK(ctx, hipStreamSynchronize(q));
   for (auto &p : ctx->pending) {
     float ms = 0;
     if (hipEventElapsedTime(&ms, p.start, p.stop) == hipSuccess) {
