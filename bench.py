@@ -39,18 +39,21 @@ HBM_PEAK_GBS = 8000.0
 # bytes the copy reads (factor 2.0), WRITE_SIZE all it writes (factor 1.0).
 C2_GF_TRAFFIC_BYTES = 2.475e9
 # The same measurement for the bit-sliced FP4 contraction that the automatic
-# choice runs in the encode step (beside the MD5 chains): FETCH_SIZE 1,522,064
-# KiB x 2.0 + WRITE_SIZE 104,448 KiB per launch.
-C2_GF_MFMA_TRAFFIC_BYTES = 3.224e9
+# choice runs in the encode step (on 160 CUs beside the MD5 chains): FETCH_SIZE
+# 1,773,930 KiB x 2.0 + WRITE_SIZE 104,448 KiB per launch.
+C2_GF_MFMA_TRAFFIC_BYTES = 3.740e9
 # Unit utilisation of the same contraction (default tile shape, alone on the
 # device) from rocprofv3 --pmc SQ_LDS_IDX_ACTIVE / SQ_ACTIVE_INST_VALU /
 # SQ_LDS_BANK_CONFLICT / GRBM_GUI_ACTIVE runs of this workload: the binding
 # units are the LDS lookups and their XOR folds, not HBM.
 C2_GF_UNITS = {"lds_util": 0.76, "valu_busy": 0.665, "lds_bank_conflict": 0.0, "sclk_ghz": 2.0,
                "source": "rocprofv3 --pmc, profiles/gf_units_utilisation_pmc.md"}
-# The same for the bit-sliced FP4 matrix-core contraction (SQ_VALU_MFMA_BUSY_CYCLES etc.).
-C2_GF_MFMA_UNITS = {"mfma_util": 0.349, "lds_util": 0.229, "valu_busy": 0.26, "lds_bank_conflict": 0.065,
-                    "sclk_ghz": 2.37, "source": "rocprofv3 --pmc, profiles/gf_units_utilisation_pmc.md"}
+# The same for the bit-sliced FP4 matrix-core contraction as the encode step
+# runs it (on 160 of the 256 CUs): utilisations over the whole chip, and the
+# matrix-core one over its own CUs.
+C2_GF_MFMA_UNITS = {"mfma_util": 0.358, "mfma_util_own_cus": 0.573, "cus": 160, "lds_util": 0.224,
+                    "valu_busy": 0.271, "lds_bank_conflict": 0.067, "sclk_ghz": 2.25,
+                    "source": "rocprofv3 --pmc, profiles/gf_units_utilisation_pmc.md"}
 FP4_DENSE_PEAK_TFLOPS = 10100.0
 INT8_DENSE_PEAK_TOPS = 5000.0
 METRIC = "encode/repair GB/s source data + p50/p99 latency vs plain GPU path, bit-exact"
