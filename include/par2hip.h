@@ -341,9 +341,12 @@ int32_t par2hip_hash_slices_dev(par2hip_ctx *ctx, const void *d_slices, size_t n
 
 /* ------------------------------------------------------------- tuning -- */
 
-/* GF contraction kernel selection: 0 = automatic (the measured winner),
- * 1 = LDS nibble tables, 2 = bit-sliced MFMA with FP4 operands, 3 = the
- * bit-sliced MFMA with int8 operands.  Returns the previous value. */
+/* GF contraction kernel selection: 0 = automatic (the LDS tables, except an
+ * encode whose MD5 chains run on the reserved CUs: there the FP4 form on as
+ * few CUs as finish under the chains, which keeps the chip's clock up for
+ * them), 1 = LDS nibble tables, 2 = bit-sliced MFMA with FP4 operands, 3 =
+ * the bit-sliced MFMA with int8 operands.  No counterpart in the reference.
+ * Returns the previous value. */
 int32_t par2hip_set_gf_kernel(par2hip_ctx *ctx, int32_t which);
 
 /* Tile-shape autotuning of the LDS contraction (no counterpart in the

@@ -247,7 +247,9 @@ int32_t launch_inverse_batch(par2hip_ctx *ctx, hipStream_t s, const InvJob *d_jo
                              long max_vand);
 int32_t run_gf_lds(par2hip_ctx *ctx, hipStream_t s, const GfProblem &p);
 int32_t run_gf_mfma(par2hip_ctx *ctx, hipStream_t s, const GfProblem &p, bool int8);
-// Kernel used when the caller leaves the choice to the library.
+// Kernel used when the caller leaves the choice to the library and the call
+// has no plan of its own (an encode beside MD5 chains plans kernel and CUs):
+// the LDS-table contraction, the faster one alone.
 constexpr int kAutoGfKernel = 1;
 
 // Slice hashes.
