@@ -490,3 +490,21 @@ def test_auto_choice_short_chains_keep_lds(ctx):
     assert "gf_muladd_lds" in stats and "gf_muladd_mfma" not in stats, stats
     cols = np.ascontiguousarray(data.view(np.uint16)[:, 1000:1064])
     assert np.array_equal(rec.view(np.uint16)[:, 1000:1064], oracle.encode(cols, np.arange(r)))
+
+
+def test_max_sources_32768(kctx):
+    """The largest set PAR2's constants allow (32768 input slices, par2_rs.rs:64-76
+    with n up to 32768): encode with exponents up to the field's order against the
+    oracle, then rebuild erased slices from the recovery slices.  64-byte slices
+    keep the oracle fast and put every word tile past the row end."""
+    ctx = kctx
+    n, s = 32768, 64
+    exps = [0, 1, 65533, 65534]
+    data = np.random.default_rng(32768).integers(0, 256, (n, s), dtype=np.uint8)
+    say("encode n=32768", s, exps)
+    rec = ctx.encode(data, exps)
+    want = oracle.encode(np.ascontiguousarray(data.view(np.uint16)), np.asarray(exps))
+    assert np.array_equal(rec.view(np.uint16), want)
+    missing = [0, 4097, 20000, 32767]
+    out = ctx.reconstruct(data, missing, rec, exps)
+    assert np.array_equal(out, data[missing])
