@@ -696,12 +696,89 @@ def bench_create(args, ctx):
     return res
 
 
+def bench_repair_files(args, ctx):
+    """End-to-end repair from files (Par2Repairer: parse the index and its
+    volumes, verify every protected file, rebuild, commit through .par2tmp
+    files; src/parser.rs:181-547, src/verify.rs:87-330, src/repair.rs:445-1125).
+    The set (16 files of 64 MiB by default, automatic block size, 10%) is
+    created once; before each step (untimed) 100 blocks spread over half the
+    files are overwritten on disk.  One step = one par2hip_repair_files call:
+    loading, device verification of all files, the rebuild, the temp files
+    written, verified and renamed."""
+    import tempfile
+
+    nfiles, fbytes = args.create_files, args.create_file_mib << 20
+    pool = random_bytes(31, nfiles * fbytes)
+    files = [(f"part{i:02d}.bin", pool[i * fbytes:(i + 1) * fbytes].tobytes()) for i in range(nfiles)]
+    out_dir = tempfile.mkdtemp(prefix="par2repair_")
+    for name, b in files:
+        with open(os.path.join(out_dir, name), "wb") as f:
+            f.write(b)
+    written = ctx.create_par2(files, os.path.join(out_dir, "set"), block_size=args.create_block,
+                              redundancy=args.create_redundancy)
+    index = written[0][0]
+    info = par2hip.par2_info(index)
+    bs = info["block_size"]
+    per_file = -(-fbytes // bs)
+    m = min(100, info["recovery_slices"])
+    rng = np.random.default_rng(7)
+    damaged_files = list(range(0, nfiles, 2)) or [0]
+
+    def damage():
+        picks = set()
+        while len(picks) < m:
+            picks.add((int(rng.choice(damaged_files)), int(rng.integers(0, per_file))))
+        for fi, k in picks:
+            with open(os.path.join(out_dir, files[fi][0]), "r+b") as f:
+                f.seek(k * bs)
+                n = min(bs, fbytes - k * bs)
+                f.write(bytes(x ^ 0xA5 for x in files[fi][1][k * bs:k * bs + min(n, 64)]))
+        return picks
+
+    damage()
+    ctx.repair_files(index)  # warm-up
+    lat, exact = [], True
+    for _ in range(args.steps):
+        picks = damage()
+        t0 = time.perf_counter()
+        rep = ctx.repair_files(index)
+        lat.append(time.perf_counter() - t0)
+        exact &= rep["repaired"] == 1 and rep["blocks_damaged"] == len(picks)
+        for fi in {fi for fi, _ in picks}:
+            with open(os.path.join(out_dir, files[fi][0]), "rb") as f:
+                exact &= f.read() == files[fi][1]
+    for name in os.listdir(out_dir):
+        os.remove(os.path.join(out_dir, name))
+    os.rmdir(out_dir)
+    src = nfiles * fbytes
+    res = {
+        "workload": "repair_files",
+        "config": {"workload": f"repair_files_{nfiles}x{fbytes >> 20}MiB", "files": nfiles, "file_bytes": fbytes,
+                   "block_size": bs, "blocks": info["blocks"], "recovery_slices": info["recovery_slices"],
+                   "damaged_blocks": m, "damaged_files": len(damaged_files),
+                   "source": ".par2 files + damaged files -> repaired files"},
+        "src_bytes": src, "total_s": float(np.sum(lat)), "lat": lat, "exact": bool(exact), "kernel_ms": {},
+    }
+    if not args.no_cpu:
+        # the reference's method on the host for the same set: MD5/CRC32 of
+        # every block and the rebuild contraction of m blocks (file reads,
+        # parsing and writes not included)
+        threads = min(16, os.cpu_count() or 1)
+        nb = nfiles * per_file
+        value, dt = cpu_baseline_repair(nb, bs, m, threads)
+        res["cpu_baseline"] = {"value": round(value, 4), "unit": "GB/s", "cores": threads, "kind": "port",
+                               "sample": f"{nb} blocks x {bs} B: MD5/CRC32 verify of every block + rebuild "
+                                         f"contraction of {m} blocks (AVX2 nibble-shuffle), {dt:.2f} s wall"}
+    return res
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--workload", choices=["encode", "repair", "micro", "batched", "served", "copy", "create"],
+    ap.add_argument("--workload", choices=["encode", "repair", "micro", "batched", "served", "copy", "create",
+                                           "repair_files"],
                     default="encode")
     ap.add_argument("--hbm-budget", type=int, default=200 * 10**9, help="served: server HBM budget in bytes")
     ap.add_argument("--lanes", type=int, default=2, help="served: concurrent batches")
@@ -709,10 +786,10 @@ def main():
     ap.add_argument("--served-requests", type=int, default=400, help="served: requests per plain run")
     ap.add_argument("--served-loads", type=float, nargs="+", default=[6, 8, 10, 12, 14, 16],
                     help="served: offered loads as multiples of the plain run's")
-    ap.add_argument("--create-files", type=int, default=16, help="create: input files")
-    ap.add_argument("--create-file-mib", type=int, default=64, help="create: MiB per input file")
-    ap.add_argument("--create-block", type=int, default=0, help="create: block size in bytes (0: automatic)")
-    ap.add_argument("--create-redundancy", type=float, default=10.0, help="create: redundancy percent")
+    ap.add_argument("--create-files", type=int, default=16, help="create / repair_files: input files")
+    ap.add_argument("--create-file-mib", type=int, default=64, help="create / repair_files: MiB per input file")
+    ap.add_argument("--create-block", type=int, default=0, help="create / repair_files: block size in bytes (0: automatic)")
+    ap.add_argument("--create-redundancy", type=float, default=10.0, help="create / repair_files: redundancy percent")
     ap.add_argument("--seed", type=int, default=4)
     ap.add_argument("--kernel", type=int, default=0, help="0 auto, 1 LDS tables, 2 MFMA FP4, 3 MFMA int8")
     ap.add_argument("--autotune", type=int, default=None,
@@ -776,6 +853,8 @@ def main():
         res = bench_batched(args, ctx)
     elif args.workload == "create":
         res = bench_create(args, ctx)
+    elif args.workload == "repair_files":
+        res = bench_repair_files(args, ctx)
     else:
         res = bench_encode(args, ctx) if args.workload == "encode" else bench_repair(args, ctx)
     lat = np.array(res["lat"]) * 1e3

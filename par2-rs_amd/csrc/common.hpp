@@ -88,6 +88,8 @@ struct par2hip_ctx {
   int *d_status = nullptr;            // device status word (singular flag)
   par2hip::ParamArena arena;
   par2hip::DevBuf scratch[11];        // grow-only device scratch, by role
+  uint8_t *host_stage = nullptr;      // grow-only pinned host staging for file I/O
+  size_t host_stage_bytes = 0;
   int gf_kernel = 0;                  // 0 auto, 1 LDS tables, 2 MFMA FP4, 3 MFMA int8
   int num_cus = 256;                  // compute units of the device
   bool profile = false;
@@ -124,6 +126,9 @@ int32_t hip_fail(par2hip_ctx *ctx, hipError_t e, const char *what);
 
 // Device scratch of at least `bytes` for a role (contents not preserved on growth).
 int32_t scratch(par2hip_ctx *ctx, int role, size_t bytes, void **out);
+// Pinned host buffer of at least `bytes` (grow-only, kept by the context):
+// files are read into it and copied to the device at pinned-memory rates.
+int32_t host_stage(par2hip_ctx *ctx, size_t bytes, uint8_t **out);
 
 // Parameter blocks: begin() waits for the previous upload, alloc() carves
 // 256-byte aligned pieces (returns host and device addresses), commit()

@@ -73,6 +73,23 @@ int32_t scratch(par2hip_ctx *ctx, int role, size_t bytes, void **out) {
   return PAR2HIP_OK;
 }
 
+int32_t host_stage(par2hip_ctx *ctx, size_t bytes, uint8_t **out) {
+  if (ctx->host_stage_bytes < bytes) {
+    if (ctx->host_stage) {
+      // copies from the old buffer may still be queued
+      P2H_CHECK(ctx, hipStreamSynchronize(ctx->stream));
+      P2H_CHECK(ctx, hipHostFree(ctx->host_stage));
+      ctx->host_stage = nullptr;
+      ctx->host_stage_bytes = 0;
+    }
+    const size_t want = (std::max(bytes, (size_t)1 << 20) + ((size_t)1 << 20) - 1) & ~(((size_t)1 << 20) - 1);
+    P2H_CHECK(ctx, hipHostMalloc((void **)&ctx->host_stage, want, hipHostMallocDefault));
+    ctx->host_stage_bytes = want;
+  }
+  *out = ctx->host_stage;
+  return PAR2HIP_OK;
+}
+
 int32_t params_begin(par2hip_ctx *ctx, size_t bytes, uint8_t *target) {
   ParamArena &a = ctx->arena;
   if (a.copied) P2H_CHECK(ctx, hipEventSynchronize(a.copied));
@@ -613,6 +630,7 @@ int32_t par2hip_ctx_destroy(par2hip_ctx *ctx) {
       if (b.ptr) (void)hipFree(b.ptr);
     if (ctx->arena.host) (void)hipHostFree(ctx->arena.host);
     if (ctx->arena.dev) (void)hipFree(ctx->arena.dev);
+    if (ctx->host_stage) (void)hipHostFree(ctx->host_stage);
     if (ctx->d_exp) (void)hipFree(ctx->d_exp);
     if (ctx->d_log) (void)hipFree(ctx->d_log);
     if (ctx->d_crc_tab) (void)hipFree(ctx->d_crc_tab);

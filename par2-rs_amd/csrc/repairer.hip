@@ -14,15 +14,18 @@
 //           seeded with the original, rebuilt blocks written, every temp
 //           verified, renamed only when all verify, temps removed otherwise)
 #include <dirent.h>
+#include <fcntl.h>
 #include <sys/stat.h>
 #include <unistd.h>
 
 #include <algorithm>
 #include <array>
+#include <atomic>
 #include <cstdio>
 #include <cstring>
 #include <map>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "common.hpp"
@@ -266,12 +269,59 @@ bool read_range(const std::string &path, uint64_t off, uint64_t n, uint8_t *dst)
   return ok;
 }
 
-struct DevMem {
-  void *p = nullptr;
-  ~DevMem() {
-    if (p) (void)hipFree(p);
-  }
+// Byte ranges of files moved with several host threads (pread / pwrite in
+// 8 MiB pieces for reads, one thread per file for writes): reading a set's
+// files and writing its temps from one thread leaves most of the host's
+// memory bandwidth unused.
+struct IoRange {
+  std::string path;
+  uint64_t off, n;
+  uint8_t *buf;
 };
+
+bool io_parallel(const std::vector<IoRange> &jobs, bool write) {
+  struct Piece {
+    size_t job;
+    uint64_t off, n;
+  };
+  std::vector<Piece> pieces;
+  const uint64_t kPiece = 8u << 20;
+  for (size_t j = 0; j < jobs.size(); j++) {
+    if (write || jobs[j].n == 0) {
+      pieces.push_back({j, 0, jobs[j].n});
+      continue;
+    }
+    for (uint64_t o = 0; o < jobs[j].n; o += kPiece) pieces.push_back({j, o, std::min(kPiece, jobs[j].n - o)});
+  }
+  std::atomic<size_t> next{0};
+  std::atomic<bool> ok{true};
+  auto worker = [&] {
+    for (size_t i; ok && (i = next++) < pieces.size();) {
+      const Piece &pc = pieces[i];
+      const IoRange &jb = jobs[pc.job];
+      const int fd = write ? open(jb.path.c_str(), O_WRONLY | O_CREAT | O_TRUNC, 0666) : open(jb.path.c_str(), O_RDONLY);
+      if (fd < 0) {
+        ok = false;
+        break;
+      }
+      uint64_t done = 0;
+      while (done < pc.n) {
+        const ssize_t r = write ? pwrite(fd, jb.buf + pc.off + done, pc.n - done, (off_t)(jb.off + pc.off + done))
+                                : pread(fd, jb.buf + pc.off + done, pc.n - done, (off_t)(jb.off + pc.off + done));
+        if (r <= 0) break;
+        done += (uint64_t)r;
+      }
+      if (close(fd) != 0 || done != pc.n) ok = false;
+    }
+  };
+  const unsigned hw = std::max(1u, std::thread::hardware_concurrency());
+  const size_t nt = std::min<size_t>({pieces.size(), (size_t)16, (size_t)hw});
+  std::vector<std::thread> th;
+  for (size_t t = 1; t < nt; t++) th.emplace_back(worker);
+  worker();
+  for (auto &t : th) t.join();
+  return ok;
+}
 
 
 // Sorted by exponent, one slice per exponent (the first found).
@@ -357,37 +407,37 @@ int32_t repair_files(par2hip_ctx *ctx, const char *index, const char *base_dir, 
   enum { kOk, kDamaged, kMissing, kSize };
   std::vector<int> status(files.size(), kOk);
   std::vector<uint64_t> present_size(files.size(), 0);
-  DevMem d_slices, d_hash;
-  P2H_CHECK(ctx, hipMalloc(&d_slices.p, n * bs));
-  P2H_CHECK(ctx, hipMalloc(&d_hash.p, n * 20));
-  P2H_CHECK(ctx, hipMemsetAsync(d_slices.p, 0, n * bs, ctx->stream));
-  std::vector<uint8_t> host;
+  // the files' blocks, zero-padded, in one pinned staging copy of the
+  // device layout, read by parallel threads and shipped in one copy
+  void *d_slices = nullptr, *d_hash = nullptr;
+  uint8_t *stage = nullptr;
+  if ((rc = scratch(ctx, kScratchIn, n * bs, &d_slices))) return rc;
+  if ((rc = scratch(ctx, kScratchHash, n * 20, &d_hash))) return rc;
+  if ((rc = host_stage(ctx, n * bs, &stage))) return rc;
+  std::vector<IoRange> reads;
   for (size_t i = 0; i < files.size(); i++) {
     const std::string path = base + "/" + files[i].name;
+    uint8_t *dst = stage + first[i] * bs;
+    uint64_t have = 0;
     if (!file_size(path, &present_size[i])) {
       status[i] = kMissing;
-      continue;
-    }
-    if (present_size[i] != files[i].length) {  // verify_file_smart: size first
+    } else if (present_size[i] != files[i].length) {  // verify_file_smart: size first
       status[i] = kSize;
-      continue;
+    } else {
+      have = files[i].length;
+      if (have) reads.push_back({path, 0, have, dst});
     }
-    host.resize(files[i].length);
-    if (files[i].length && !read_range(path, 0, files[i].length, host.data()))
-      return fail(ctx, PAR2HIP_EINVAL, "cannot read " + path);
-    if (files[i].length) {
-      P2H_CHECK(ctx, hipMemcpyAsync((uint8_t *)d_slices.p + first[i] * bs, host.data(), files[i].length,
-                                    hipMemcpyHostToDevice, ctx->stream));
-      P2H_CHECK(ctx, hipStreamSynchronize(ctx->stream));
-    }
+    memset(dst + have, 0, nblk[i] * bs - have);
   }
-  if ((rc = par2hip_hash_slices_dev(ctx, d_slices.p, n, bs, bs, nullptr, d_hash.p, (uint8_t *)d_hash.p + n * 16)))
+  if (!io_parallel(reads, false)) return fail(ctx, PAR2HIP_EINVAL, "cannot read the protected files");
+  P2H_CHECK(ctx, hipMemcpyAsync(d_slices, stage, n * bs, hipMemcpyHostToDevice, ctx->stream));
+  if ((rc = par2hip_hash_slices_dev(ctx, d_slices, n, bs, bs, nullptr, d_hash, (uint8_t *)d_hash + n * 16)))
     return rc;
   std::vector<uint8_t> md5s(n * 16);
   std::vector<uint32_t> crcs(n);
-  P2H_CHECK(ctx, hipMemcpyAsync(md5s.data(), d_hash.p, n * 16, hipMemcpyDeviceToHost, ctx->stream));
+  P2H_CHECK(ctx, hipMemcpyAsync(md5s.data(), d_hash, n * 16, hipMemcpyDeviceToHost, ctx->stream));
   P2H_CHECK(ctx, hipStreamSynchronize(ctx->stream));
-  P2H_CHECK(ctx, hipMemcpyAsync(crcs.data(), (uint8_t *)d_hash.p + n * 16, n * 4, hipMemcpyDeviceToHost,
+  P2H_CHECK(ctx, hipMemcpyAsync(crcs.data(), (uint8_t *)d_hash + n * 16, n * 4, hipMemcpyDeviceToHost,
                                 ctx->stream));
   P2H_CHECK(ctx, hipStreamSynchronize(ctx->stream));
   std::vector<uint32_t> missing;  // global block indices, canonical order
@@ -425,90 +475,90 @@ int32_t repair_files(par2hip_ctx *ctx, const char *index, const char *base_dir, 
     return fail(ctx, PAR2HIP_ERECOVERY,
                 "need " + std::to_string(m) + " recovery blocks, have " + std::to_string(recs.size()));
 
-  // ---- rebuild on the device from the m lowest exponents (repair.rs:575-604)
-  std::vector<uint8_t> rec(m * bs);
+  // ---- rebuild on the device from the m lowest exponents (repair.rs:575-604);
+  // the data is on the device, so the staging copy now takes the recovery
+  // slices (m <= n)
   std::vector<uint32_t> exps(m);
+  std::vector<IoRange> rreads;
   for (size_t k = 0; k < m; k++) {
     exps[k] = recs[k].exponent;
-    if (!read_range(recs[k].path, recs[k].data_offset, bs, &rec[k * bs]))
-      return fail(ctx, PAR2HIP_EINVAL, "cannot read recovery slice from " + recs[k].path);
+    rreads.push_back({recs[k].path, recs[k].data_offset, bs, stage + k * bs});
   }
-  DevMem d_rec, d_out;
-  P2H_CHECK(ctx, hipMalloc(&d_rec.p, m * bs));
-  P2H_CHECK(ctx, hipMalloc(&d_out.p, m * bs));
-  P2H_CHECK(ctx, hipMemcpyAsync(d_rec.p, rec.data(), m * bs, hipMemcpyHostToDevice, ctx->stream));
-  P2H_CHECK(ctx, hipStreamSynchronize(ctx->stream));
-  if ((rc = par2hip_reconstruct_dev(ctx, d_slices.p, n, bs, missing.data(), m, d_rec.p, exps.data(), d_out.p, 0)))
+  P2H_CHECK(ctx, hipStreamSynchronize(ctx->stream));  // the staging copy of the data has left
+  if (!io_parallel(rreads, false)) return fail(ctx, PAR2HIP_EINVAL, "cannot read the recovery slices");
+  void *d_rec = nullptr, *d_out = nullptr;
+  if ((rc = scratch(ctx, kScratchAux, m * bs, &d_rec))) return rc;
+  if ((rc = scratch(ctx, kScratchOut, m * bs, &d_out))) return rc;
+  P2H_CHECK(ctx, hipMemcpyAsync(d_rec, stage, m * bs, hipMemcpyHostToDevice, ctx->stream));
+  if ((rc = par2hip_reconstruct_dev(ctx, d_slices, n, bs, missing.data(), m, d_rec, exps.data(), d_out, 0)))
     return rc;
-  std::vector<uint8_t> out(m * bs);
-  P2H_CHECK(ctx, hipMemcpyAsync(out.data(), d_out.p, m * bs, hipMemcpyDeviceToHost, ctx->stream));
-  P2H_CHECK(ctx, hipStreamSynchronize(ctx->stream));
 
-  // ---- temp files (repair.rs:736-1040): originals untouched until commit
+  // ---- temp files (repair.rs:736-1040): originals untouched until commit.
+  // Each damaged file's full contents are assembled on the device (its
+  // blocks as read, with the rebuilt ones copied over them) and written in
+  // one piece; the reference seeds the temp with the original's bytes and
+  // writes the rebuilt blocks into it, which gives the same file.
   std::vector<std::pair<std::string, std::string>> temps;  // tmp, final
   auto cleanup = [&] {
     for (auto &t : temps) (void)remove(t.first.c_str());
   };
   std::vector<size_t> touched;
+  std::vector<IoRange> writes;
   for (size_t i = 0; i < files.size(); i++) {
     if (status[i] == kOk) continue;
-    const std::string final_path = base + "/" + files[i].name, tmp = final_path + kTmpSuffix;
-    FILE *f = fopen(tmp.c_str(), "wb+");
-    if (!f) {
-      cleanup();
-      return fail(ctx, PAR2HIP_EINVAL, "cannot create " + tmp);
-    }
-    temps.push_back({tmp, final_path});
     touched.push_back(i);
-    bool ok = ftruncate(fileno(f), (off_t)files[i].length) == 0;
-    if (ok && status[i] != kMissing && present_size[i]) {  // seed with the original's bytes
-      const uint64_t copy = std::min(present_size[i], files[i].length);
-      std::vector<uint8_t> orig(copy);
-      ok = read_range(final_path, 0, copy, orig.data()) && fseeko(f, 0, SEEK_SET) == 0 &&
-           fwrite(orig.data(), 1, copy, f) == copy;
-    }
-    for (size_t k = 0; k < m && ok; k++) {
-      const uint64_t g = missing[k];
-      if (g < first[i] || g >= first[i] + nblk[i]) continue;
-      const uint64_t off = (g - first[i]) * bs;
-      const uint64_t len = std::min<uint64_t>(bs, files[i].length - off);
-      ok = fseeko(f, (off_t)off, SEEK_SET) == 0 && fwrite(&out[k * bs], 1, len, f) == len;
-    }
-    ok = (fclose(f) == 0) && ok;
-    if (!ok) {
-      cleanup();
-      return fail(ctx, PAR2HIP_EINVAL, "cannot write " + tmp);
-    }
+    const std::string final_path = base + "/" + files[i].name;
+    temps.push_back({final_path + kTmpSuffix, final_path});
+    writes.push_back({temps.back().first, 0, files[i].length, stage + first[i] * bs});
+  }
+  for (size_t k = 0; k < m; k++)
+    P2H_CHECK(ctx, hipMemcpyAsync((uint8_t *)d_slices + (uint64_t)missing[k] * bs, (const uint8_t *)d_out + k * bs,
+                                  bs, hipMemcpyDeviceToDevice, ctx->stream));
+  for (size_t i : touched)
+    if (files[i].length)
+      P2H_CHECK(ctx, hipMemcpyAsync(stage + first[i] * bs, (const uint8_t *)d_slices + first[i] * bs, files[i].length,
+                                    hipMemcpyDeviceToHost, ctx->stream));
+  P2H_CHECK(ctx, hipStreamSynchronize(ctx->stream));
+  if (!io_parallel(writes, true)) {
+    cleanup();
+    return fail(ctx, PAR2HIP_EINVAL, "cannot write the temp files");
   }
 
-  // ---- every temp verified before anything is renamed (repair.rs:1050-1080)
+  // ---- every temp verified before anything is renamed (repair.rs:1050-1080):
+  // read back from disk (parallel), hashed on the device against IFSC
+  std::vector<IoRange> vreads;
+  for (size_t t = 0; t < touched.size(); t++) {
+    const size_t i = touched[t];
+    if (s.ifsc.count(files[i].id) && nblk[i]) {
+      uint8_t *dst = stage + first[i] * bs;
+      memset(dst + files[i].length, 0, nblk[i] * bs - files[i].length);
+      vreads.push_back({temps[t].first, 0, files[i].length, dst});
+    }
+  }
+  if (!io_parallel(vreads, false)) {
+    cleanup();
+    return fail(ctx, PAR2HIP_EREPAIR, "repair completed but verification failed - files may still be damaged");
+  }
   for (size_t t = 0; t < touched.size(); t++) {
     const size_t i = touched[t];
     const std::string &tmp = temps[t].first;
     bool ok = true;
     auto it = s.ifsc.find(files[i].id);
     if (it != s.ifsc.end() && nblk[i]) {
-      host.assign(nblk[i] * bs, 0);
-      ok = read_range(tmp, 0, files[i].length, host.data());
-      if (ok) {
-        P2H_CHECK(ctx,
-                  hipMemcpyAsync(d_slices.p, host.data(), nblk[i] * bs, hipMemcpyHostToDevice, ctx->stream));
-        P2H_CHECK(ctx, hipStreamSynchronize(ctx->stream));
-        if ((rc = par2hip_hash_slices_dev(ctx, d_slices.p, nblk[i], bs, bs, nullptr, d_hash.p,
-                                          (uint8_t *)d_hash.p + nblk[i] * 16))) {
-          cleanup();
-          return rc;
-        }
-        P2H_CHECK(ctx,
-                  hipMemcpyAsync(md5s.data(), d_hash.p, nblk[i] * 16, hipMemcpyDeviceToHost, ctx->stream));
-        P2H_CHECK(ctx, hipStreamSynchronize(ctx->stream));
-        P2H_CHECK(ctx, hipMemcpyAsync(crcs.data(), (uint8_t *)d_hash.p + nblk[i] * 16, nblk[i] * 4,
-                                      hipMemcpyDeviceToHost, ctx->stream));
-        P2H_CHECK(ctx, hipStreamSynchronize(ctx->stream));
-        for (uint64_t k = 0; k < nblk[i] && ok; k++)
-          ok = k < it->second.size() && crcs[k] == it->second[k].second &&
-               memcmp(&md5s[k * 16], it->second[k].first.data(), 16) == 0;
+      uint8_t *dev = (uint8_t *)d_slices + first[i] * bs;
+      P2H_CHECK(ctx, hipMemcpyAsync(dev, stage + first[i] * bs, nblk[i] * bs, hipMemcpyHostToDevice, ctx->stream));
+      if ((rc = par2hip_hash_slices_dev(ctx, dev, nblk[i], bs, bs, nullptr, d_hash,
+                                        (uint8_t *)d_hash + nblk[i] * 16))) {
+        cleanup();
+        return rc;
       }
+      P2H_CHECK(ctx, hipMemcpyAsync(md5s.data(), d_hash, nblk[i] * 16, hipMemcpyDeviceToHost, ctx->stream));
+      P2H_CHECK(ctx, hipMemcpyAsync(crcs.data(), (uint8_t *)d_hash + nblk[i] * 16, nblk[i] * 4,
+                                    hipMemcpyDeviceToHost, ctx->stream));
+      P2H_CHECK(ctx, hipStreamSynchronize(ctx->stream));
+      for (uint64_t k = 0; k < nblk[i] && ok; k++)
+        ok = k < it->second.size() && crcs[k] == it->second[k].second &&
+             memcmp(&md5s[k * 16], it->second[k].first.data(), 16) == 0;
     } else {
       uint8_t d[16];
       ok = md5_file_prefix(tmp, files[i].length, d) && memcmp(d, files[i].md5.data(), 16) == 0;
