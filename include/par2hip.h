@@ -315,7 +315,10 @@ int32_t par2hip_par2_info(const char *par2_path, par2hip_set_info *info);
  * original, verify every temp on the device and rename them over the
  * originals only when all verify (src/repair.rs:445-1125).  verify_only: stop
  * after verification.  Errors: PAR2HIP_ERECOVERY (nothing written),
- * PAR2HIP_ESINGULAR, PAR2HIP_EREPAIR (temps removed, originals untouched). */
+ * PAR2HIP_ESINGULAR, PAR2HIP_EREPAIR (temps removed, originals untouched).
+ * The context keeps a pinned host staging buffer of the set's block bytes
+ * (and device buffers of the same size) for later calls until it is
+ * destroyed. */
 typedef struct {
     uint64_t block_size, blocks, blocks_damaged;
     uint32_t files, files_damaged, files_missing;
