@@ -4,16 +4,21 @@ Default workload (plain GPU encode): one 1 GiB set of 1024 input slices of
 1 MiB (uniform random bytes, seeded), 102 recovery slices with exponents
 0..101 over GF(2^16) poly 0x1100B, plus the IFSC MD5 and CRC32 of every input
 slice.  Inputs are resident in HBM before timing starts; one step is one
-par2hip_encode_dev call (coefficients, tables, contraction, both hashes) and
-its completion.  ``--workload repair`` times the plain repair instead
-(2000 x 512 KiB, 100 erasures, 100 recovery slices, verify hashes of all
-slices + Gauss-Jordan + rebuild).
+par2hip_encode_dev call (coefficients, the contraction as the library plans
+it beside the MD5 chains, both hashes) and its completion.  Other workloads
+(``--workload``): repair (2000 x 512 KiB, 100 erasures, verify hashes of all
+slices + inverse + rebuild), micro (each contraction variant and hash alone),
+batched (256 sets per planned, graph-replayed batch), served (Poisson
+requests through the serving loop against the plain path at equal p99),
+create and repair_files (whole files to and from .par2 sets), copy (the
+calibration copy for the HBM traffic counters).
 
 Prints ONE JSON line (rank 0): metric/value/unit, ms_per_step, p50/p99 step
-latency, the dominant kernel's roofline object (HIP-event durations measured
-on the kernel's own stream inside the timed region) and the CPU baseline
-(the oracle's restatement of the reference's AVX2 strategy and hashing, timed
-on this host on a bounded sample).
+latency, the roofline object of the step's contraction (HIP-event durations
+measured on the kernel's own stream inside the timed region; HBM bytes, or
+matrix-core operations for the bit-sliced form) and the CPU baseline (the
+oracle's restatement of the reference's AVX2 strategy and hashing, timed on
+this host on a bounded sample).
 """
 from __future__ import annotations
 
