@@ -240,9 +240,10 @@ struct Overlap {
 // form on only as many CUs as it needs to finish under the chains (here:
 // the narrowest of the narrow streams and the full complement that brings
 // its estimate under 88% of the chains' lower-bound time) leaves them the
-// clock (1 GiB / R=102: 10.7 ms with the LDS form on 248 CUs, 10.75 ms with
-// the FP4 form on 248, 9.98 ms on 160).  If no width fits, the contraction
-// is the critical path and the LDS form on every other CU runs.
+// clock (1 GiB / R=102, 8 MD5 CUs: 10.7 ms with the LDS form on the other
+// 248 CUs, 10.75 ms with the FP4 form on 248, 9.98 ms on 160).  If no width
+// fits, the contraction is the critical path and the LDS form on every
+// other CU runs.
 struct GfPlan {
   int kernel = 0;             // 0: the context's choice
   hipStream_t work = nullptr;  // nullptr: rest
@@ -571,10 +572,16 @@ int32_t par2hip_ctx_create(int32_t device, par2hip_ctx **out) {
   ctx->cur = ctx->stream;
   if (hipGetDeviceProperties(&prop, device) == hipSuccess) strncpy(ctx->arch, prop.gcnArchName, sizeof ctx->arch - 1);
   {
-    // MD5 chains get a few CUs of their own (enough for 64 chains per SIMD-wave,
-    // 2048 slices); overlapped work uses the complementary mask.  Without CU
-    // masks the side stream is an ordinary stream and `rest` is the main stream.
-    const int ncu = ctx->num_cus, hash_cus = ncu >= 64 ? 8 : 0;
+    // MD5 chains get CUs of their own; overlapped work uses the complementary
+    // mask.  32 CUs hold one 64-chain wave each up to 2048 slices: a chain
+    // wave alone on its CU, with that CU's vector cache and memory path to
+    // itself, keeps its prefetched lines and runs ~10% faster beside a
+    // contraction than two or four waves sharing a CU, and the contraction
+    // loses less to that interference than it gains from the extra CUs
+    // (1 GiB encode 10.13 -> 9.74 ms, 2000 x 512 KiB repair 5.81 -> 5.10 ms
+    // against 8 CUs).  Without CU masks the side stream is an ordinary stream
+    // and `rest` is the main stream.
+    const int ncu = ctx->num_cus, hash_cus = ncu >= 128 ? 32 : ncu >= 64 ? 8 : 0;
     std::vector<uint32_t> side_mask((ncu + 31) / 32, 0u), rest_mask((ncu + 31) / 32, 0u);
     for (int c = 0; c < ncu; c++) (c < hash_cus ? side_mask : rest_mask)[c / 32] |= 1u << (c % 32);
     bool ok = hash_cus > 0 &&
