@@ -172,23 +172,15 @@ int32_t parse_file(const std::string &path, Par2Set &s, bool recovery_only, std:
   return PAR2HIP_OK;
 }
 
-// The recovery set ID of a .par2 file: that of its first packet.
+// The recovery set ID of a .par2 file: that of the packet header at offset
+// 0; a file that does not start with one is not a volume of any set
+// (parser.rs:606-623).
 bool first_set_id(const std::string &path, Id *out) {
   FILE *f = fopen(path.c_str(), "rb");
   if (!f) return false;
   uint8_t h[64];
-  uint64_t size = 0, pos = 0;
-  file_size(path, &size);
-  bool ok = false;
-  while (pos + 64 <= size) {
-    if (fseeko(f, (off_t)pos, SEEK_SET) != 0 || fread(h, 1, 64, f) < 64) break;
-    if (memcmp(h, kMagic, 8) == 0) {
-      *out = id_at(h + 32);
-      ok = true;
-      break;
-    }
-    pos++;
-  }
+  const bool ok = fread(h, 1, 64, f) == 64 && memcmp(h, kMagic, 8) == 0;
+  if (ok) *out = id_at(h + 32);
   fclose(f);
   return ok;
 }
@@ -261,12 +253,20 @@ std::vector<FileRec> canonical_files(const Par2Set &s) {
   return out;
 }
 
-bool read_range(const std::string &path, uint64_t off, uint64_t n, uint8_t *dst) {
-  FILE *f = fopen(path.c_str(), "rb");
-  if (!f) return false;
-  const bool ok = fseeko(f, (off_t)off, SEEK_SET) == 0 && fread(dst, 1, n, f) == n;
-  fclose(f);
-  return ok;
+// A stored file name the repairer will open, write and rename under the base
+// directory: relative, non-empty, no ".." component.  The reference joins
+// the name to the base path as it stands (repair.rs:731, :772), so a crafted
+// set could make it write outside that directory; such a set is refused.
+bool safe_name(const std::string &name) {
+  if (name.empty() || name[0] == '/') return false;
+  size_t b = 0;
+  while (b <= name.size()) {
+    size_t e = name.find('/', b);
+    if (e == std::string::npos) e = name.size();
+    if (name.compare(b, e - b, "..") == 0 && e - b == 2) return false;
+    b = e + 1;
+  }
+  return true;
 }
 
 // Byte ranges of files moved with several host threads (pread / pwrite in
@@ -383,6 +383,8 @@ int32_t repair_files(par2hip_ctx *ctx, const char *index, const char *base_dir, 
   int32_t rc = load_set(index, s, &err);
   if (rc) return fail(ctx, rc, err);
   const std::vector<FileRec> files = canonical_files(s);
+  for (const FileRec &f : files)
+    if (!safe_name(f.name)) return fail(ctx, PAR2HIP_EINVAL, "unsafe file name in the set: " + f.name);
   const uint64_t bs = s.block_size;
   const std::string base = base_dir && *base_dir ? base_dir : dir_of(index);
 

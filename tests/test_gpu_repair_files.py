@@ -11,6 +11,7 @@ import sys
 import pytest
 
 import c1
+import oracle
 import par2hip
 
 pytestmark = pytest.mark.gpu
@@ -144,3 +145,20 @@ def test_c1_created_damaged_and_repaired(ctx, tmp_path):
     rep = ctx.repair_files(written[0][0])
     assert (rep["blocks_damaged"], rep["repaired"]) == (3, 1)
     assert path.read_bytes() == data
+
+
+@pytest.mark.parametrize("name", ["../escape.bin", "/tmp/escape.bin", "sub/../../escape.bin"])
+def test_unsafe_stored_name_is_refused(ctx, tmp_path, name):
+    """A set whose FileDesc names a path outside the base directory (absolute or
+    with a '..' component) is refused before any file is read or written; the
+    reference would join the name to the base path as it stands."""
+    data = c1.seeded_bytes(5, 20000).tobytes()
+    sub = tmp_path / "set"
+    sub.mkdir()
+    for fname, b in oracle.par2_create([(name, data)], block_size=4096, redundancy=20.0, stem=str(sub / "evil")):
+        open(fname, "wb").write(b)
+    say("unsafe name", name)
+    with pytest.raises(par2hip.InvalidFormat, match="unsafe file name"):
+        ctx.repair_files(str(sub / "evil.par2"))
+    assert not (tmp_path / "escape.bin").exists() and not (tmp_path / "escape.bin.par2tmp").exists()
+    assert all(p.endswith(".par2") for p in os.listdir(sub))

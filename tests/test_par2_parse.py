@@ -75,3 +75,14 @@ def test_damaged_packets_and_garbage_are_skipped(tmp_path):
     (tmp_path / "empty.par2").write_bytes(b"")
     with pytest.raises(par2hip.InvalidFormat):
         par2hip.par2_info(str(tmp_path / "empty.par2"))
+
+
+def test_volume_must_start_with_a_packet(tmp_path):
+    """A volume is recognised by the packet header at its offset 0
+    (parser.rs:606-623): leading junk hides its recovery slices."""
+    for f in os.listdir(QP):
+        shutil.copy(os.path.join(QP, f), tmp_path / f)
+    vol = tmp_path / "testdata.vol31+29.par2"
+    vol.write_bytes(b"junk" + vol.read_bytes())
+    info = par2hip.par2_info(str(tmp_path / "testdata.par2"))
+    assert (info["recovery_slices"], info["max_exponent"]) == (31, 30)
