@@ -353,11 +353,11 @@ __global__ __launch_bounds__(kTpb, 4) void k_gf_lds_batch(const GfTaskDev *__res
     const GfItem w = items[it];
     const GfTaskDev t = tasks[w.task];
     if (t.groups == 1)
-      gf_lds_tile<8, 1, 16>(lds, t.srcs, t.nsrc, t.dsts, t.ndst, t.tabs, t.words, t.accumulate, w.tile, w.rowblk,
-                            t.align);
+      gf_lds_tile<8, 1, 16, true>(lds, t.srcs, t.nsrc, t.dsts, t.ndst, t.tabs, t.words, t.accumulate, w.tile,
+                                  w.rowblk, t.align);
     else
-      gf_lds_tile<8, 2, 16>(lds, t.srcs, t.nsrc, t.dsts, t.ndst, t.tabs, t.words, t.accumulate, w.tile, w.rowblk,
-                            t.align);
+      gf_lds_tile<8, 2, 16, true>(lds, t.srcs, t.nsrc, t.dsts, t.ndst, t.tabs, t.words, t.accumulate, w.tile,
+                                  w.rowblk, t.align);
   }
 }
 
