@@ -10,7 +10,11 @@
 // recovery slice into a packet image row that already holds the set ID, the
 // packet type and the exponent, so the packet digest (MD5 over set ID, type
 // and body, src/writer.rs:97-105) is one more slice hash.
+#include <fcntl.h>
+#include <unistd.h>
+
 #include <algorithm>
+#include <atomic>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -102,12 +106,29 @@ int digits(uint64_t x) {  // src/volumes.rs:47-59
   return w;
 }
 
-struct DevMem {
-  void *p = nullptr;
-  ~DevMem() {
-    if (p) (void)hipFree(p);
-  }
+// memcpy of many ranges in 8 MiB pieces over a few threads (a single thread
+// copying a set's files into the pinned staging buffer runs at a fraction of
+// the host's memory bandwidth)
+struct CopyJob {
+  uint8_t *dst;
+  const uint8_t *src;
+  uint64_t n;
 };
+void copy_parallel(const std::vector<CopyJob> &jobs, size_t max_threads) {
+  std::vector<CopyJob> pieces;
+  const uint64_t kPiece = 8u << 20;
+  for (const CopyJob &j : jobs)
+    for (uint64_t o = 0; o < j.n; o += kPiece) pieces.push_back({j.dst + o, j.src + o, std::min(kPiece, j.n - o)});
+  std::atomic<size_t> next{0};
+  auto worker = [&] {
+    for (size_t i; (i = next++) < pieces.size();) memcpy(pieces[i].dst, pieces[i].src, pieces[i].n);
+  };
+  const size_t nt = std::min(pieces.size(), max_threads);
+  std::vector<std::thread> th;
+  for (size_t t = 1; t < nt; t++) th.emplace_back(worker);
+  worker();
+  for (auto &t : th) t.join();
+}
 
 }  // namespace
 
@@ -205,105 +226,158 @@ int32_t create_par2(par2hip_ctx *ctx, const par2hip_input_file *files, size_t nf
   uint8_t set_id[16];
   Md5Host::digest(mainb.v.data(), mainb.v.size(), set_id);
 
-  // ---- device: blocks in canonical order, IFSC hashes, recovery packet image
+  // ---- device: blocks in canonical order, IFSC hashes, recovery packet image.
+  // The blocks go through the context's pinned staging buffer (filled by
+  // several threads, one DMA); the packet image comes back through it.
   const uint64_t row = 36 + block_size;  // set ID, type, exponent, slice
-  std::vector<uint8_t> md5s(n * 16), image(r * row), pkt_md5(r * 16);
+  std::vector<uint8_t> md5s(n * 16), pkt_md5(r * 16), host_img;
   std::vector<uint32_t> crcs(n);
+  const uint8_t *img = nullptr;  // r rows of `row` bytes
+  int32_t rc;
   if (n) {
-    DevMem d_blocks, d_hash, d_img, d_pkt;
-    P2H_CHECK(ctx, hipMalloc(&d_blocks.p, n * block_size));
-    P2H_CHECK(ctx, hipMalloc(&d_hash.p, n * 20));
-    P2H_CHECK(ctx, hipMemsetAsync(d_blocks.p, 0, n * block_size, ctx->stream));
-    for (const Input &x : in)
-      if (x.f->len)
-        P2H_CHECK(ctx, hipMemcpyAsync((uint8_t *)d_blocks.p + x.first * block_size, x.f->data, x.f->len,
-                                      hipMemcpyHostToDevice, ctx->stream));
+    uint8_t *stage = nullptr;
+    void *d_blocks, *d_hash, *d_img = nullptr, *d_pkt = nullptr;
+    if ((rc = host_stage(ctx, std::max(n * block_size, r * row), &stage))) return rc;
+    if ((rc = scratch(ctx, kScratchIn, n * block_size, &d_blocks))) return rc;
+    if ((rc = scratch(ctx, kScratchHash, n * 20, &d_hash))) return rc;
+    if (r && ((rc = scratch(ctx, kScratchAux, r * row, &d_img)) || (rc = scratch(ctx, kScratchAux2, r * 16, &d_pkt))))
+      return rc;
+    std::vector<CopyJob> copies;
+    for (const Input &x : in) {
+      uint8_t *dst = stage + x.first * block_size;
+      if (x.f->len) copies.push_back({dst, static_cast<const uint8_t *>(x.f->data), x.f->len});
+      memset(dst + x.f->len, 0, x.blocks * block_size - x.f->len);  // zero padding of the last block
+    }
+    copy_parallel(copies, 8);
+    P2H_CHECK(ctx, hipMemcpyAsync(d_blocks, stage, n * block_size, hipMemcpyHostToDevice, ctx->stream));
     if (r) {
-      P2H_CHECK(ctx, hipMalloc(&d_img.p, r * row));
-      P2H_CHECK(ctx, hipMalloc(&d_pkt.p, r * 16));
       std::vector<uint8_t> prefix(r * 36);
       for (uint64_t e = 0; e < r; e++) {
         memcpy(&prefix[e * 36], set_id, 16);
         memcpy(&prefix[e * 36 + 16], kRecvSlic, 16);
         for (int k = 0; k < 4; k++) prefix[e * 36 + 32 + k] = (uint8_t)(e >> (8 * k));
       }
-      P2H_CHECK(ctx, hipMemcpy2DAsync(d_img.p, row, prefix.data(), 36, 36, r, hipMemcpyHostToDevice, ctx->stream));
+      P2H_CHECK(ctx, hipMemcpy2DAsync(d_img, row, prefix.data(), 36, 36, r, hipMemcpyHostToDevice, ctx->stream));
       // the prefix buffer must stay alive until the copy has run
       P2H_CHECK(ctx, hipStreamSynchronize(ctx->stream));
     }
     std::vector<uint32_t> exps(r);
     for (uint64_t e = 0; e < r; e++) exps[e] = (uint32_t)e;
-    int32_t rc = encode_dev_impl(ctx, (const uint8_t *)d_blocks.p, n, block_size, exps.data(), r,
-                                 r ? (uint8_t *)d_img.p + 36 : nullptr, (uint32_t *)d_hash.p,
-                                 (uint32_t *)((uint8_t *)d_hash.p + n * 16), row);
-    if (rc) return rc;
-    if (r && (rc = launch_md5(ctx, ctx->stream, (const uint8_t *)d_img.p, r, row, row, nullptr,
-                              (uint32_t *)d_pkt.p, false)))
+    if ((rc = encode_dev_impl(ctx, (const uint8_t *)d_blocks, n, block_size, exps.data(), r,
+                              r ? (uint8_t *)d_img + 36 : nullptr, (uint32_t *)d_hash,
+                              (uint32_t *)((uint8_t *)d_hash + n * 16), row)))
       return rc;
-    P2H_CHECK(ctx, hipMemcpyAsync(md5s.data(), d_hash.p, n * 16, hipMemcpyDeviceToHost, ctx->stream));
-    P2H_CHECK(ctx, hipMemcpyAsync(crcs.data(), (uint8_t *)d_hash.p + n * 16, n * 4, hipMemcpyDeviceToHost,
+    if (r && (rc = launch_md5(ctx, ctx->stream, (const uint8_t *)d_img, r, row, row, nullptr, (uint32_t *)d_pkt,
+                              false)))
+      return rc;
+    P2H_CHECK(ctx, hipMemcpyAsync(md5s.data(), d_hash, n * 16, hipMemcpyDeviceToHost, ctx->stream));
+    P2H_CHECK(ctx, hipMemcpyAsync(crcs.data(), (uint8_t *)d_hash + n * 16, n * 4, hipMemcpyDeviceToHost,
                                   ctx->stream));
     if (r) {
-      P2H_CHECK(ctx, hipMemcpyAsync(image.data(), d_img.p, r * row, hipMemcpyDeviceToHost, ctx->stream));
-      P2H_CHECK(ctx, hipMemcpyAsync(pkt_md5.data(), d_pkt.p, r * 16, hipMemcpyDeviceToHost, ctx->stream));
+      // the staging buffer's blocks have reached the device (same stream)
+      P2H_CHECK(ctx, hipMemcpyAsync(stage, d_img, r * row, hipMemcpyDeviceToHost, ctx->stream));
+      P2H_CHECK(ctx, hipMemcpyAsync(pkt_md5.data(), d_pkt, r * 16, hipMemcpyDeviceToHost, ctx->stream));
     }
     P2H_CHECK(ctx, hipStreamSynchronize(ctx->stream));
+    img = stage;
   } else if (r) {
     // no data blocks: every recovery slice is zero
+    host_img.assign(r * row, 0);
     for (uint64_t e = 0; e < r; e++) {
-      uint8_t *p = &image[e * row];
+      uint8_t *p = &host_img[e * row];
       memcpy(p, set_id, 16);
       memcpy(p + 16, kRecvSlic, 16);
       for (int k = 0; k < 4; k++) p[32 + k] = (uint8_t)(e >> (8 * k));
       Md5Host::digest(p, row, &pkt_md5[e * 16]);
     }
+    img = host_img.data();
   }
 
-  for (auto &t : hashers) t.join();
-
-  // ---- the packets every file starts with (creator.rs:483-516)
-  Bytes head;
-  append_packet(head, kCreator, set_id, std::vector<uint8_t>(kCreatorText, kCreatorText + strlen(kCreatorText)));
-  append_packet(head, kMain, set_id, mainb.v);
-  for (const Input &x : in) {
-    Bytes b;
-    b.put(x.id, 16);
-    b.put(x.full, 16);
-    b.put(x.h16k, 16);
-    b.u64(x.f->len);
-    b.put(x.f->name, strlen(x.f->name) + 1);
-    append_packet(head, kFileDesc, set_id, b.v);
-  }
-  for (const Input &x : in) {
-    Bytes b;
-    b.put(x.id, 16);
-    for (uint64_t k = 0; k < x.blocks; k++) {
-      b.put(&md5s[(x.first + k) * 16], 16);
-      b.u32(crcs[x.first + k]);
+  // ---- the packets every file starts with (creator.rs:483-516).  Only the
+  // FileDesc packets need the whole-file MD5s; the head's size does not
+  // depend on them, so each file's recovery packets are written at their
+  // final offset while the MD5 threads finish, and the head last.
+  auto build_head = [&](bool with_full) {
+    Bytes head;
+    append_packet(head, kCreator, set_id, std::vector<uint8_t>(kCreatorText, kCreatorText + strlen(kCreatorText)));
+    append_packet(head, kMain, set_id, mainb.v);
+    const uint8_t zero[16] = {};
+    for (const Input &x : in) {
+      Bytes b;
+      b.put(x.id, 16);
+      b.put(with_full ? x.full : zero, 16);
+      b.put(x.h16k, 16);
+      b.u64(x.f->len);
+      b.put(x.f->name, strlen(x.f->name) + 1);
+      append_packet(head, kFileDesc, set_id, b.v);
     }
-    append_packet(head, kIfsc, set_id, b.v);
-  }
+    for (const Input &x : in) {
+      Bytes b;
+      b.put(x.id, 16);
+      for (uint64_t k = 0; k < x.blocks; k++) {
+        b.put(&md5s[(x.first + k) * 16], 16);
+        b.u32(crcs[x.first + k]);
+      }
+      append_packet(head, kIfsc, set_id, b.v);
+    }
+    return head;
+  };
+  const uint64_t head_size = build_head(false).v.size();
 
-  // ---- write the files: head, then the volume's recovery packets
+  // ---- the files: one writer thread per file for its recovery packets
+  std::vector<int> fds(vols.size(), -1);
+  struct Closer {
+    std::vector<int> &fds;
+    ~Closer() {
+      for (int fd : fds)
+        if (fd >= 0) (void)close(fd);
+    }
+  } closer{fds};
   for (size_t i = 0; i < vols.size(); i++) {
-    const Vol &v = vols[i];
-    FILE *f = fopen(v.path.c_str(), "wb");
-    if (!f) return fail(ctx, PAR2HIP_EINVAL, "create: cannot write " + v.path);
-    bool ok = fwrite(head.v.data(), 1, head.v.size(), f) == head.v.size();
-    uint64_t bytes = head.v.size();
-    for (uint64_t e = v.start; e < v.end && ok; e++) {
-      Bytes hdr;
-      hdr.put(kMagic, 8);
-      hdr.u64(64 + 4 + block_size);
-      hdr.put(&pkt_md5[e * 16], 16);
-      ok = fwrite(hdr.v.data(), 1, hdr.v.size(), f) == hdr.v.size() &&
-           fwrite(&image[e * row], 1, row, f) == row;
-      bytes += hdr.v.size() + row;
+    fds[i] = open(vols[i].path.c_str(), O_WRONLY | O_CREAT | O_TRUNC, 0666);
+    if (fds[i] < 0) return fail(ctx, PAR2HIP_EINVAL, "create: cannot write " + vols[i].path);
+  }
+  auto pwrite_all = [](int fd, const uint8_t *p, uint64_t len, uint64_t off) {
+    while (len) {
+      const ssize_t w = pwrite(fd, p, len, (off_t)off);
+      if (w <= 0) return false;
+      p += w;
+      off += (uint64_t)w;
+      len -= (uint64_t)w;
     }
-    ok = (fclose(f) == 0) && ok;
-    if (!ok) return fail(ctx, PAR2HIP_EINVAL, "create: write failed for " + v.path);
-    snprintf(outs[i].path, sizeof outs[i].path, "%s", v.path.c_str());
-    outs[i].bytes = bytes;
+    return true;
+  };
+  std::vector<char> wrote(vols.size(), 1);
+  std::vector<std::thread> writers;
+  for (size_t i = 0; i < vols.size(); i++)
+    writers.emplace_back([&, i] {
+      const Vol &v = vols[i];
+      uint64_t off = head_size;
+      for (uint64_t e = v.start; e < v.end && wrote[i]; e++) {
+        Bytes hdr;
+        hdr.put(kMagic, 8);
+        hdr.u64(64 + 4 + block_size);
+        hdr.put(&pkt_md5[e * 16], 16);
+        wrote[i] = pwrite_all(fds[i], hdr.v.data(), hdr.v.size(), off) &&
+                   pwrite_all(fds[i], img + e * row, row, off + hdr.v.size());
+        off += hdr.v.size() + row;
+      }
+    });
+  for (auto &t : hashers) t.join();
+  const Bytes head = build_head(true);
+  bool ok = head.v.size() == head_size;
+  for (size_t i = 0; i < vols.size() && ok; i++) ok = pwrite_all(fds[i], head.v.data(), head.v.size(), 0);
+  for (auto &t : writers) t.join();
+  for (size_t i = 0; i < vols.size(); i++) {
+    ok = ok && wrote[i];
+    const int fd = fds[i];
+    fds[i] = -1;
+    ok = (close(fd) == 0) && ok;
+  }
+  if (!ok) return fail(ctx, PAR2HIP_EINVAL, "create: write failed for " + vols[0].path + " or a volume");
+  for (size_t i = 0; i < vols.size(); i++) {
+    snprintf(outs[i].path, sizeof outs[i].path, "%s", vols[i].path.c_str());
+    outs[i].bytes = head_size + (vols[i].end - vols[i].start) * (64 + 4 + block_size);
   }
   return PAR2HIP_OK;
 }
