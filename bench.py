@@ -789,7 +789,7 @@ def main():
     ap.add_argument("--lanes", type=int, default=2, help="served: concurrent batches")
     ap.add_argument("--max-batch", type=int, default=512, help="served: requests per batch at most")
     ap.add_argument("--served-requests", type=int, default=400, help="served: requests per plain run")
-    ap.add_argument("--served-loads", type=float, nargs="+", default=[6, 8, 10, 12, 14, 16],
+    ap.add_argument("--served-loads", type=float, nargs="+", default=[6, 8, 10, 11, 12, 13, 14, 16],
                     help="served: offered loads as multiples of the plain run's")
     ap.add_argument("--create-files", type=int, default=16, help="create / repair_files: input files")
     ap.add_argument("--create-file-mib", type=int, default=64, help="create / repair_files: MiB per input file")
