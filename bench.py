@@ -47,6 +47,10 @@ C2_GF_TRAFFIC_BYTES = 2.475e9
 # choice runs in the encode step (on 160 CUs beside the MD5 chains): FETCH_SIZE
 # 1,773,930 KiB x 2.0 + WRITE_SIZE 104,448 KiB per launch.
 C2_GF_MFMA_TRAFFIC_BYTES = 3.740e9
+# The same two counters for the repair step's rebuild contraction (2000 x 512
+# KiB, 100 erasures, default tile shape): FETCH_SIZE 668,054.7 KiB x 2.0 +
+# WRITE_SIZE 51,200 KiB per launch.
+C3_GF_TRAFFIC_BYTES = 1.421e9
 # Unit utilisation of the same contraction (default tile shape, alone on the
 # device) from rocprofv3 --pmc SQ_LDS_IDX_ACTIVE / SQ_ACTIVE_INST_VALU /
 # SQ_LDS_BANK_CONFLICT / GRBM_GUI_ACTIVE runs of this workload: the binding
@@ -295,6 +299,9 @@ def bench_repair(args, ctx):
     gf_avg = gf_ms / max(launches, 1)
     algo_bytes = n * s + m * s + m * s
     achieved = algo_bytes / (gf_avg * 1e-3) / 1e9 if gf_avg > 0 else 0.0
+    traffic = args.traffic
+    if traffic is None and gf_name == "gf_muladd_lds" and not args.autotune:
+        traffic = C3_GF_TRAFFIC_BYTES
     return {
         "workload": "plain_repair",
         "config": {"workload": "plain_repair_1GiB", "n_slices": n, "slice_bytes": s, "erasures": m,
@@ -302,7 +309,8 @@ def bench_repair(args, ctx):
         "src_bytes": n * s, "total_s": total, "lat": lat, "exact": exact,
         "kernel_ms": {k: v[1] / max(v[0], 1) for k, v in stats.items()},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": args.traffic,
+                     "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": traffic,
+                     "traffic_x_algorithmic": round(traffic / algo_bytes, 3) if traffic else None,
                      "kernel": gf_name, "kernel_ms": round(gf_avg, 4), "algorithmic_bytes": algo_bytes,
                      "gf_muladds": m * n * (s // 2)},
         "cpu": (n, s, m),
