@@ -90,6 +90,7 @@ struct par2hip_ctx {
   par2hip::DevBuf scratch[11];        // grow-only device scratch, by role
   uint8_t *host_stage = nullptr;      // grow-only pinned host staging for file I/O
   size_t host_stage_bytes = 0;
+  hipEvent_t stage_ev[2] = {nullptr, nullptr};  // the staging halves' last copies (h2d_staged / d2h_staged)
   int gf_kernel = 0;                  // 0 auto, 1 LDS tables, 2 MFMA FP4, 3 MFMA int8
   int num_cus = 256;                  // compute units of the device
   bool profile = false;
@@ -129,6 +130,19 @@ int32_t scratch(par2hip_ctx *ctx, int role, size_t bytes, void **out);
 // Pinned host buffer of at least `bytes` (grow-only, kept by the context):
 // files are read into it and copied to the device at pinned-memory rates.
 int32_t host_stage(par2hip_ctx *ctx, size_t bytes, uint8_t **out);
+// memcpy of many ranges in 8 MiB pieces over up to max_threads host threads
+struct CopyJob {
+  uint8_t *dst;
+  const uint8_t *src;
+  uint64_t n;
+};
+void copy_parallel(const std::vector<CopyJob> &jobs, size_t max_threads);
+// Copies between caller (pageable) memory and the device through the pinned
+// staging buffer, in chunks: host threads fill or drain one half while the
+// other half's DMA runs (on ctx->stream).  h2d_staged returns once the
+// caller's bytes have been read; d2h_staged once they have been written.
+int32_t h2d_staged(par2hip_ctx *ctx, void *d_dst, const void *h_src, size_t bytes);
+int32_t d2h_staged(par2hip_ctx *ctx, void *h_dst, const void *d_src, size_t bytes);
 
 // Parameter blocks: begin() waits for the previous upload, alloc() carves
 // 256-byte aligned pieces (returns host and device addresses), commit()

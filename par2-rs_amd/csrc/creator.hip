@@ -106,30 +106,6 @@ int digits(uint64_t x) {  // src/volumes.rs:47-59
   return w;
 }
 
-// memcpy of many ranges in 8 MiB pieces over a few threads (a single thread
-// copying a set's files into the pinned staging buffer runs at a fraction of
-// the host's memory bandwidth)
-struct CopyJob {
-  uint8_t *dst;
-  const uint8_t *src;
-  uint64_t n;
-};
-void copy_parallel(const std::vector<CopyJob> &jobs, size_t max_threads) {
-  std::vector<CopyJob> pieces;
-  const uint64_t kPiece = 8u << 20;
-  for (const CopyJob &j : jobs)
-    for (uint64_t o = 0; o < j.n; o += kPiece) pieces.push_back({j.dst + o, j.src + o, std::min(kPiece, j.n - o)});
-  std::atomic<size_t> next{0};
-  auto worker = [&] {
-    for (size_t i; (i = next++) < pieces.size();) memcpy(pieces[i].dst, pieces[i].src, pieces[i].n);
-  };
-  const size_t nt = std::min(pieces.size(), max_threads);
-  std::vector<std::thread> th;
-  for (size_t t = 1; t < nt; t++) th.emplace_back(worker);
-  worker();
-  for (auto &t : th) t.join();
-}
-
 }  // namespace
 
 int32_t create_par2(par2hip_ctx *ctx, const par2hip_input_file *files, size_t nfiles, uint64_t block_size,

@@ -1,9 +1,11 @@
 // par2hip_api.hip -- the C ABI (include/par2hip.h): context management,
 // parameter upload, and the encode / repair / hash / strategy entry points.
 #include <algorithm>
+#include <atomic>
 #include <cstdio>
 #include <cstring>
 #include <memory>
+#include <thread>
 
 #include "common.hpp"
 
@@ -87,6 +89,84 @@ int32_t host_stage(par2hip_ctx *ctx, size_t bytes, uint8_t **out) {
     ctx->host_stage_bytes = want;
   }
   *out = ctx->host_stage;
+  return PAR2HIP_OK;
+}
+
+void copy_parallel(const std::vector<CopyJob> &jobs, size_t max_threads) {
+  std::vector<CopyJob> pieces;
+  const uint64_t kPiece = 8u << 20;
+  for (const CopyJob &j : jobs)
+    for (uint64_t o = 0; o < j.n; o += kPiece) pieces.push_back({j.dst + o, j.src + o, std::min(kPiece, j.n - o)});
+  std::atomic<size_t> next{0};
+  auto worker = [&] {
+    for (size_t i; (i = next++) < pieces.size();) memcpy(pieces[i].dst, pieces[i].src, pieces[i].n);
+  };
+  const size_t nt = std::min(pieces.size(), max_threads);
+  std::vector<std::thread> th;
+  for (size_t t = 1; t < nt; t++) th.emplace_back(worker);
+  worker();
+  for (auto &t : th) t.join();
+}
+
+namespace {
+constexpr size_t kStageChunk = (size_t)64 << 20;  // bytes per staging half
+constexpr size_t kStageDirect = (size_t)4 << 20;  // below this a plain (pageable) copy
+constexpr size_t kStageThreads = 8;
+}  // namespace
+
+int32_t h2d_staged(par2hip_ctx *ctx, void *d_dst, const void *h_src, size_t bytes) {
+  if (bytes < kStageDirect) {
+    if (bytes) P2H_CHECK(ctx, hipMemcpyAsync(d_dst, h_src, bytes, hipMemcpyHostToDevice, ctx->stream));
+    return PAR2HIP_OK;
+  }
+  const size_t half = std::min(bytes, kStageChunk);
+  uint8_t *stage;
+  int32_t rc;
+  if ((rc = host_stage(ctx, 2 * half, &stage))) return rc;
+  const uint8_t *src = static_cast<const uint8_t *>(h_src);
+  uint8_t *dst = static_cast<uint8_t *>(d_dst);
+  size_t k = 0;
+  for (size_t off = 0; off < bytes; off += half, k++) {
+    const size_t len = std::min(half, bytes - off);
+    uint8_t *buf = stage + (k & 1) * half;
+    // the half's last DMA (chunk k-2, or a previous call's) has read it
+    P2H_CHECK(ctx, hipEventSynchronize(ctx->stage_ev[k & 1]));
+    copy_parallel({{buf, src + off, len}}, kStageThreads);
+    P2H_CHECK(ctx, hipMemcpyAsync(dst + off, buf, len, hipMemcpyHostToDevice, ctx->stream));
+    P2H_CHECK(ctx, hipEventRecord(ctx->stage_ev[k & 1], ctx->stream));
+  }
+  return PAR2HIP_OK;
+}
+
+int32_t d2h_staged(par2hip_ctx *ctx, void *h_dst, const void *d_src, size_t bytes) {
+  if (bytes < kStageDirect) {
+    if (bytes) {
+      P2H_CHECK(ctx, hipMemcpyAsync(h_dst, d_src, bytes, hipMemcpyDeviceToHost, ctx->stream));
+      P2H_CHECK(ctx, hipStreamSynchronize(ctx->stream));
+    }
+    return PAR2HIP_OK;
+  }
+  const size_t half = std::min(bytes, kStageChunk);
+  uint8_t *stage;
+  int32_t rc;
+  if ((rc = host_stage(ctx, 2 * half, &stage))) return rc;
+  const uint8_t *src = static_cast<const uint8_t *>(d_src);
+  uint8_t *dst = static_cast<uint8_t *>(h_dst);
+  const size_t nchunks = (bytes + half - 1) / half;
+  auto issue = [&](size_t k) -> int32_t {
+    const size_t off = k * half, len = std::min(half, bytes - off);
+    P2H_CHECK(ctx, hipMemcpyAsync(stage + (k & 1) * half, src + off, len, hipMemcpyDeviceToHost, ctx->stream));
+    P2H_CHECK(ctx, hipEventRecord(ctx->stage_ev[k & 1], ctx->stream));
+    return PAR2HIP_OK;
+  };
+  if ((rc = issue(0))) return rc;
+  for (size_t k = 0; k < nchunks; k++) {
+    // the half chunk k+1 goes to held chunk k-1, drained in the previous pass
+    if (k + 1 < nchunks && (rc = issue(k + 1))) return rc;
+    P2H_CHECK(ctx, hipEventSynchronize(ctx->stage_ev[k & 1]));
+    const size_t off = k * half, len = std::min(half, bytes - off);
+    copy_parallel({{dst + off, stage + (k & 1) * half, len}}, kStageThreads);
+  }
   return PAR2HIP_OK;
 }
 
@@ -610,6 +690,7 @@ int32_t par2hip_ctx_create(int32_t device, par2hip_ctx **out) {
   P2H_CHECK(ctx, hipEventCreateWithFlags(&ctx->fork, hipEventDisableTiming));
   P2H_CHECK(ctx, hipEventCreateWithFlags(&ctx->join, hipEventDisableTiming));
   P2H_CHECK(ctx, hipEventCreateWithFlags(&ctx->join2, hipEventDisableTiming));
+  for (hipEvent_t &e : ctx->stage_ev) P2H_CHECK(ctx, hipEventCreateWithFlags(&e, hipEventDisableTiming));
   P2H_CHECK(ctx, hipEventCreateWithFlags(&ctx->arena.copied, hipEventDisableTiming));
   const HostField &f = field();
   P2H_CHECK(ctx, hipMalloc(&ctx->d_exp, sizeof f.exp));
@@ -650,6 +731,8 @@ int32_t par2hip_ctx_destroy(par2hip_ctx *ctx) {
     (void)hipEventDestroy(ctx->fork);
     (void)hipEventDestroy(ctx->join);
     (void)hipEventDestroy(ctx->join2);
+    for (hipEvent_t e : ctx->stage_ev)
+      if (e) (void)hipEventDestroy(e);
     if (ctx->rest != ctx->stream) (void)hipStreamDestroy(ctx->rest);
     (void)hipStreamDestroy(ctx->wide);
     for (hipStream_t q : ctx->narrow)
@@ -692,7 +775,8 @@ int32_t par2hip_dev_free(par2hip_ctx *ctx, void *dptr) {
 int32_t par2hip_h2d(par2hip_ctx *ctx, void *dst, const void *src, size_t bytes) {
   if (!ctx || (bytes && (!dst || !src))) return PAR2HIP_EINVAL;
   Guard g(ctx);
-  P2H_CHECK(ctx, hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, ctx->stream));
+  int32_t rc = h2d_staged(ctx, dst, src, bytes);
+  if (rc) return rc;
   P2H_CHECK(ctx, hipStreamSynchronize(ctx->stream));
   return PAR2HIP_OK;
 }
@@ -700,9 +784,7 @@ int32_t par2hip_h2d(par2hip_ctx *ctx, void *dst, const void *src, size_t bytes) 
 int32_t par2hip_d2h(par2hip_ctx *ctx, void *dst, const void *src, size_t bytes) {
   if (!ctx || (bytes && (!dst || !src))) return PAR2HIP_EINVAL;
   Guard g(ctx);
-  P2H_CHECK(ctx, hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, ctx->stream));
-  P2H_CHECK(ctx, hipStreamSynchronize(ctx->stream));
-  return PAR2HIP_OK;
+  return d2h_staged(ctx, dst, src, bytes);
 }
 
 int32_t par2hip_d2d(par2hip_ctx *ctx, void *dst, const void *src, size_t bytes) {
@@ -845,12 +927,12 @@ int32_t par2hip_encode(par2hip_ctx *ctx, const uint8_t *slices, size_t n, size_t
   if ((rc = scratch(ctx, kScratchIn, n * slice_size, &din))) return rc;
   if ((rc = scratch(ctx, kScratchOut, std::max<size_t>(r, 1) * slice_size, &dout))) return rc;
   if ((md5_out || crc_out) && (rc = scratch(ctx, kScratchHash, n * 20, &dh))) return rc;
-  P2H_CHECK(ctx, hipMemcpyAsync(din, slices, n * slice_size, hipMemcpyHostToDevice, ctx->stream));
+  if ((rc = h2d_staged(ctx, din, slices, n * slice_size))) return rc;
   uint32_t *dmd5 = md5_out ? (uint32_t *)dh : nullptr;
   uint32_t *dcrc = crc_out ? (uint32_t *)dh + 4 * n : nullptr;
   if ((rc = encode_body(ctx, (const uint8_t *)din, n, slice_size, exps, r, (uint8_t *)dout, dmd5, dcrc, 0)))
     return rc;
-  if (r) P2H_CHECK(ctx, hipMemcpyAsync(recovery, dout, r * slice_size, hipMemcpyDeviceToHost, ctx->stream));
+  if (r && (rc = d2h_staged(ctx, recovery, dout, r * slice_size))) return rc;
   if (md5_out) P2H_CHECK(ctx, hipMemcpyAsync(md5_out, dmd5, n * 16, hipMemcpyDeviceToHost, ctx->stream));
   if (crc_out) P2H_CHECK(ctx, hipMemcpyAsync(crc_out, dcrc, n * 4, hipMemcpyDeviceToHost, ctx->stream));
   P2H_CHECK(ctx, hipStreamSynchronize(ctx->stream));
@@ -904,14 +986,12 @@ int32_t par2hip_reconstruct(par2hip_ctx *ctx, const uint8_t *slices, size_t n, s
   if ((rc = scratch(ctx, kScratchIn, n * slice_size, &din))) return rc;
   if ((rc = scratch(ctx, kScratchAux, m * slice_size, &drec))) return rc;
   if ((rc = scratch(ctx, kScratchOut, m * slice_size, &dout))) return rc;
-  P2H_CHECK(ctx, hipMemcpyAsync(din, slices, n * slice_size, hipMemcpyHostToDevice, ctx->stream));
-  P2H_CHECK(ctx, hipMemcpyAsync(drec, recovery, m * slice_size, hipMemcpyHostToDevice, ctx->stream));
+  if ((rc = h2d_staged(ctx, din, slices, n * slice_size))) return rc;
+  if ((rc = h2d_staged(ctx, drec, recovery, m * slice_size))) return rc;
   if ((rc = reconstruct_dev_impl(ctx, (uint8_t *)din, n, slice_size, missing, m, (const uint8_t *)drec,
                                  rec_exps, (uint8_t *)dout, false)))
     return rc;
-  P2H_CHECK(ctx, hipMemcpyAsync(out, dout, m * slice_size, hipMemcpyDeviceToHost, ctx->stream));
-  P2H_CHECK(ctx, hipStreamSynchronize(ctx->stream));
-  return PAR2HIP_OK;
+  return d2h_staged(ctx, out, dout, m * slice_size);
 }
 
 int32_t par2hip_repair_set_dev(par2hip_ctx *ctx, void *d_slices, size_t n, size_t slice_size,
@@ -973,7 +1053,7 @@ int32_t par2hip_hash_slices(par2hip_ctx *ctx, const uint8_t *slices, size_t n, s
   if ((rc = scratch(ctx, kScratchIn, n * dstride + 16, &din))) return rc;
   if ((rc = scratch(ctx, kScratchHash, n * 20, &dh))) return rc;
   if (dstride == slice_stride) {
-    P2H_CHECK(ctx, hipMemcpyAsync(din, slices, span, hipMemcpyHostToDevice, ctx->stream));
+    if ((rc = h2d_staged(ctx, din, slices, span))) return rc;
   } else {
     if (n > 1)
       P2H_CHECK(ctx, hipMemcpy2DAsync(din, dstride, slices, slice_stride, std::min(slice_stride, slice_size), n - 1,
