@@ -344,11 +344,20 @@ int32_t create_par2(par2hip_ctx *ctx, const par2hip_input_file *files, size_t nf
   bool ok = head.v.size() == head_size;
   for (size_t i = 0; i < vols.size() && ok; i++) ok = pwrite_all(fds[i], head.v.data(), head.v.size(), 0);
   for (auto &t : writers) t.join();
+  // writeback started and files closed in parallel: ext4 forces the block
+  // allocation of a file truncated and rewritten at its close, one file after
+  // another if the closes run on one thread
+  std::vector<char> closed(vols.size(), 0);
+  std::vector<std::thread> closers;
+  for (size_t i = 0; i < vols.size(); i++)
+    closers.emplace_back([&, i] {
+      (void)sync_file_range(fds[i], 0, 0, SYNC_FILE_RANGE_WRITE);
+      closed[i] = close(fds[i]) == 0;
+    });
+  for (auto &t : closers) t.join();
   for (size_t i = 0; i < vols.size(); i++) {
-    ok = ok && wrote[i];
-    const int fd = fds[i];
+    ok = ok && wrote[i] && closed[i];
     fds[i] = -1;
-    ok = (close(fd) == 0) && ok;
   }
   if (!ok) return fail(ctx, PAR2HIP_EINVAL, "create: write failed for " + vols[0].path + " or a volume");
   for (size_t i = 0; i < vols.size(); i++) {

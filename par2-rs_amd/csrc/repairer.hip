@@ -311,6 +311,10 @@ bool io_parallel(const std::vector<IoRange> &jobs, bool write) {
         if (r <= 0) break;
         done += (uint64_t)r;
       }
+      // start the writeback here, in parallel: ext4 otherwise forces the
+      // block allocation of a file renamed over an existing one inside the
+      // rename (one file after another under the directory lock)
+      if (write && done == pc.n) (void)sync_file_range(fd, 0, 0, SYNC_FILE_RANGE_WRITE);
       if (close(fd) != 0 || done != pc.n) ok = false;
     }
   };
@@ -527,19 +531,38 @@ int32_t repair_files(par2hip_ctx *ctx, const char *index, const char *base_dir, 
   }
 
   // ---- every temp verified before anything is renamed (repair.rs:1050-1080):
-  // read back from disk (parallel), hashed on the device against IFSC
+  // read back from disk (parallel) into the set's device layout, then one
+  // hash pass over every block (the chains run in parallel, so the pass takes
+  // one chain's time however many files were rewritten)
   std::vector<IoRange> vreads;
+  bool any_ifsc = false;
   for (size_t t = 0; t < touched.size(); t++) {
     const size_t i = touched[t];
     if (s.ifsc.count(files[i].id) && nblk[i]) {
       uint8_t *dst = stage + first[i] * bs;
       memset(dst + files[i].length, 0, nblk[i] * bs - files[i].length);
       vreads.push_back({temps[t].first, 0, files[i].length, dst});
+      any_ifsc = true;
     }
   }
   if (!io_parallel(vreads, false)) {
     cleanup();
     return fail(ctx, PAR2HIP_EREPAIR, "repair completed but verification failed - files may still be damaged");
+  }
+  if (any_ifsc) {
+    for (size_t t = 0; t < touched.size(); t++) {
+      const size_t i = touched[t];
+      if (s.ifsc.count(files[i].id) && nblk[i])
+        P2H_CHECK(ctx, hipMemcpyAsync((uint8_t *)d_slices + first[i] * bs, stage + first[i] * bs, nblk[i] * bs,
+                                      hipMemcpyHostToDevice, ctx->stream));
+    }
+    if ((rc = par2hip_hash_slices_dev(ctx, d_slices, n, bs, bs, nullptr, d_hash, (uint8_t *)d_hash + n * 16))) {
+      cleanup();
+      return rc;
+    }
+    P2H_CHECK(ctx, hipMemcpyAsync(md5s.data(), d_hash, n * 16, hipMemcpyDeviceToHost, ctx->stream));
+    P2H_CHECK(ctx, hipMemcpyAsync(crcs.data(), (uint8_t *)d_hash + n * 16, n * 4, hipMemcpyDeviceToHost, ctx->stream));
+    P2H_CHECK(ctx, hipStreamSynchronize(ctx->stream));
   }
   for (size_t t = 0; t < touched.size(); t++) {
     const size_t i = touched[t];
@@ -547,20 +570,11 @@ int32_t repair_files(par2hip_ctx *ctx, const char *index, const char *base_dir, 
     bool ok = true;
     auto it = s.ifsc.find(files[i].id);
     if (it != s.ifsc.end() && nblk[i]) {
-      uint8_t *dev = (uint8_t *)d_slices + first[i] * bs;
-      P2H_CHECK(ctx, hipMemcpyAsync(dev, stage + first[i] * bs, nblk[i] * bs, hipMemcpyHostToDevice, ctx->stream));
-      if ((rc = par2hip_hash_slices_dev(ctx, dev, nblk[i], bs, bs, nullptr, d_hash,
-                                        (uint8_t *)d_hash + nblk[i] * 16))) {
-        cleanup();
-        return rc;
+      for (uint64_t k = 0; k < nblk[i] && ok; k++) {
+        const uint64_t g = first[i] + k;
+        ok = k < it->second.size() && crcs[g] == it->second[k].second &&
+             memcmp(&md5s[g * 16], it->second[k].first.data(), 16) == 0;
       }
-      P2H_CHECK(ctx, hipMemcpyAsync(md5s.data(), d_hash, nblk[i] * 16, hipMemcpyDeviceToHost, ctx->stream));
-      P2H_CHECK(ctx, hipMemcpyAsync(crcs.data(), (uint8_t *)d_hash + nblk[i] * 16, nblk[i] * 4,
-                                    hipMemcpyDeviceToHost, ctx->stream));
-      P2H_CHECK(ctx, hipStreamSynchronize(ctx->stream));
-      for (uint64_t k = 0; k < nblk[i] && ok; k++)
-        ok = k < it->second.size() && crcs[k] == it->second[k].second &&
-             memcmp(&md5s[k * 16], it->second[k].first.data(), 16) == 0;
     } else {
       uint8_t d[16];
       ok = md5_file_prefix(tmp, files[i].length, d) && memcmp(d, files[i].md5.data(), 16) == 0;
