@@ -130,6 +130,8 @@ int32_t scratch(par2hip_ctx *ctx, int role, size_t bytes, void **out);
 // Pinned host buffer of at least `bytes` (grow-only, kept by the context):
 // files are read into it and copied to the device at pinned-memory rates.
 int32_t host_stage(par2hip_ctx *ctx, size_t bytes, uint8_t **out);
+// par2hip_ctx_create, optionally without the narrow CU-masked streams
+int32_t ctx_create(int32_t device, bool narrow_streams, par2hip_ctx **out);
 // memcpy of many ranges in 8 MiB pieces over up to max_threads host threads
 struct CopyJob {
   uint8_t *dst;

@@ -634,7 +634,15 @@ int32_t par2hip_available(void) {
 const char *par2hip_strategy_name(void) { return "HIP-gfx950"; }
 int32_t par2hip_strategy_priority(void) { return PAR2HIP_PRIORITY_OPTIMAL; }
 
-int32_t par2hip_ctx_create(int32_t device, par2hip_ctx **out) {
+int32_t par2hip_ctx_create(int32_t device, par2hip_ctx **out) { return par2hip::ctx_create(device, true, out); }
+
+}  // extern "C"
+
+// narrow_streams: also create the narrower CU-masked streams the single-set
+// encode plans its contraction on.  Each CU-masked stream holds a hardware
+// queue of its own; contexts that only run batches (the server's lanes) skip
+// them so that several lanes do not oversubscribe the device's queues.
+int32_t par2hip::ctx_create(int32_t device, bool narrow_streams, par2hip_ctx **out) {
   if (!out) return PAR2HIP_EINVAL;
   *out = nullptr;
   if (par2hip_available() <= 0) return PAR2HIP_ENODEV;
@@ -672,7 +680,7 @@ int32_t par2hip_ctx_create(int32_t device, par2hip_ctx **out) {
     }
     ctx->masked = ok;
     ctx->hash_cus = ok ? hash_cus : 0;
-    for (int i = 0; ok && i < 3; i++) {
+    for (int i = 0; ok && narrow_streams && i < 3; i++) {
       const int k = 128 + 32 * i;  // 128, 160, 192 CUs after the side stream's
       if (hash_cus + k >= ncu) break;
       std::vector<uint32_t> m((ncu + 31) / 32, 0u);
@@ -703,6 +711,8 @@ int32_t par2hip_ctx_create(int32_t device, par2hip_ctx **out) {
   *out = c.release();
   return PAR2HIP_OK;
 }
+
+extern "C" {
 
 int32_t par2hip_ctx_destroy(par2hip_ctx *ctx) {
   if (!ctx) return PAR2HIP_EINVAL;

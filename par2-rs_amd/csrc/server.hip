@@ -225,7 +225,7 @@ int32_t par2hip_server_create(int32_t device, uint64_t hbm_budget, int32_t lanes
   int32_t rc;
   for (int i = 0; i < lanes; i++) {
     par2hip_ctx *c = nullptr;
-    if ((rc = par2hip_ctx_create(device, &c))) {
+    if ((rc = par2hip::ctx_create(device, false, &c))) {  // lanes run batches only
       for (par2hip_ctx *x : srv->ctxs) par2hip_ctx_destroy(x);
       (void)hipFree(srv->pool_base);
       return rc;
