@@ -10,8 +10,9 @@ it beside the MD5 chains, both hashes) and its completion.  Other workloads
 slices + inverse + rebuild), micro (each contraction variant and hash alone),
 batched (256 sets per planned, graph-replayed batch), served (Poisson
 requests through the serving loop against the plain path at equal p99),
-create and repair_files (whole files to and from .par2 sets), copy (the
-calibration copy for the HBM traffic counters).
+create and repair_files (whole files to and from .par2 sets), encode_host
+(the default set handed over in host memory, both transfers timed), copy
+(the calibration copy for the HBM traffic counters).
 
 Prints ONE JSON line (rank 0): metric/value/unit, ms_per_step, p50/p99 step
 latency, the roofline object of the step's contraction (HIP-event durations
