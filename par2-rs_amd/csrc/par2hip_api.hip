@@ -317,13 +317,16 @@ struct Overlap {
 // faster one; beside the chains the chip is power-bound: the LDS form holds
 // every CU busy on lookups and pulls the shader clock to ~2.0-2.2 GHz, and
 // the chains, the step's critical path, slow with it.  The bit-sliced FP4
-// form on only as many CUs as it needs to finish under the chains (here:
-// the narrowest of the narrow streams and the full complement that brings
-// its estimate under 88% of the chains' lower-bound time) leaves them the
-// clock (1 GiB / R=102, 8 MD5 CUs: 10.7 ms with the LDS form on the other
-// 248 CUs, 10.75 ms with the FP4 form on 248, 9.98 ms on 160).  If no width
-// fits, the contraction is the critical path and the LDS form on every
-// other CU runs.
+// form on only as many CUs as it needs to finish well under the chains
+// (here: the narrowest of the narrow streams and the full complement that
+// brings its estimate under kPlanSlack of the chains' lower-bound time)
+// leaves them the clock.  1 GiB / R=102 with 32 MD5 CUs: the FP4 form on 128
+// CUs 10.88 ms per step (its contraction outlasts the chains), on 160 CUs
+// 9.80 ms, on 192 CUs 9.80 ms with 1.2 ms more slack, on all 224 10.3 ms
+// (the chains lose clock); with 8 MD5 CUs the LDS form on the other 248 CUs
+// took 10.7 ms.  If no width fits, the contraction is the critical path and
+// the LDS form on every other CU runs.
+constexpr double kPlanSlack = 0.75;
 struct GfPlan {
   int kernel = 0;             // 0: the context's choice
   hipStream_t work = nullptr;  // nullptr: rest
@@ -338,7 +341,7 @@ static GfPlan plan_overlapped_gf(const par2hip_ctx *ctx, size_t n, size_t slice_
   for (int i = 0; i <= 3; i++) {
     const int cus = i < 3 ? ctx->narrow_cus[i] : ctx->num_cus - ctx->hash_cus;
     if (cus <= 0 || (i < 3 && !ctx->narrow[i])) continue;
-    if (macs / (kFp4MacPerCuSec * cus) * 1e3 <= 0.88 * md5_ms) {
+    if (macs / (kFp4MacPerCuSec * cus) * 1e3 <= kPlanSlack * md5_ms) {
       g.kernel = 2;
       g.work = i < 3 ? ctx->narrow[i] : nullptr;
       break;
