@@ -46,8 +46,8 @@ HBM_PEAK_GBS = 8000.0
 C2_GF_TRAFFIC_BYTES = 2.475e9
 # The same measurement for the bit-sliced FP4 contraction that the automatic
 # choice runs in the encode step (on 192 CUs beside the MD5 chains): FETCH_SIZE
-# 1,662,258.2 KiB x 2.0 + WRITE_SIZE 104,448 KiB per launch.
-C2_GF_MFMA_TRAFFIC_BYTES = 3.511e9
+# 1,647,433.7 KiB x 2.0 + WRITE_SIZE 104,448 KiB per launch.
+C2_GF_MFMA_TRAFFIC_BYTES = 3.481e9
 # The same two counters for the repair step's rebuild contraction (2000 x 512
 # KiB, 100 erasures, default tile shape): FETCH_SIZE 668,054.7 KiB x 2.0 +
 # WRITE_SIZE 51,200 KiB per launch.
@@ -61,8 +61,8 @@ C2_GF_UNITS = {"lds_util": 0.76, "valu_busy": 0.665, "lds_bank_conflict": 0.0, "
 # The same for the bit-sliced FP4 matrix-core contraction as the encode step
 # runs it (on 192 of the 256 CUs): utilisations over the whole chip, and the
 # matrix-core one over its own CUs.
-C2_GF_MFMA_UNITS = {"mfma_util": 0.452, "mfma_util_own_cus": 0.602, "cus": 192, "lds_util": 0.287,
-                    "valu_busy": 0.342, "lds_bank_conflict": 0.087, "sclk_ghz": 2.18,
+C2_GF_MFMA_UNITS = {"mfma_util": 0.427, "mfma_util_own_cus": 0.570, "cus": 192, "lds_util": 0.267,
+                    "valu_busy": 0.324, "lds_bank_conflict": 0.080, "sclk_ghz": 2.19,
                     "source": "rocprofv3 --pmc, profiles/gf_units_utilisation_pmc.md"}
 FP4_DENSE_PEAK_TFLOPS = 10100.0
 INT8_DENSE_PEAK_TOPS = 5000.0

@@ -236,11 +236,9 @@ __global__ __launch_bounds__(kMfThreads, 1) void k_gf_mfma(const uint16_t *const
     // all 8 K-steps of the stage (K-steps past the last source carry zero
     // operands); each wave expands its own B columns, A comes pre-expanded.
     // Kept rolled: unrolled, the operands of 8 steps spill at 2 waves/SIMD.
-    // The other buffer, filled with the next stage, was last read in stage
-    // c-1, which every wave finished before the previous barrier.
-    if constexpr (kI8) {
 #pragma unroll 1
-      for (int ksl = 0; ksl < kMfKc; ksl++) {
+    for (int ksl = 0; ksl < kMfKc; ksl++) {
+      if constexpr (kI8) {
 #pragma unroll
         for (int hh = 0; hh < 2; hh++) {
           const uint32_t x0 = reinterpret_cast<const uint32_t *>(&sB[buf][4 * ksl + 2 * hh][0])[bcol >> 1];
@@ -256,10 +254,7 @@ __global__ __launch_bounds__(kMfThreads, 1) void k_gf_mfma(const uint16_t *const
               acc[i][f] = __builtin_amdgcn_mfma_i32_32x32x32_i8(a, b[f], acc[i][f], 0, 0, 0);
           }
         }
-      }
-      if (c + 1 < nchunks) store_stage(buf ^ 1);
-    } else {
-      auto kstep = [&](int ksl) {
+      } else {
         const uint32_t x0 = reinterpret_cast<const uint32_t *>(&sB[buf][4 * ksl + 2 * g][0])[bcol >> 1];
         const uint32_t x1 = reinterpret_cast<const uint32_t *>(&sB[buf][4 * ksl + 2 * g + 1][0])[bcol >> 1];
         v8i b[kMfCols];
@@ -273,22 +268,11 @@ __global__ __launch_bounds__(kMfThreads, 1) void k_gf_mfma(const uint16_t *const
           for (int f = 0; f < kMfCols; f++)
             acc[i][f] = __builtin_amdgcn_mfma_scale_f32_32x32x64_f8f6f4(a, b[f], acc[i][f], 4, 4, 0, 127, 0, 127);
         }
-      };
-#pragma unroll 1
-      for (int ksl = 0; ksl < kMfKc - 1; ksl++) kstep(ksl);
-      // The last K-step carries the next stage's eight LDS stores, one
-      // between each pair of its MFMAs, so the matrix pipe keeps running
-      // while they issue (stored after the MFMAs, they left it idle before
-      // every barrier).  After the last stage the store goes to a buffer
-      // that is not read again, so it needs no branch.
-      kstep(kMfKc - 1);
-      store_stage(buf ^ 1);
-#pragma unroll
-      for (int i = 0; i < 8; i++) {
-        __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);  // one MFMA
-        __builtin_amdgcn_sched_group_barrier(0x200, 1, 0);  // one LDS write
       }
     }
+    // the other buffer was last read in stage c-1, which every wave finished
+    // before the previous barrier
+    if (c + 1 < nchunks) store_stage(buf ^ 1);
     __syncthreads();
   }
 
