@@ -161,11 +161,13 @@ __device__ __forceinline__ void md5_slice(const P2H_GLOBAL uint8_t *__restrict__
     // compressed.  Every fetch is unconditional (indexes past the end re-read
     // the last block), so each wait covers exactly the set about to be
     // compressed.
-    const uint64_t nblk = full >> 6;
+    // 32-bit block indices (slices under 256 GiB): the clamp below is then a
+    // 32-bit compare, which the scalar unit has for uniform lengths
+    const uint32_t nblk = (uint32_t)(full >> 6);
     if (nblk) {
-      const uint64_t last = nblk - 1;
+      const uint32_t last = nblk - 1;
       uint32_t S[kMd5Sets][16];
-      auto fetch = [&](uint32_t (&D)[16], uint64_t blk) {
+      auto fetch = [&](uint32_t (&D)[16], uint32_t blk) {
         blk = blk < last ? blk : last;
         if (kAlign == 16) {
           const P2H_GLOBAL u32x4 *r = reinterpret_cast<const P2H_GLOBAL u32x4 *>(p) + 4 * blk;
@@ -185,7 +187,7 @@ __device__ __forceinline__ void md5_slice(const P2H_GLOBAL uint8_t *__restrict__
       };
 #pragma unroll
       for (int j = 0; j < kMd5Sets - 1; j++) fetch(S[j], j);
-      uint64_t blk = 0;
+      uint32_t blk = 0;
       for (; blk + kMd5Sets <= nblk; blk += kMd5Sets) {
 #pragma unroll
         for (int j = 0; j < kMd5Sets; j++) {
@@ -256,9 +258,17 @@ __global__ __launch_bounds__(64) void k_md5_slices(const uint8_t *__restrict__ d
   if (high_priority) __builtin_amdgcn_s_setprio(3);
   const long i = (long)blockIdx.x * 64 + threadIdx.x;
   if (i >= n) return;
-  uint64_t len = lens ? lens[i] : slice_size;
+  const P2H_GLOBAL uint8_t *p = gptr(data) + (uint64_t)i * stride;
+  if (!lens) {
+    // every slice full: the block count and the prefetch clamp are the same
+    // for all lanes, so they stay in scalar registers (about 6 vector
+    // instructions less per 64-byte block than with per-lane lengths)
+    md5_slice<kAlign>(p, slice_size, slice_size, gptr(out) + 4 * i);
+    return;
+  }
+  uint64_t len = lens[i];
   if (len > slice_size) len = slice_size;
-  md5_slice<kAlign>(gptr(data) + (uint64_t)i * stride, len, slice_size, gptr(out) + 4 * i);
+  md5_slice<kAlign>(p, len, slice_size, gptr(out) + 4 * i);
 }
 
 // Batched form: one lane per entry of a slice list (slices of many sets,
@@ -276,8 +286,10 @@ __global__ __launch_bounds__(64) void k_md5_list(const HashEntry *__restrict__ l
 }
 
 double md5_chains_ms(size_t n, uint64_t bytes, int simds) {
-  // 337 instructions per 64-byte block in md5_compress's loop, one issue per
-  // 4 clocks per wave (a wave64 VALU instruction occupies its SIMD 4 clocks)
+  // ~337 instructions issued per 64-byte block in md5_compress's loop (324
+  // vector ones with uniform slice lengths, 330 with per-slice lengths), one
+  // issue per 4 clocks per wave (a wave64 VALU instruction occupies its SIMD
+  // 4 clocks)
   const double blocks = (double)((bytes + 8) / 64 + 1);
   const double waves = (double)((n + 63) / 64);
   const double rounds = std::ceil(waves / std::max(simds, 1));
@@ -298,6 +310,8 @@ int32_t launch_md5(par2hip_ctx *ctx, hipStream_t s, const uint8_t *d_data, size_
                    size_t slice_size, size_t stride, const uint64_t *d_lens, uint32_t *d_md5,
                    bool high_priority) {
   if (n == 0) return PAR2HIP_OK;
+  // md5_slice counts 64-byte blocks in 32 bits
+  if (slice_size >= (size_t(1) << 38)) return fail(ctx, PAR2HIP_EINVAL, "slice of 256 GiB or more");
   const bool a16 = ((uintptr_t)d_data % 16 == 0) && (stride % 16 == 0);
   const bool a4 = ((uintptr_t)d_data % 4 == 0) && (stride % 4 == 0);
   ScopedTiming t(ctx, "md5_slices", s);
