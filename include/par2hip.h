@@ -46,8 +46,12 @@ enum {
     PAR2HIP_ESINGULAR = -4, /* erasure matrix singular ("Singular matrix", par2_rs.rs:369) */
     PAR2HIP_EHIP = -5,      /* HIP runtime error; see par2hip_last_error */
     PAR2HIP_ERECOVERY = -6, /* fewer recovery slices than erasures (par2_rs.rs:332-338) */
-    PAR2HIP_EREPAIR = -7    /* rebuilt files failed verification; nothing was committed
+    PAR2HIP_EREPAIR = -7,   /* rebuilt files failed verification; nothing was committed
                                (RepairFailed, src/repair.rs:1050-1080) */
+    PAR2HIP_ECANCELLED = -8, /* the caller's cancel flag was set; no original file was
+                                changed (Par2Error::Cancelled, src/error.rs:35) */
+    PAR2HIP_ETIMEDOUT = -9   /* par2hip_server_wait_timeout: not done in time (no
+                                counterpart in the reference) */
 };
 
 /* Strategy priority, same scale as `Priority` (src/galois/simd/mod.rs:15-21). */
@@ -99,12 +103,16 @@ int32_t par2hip_gf_muladd(par2hip_ctx *ctx, uint16_t *dst, const uint16_t *src, 
                           uint16_t scalar);
 
 /* GaloisSimdStrategy::muladd_region (simd/mod.rs:50-57): for every dst i and
- * src j < nsrc: dst_i[off..off+len) ^= coeff_rows[i][j] * src_j[off..off+len)
- * (src/galois/scalar.rs:82-113).  coeff_rows[i] points at >= nsrc values. */
+ * src j < min(coeff_lens[i], nsrc):
+ *   dst_i[off..off+len) ^= coeff_rows[i][j] * src_j[off..off+len)
+ * (src/galois/scalar.rs:82-113; a coefficient row shorter than the source
+ * list applies only its own entries, as `.enumerate().take(num_srcs)` does at
+ * scalar.rs:100).  coeff_rows[i] points at coeff_lens[i] values; coeff_lens
+ * NULL means every row holds at least nsrc. */
 int32_t par2hip_gf_muladd_region(par2hip_ctx *ctx, uint16_t *const *dsts, size_t ndst,
                                  const uint16_t *const *srcs, size_t nsrc,
-                                 const uint16_t *const *coeff_rows, size_t region_offset,
-                                 size_t region_size);
+                                 const uint16_t *const *coeff_rows, const size_t *coeff_lens,
+                                 size_t region_offset, size_t region_size);
 
 /* ------------------------------------------------------------ encode -- */
 
@@ -116,7 +124,9 @@ int32_t par2hip_gf_muladd_region(par2hip_ctx *ctx, uint16_t *const *dsts, size_t
  *           c_d the PAR2 constants (par2_rs.rs:64-76).
  * md5_out / crc_out (nullable): IFSC MD5 (16 B) and CRC32 of every input
  *           slice (creator.rs:397-409).
- * slice_size must be even and non-zero; n <= 32768 (distinct constants). */
+ * slice_size must be even and non-zero; n <= 32768 (distinct constants);
+ * with MD5 requested, slice_size < 2^37 (128 GiB: the MD5 kernels count
+ * 64-byte blocks in 32 bits), else PAR2HIP_EINVAL before any launch. */
 int32_t par2hip_encode(par2hip_ctx *ctx, const uint8_t *slices, size_t n, size_t slice_size,
                        const uint32_t *exps, size_t r, uint8_t *recovery, uint8_t *md5_out,
                        uint32_t *crc_out);
@@ -261,8 +271,18 @@ double par2hip_server_now_ms(par2hip_server *srv);
 /* Non-blocking.  PAR2HIP_EINVAL for a malformed job, PAR2HIP_ENOMEM for one
  * whose footprint exceeds the whole budget. */
 int32_t par2hip_server_submit(par2hip_server *srv, const par2hip_request *req, uint64_t *ticket);
-/* Blocks until the request is done; its outputs stay valid until release. */
+/* Blocks until the request is done; its outputs stay valid until release.
+ * Admission is strict FIFO: a request whose footprint does not fit the free
+ * budget holds back every request submitted after it until earlier requests
+ * are released.  A client that waits on such a request while it still holds
+ * unreleased results it needs to free the budget therefore waits forever;
+ * par2hip_server_wait_timeout bounds that wait. */
 int32_t par2hip_server_wait(par2hip_server *srv, uint64_t ticket, par2hip_result *res);
+/* par2hip_server_wait with a limit: PAR2HIP_ETIMEDOUT (res untouched, the
+ * request keeps its place) when the request is not done within timeout_ms
+ * milliseconds; 0 polls; an infinite value waits as par2hip_server_wait. */
+int32_t par2hip_server_wait_timeout(par2hip_server *srv, uint64_t ticket, double timeout_ms,
+                                    par2hip_result *res);
 int32_t par2hip_server_release(par2hip_server *srv, uint64_t ticket);
 int32_t par2hip_server_get_stats(par2hip_server *srv, par2hip_server_stats *st);
 
@@ -328,13 +348,27 @@ typedef struct {
 } par2hip_repair_report;
 int32_t par2hip_repair_files(par2hip_ctx *ctx, const char *par2_path, const char *base_dir,
                              int32_t verify_only, par2hip_repair_report *report);
+/* Par2Repairer::repair_with_callbacks_cancellable (src/lib.rs:196-345):
+ * par2hip_repair_files with a cancel flag, read with relaxed atomic loads at
+ * the reference's safe points: before loading, before and after
+ * verification, after the rebuild, after the temps are written and before
+ * the first rename (src/repair.rs:808-813, :972-977, :1043-1067).  cancel
+ * points at one byte, 0 or 1 (the layout of a Rust AtomicBool); NULL never
+ * cancels.  A cancelled call returns PAR2HIP_ECANCELLED, removes every
+ * .par2tmp it wrote and leaves every original byte-identical; once the
+ * renames have started the repair is committed and reported as done. */
+int32_t par2hip_repair_files_cancellable(par2hip_ctx *ctx, const char *par2_path, const char *base_dir,
+                                         int32_t verify_only, const uint8_t *cancel,
+                                         par2hip_repair_report *report);
 
 /* -------------------------------------------------------------- hashes -- */
 
 /* IFSC MD5 + CRC32 of n slices, each zero-padded to slice_size
  * (creator.rs:389-409; verify.rs:592-595).  Slice i is read from
  * slices + i*slice_stride for lens[i] bytes (lens NULL: slice_size each).
- * md5_out: n*16 bytes (nullable); crc_out: n u32 (nullable). */
+ * md5_out: n*16 bytes (nullable); crc_out: n u32 (nullable).  With md5_out,
+ * slice_size < 2^37 (128 GiB), else PAR2HIP_EINVAL before any launch; the
+ * same limit holds for par2hip_repair_set_dev and for hashed batch jobs. */
 int32_t par2hip_hash_slices(par2hip_ctx *ctx, const uint8_t *slices, size_t n, size_t slice_size,
                             size_t slice_stride, const uint64_t *lens, uint8_t *md5_out,
                             uint32_t *crc_out);

@@ -144,6 +144,17 @@ def muladd_region(dsts, srcs, coeff: np.ndarray, off: int, length: int) -> None:
                            coeff.shape[1], off, length)
 
 
+def muladd_region_rows(dsts, srcs, rows, off: int, length: int) -> None:
+    """gf_muladd_region_scalar (src/galois/scalar.rs:82-113) with coefficient
+    rows of any length: row i applies entries j < min(len(rows[i]), nsrc),
+    the `.enumerate().take(num_srcs)` of scalar.rs:100."""
+    for d, row in zip(dsts, rows):
+        row = np.ascontiguousarray(row, np.uint16).reshape(1, -1)
+        k = min(row.shape[1], len(srcs))
+        if k:
+            muladd_region([d], srcs[:k], row, off, length)
+
+
 # ---- codec (src/par2_rs.rs) ------------------------------------------------
 def encode(data: np.ndarray, exps) -> np.ndarray:
     """data: (n, words) uint16 -> (r, words) recovery slices."""

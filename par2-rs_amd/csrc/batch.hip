@@ -85,7 +85,7 @@ static int32_t plan_build(par2hip_ctx *ctx, const par2hip_job *jobs, size_t njob
       return fail(ctx, PAR2HIP_EINVAL, "batch job: need slices, n >= 1 and an even slice size");
     if (J.n > 32768) return fail(ctx, PAR2HIP_EINVAL, "batch job: more than 32768 input slices");
     // the MD5 kernels count 64-byte blocks in 32 bits
-    if (J.slice_size >= (size_t(1) << 38)) return fail(ctx, PAR2HIP_EINVAL, "batch job: slice of 256 GiB or more");
+    if (J.slice_size >= kMd5MaxSliceBytes) return fail(ctx, PAR2HIP_EINVAL, "batch job: slice of 128 GiB or more");
     if (J.kind == PAR2HIP_JOB_ENCODE) {
       if (J.count && (!J.exps || !J.d_recovery)) return fail(ctx, PAR2HIP_EINVAL, "batch encode job: exponents and recovery buffer");
       if (J.count > 65535) return fail(ctx, PAR2HIP_EINVAL, "batch encode job: too many recovery slices");

@@ -274,6 +274,10 @@ int32_t run_gf_mfma(par2hip_ctx *ctx, hipStream_t s, const GfProblem &p, bool in
 constexpr int kAutoGfKernel = 1;
 
 // Slice hashes.
+// Largest slice (exclusive) the MD5 kernels take: they count 64-byte blocks
+// in 32 bits with room for the prefetch distance.  Every entry point that
+// hashes refuses larger slices with PAR2HIP_EINVAL before any launch.
+constexpr uint64_t kMd5MaxSliceBytes = uint64_t(1) << 37;
 constexpr int kCrcLanes = 256;  // CRC-32: lanes (one workgroup) per slice
 // Per-slice-size constants of the CRC-32 tree combine.
 struct CrcParams {

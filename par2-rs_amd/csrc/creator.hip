@@ -25,6 +25,7 @@
 
 #include "common.hpp"
 #include "md5_host.hpp"
+#include "par2set.hpp"
 
 namespace par2hip {
 namespace {
@@ -72,18 +73,6 @@ void append_packet(Bytes &out, const uint8_t type[16], const uint8_t set_id[16],
   out.put(padded.data(), padded.size());
 }
 
-// calculate_block_size (src/creator.rs:29-67): about 2000 blocks, 2 KiB to
-// 4 MiB, multiple of 4, never more than 65535 blocks.
-uint64_t auto_block_size(uint64_t total) {
-  if (total == 0) return 2048;
-  uint64_t bs = (total / 2000 + 3) & ~(uint64_t)3;
-  const uint64_t kMax = (uint64_t)4 << 20;
-  uint64_t floor_bs = std::max<uint64_t>((total + 65534) / 65535, 2048);
-  floor_bs = (floor_bs + 3) & ~(uint64_t)3;
-  if (floor_bs > kMax) return floor_bs;
-  return std::max<uint64_t>(std::min<uint64_t>(std::max<uint64_t>(bs, 2048), kMax), floor_bs);
-}
-
 struct Input {
   const par2hip_input_file *f;
   uint8_t full[16], h16k[16], id[16];
@@ -126,11 +115,9 @@ int32_t create_par2(par2hip_ctx *ctx, const par2hip_input_file *files, size_t nf
   // whole-file MD5s (creator.rs:80-135, one serial chain each) run on host
   // threads while the device works and are needed only for FileDesc.
   std::vector<Input> in(nfiles);
-  uint64_t total = 0;
   for (size_t i = 0; i < nfiles; i++) {
     Input &x = in[i];
     x.f = &files[i];
-    total += files[i].len;
     Md5Host::digest(x.f->data, std::min<uint64_t>(x.f->len, 16384), x.h16k);
     Md5Host h;
     h.update(x.h16k, 16);
@@ -143,7 +130,12 @@ int32_t create_par2(par2hip_ctx *ctx, const par2hip_input_file *files, size_t nf
   std::stable_sort(in.begin(), in.end(), id_less);
 
   // ---- block and recovery counts (creator.rs:291-347)
-  if (block_size == 0) block_size = auto_block_size(total);
+  if (block_size == 0) {
+    std::vector<uint64_t> lens(nfiles);
+    for (size_t i = 0; i < nfiles; i++) lens[i] = files[i].len;
+    block_size = create_block_size(lens.data(), nfiles, 32768);
+    if (block_size == 0) return fail(ctx, PAR2HIP_EINVAL, "create: 32768 or more non-empty files");
+  }
   uint64_t n = 0;
   for (Input &x : in) {
     x.first = n;

@@ -161,8 +161,10 @@ __device__ __forceinline__ void md5_slice(const P2H_GLOBAL uint8_t *__restrict__
     // compressed.  Every fetch is unconditional (indexes past the end re-read
     // the last block), so each wait covers exactly the set about to be
     // compressed.
-    // 32-bit block indices (slices under 256 GiB): the clamp below is then a
-    // 32-bit compare, which the scalar unit has for uniform lengths
+    // 32-bit block counts (slices under kMd5MaxSliceBytes = 128 GiB, so
+    // every block index and index + kMd5Sets stays below 2^31): the clamp
+    // below is then a 32-bit compare, which the scalar unit has for uniform
+    // lengths.  Addresses are formed in 64 bits.
     const uint32_t nblk = (uint32_t)(full >> 6);
     if (nblk) {
       const uint32_t last = nblk - 1;
@@ -170,14 +172,14 @@ __device__ __forceinline__ void md5_slice(const P2H_GLOBAL uint8_t *__restrict__
       auto fetch = [&](uint32_t (&D)[16], uint32_t blk) {
         blk = blk < last ? blk : last;
         if (kAlign == 16) {
-          const P2H_GLOBAL u32x4 *r = reinterpret_cast<const P2H_GLOBAL u32x4 *>(p) + 4 * blk;
+          const P2H_GLOBAL u32x4 *r = reinterpret_cast<const P2H_GLOBAL u32x4 *>(p + (uint64_t)blk * 64);
           const u32x4 t0 = r[0], t1 = r[1], t2 = r[2], t3 = r[3];
           D[0] = t0.x; D[1] = t0.y; D[2] = t0.z; D[3] = t0.w;
           D[4] = t1.x; D[5] = t1.y; D[6] = t1.z; D[7] = t1.w;
           D[8] = t2.x; D[9] = t2.y; D[10] = t2.z; D[11] = t2.w;
           D[12] = t3.x; D[13] = t3.y; D[14] = t3.z; D[15] = t3.w;
         } else {
-          const P2H_GLOBAL uint32_t *r = reinterpret_cast<const P2H_GLOBAL uint32_t *>(p) + 16 * blk;
+          const P2H_GLOBAL uint32_t *r = reinterpret_cast<const P2H_GLOBAL uint32_t *>(p + (uint64_t)blk * 64);
 #pragma unroll
           for (int k = 0; k < 16; k++) D[k] = r[k];
         }
@@ -286,14 +288,14 @@ __global__ __launch_bounds__(64) void k_md5_list(const HashEntry *__restrict__ l
 }
 
 double md5_chains_ms(size_t n, uint64_t bytes, int simds) {
-  // ~337 instructions issued per 64-byte block in md5_compress's loop (324
-  // vector ones with uniform slice lengths, 330 with per-slice lengths), one
-  // issue per 4 clocks per wave (a wave64 VALU instruction occupies its SIMD
-  // 4 clocks)
+  // 324 vector instructions per 64-byte block in md5_compress's loop with
+  // uniform slice lengths (the form every whole-set encode and verify takes;
+  // 330 with per-slice lengths), one issue per 4 clocks per wave (a wave64
+  // VALU instruction occupies its SIMD 4 clocks)
   const double blocks = (double)((bytes + 8) / 64 + 1);
   const double waves = (double)((n + 63) / 64);
   const double rounds = std::ceil(waves / std::max(simds, 1));
-  return rounds * blocks * 337.0 * 4.0 / 2.4e9 * 1e3;
+  return rounds * blocks * 324.0 * 4.0 / 2.4e9 * 1e3;
 }
 
 int32_t launch_md5_list(par2hip_ctx *ctx, hipStream_t s, const HashEntry *d_list, size_t n, bool aligned4,
@@ -310,8 +312,7 @@ int32_t launch_md5(par2hip_ctx *ctx, hipStream_t s, const uint8_t *d_data, size_
                    size_t slice_size, size_t stride, const uint64_t *d_lens, uint32_t *d_md5,
                    bool high_priority) {
   if (n == 0) return PAR2HIP_OK;
-  // md5_slice counts 64-byte blocks in 32 bits
-  if (slice_size >= (size_t(1) << 38)) return fail(ctx, PAR2HIP_EINVAL, "slice of 256 GiB or more");
+  if (slice_size >= kMd5MaxSliceBytes) return fail(ctx, PAR2HIP_EINVAL, "slice of 128 GiB or more");
   const bool a16 = ((uintptr_t)d_data % 16 == 0) && (stride % 16 == 0);
   const bool a4 = ((uintptr_t)d_data % 4 == 0) && (stride % 4 == 0);
   ScopedTiming t(ctx, "md5_slices", s);

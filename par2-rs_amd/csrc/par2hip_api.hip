@@ -366,6 +366,7 @@ int32_t encode_body(par2hip_ctx *ctx, const uint8_t *d_slices, size_t n, size_t 
     return fail(ctx, PAR2HIP_EINVAL, "need n >= 1 and an even, non-zero slice size");
   if (n > 32768) return fail(ctx, PAR2HIP_EINVAL, "more than 32768 input slices repeat PAR2 constants");
   if (r > 65535 || (r && !exps)) return fail(ctx, PAR2HIP_EINVAL, "bad recovery exponent list");
+  if (d_md5 && slice_size >= kMd5MaxSliceBytes) return fail(ctx, PAR2HIP_EINVAL, "hashed slice of 128 GiB or more");
   const size_t words = slice_size / 2;
   const std::vector<uint32_t> logs = constant_logs(n);
   void *coeff = nullptr;
@@ -539,6 +540,7 @@ int32_t repair_set_impl(par2hip_ctx *ctx, uint8_t *d_slices, size_t n, size_t sl
                         size_t *ndamaged) {
   if (n == 0 || slice_size == 0 || (slice_size & 1))
     return fail(ctx, PAR2HIP_EINVAL, "need n >= 1 and an even, non-zero slice size");
+  if (slice_size >= kMd5MaxSliceBytes) return fail(ctx, PAR2HIP_EINVAL, "hashed slice of 128 GiB or more");
   int32_t rc;
   void *dh;
   if ((rc = scratch(ctx, kScratchHashD, n * 20 + 16, &dh))) return rc;
@@ -822,8 +824,8 @@ int32_t par2hip_sync(par2hip_ctx *ctx) {
 // ---------------------------------------------------- strategy-shaped --
 int32_t par2hip_gf_muladd_region(par2hip_ctx *ctx, uint16_t *const *dsts, size_t ndst,
                                  const uint16_t *const *srcs, size_t nsrc,
-                                 const uint16_t *const *coeff_rows, size_t region_offset,
-                                 size_t region_size) {
+                                 const uint16_t *const *coeff_rows, const size_t *coeff_lens,
+                                 size_t region_offset, size_t region_size) {
   if (!ctx) return PAR2HIP_EINVAL;
   if (ndst == 0 || region_size == 0) return PAR2HIP_OK;
   if (!dsts || (nsrc && (!srcs || !coeff_rows))) return PAR2HIP_EINVAL;
@@ -841,7 +843,13 @@ int32_t par2hip_gf_muladd_region(par2hip_ctx *ctx, uint16_t *const *dsts, size_t
   uint16_t *h_coeff = (uint16_t *)params_alloc(ctx, ndst * nsrc * 2, &d_coeff);
   const uint16_t **h_srcs = (const uint16_t **)params_alloc(ctx, nsrc * 8, &d_srcs);
   uint16_t **h_dsts = (uint16_t **)params_alloc(ctx, ndst * 8, &d_dsts);
-  for (size_t i = 0; i < ndst; i++) memcpy(h_coeff + i * nsrc, coeff_rows[i], nsrc * 2);
+  // row i applies its first min(len_i, nsrc) entries; sources past a short
+  // row's end are not applied (scalar.rs:100, `.enumerate().take(num_srcs)`)
+  for (size_t i = 0; i < ndst; i++) {
+    const size_t k = coeff_lens ? std::min(coeff_lens[i], nsrc) : nsrc;
+    if (k) memcpy(h_coeff + i * nsrc, coeff_rows[i], k * 2);
+    memset(h_coeff + i * nsrc + k, 0, (nsrc - k) * 2);
+  }
   for (size_t j = 0; j < nsrc; j++) h_srcs[j] = (const uint16_t *)((uint8_t *)din + j * pitch);
   for (size_t i = 0; i < ndst; i++) h_dsts[i] = (uint16_t *)((uint8_t *)dout + i * pitch);
   if ((rc = params_commit(ctx))) return rc;
@@ -876,7 +884,7 @@ int32_t par2hip_gf_muladd(par2hip_ctx *ctx, uint16_t *dst, const uint16_t *src, 
   const uint16_t *cr[1] = {&scalar};
   uint16_t *d[1] = {dst};
   const uint16_t *s[1] = {src};
-  return par2hip_gf_muladd_region(ctx, d, 1, s, 1, cr, 0, len);
+  return par2hip_gf_muladd_region(ctx, d, 1, s, 1, cr, nullptr, 0, len);
 }
 
 int32_t par2hip_gf_mul_slice(par2hip_ctx *ctx, uint16_t scalar, uint16_t *data, size_t len) {
@@ -1028,6 +1036,7 @@ int32_t par2hip_hash_slices_dev(par2hip_ctx *ctx, const void *d_slices, size_t n
   Guard g(ctx);
   if (slice_stride == 0) slice_stride = slice_size;
   if (slice_stride < slice_size && !lens) return fail(ctx, PAR2HIP_EINVAL, "stride shorter than the slice size");
+  if (d_md5 && slice_size >= kMd5MaxSliceBytes) return fail(ctx, PAR2HIP_EINVAL, "hashed slice of 128 GiB or more");
   const uint64_t *d_lens = nullptr;
   int32_t rc;
   if (lens) {

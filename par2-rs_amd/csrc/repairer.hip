@@ -1,18 +1,19 @@
-// repairer.hip -- loading a PAR2 set and repairing its files over the GPU
-// entry points: what Par2Repairer does with the reference's parser, verifier
-// and repair driver, with the slice hashes (verify and pre-commit check) and
-// the rebuild on the device.
+// repairer.hip -- repairing the files of a PAR2 set over the GPU entry
+// points: what Par2Repairer does with the reference's verifier and repair
+// driver, with the slice hashes (verify and pre-commit check) and the rebuild
+// on the device.  Loading the set (parser, volumes, canonical order, block
+// layout) is host-only code in par2set.cpp.
 //
-//   parse   src/parser.rs:181-547 (packets, resync, MD5 check of every packet
-//           but the recovery slices, which are located and read at repair
-//           time; volumes = every .par2 beside the index with the same
-//           recovery set ID), canonical order src/parser.rs:556-596
 //   verify  src/verify.rs:87-330, :662-760 (size first; IFSC block checks, or
 //           the 16 KiB and whole-file MD5 without IFSC; a missing file or a
 //           size mismatch makes every block of the file damaged)
 //   repair  src/repair.rs:445-1125 (lowest exponents first, <final>.par2tmp
 //           seeded with the original, rebuilt blocks written, every temp
 //           verified, renamed only when all verify, temps removed otherwise)
+//   cancel  src/lib.rs:196-345, src/repair.rs:56-59, :808-813, :972-977,
+//           :1043-1067 (a flag checked at safe points; before the renames
+//           only temps exist, so a cancelled call leaves every original
+//           byte-identical)
 #include <dirent.h>
 #include <fcntl.h>
 #include <sys/stat.h>
@@ -30,242 +31,17 @@
 
 #include "common.hpp"
 #include "md5_host.hpp"
+#include "par2set.hpp"
 
 namespace par2hip {
 namespace {
 
-const uint8_t kMagic[8] = {'P', 'A', 'R', '2', 0, 'P', 'K', 'T'};
-const uint8_t kMain[16] = {'P', 'A', 'R', ' ', '2', '.', '0', 0, 'M', 'a', 'i', 'n', 0, 0, 0, 0};
-const uint8_t kFileDesc[16] = {'P', 'A', 'R', ' ', '2', '.', '0', 0, 'F', 'i', 'l', 'e', 'D', 'e', 's', 'c'};
-const uint8_t kIfsc[16] = {'P', 'A', 'R', ' ', '2', '.', '0', 0, 'I', 'F', 'S', 'C', 0, 0, 0, 0};
-const uint8_t kRecvSlic[16] = {'P', 'A', 'R', ' ', '2', '.', '0', 0, 'R', 'e', 'c', 'v', 'S', 'l', 'i', 'c'};
 const char kTmpSuffix[] = ".par2tmp";  // src/repair.rs:63-67
-
-using Id = std::array<uint8_t, 16>;
-
-uint64_t le64(const uint8_t *p) {
-  uint64_t v = 0;
-  for (int i = 7; i >= 0; i--) v = v << 8 | p[i];
-  return v;
-}
-uint32_t le32(const uint8_t *p) { return (uint32_t)p[0] | p[1] << 8 | p[2] << 16 | (uint32_t)p[3] << 24; }
-Id id_at(const uint8_t *p) {
-  Id x;
-  memcpy(x.data(), p, 16);
-  return x;
-}
-
-struct FileRec {
-  Id id, md5, md5k;
-  uint64_t length = 0;
-  std::string name;
-};
-struct RecSlice {
-  uint32_t exponent;
-  std::string path;
-  uint64_t data_offset;  // of the recovery bytes (after the exponent)
-};
-struct Par2Set {
-  bool have_set_id = false;
-  Id set_id{};
-  uint64_t block_size = 0;
-  std::vector<Id> main_order;
-  std::vector<FileRec> files_in_order;  // FileDesc order of first appearance
-  std::map<Id, FileRec> files;
-  std::map<Id, std::vector<std::pair<Id, uint32_t>>> ifsc;  // md5, crc32 per slice
-  std::vector<RecSlice> recs;
-};
 
 bool file_size(const std::string &p, uint64_t *sz) {
   struct stat st;
   if (stat(p.c_str(), &st) != 0 || !S_ISREG(st.st_mode)) return false;
   *sz = (uint64_t)st.st_size;
-  return true;
-}
-
-// One .par2 file (parser.rs:181-400).  recovery_only: a volume beside the
-// index (only its recovery slices are taken, parser.rs:443-520).
-int32_t parse_file(const std::string &path, Par2Set &s, bool recovery_only, std::string *err) {
-  FILE *f = fopen(path.c_str(), "rb");
-  if (!f) {
-    *err = "cannot open " + path;
-    return PAR2HIP_EINVAL;
-  }
-  uint64_t size = 0;
-  file_size(path, &size);
-  std::vector<std::pair<uint64_t, uint64_t>> deferred;  // recovery packets seen before the block size
-  std::vector<uint8_t> body;
-  uint64_t pos = 0;
-  uint8_t h[64];
-  while (pos < size) {
-    if (fseeko(f, (off_t)pos, SEEK_SET) != 0 || fread(h, 1, 64, f) < 64) break;
-    if (memcmp(h, kMagic, 8) != 0) {
-      pos++;
-      continue;
-    }
-    if (!s.have_set_id && !recovery_only) {
-      s.set_id = id_at(h + 32);
-      s.have_set_id = true;
-    }
-    const uint64_t len = le64(h + 8);
-    if (len < 64 || len > size - pos) {
-      pos++;
-      continue;
-    }
-    const uint64_t next = pos + len + ((len % 4) ? 4 - len % 4 : 0);
-    if (memcmp(h + 48, kRecvSlic, 16) == 0) {
-      if (len >= 68) {
-        if (s.block_size == 0) {
-          deferred.push_back({pos, len});
-        } else {
-          uint8_t e[4];
-          if (fread(e, 1, 4, f) == 4 && len >= 68 + s.block_size) s.recs.push_back({le32(e), path, pos + 68});
-        }
-      }
-      pos = next;
-      continue;
-    }
-    body.resize(len - 64);
-    if (fread(body.data(), 1, body.size(), f) < body.size()) break;
-    Md5Host md;
-    md.update(h + 32, 32);
-    md.update(body.data(), body.size());
-    uint8_t digest[16];
-    md.finish(digest);
-    if (memcmp(digest, h + 16, 16) != 0) {  // a damaged packet is skipped
-      pos = next;
-      continue;
-    }
-    if (!recovery_only) {
-      if (memcmp(h + 48, kMain, 16) == 0 && body.size() >= 12) {
-        const uint64_t bs = le64(body.data());
-        const uint32_t nrec = le32(body.data() + 8);
-        if (body.size() >= 12 + 16ull * nrec) {
-          s.block_size = bs;
-          if (s.main_order.empty())
-            for (uint32_t k = 0; k < nrec; k++) s.main_order.push_back(id_at(body.data() + 12 + 16 * k));
-        }
-      } else if (memcmp(h + 48, kFileDesc, 16) == 0 && body.size() >= 56) {
-        FileRec r;
-        r.id = id_at(body.data());
-        r.md5 = id_at(body.data() + 16);
-        r.md5k = id_at(body.data() + 32);
-        r.length = le64(body.data() + 48);
-        size_t nl = 0;
-        while (56 + nl < body.size() && body[56 + nl]) nl++;
-        r.name.assign((const char *)body.data() + 56, nl);
-        if (s.files.emplace(r.id, r).second) s.files_in_order.push_back(r);
-      } else if (memcmp(h + 48, kIfsc, 16) == 0 && body.size() >= 16) {
-        std::vector<std::pair<Id, uint32_t>> ent;
-        for (size_t k = 16; k + 20 <= body.size(); k += 20) ent.push_back({id_at(body.data() + k), le32(&body[k + 16])});
-        s.ifsc[id_at(body.data())] = ent;
-      }
-    }
-    pos = next;
-  }
-  for (auto &d : deferred) {  // parser.rs:415-440
-    uint8_t e[4];
-    if (fseeko(f, (off_t)(d.first + 64), SEEK_SET) == 0 && fread(e, 1, 4, f) == 4 && d.second >= 68 + s.block_size)
-      s.recs.push_back({le32(e), path, d.first + 68});
-  }
-  fclose(f);
-  return PAR2HIP_OK;
-}
-
-// The recovery set ID of a .par2 file: that of the packet header at offset
-// 0; a file that does not start with one is not a volume of any set
-// (parser.rs:606-623).
-bool first_set_id(const std::string &path, Id *out) {
-  FILE *f = fopen(path.c_str(), "rb");
-  if (!f) return false;
-  uint8_t h[64];
-  const bool ok = fread(h, 1, 64, f) == 64 && memcmp(h, kMagic, 8) == 0;
-  if (ok) *out = id_at(h + 32);
-  fclose(f);
-  return ok;
-}
-
-std::string dir_of(const std::string &p) {
-  const size_t k = p.find_last_of('/');
-  return k == std::string::npos ? "." : p.substr(0, k);
-}
-
-int32_t load_set(const std::string &index, Par2Set &s, std::string *err) {
-  int32_t rc = parse_file(index, s, false, err);
-  if (rc) return rc;
-  if (!s.have_set_id) {
-    *err = "no packet in " + index;
-    return PAR2HIP_EINVAL;
-  }
-  if (s.block_size == 0 || s.block_size % 4) {  // parser.rs:364-380
-    *err = "block size missing, zero or not a multiple of 4";
-    return PAR2HIP_EINVAL;
-  }
-  // the other .par2 files beside the index with the same set ID
-  std::vector<std::string> names;
-  {
-    const std::string dir = dir_of(index);
-    DIR *dp = opendir(dir.c_str());
-    if (dp) {
-      while (dirent *e = readdir(dp)) {
-        std::string n = e->d_name;
-        if (n.size() < 5) continue;
-        std::string ext = n.substr(n.size() - 5);
-        std::transform(ext.begin(), ext.end(), ext.begin(), ::tolower);
-        if (ext != ".par2") continue;
-        const std::string full = dir + "/" + n;
-        const size_t slash = index.find_last_of('/');
-        if (n == (slash == std::string::npos ? index : index.substr(slash + 1))) continue;  // the index itself
-        names.push_back(full);
-      }
-      closedir(dp);
-    }
-  }
-  std::sort(names.begin(), names.end());
-  for (const std::string &v : names) {
-    Id sid;
-    if (first_set_id(v, &sid) && sid == s.set_id) (void)parse_file(v, s, true, err);
-  }
-  return PAR2HIP_OK;
-}
-
-// Files in the canonical block order (parser.rs:566-596): the Main packet's
-// list first, then any other described file by File ID (byte 15 first).
-std::vector<FileRec> canonical_files(const Par2Set &s) {
-  std::vector<FileRec> out;
-  std::vector<Id> seen;
-  for (const Id &id : s.main_order) {
-    auto it = s.files.find(id);
-    if (it != s.files.end() && std::find(seen.begin(), seen.end(), id) == seen.end()) {
-      out.push_back(it->second);
-      seen.push_back(id);
-    }
-  }
-  std::vector<FileRec> rest;
-  for (const FileRec &r : s.files_in_order)
-    if (std::find(seen.begin(), seen.end(), r.id) == seen.end()) rest.push_back(r);
-  std::sort(rest.begin(), rest.end(), [](const FileRec &a, const FileRec &b) {
-    for (int i = 15; i >= 0; i--)
-      if (a.id[i] != b.id[i]) return a.id[i] < b.id[i];
-    return false;
-  });
-  out.insert(out.end(), rest.begin(), rest.end());
-  return out;
-}
-
-// A stored file name the repairer will open, write and rename under the base
-// directory: relative, non-empty, no ".." component.  The reference joins
-// the name to the base path as it stands (repair.rs:731, :772), so a crafted
-// set could make it write outside that directory; such a set is refused.
-bool safe_name(const std::string &name) {
-  if (name.empty() || name[0] == '/') return false;
-  size_t b = 0;
-  while (b <= name.size()) {
-    size_t e = name.find('/', b);
-    if (e == std::string::npos) e = name.size();
-    if (name.compare(b, e - b, "..") == 0 && e - b == 2) return false;
-    b = e + 1;
-  }
   return true;
 }
 
@@ -328,15 +104,6 @@ bool io_parallel(const std::vector<IoRange> &jobs, bool write) {
 }
 
 
-// Sorted by exponent, one slice per exponent (the first found).
-std::vector<RecSlice> recovery_by_exponent(const Par2Set &s) {
-  std::vector<RecSlice> r = s.recs;
-  std::stable_sort(r.begin(), r.end(), [](const RecSlice &a, const RecSlice &b) { return a.exponent < b.exponent; });
-  r.erase(std::unique(r.begin(), r.end(), [](const RecSlice &a, const RecSlice &b) { return a.exponent == b.exponent; }),
-          r.end());
-  return r;
-}
-
 bool md5_file_prefix(const std::string &path, uint64_t n, uint8_t out[16]) {
   FILE *f = fopen(path.c_str(), "rb");
   if (!f) return false;
@@ -363,13 +130,16 @@ bool md5_file_prefix(const std::string &path, uint64_t n, uint8_t out[16]) {
 int32_t set_info(const char *index, par2hip_set_info *info) {
   Par2Set s;
   std::string err;
-  int32_t rc = load_set(index, s, &err);
+  int32_t rc = load_par2_set(index, s, &err);
   if (rc) return rc;
   memset(info, 0, sizeof *info);
   info->block_size = s.block_size;
   info->files = (uint32_t)s.files.size();
   memcpy(info->set_id, s.set_id.data(), 16);
-  for (const FileRec &r : s.files_in_order) info->blocks += (r.length + s.block_size - 1) / s.block_size;
+  for (const FileRec &r : s.files_in_order) {
+    const uint64_t k = blocks_of(r.length, s.block_size);
+    info->blocks = k > UINT64_MAX - info->blocks ? UINT64_MAX : info->blocks + k;  // saturates
+  }
   const std::vector<RecSlice> recs = recovery_by_exponent(s);
   info->recovery_slices = (uint32_t)recs.size();
   if (!recs.empty()) {
@@ -379,12 +149,17 @@ int32_t set_info(const char *index, par2hip_set_info *info) {
   return PAR2HIP_OK;
 }
 
+// Relaxed read of the caller's cancel flag (a Rust AtomicBool has the layout
+// of a bool: one byte, 0 or 1).
+bool cancelled(const uint8_t *flag) { return flag && __atomic_load_n(flag, __ATOMIC_RELAXED) != 0; }
+
 int32_t repair_files(par2hip_ctx *ctx, const char *index, const char *base_dir, int verify_only,
-                     par2hip_repair_report *rep) {
+                     const uint8_t *cancel, par2hip_repair_report *rep) {
   memset(rep, 0, sizeof *rep);
+  if (cancelled(cancel)) return fail(ctx, PAR2HIP_ECANCELLED, "cancelled");
   Par2Set s;
   std::string err;
-  int32_t rc = load_set(index, s, &err);
+  int32_t rc = load_par2_set(index, s, &err);
   if (rc) return fail(ctx, rc, err);
   const std::vector<FileRec> files = canonical_files(s);
   for (const FileRec &f : files)
@@ -392,22 +167,23 @@ int32_t repair_files(par2hip_ctx *ctx, const char *index, const char *base_dir, 
   const uint64_t bs = s.block_size;
   const std::string base = base_dir && *base_dir ? base_dir : dir_of(index);
 
-  // ---- block layout of the files in canonical order (repair.rs:499-545)
-  std::vector<uint64_t> first(files.size()), nblk(files.size());
-  uint64_t n = 0;
-  for (size_t i = 0; i < files.size(); i++) {
-    first[i] = n;
-    nblk[i] = (files[i].length + bs - 1) / bs;
-    n += nblk[i];
-  }
+  // ---- block layout of the files in canonical order (repair.rs:499-545),
+  // refused before any buffer is sized from it when the block count passes
+  // the 32768 distinct PAR2 constants or the blocks do not fit the device
   const std::vector<RecSlice> recs = recovery_by_exponent(s);
   rep->files = (uint32_t)files.size();
-  rep->blocks = n;
   rep->block_size = bs;
   rep->recovery_slices = (uint32_t)recs.size();
-  if (n > 32768) return fail(ctx, PAR2HIP_EINVAL, "more than 32768 input blocks repeat PAR2 constants");
+  size_t dev_free = 0, dev_total = 0;
+  P2H_CHECK(ctx, hipMemGetInfo(&dev_free, &dev_total));
+  BlockLayout lay;
+  if ((rc = block_layout(files, bs, 32768, dev_total, &lay, &err))) return fail(ctx, rc, err);
+  const std::vector<uint64_t> &first = lay.first, &nblk = lay.nblk;
+  const uint64_t n = lay.n;
+  rep->blocks = n;
   if (n + recs.size() > 65535) return fail(ctx, PAR2HIP_EINVAL, "data + recovery blocks exceed 65535");
   if (n == 0) return PAR2HIP_OK;
+  if (cancelled(cancel)) return fail(ctx, PAR2HIP_ECANCELLED, "cancelled");
 
   // ---- verify (verify.rs:87-330): every file's blocks on the device
   enum { kOk, kDamaged, kMissing, kSize };
@@ -475,6 +251,7 @@ int32_t repair_files(par2hip_ctx *ctx, const char *index, const char *base_dir, 
     else if (status[i] != kOk) rep->files_damaged++;
   }
   rep->blocks_damaged = missing.size();
+  if (cancelled(cancel)) return fail(ctx, PAR2HIP_ECANCELLED, "cancelled");  // lib.rs:246-249, :339-343
   if (missing.empty() || verify_only) return PAR2HIP_OK;
   const size_t m = missing.size();
   if (m > recs.size())
@@ -498,6 +275,10 @@ int32_t repair_files(par2hip_ctx *ctx, const char *index, const char *base_dir, 
   P2H_CHECK(ctx, hipMemcpyAsync(d_rec, stage, m * bs, hipMemcpyHostToDevice, ctx->stream));
   if ((rc = par2hip_reconstruct_dev(ctx, d_slices, n, bs, missing.data(), m, d_rec, exps.data(), d_out, 0)))
     return rc;
+  if (cancelled(cancel)) {  // repair.rs:972-977: nothing written yet
+    P2H_CHECK(ctx, hipStreamSynchronize(ctx->stream));
+    return fail(ctx, PAR2HIP_ECANCELLED, "cancelled");
+  }
 
   // ---- temp files (repair.rs:736-1040): originals untouched until commit.
   // Each damaged file's full contents are assembled on the device (its
@@ -528,6 +309,10 @@ int32_t repair_files(par2hip_ctx *ctx, const char *index, const char *base_dir, 
   if (!io_parallel(writes, true)) {
     cleanup();
     return fail(ctx, PAR2HIP_EINVAL, "cannot write the temp files");
+  }
+  if (cancelled(cancel)) {  // repair.rs:1043-1047: only temps exist
+    cleanup();
+    return fail(ctx, PAR2HIP_ECANCELLED, "cancelled");
   }
 
   // ---- every temp verified before anything is renamed (repair.rs:1050-1080):
@@ -584,6 +369,12 @@ int32_t repair_files(par2hip_ctx *ctx, const char *index, const char *base_dir, 
       return fail(ctx, PAR2HIP_EREPAIR, "repair completed but verification failed - files may still be damaged");
     }
   }
+  // last safe point (repair.rs:1064-1067); once the renames start the
+  // repair is committed and the call reports it
+  if (cancelled(cancel)) {
+    cleanup();
+    return fail(ctx, PAR2HIP_ECANCELLED, "cancelled");
+  }
   for (size_t t = 0; t < temps.size(); t++) {
     if (rename(temps[t].first.c_str(), temps[t].second.c_str()) != 0) {
       for (size_t u = t; u < temps.size(); u++) (void)remove(temps[u].first.c_str());
@@ -601,10 +392,16 @@ extern "C" int32_t par2hip_par2_info(const char *par2_path, par2hip_set_info *in
   return par2hip::set_info(par2_path, info);
 }
 
-extern "C" int32_t par2hip_repair_files(par2hip_ctx *ctx, const char *par2_path, const char *base_dir,
-                                        int32_t verify_only, par2hip_repair_report *report) {
+extern "C" int32_t par2hip_repair_files_cancellable(par2hip_ctx *ctx, const char *par2_path, const char *base_dir,
+                                                    int32_t verify_only, const uint8_t *cancel,
+                                                    par2hip_repair_report *report) {
   if (!ctx || !par2_path || !report) return PAR2HIP_EINVAL;
   std::lock_guard<std::recursive_mutex> lk(ctx->mu);
   (void)hipSetDevice(ctx->device);
-  return par2hip::repair_files(ctx, par2_path, base_dir, verify_only, report);
+  return par2hip::repair_files(ctx, par2_path, base_dir, verify_only, cancel, report);
+}
+
+extern "C" int32_t par2hip_repair_files(par2hip_ctx *ctx, const char *par2_path, const char *base_dir,
+                                        int32_t verify_only, par2hip_repair_report *report) {
+  return par2hip_repair_files_cancellable(ctx, par2_path, base_dir, verify_only, nullptr, report);
 }

@@ -18,6 +18,7 @@
 // memory (coefficients, inverses and nibble tables).
 #include <algorithm>
 #include <chrono>
+#include <cmath>
 #include <condition_variable>
 #include <cstring>
 #include <deque>
@@ -262,6 +263,7 @@ int32_t par2hip_server_submit(par2hip_server *srv, const par2hip_request *req, u
   if (j.kind != PAR2HIP_JOB_ENCODE && j.kind != PAR2HIP_JOB_REPAIR) return PAR2HIP_EINVAL;
   if (j.count && !j.exps) return PAR2HIP_EINVAL;
   if (j.kind == PAR2HIP_JOB_ENCODE && j.count > 65535) return PAR2HIP_EINVAL;
+  if (req->hashes && j.slice_size >= par2hip::kMd5MaxSliceBytes) return PAR2HIP_EINVAL;
   if (j.kind == PAR2HIP_JOB_REPAIR) {
     if (j.count > j.n || (j.count && (!j.missing || !j.d_recovery))) return PAR2HIP_EINVAL;
     std::vector<char> seen(j.n, 0);
@@ -302,13 +304,21 @@ int32_t par2hip_server_submit(par2hip_server *srv, const par2hip_request *req, u
   return PAR2HIP_OK;
 }
 
-int32_t par2hip_server_wait(par2hip_server *srv, uint64_t ticket, par2hip_result *res) {
-  if (!srv || !res) return PAR2HIP_EINVAL;
+int32_t par2hip_server_wait_timeout(par2hip_server *srv, uint64_t ticket, double timeout_ms, par2hip_result *res) {
+  if (!srv || !res || !(timeout_ms >= 0)) return PAR2HIP_EINVAL;
   std::unique_lock<std::mutex> lk(srv->mu);
   auto it = srv->reqs.find(ticket);
   if (it == srv->reqs.end()) return PAR2HIP_EINVAL;
   Request *r = it->second.get();
-  srv->cv_done.wait(lk, [&] { return r->state == State::kDone; });
+  auto done = [&] { return r->state == State::kDone; };
+  if (std::isinf(timeout_ms)) {
+    srv->cv_done.wait(lk, done);
+  } else {
+    const auto limit = std::chrono::steady_clock::now() +
+                       std::chrono::duration_cast<std::chrono::steady_clock::duration>(
+                           std::chrono::duration<double, std::milli>(std::min(timeout_ms, 1e12)));
+    if (!srv->cv_done.wait_until(lk, limit, done)) return PAR2HIP_ETIMEDOUT;
+  }
   res->status = r->status;
   res->batch_size = r->batch_size;
   res->d_out = r->out_off == UINT64_MAX ? nullptr : srv->ptr(r->out_off);
@@ -320,6 +330,10 @@ int32_t par2hip_server_wait(par2hip_server *srv, uint64_t ticket, par2hip_result
   res->t_start_ms = r->t_start;
   res->t_done_ms = r->t_done;
   return PAR2HIP_OK;
+}
+
+int32_t par2hip_server_wait(par2hip_server *srv, uint64_t ticket, par2hip_result *res) {
+  return par2hip_server_wait_timeout(srv, ticket, HUGE_VAL, res);
 }
 
 int32_t par2hip_server_release(par2hip_server *srv, uint64_t ticket) {

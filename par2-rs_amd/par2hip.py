@@ -30,7 +30,8 @@ import numpy as np
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libpar2hip.so")
 
-OK, EINVAL, ENODEV, ENOMEM, ESINGULAR, EHIP, ERECOVERY, EREPAIR = 0, -1, -2, -3, -4, -5, -6, -7
+OK, EINVAL, ENODEV, ENOMEM, ESINGULAR, EHIP, ERECOVERY, EREPAIR, ECANCELLED, ETIMEDOUT = (
+    0, -1, -2, -3, -4, -5, -6, -7, -8, -9)
 PRIORITY_OPTIMAL = 40  # Priority::Optimal (src/galois/simd/mod.rs:20)
 
 
@@ -52,6 +53,10 @@ class InsufficientRecovery(Par2Error):
 
 class DeviceError(Par2Error):
     pass
+
+
+class Cancelled(Par2Error):
+    """Par2Error::Cancelled (src/error.rs:35)."""
 
 
 JOB_ENCODE, JOB_REPAIR = 0, 1
@@ -144,7 +149,7 @@ def lib():
                 "par2hip_sync": ([P], I32),
                 "par2hip_gf_mul_slice": ([P, C.c_uint16, P, S], I32),
                 "par2hip_gf_muladd": ([P, P, P, S, C.c_uint16], I32),
-                "par2hip_gf_muladd_region": ([P, P, S, P, S, P, S, S], I32),
+                "par2hip_gf_muladd_region": ([P, P, S, P, S, P, P, S, S], I32),
                 "par2hip_encode": ([P, P, S, S, P, S, P, P, P], I32),
                 "par2hip_encode_dev": ([P, P, S, S, P, S, P, P, P], I32),
                 "par2hip_gf_invert": ([P, P, S, P], I32),
@@ -161,6 +166,7 @@ def lib():
                 "par2hip_create_par2": ([P, P, S, C.c_uint64, C.c_float, I32, C.c_char_p, P, S, C.POINTER(S)], I32),
                 "par2hip_par2_info": ([C.c_char_p, P], I32),
                 "par2hip_repair_files": ([P, C.c_char_p, C.c_char_p, I32, P], I32),
+                "par2hip_repair_files_cancellable": ([P, C.c_char_p, C.c_char_p, I32, P, P], I32),
                 "par2hip_set_autotune": ([P, I32], I32),
                 "par2hip_force_tile_shape": ([P, I32, I32, I32], I32),
                 "par2hip_autotune_save": ([C.c_char_p], I32),
@@ -173,6 +179,7 @@ def lib():
                 "par2hip_server_now_ms": ([P], C.c_double),
                 "par2hip_server_submit": ([P, P, C.POINTER(C.c_uint64)], I32),
                 "par2hip_server_wait": ([P, C.c_uint64, P], I32),
+                "par2hip_server_wait_timeout": ([P, C.c_uint64, C.c_double, P], I32),
                 "par2hip_server_release": ([P, C.c_uint64], I32),
                 "par2hip_server_get_stats": ([P, P], I32),
             }
@@ -248,6 +255,20 @@ class DeviceView:
         self.owner = owner
         self.nbytes = owner.nbytes - offset
         self.ptr = C.c_void_p(owner.addr + offset)
+
+
+class CancelFlag:
+    """A one-byte flag shared with the library (the layout of a Rust
+    AtomicBool); set() from any thread requests cancellation."""
+
+    def __init__(self, value: bool = False):
+        self.flag = C.c_uint8(1 if value else 0)
+
+    def set(self) -> None:
+        self.flag.value = 1
+
+    def is_set(self) -> bool:
+        return bool(self.flag.value)
 
 
 class Context:
@@ -455,14 +476,23 @@ class Context:
             self._check(rc)
             return [(outs[i].path.decode(), int(outs[i].bytes)) for i in range(n.value)]
 
-    def repair_files(self, par2_path: str, base_dir: str | None = None, verify_only: bool = False) -> dict:
+    def repair_files(self, par2_path: str, base_dir: str | None = None, verify_only: bool = False,
+                     cancel: "CancelFlag | None" = None) -> dict:
         """Par2Repairer: verify the set's files and repair them in place through
-        temp files (par2hip_repair_files); returns the report."""
+        temp files (par2hip_repair_files); returns the report.  With a
+        CancelFlag this is repair_with_callbacks_cancellable (src/lib.rs:196):
+        a set flag makes the call raise Cancelled with every original intact."""
         rep = RepairReport()
-        rc = lib().par2hip_repair_files(self.handle, par2_path.encode(), base_dir.encode() if base_dir else None,
-                                        1 if verify_only else 0, C.byref(rep))
+        path_b, base_b = par2_path.encode(), base_dir.encode() if base_dir else None
+        if cancel is None:
+            rc = lib().par2hip_repair_files(self.handle, path_b, base_b, 1 if verify_only else 0, C.byref(rep))
+        else:
+            rc = lib().par2hip_repair_files_cancellable(self.handle, path_b, base_b, 1 if verify_only else 0,
+                                                        C.addressof(cancel.flag), C.byref(rep))
         if rc == EREPAIR:
             raise RepairFailed((lib().par2hip_last_error(self.handle) or b"").decode())
+        if rc == ECANCELLED:
+            raise Cancelled("cancelled")
         self._check(rc)
         return {k: int(getattr(rep, k)) for k, _ in RepairReport._fields_ if k != "reserved"}
 
@@ -600,9 +630,17 @@ class Server:
         self._keep[t.value] = (job,)  # the input buffers must outlive the request
         return t.value
 
-    def wait(self, ticket: int) -> dict:
+    def wait(self, ticket: int, timeout_ms: float | None = None) -> dict:
+        """Wait for a request; with timeout_ms, raise TimeoutError when it is
+        not done in time (the request keeps its place)."""
         res = Result()
-        _status(lib().par2hip_server_wait(self.handle, ticket, C.byref(res)), "server_wait")
+        if timeout_ms is None:
+            rc = lib().par2hip_server_wait(self.handle, ticket, C.byref(res))
+        else:
+            rc = lib().par2hip_server_wait_timeout(self.handle, ticket, float(timeout_ms), C.byref(res))
+            if rc == ETIMEDOUT:
+                raise TimeoutError(f"request {ticket} not done within {timeout_ms} ms")
+        _status(rc, "server_wait")
         out = {k: getattr(res, k) for k, _ in Result._fields_}
         _status(res.status, f"request {ticket}")
         return out
@@ -664,8 +702,9 @@ class HipStrategy:
         dp = (C.c_void_p * nd)(*[d.ctypes.data for d in destinations])
         sp = (C.c_void_p * max(ns, 1))(*[s.ctypes.data for s in sources])
         rows = [np.ascontiguousarray(c, np.uint16) for c in coefficients[:nd]]
-        cp = (C.c_void_p * nd)(*[r.ctypes.data for r in rows])
-        self.ctx._check(lib().par2hip_gf_muladd_region(self.ctx.handle, dp, nd, sp, ns, cp, region_offset,
+        cp = (C.c_void_p * nd)(*[r.ctypes.data if r.size else None for r in rows])
+        lens = (C.c_size_t * nd)(*[r.size for r in rows])
+        self.ctx._check(lib().par2hip_gf_muladd_region(self.ctx.handle, dp, nd, sp, ns, cp, lens, region_offset,
                                                        region_size))
 
     # shuffle2x hooks: the GPU consumes interleaved words, so these are the

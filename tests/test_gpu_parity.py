@@ -320,6 +320,32 @@ def test_strategy_muladd_region(kctx):
         assert np.array_equal(a, b)
 
 
+def test_strategy_muladd_region_ragged_rows(kctx):
+    """Coefficient rows shorter and longer than the source list: a short row
+    applies only its own entries (src/galois/scalar.rs:100)."""
+    s = par2hip.HipStrategy(kctx)
+    rng = np.random.default_rng(11)
+    ns, L = 7, 777
+    srcs = [rng.integers(0, 65536, L).astype(np.uint16) for _ in range(ns)]
+    lens = [0, 1, 3, 6, 7, 9, 12]
+    rows = [rng.integers(1, 65536, k).astype(np.uint16) for k in lens]
+    dsts = [rng.integers(0, 65536, L).astype(np.uint16) for _ in lens]
+    want = [d.copy() for d in dsts]
+    oracle.muladd_region_rows(want, srcs, rows, 5, 700)
+    # the short rows really leave sources out: the full-width product differs
+    full = [d.copy() for d in dsts]
+    padded = np.zeros((len(lens), ns), np.uint16)
+    for i, r in enumerate(rows):
+        padded[i, :min(len(r), ns)] = r[:ns]
+    padded[0] = 1
+    oracle.muladd_region(full, srcs, padded, 5, 700)
+    assert not np.array_equal(full[0], want[0])
+    say("muladd_region ragged rows", lens)
+    s.muladd_region(dsts, srcs, rows, 5, 700)
+    for a, b in zip(dsts, want):
+        assert np.array_equal(a, b)
+
+
 # ---------------------------------------------------- verify + repair set --
 CRC_SILENT = bytes.fromhex("410671db01")  # the CRC-32 polynomial: XOR-ing it never changes CRC-32
 

@@ -162,3 +162,109 @@ def test_unsafe_stored_name_is_refused(ctx, tmp_path, name):
         ctx.repair_files(str(sub / "evil.par2"))
     assert not (tmp_path / "escape.bin").exists() and not (tmp_path / "escape.bin.par2tmp").exists()
     assert all(p.endswith(".par2") for p in os.listdir(sub))
+
+
+def _snapshot(d):
+    return {f: open(os.path.join(d, f), "rb").read() for f in sorted(os.listdir(d)) if f.endswith(".data")}
+
+
+def test_cancel_preset_flag_leaves_originals(ctx, tmp_path):
+    """tests/info_and_cancel.rs:216-252: a flag set before the call cancels it;
+    the damaged files stay exactly as they were and no temp is left."""
+    idx = fixture(tmp_path)
+    for name in ("test-1.data", "test-6.data"):
+        corrupt_at(tmp_path / name, 2048, 8192)
+    before = _snapshot(tmp_path)
+    flag = par2hip.CancelFlag(True)
+    say("cancel preset")
+    with pytest.raises(par2hip.Cancelled):
+        ctx.repair_files(idx, cancel=flag)
+    assert _snapshot(tmp_path) == before
+    assert not any(f.endswith(".par2tmp") for f in os.listdir(tmp_path))
+    # verify-only is cancelled too, and a clear flag repairs
+    with pytest.raises(par2hip.Cancelled):
+        ctx.repair_files(idx, verify_only=True, cancel=flag)
+    rep = ctx.repair_files(idx, cancel=par2hip.CancelFlag(False))
+    assert rep["repaired"] == 1
+    assert open(tmp_path / "test-1.data", "rb").read() == pristine("test-1.data")
+
+
+def test_cancel_from_thread_is_safe(ctx, tmp_path):
+    """tests/info_and_cancel.rs:254-308: the flag set from another thread at
+    once; a cancelled call left the originals byte-identical, a completed one
+    repaired them, and neither leaves a temp."""
+    import threading
+
+    idx = fixture(tmp_path)
+    corrupt_at(tmp_path / "test-4.data", 512, 16384)
+    before = _snapshot(tmp_path)
+    flag = par2hip.CancelFlag(False)
+    t = threading.Thread(target=flag.set)
+    say("cancel from thread")
+    t.start()
+    try:
+        ctx.repair_files(idx, cancel=flag)
+        assert open(tmp_path / "test-4.data", "rb").read() == pristine("test-4.data")
+    except par2hip.Cancelled:
+        assert _snapshot(tmp_path) == before
+    t.join()
+    assert not any(f.endswith(".par2tmp") for f in os.listdir(tmp_path))
+
+
+def test_cancel_at_growing_delays(ctx, tmp_path):
+    """The flag raised by a timer at growing delays, so that successive calls
+    are cancelled at later and later safe points (loading, verification, the
+    rebuild, the temps, the renames) until one completes: every cancelled call
+    leaves the originals byte-identical and no temp, the completed one
+    repairs them."""
+    import threading
+
+    idx = fixture(tmp_path)
+    corrupt_at(tmp_path / "test-1.data", 10, 64)
+    corrupt_at(tmp_path / "test-4.data", BS * 2 + 5, 40)
+    before = _snapshot(tmp_path)
+    done = False
+    for delay_ms in (0.0, 0.5, 1, 2, 4, 8, 16, 32, 64, 128, 256, 1000):
+        flag = par2hip.CancelFlag(False)
+        tm = threading.Timer(delay_ms / 1e3, flag.set)
+        tm.start()
+        say("cancel after ms", delay_ms)
+        try:
+            ctx.repair_files(idx, cancel=flag)
+            done = True
+        except par2hip.Cancelled:
+            assert _snapshot(tmp_path) == before
+        tm.cancel()
+        tm.join()
+        assert not any(f.endswith(".par2tmp") for f in os.listdir(tmp_path))
+        if done:
+            break
+    assert done
+    assert open(tmp_path / "test-1.data", "rb").read() == pristine("test-1.data")
+    assert open(tmp_path / "test-4.data", "rb").read() == pristine("test-4.data")
+
+
+def test_overflowing_layout_refused(ctx, tmp_path):
+    """An index whose block size (2^62) and file lengths would wrap the block
+    layout's byte count is refused with EINVAL before any buffer is sized."""
+    import struct
+    import hashlib
+
+    sid = bytes([7]) * 16
+
+    def packet(ptype, body):
+        body = body + b"\0" * (-len(body) % 4)
+        md5 = hashlib.md5(sid + ptype + body).digest()
+        return b"PAR2\0PKT" + struct.pack("<Q", 64 + len(body)) + md5 + sid + ptype + body
+
+    ids = [bytes([k + 1]) + b"\0" * 15 for k in range(4)]
+    out = packet(b"PAR 2.0\0Main\0\0\0\0", struct.pack("<QI", 1 << 62, 4) + b"".join(ids))
+    for k, fid in enumerate(ids):
+        out += packet(b"PAR 2.0\0FileDesc", fid + b"\0" * 32 + struct.pack("<Q", 1 << 62) + b"f%d" % k)
+    p = tmp_path / "crafted.par2"
+    p.write_bytes(out)
+    say("crafted overflow index")
+    with pytest.raises(par2hip.InvalidFormat):
+        ctx.repair_files(str(p), verify_only=True)
+    with pytest.raises(par2hip.InvalidFormat):
+        ctx.repair_files(str(p))

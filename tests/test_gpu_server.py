@@ -158,3 +158,41 @@ def test_server_rejects_bad_requests(ctx):
         assert srv.stats()["submitted"] == 0
     finally:
         srv.close()
+
+
+def test_server_wait_timeout_on_budget_blocked_ticket(ctx):
+    """FIFO admission: with the budget held by an unreleased result, a later
+    request cannot be admitted; waiting on it with a limit times out instead
+    of hanging, the request keeps its place, and it runs once the first
+    result is released."""
+    rng = np.random.default_rng(13)
+    n, s = 64, 65536
+    data = rng.integers(0, 256, (2, n, s), dtype=np.uint8)
+    bufs = []
+    for k in range(2):
+        d = ctx.alloc(n * s)
+        d.upload(data[k])
+        bufs.append(d)
+    exps = np.arange(8, dtype=np.uint32)
+    one = n * s + 8 * s + n * 20 + 8 * n * 2 + n * 64 * 16
+    srv = par2hip.Server(0, hbm_budget=one + 4096, lanes=1, max_batch=4)
+    try:
+        t0 = srv.submit({"kind": "encode", "slices": bufs[0], "n": n, "slice_size": s, "exps": exps})
+        t1 = srv.submit({"kind": "encode", "slices": bufs[1], "n": n, "slice_size": s, "exps": exps})
+        res0 = srv.wait(t0, timeout_ms=60000)  # done: the first one fits
+        say("timeout: second request pending", srv.stats()["pending"])
+        assert srv.stats()["pending"] == 1
+        with pytest.raises(TimeoutError):
+            srv.wait(t1, timeout_ms=200)
+        with pytest.raises(TimeoutError):
+            srv.wait(t1, timeout_ms=0)
+        out0 = par2hip.d2h(res0["d_out"], 8 * s, ctx).reshape(8, s)
+        srv.release(t0)
+        res1 = srv.wait(t1, timeout_ms=60000)
+        out1 = par2hip.d2h(res1["d_out"], 8 * s, ctx).reshape(8, s)
+        for k, out in enumerate((out0, out1)):
+            want = oracle.encode(data[k].view(np.uint16), exps).view(np.uint8).reshape(8, s)
+            assert np.array_equal(out, want)
+        srv.release(t1)
+    finally:
+        srv.close()

@@ -162,6 +162,25 @@ def test_region_brute_force():
         assert np.array_equal(a, b)
 
 
+def test_region_ragged_rows_brute_force():
+    """Rows shorter than the source list apply only their own entries
+    (src/galois/scalar.rs:100): against the carry-less product per entry."""
+    rng = np.random.default_rng(4)
+    ns, L = 6, 131
+    srcs = [rng.integers(0, 65536, L).astype(np.uint16) for _ in range(ns)]
+    lens = [0, 2, 5, 6, 9]
+    rows = [rng.integers(0, 65536, k).astype(np.uint16) for k in lens]
+    dsts = [rng.integers(0, 65536, L).astype(np.uint16) for _ in lens]
+    want = [d.copy() for d in dsts]
+    off, ln = 7, 100
+    for i, row in enumerate(rows):
+        for j in range(min(len(row), ns)):
+            want[i][off:off + ln] ^= clmul_mod_np(srcs[j][off:off + ln], np.full(ln, row[j], np.uint16))
+    oracle.muladd_region_rows(dsts, srcs, rows, off, ln)
+    for a, b in zip(dsts, want):
+        assert np.array_equal(a, b)
+
+
 # ------------------------------------------------------ golden fixture --
 def test_quickpar_recovery_slices_bit_exact():
     qp = quickpar()
