@@ -35,6 +35,7 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.join(ROOT, "par2-rs_amd"))
 
 import par2hip  # noqa: E402
+from workloads import draw_requests, draw_sets, random_bytes  # noqa: E402
 
 HBM_PEAK_GBS = 8000.0
 # HBM bytes per launch of the plain-encode contraction (1024 x 1 MiB, R=102)
@@ -73,8 +74,58 @@ def log(*a):
     print("[bench]", *a, file=sys.stderr, flush=True)
 
 
-def random_bytes(seed: int, n: int) -> np.ndarray:
-    return np.frombuffer(np.random.default_rng(seed).bytes(n), np.uint8)
+def host_cpus():
+    """Host cores this process may use: its CPU affinity, capped by a cgroup
+    CPU quota when one is set (a container's share of a larger host)."""
+    try:
+        n = len(os.sched_getaffinity(0))
+    except AttributeError:
+        n = os.cpu_count() or 1
+    how = f"affinity {n}"
+    try:
+        quota, period = open("/sys/fs/cgroup/cpu.max").read().split()[:2]
+        if quota != "max":
+            q = max(1, int(int(quota) // int(period)))
+            if q < n:
+                n, how = q, how + f", cgroup quota {q}"
+    except (OSError, ValueError):
+        pass
+    return n, how
+
+
+_CPU_BUILD = None
+
+
+def cpu_port_build():
+    """Load the oracle's restatement compiled for this host (-O3
+    -march=native, the reference's benchmark flags) when the host has a C
+    compiler; otherwise the shipped -O3 x86-64-v3 build.  The native build
+    must agree with the shipped one on a small encode before it is used."""
+    global _CPU_BUILD
+    if _CPU_BUILD is not None:
+        return _CPU_BUILD
+    import subprocess
+    import tempfile
+
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle
+
+    probe = random_bytes(13, 64 * 4096).reshape(64, 4096).view(np.uint16)
+    want = oracle.encode_avx2(probe, np.arange(7), threads=2)
+    out = os.path.join(tempfile.mkdtemp(prefix="par2cpu_"), "liboracle_native.so")
+    try:
+        subprocess.run(["make", "-s", "-C", os.path.join(ROOT, "oracle"), "native", f"OUT={out}"], check=True,
+                       capture_output=True, timeout=120)
+        shipped = oracle.LIB_PATH
+        oracle.use_library(out)
+        if not np.array_equal(oracle.encode_avx2(probe, np.arange(7), threads=2), want):
+            oracle.use_library(shipped)
+            raise RuntimeError("native build disagrees with the shipped one")
+        _CPU_BUILD = "gcc -O3 -march=native (built on this host)"
+    except Exception as e:  # no compiler, or a build that does not agree
+        log("cpu baseline: native build unavailable:", e)
+        _CPU_BUILD = "gcc -O3 -march=x86-64-v3 (shipped build)"
+    return _CPU_BUILD
 
 
 def cpu_baseline_encode(n, s_sample, r, threads):
@@ -368,30 +419,6 @@ def bench_repair(args, ctx):
     }
 
 
-def draw_sets(seed, count=256, cap_bytes=48 << 30):
-    """Batched workload: count sets, N ~ logU[8,4096], S ~ logU[4 KiB,1 MiB] (multiple
-    of 4 B), redundancy ~ U[1,20]% with R by the creator's rule
-    (creator.rs:324-331), half repairs with m = ceil(U[1,10]% * N) clamped to R;
-    redrawn until the source total fits cap_bytes."""
-    rng = np.random.default_rng(seed)
-    while True:
-        sets = []
-        for i in range(count):
-            n = int(round(np.exp(rng.uniform(np.log(8), np.log(4096)))))
-            s = int(round(np.exp(rng.uniform(np.log(4096), np.log(1 << 20))) / 4)) * 4
-            pct = rng.uniform(1, 20)
-            r = -(-int(n * s * pct / 100.0) // s)
-            r = max(r, 1)
-            kind = "repair" if i % 2 else "encode"
-            m = 0
-            if kind == "repair":
-                m = min(r, max(1, int(np.ceil(rng.uniform(0.01, 0.10) * n))))
-            sets.append({"n": n, "s": s, "r": r, "kind": kind, "m": m})
-        total = sum(x["n"] * x["s"] for x in sets)
-        if total <= cap_bytes:
-            return sets, total
-
-
 def bench_batched(args, ctx):
     """256 PAR2 sets of uneven shapes (half create, half repair) in one batched
     run vs the plain path (one set at a time), both with IFSC hashes.  A step
@@ -480,6 +507,10 @@ def bench_batched(args, ctx):
     lat_b = timed(replay, args.steps)
     plan.close()
     lat_p = timed(plain, max(1, args.steps // 2))
+    for j in jobs:  # the sets' device memory goes back before the next workload
+        for k in ("slices", "md5", "crc", "recovery", "out"):
+            if k in j:
+                j[k].free()
     return {
         "workload": "batched",
         "config": {"workload": "batched_256_sets", "sets": len(sets), "source_GiB": round(total / 2**30, 3),
@@ -491,28 +522,6 @@ def bench_batched(args, ctx):
         "eager_ms": float(np.mean(lat_e) * 1e3),
         "plain_ms": float(np.mean(lat_p) * 1e3), "plain_gbs": total / float(np.mean(lat_p)) / 1e9,
     }
-
-
-def draw_requests(seed, count, sigma=1.0):
-    """Served workload: request sizes lognormal (median 64 MiB, sigma given),
-    capped at 8 GiB; slice size uniform over 256 KiB..1 MiB in 4-byte steps;
-    30% repairs with ceil(U[1,10]% * N) erasures (recovery rows 0..m-1),
-    creates with redundancy U[1,20]% (R by the creator's rule,
-    creator.rs:324-331).  Arrival gaps are Exp(1), scaled by 1/rate later."""
-    rng = np.random.default_rng(seed)
-    reqs, t = [], 0.0
-    for _ in range(count):
-        t += float(rng.exponential(1.0))
-        size = min(float(np.exp(np.log(64 * 2**20) + sigma * rng.standard_normal())), 8.0 * 2**30)
-        s = int(rng.integers(65536, 262145)) * 4
-        n = int(min(32768, max(1, -(-int(size) // s))))
-        if rng.random() < 0.3:
-            m = max(1, int(np.ceil(rng.uniform(0.01, 0.10) * n)))
-            reqs.append({"t": t, "kind": "repair", "n": n, "s": s, "k": m})
-        else:
-            r = max(1, -(-int(n * s * rng.uniform(1, 20) / 100.0) // s))
-            reqs.append({"t": t, "kind": "encode", "n": n, "s": s, "k": r})
-    return reqs
 
 
 def bench_served(args, ctx):
@@ -648,6 +657,7 @@ def bench_served(args, ctx):
     best = max(ok, key=lambda r: r["GBps"]) if ok else runs[0]
     st = srv.stats()
     srv.close()
+    d_pool.free()
     mean_req = float(np.mean([q["n"] * q["s"] for q in template]))
     return {
         "workload": "served",
@@ -740,7 +750,8 @@ def bench_create(args, ctx):
         # the reference's method on the host: AVX2 encode with the
         # StreamingEncoder batching, IFSC hashing and the whole-file MD5s,
         # all multithreaded; packet assembly and writes not included
-        threads = min(16, os.cpu_count() or 1)
+        threads, how = host_cpus()
+        build = cpu_port_build()
         data = np.concatenate([np.frombuffer(b, np.uint8) for _, b in files])
         nb = n_blocks
         padded = np.zeros(nb * bs, np.uint8)
@@ -755,6 +766,7 @@ def bench_create(args, ctx):
             list(digests)
         dt = time.perf_counter() - t0
         res["cpu_baseline"] = {"value": round(src / dt / 1e9, 4), "unit": "GB/s", "cores": threads, "kind": "port",
+                               "cores_from": how, "build": build,
                                "sample": f"the same {src / 2**30:.3g} GiB set: AVX2 encode of {nb} x {bs} B into {r} recovery "
                                          f"slices + IFSC MD5/CRC32 + whole-file MD5s, {dt:.2f} s wall"}
     return res
@@ -827,10 +839,12 @@ def bench_repair_files(args, ctx):
         # the reference's method on the host for the same set: MD5/CRC32 of
         # every block and the rebuild contraction of m blocks (file reads,
         # parsing and writes not included)
-        threads = min(16, os.cpu_count() or 1)
+        threads, how = host_cpus()
+        build = cpu_port_build()
         nb = nfiles * per_file
         value, dt = cpu_baseline_repair(nb, bs, m, threads)
         res["cpu_baseline"] = {"value": round(value, 4), "unit": "GB/s", "cores": threads, "kind": "port",
+                               "cores_from": how, "build": build,
                                "sample": f"{nb} blocks x {bs} B: MD5/CRC32 verify of every block + rebuild "
                                          f"contraction of {m} blocks (AVX2 nibble-shuffle), {dt:.2f} s wall"}
     return res
@@ -848,7 +862,7 @@ def main():
     ap.add_argument("--lanes", type=int, default=2, help="served: concurrent batches")
     ap.add_argument("--max-batch", type=int, default=512, help="served: requests per batch at most")
     ap.add_argument("--served-requests", type=int, default=400, help="served: requests per plain run")
-    ap.add_argument("--served-loads", type=float, nargs="+", default=[6, 8, 10, 11, 12, 13, 14, 16],
+    ap.add_argument("--served-loads", type=float, nargs="+", default=[6, 8, 10, 11, 12, 13, 14],
                     help="served: offered loads as multiples of the plain run's")
     ap.add_argument("--create-files", type=int, default=16, help="create / repair_files: input files")
     ap.add_argument("--create-file-mib", type=int, default=64, help="create / repair_files: MiB per input file")
@@ -861,6 +875,11 @@ def main():
                          "(default: 0 for encode/repair, whose steps the MD5 chains bound, so the contraction's "
                          "shape and its measured HBM traffic stay fixed; 1 otherwise)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--legs", nargs="*", default=["repair", "batched", "served"],
+                    choices=["repair", "batched", "served"],
+                    help="encode: other workloads measured in the same run and reported as sub-objects "
+                         "(--legs with no value: none)")
+    ap.add_argument("--leg-steps", type=int, default=5, help="steps of the batched leg")
     ap.add_argument("--traffic", type=float, default=None,
                     help="HBM bytes per GF launch measured with rocprofv3 --pmc (FETCH_SIZE+WRITE_SIZE)")
     args = ap.parse_args()
@@ -901,15 +920,9 @@ def main():
         ctx.close()
         return
     if args.workload == "served":
-        res = bench_served(args, ctx)
-        out = {
-            "metric": METRIC, "value": res["value"], "unit": "GB/s", "n_gpus": 1, "steps": len(res["served"]),
-            "warmup": 0, "ms_per_step": None, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
-            "dtype": "u16", "data": "synthetic (seeded uniform random bytes)", "config": res["config"],
-            "p50_ms": res["p50_ms"], "p99_ms": res["p99_ms"], "bit_exact_sample": res["exact"],
-            "device": ctx.device_info(), "served": res["served"], "server_stats": res["server_stats"],
-            "plain_path": res["plain_path"],
-        }
+        out = {"metric": METRIC, "n_gpus": 1, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+               "dtype": "u16", "data": "synthetic (seeded uniform random bytes)", **served_line(bench_served(args, ctx)),
+               "device": ctx.device_info()}
         print(json.dumps(out), flush=True)
         ctx.close()
         return
@@ -923,19 +936,51 @@ def main():
         res = bench_encode_host(args, ctx)
     else:
         res = bench_encode(args, ctx) if args.workload == "encode" else bench_repair(args, ctx)
+    out = {"metric": METRIC, **step_line(res, args), "n_gpus": 1, "higher_is_better": True, "scaling": "weak",
+           "vs_baseline": None, "dtype": "u16", "data": "synthetic (seeded uniform random bytes)",
+           "device": ctx.device_info()}
+    out = {k: out[k] for k in ("metric", "value", "unit", "n_gpus", "steps", "warmup", "ms_per_step",
+                               "higher_is_better", "scaling", "vs_baseline", "dtype", "data")} | out
+    if not args.no_cpu:
+        cpu = cpu_leg(args, res)
+        if cpu:
+            out["cpu_baseline"] = cpu
+    if args.workload == "encode" and args.legs:
+        # the metric's other halves in the same run: plain repair, the batched
+        # sets against the plain path, and the served stream against the plain
+        # per-request path at equal p99
+        legs = {}
+        for leg in args.legs:
+            log("leg", leg)
+            if leg == "repair":
+                r = bench_repair(args, ctx)
+                legs["repair"] = step_line(r, args)
+                if not args.no_cpu:
+                    legs["repair"]["cpu_baseline"] = cpu_leg(args, r)
+            elif leg == "batched":
+                steps = args.steps
+                args.steps = max(3, min(args.steps, args.leg_steps))
+                legs["batched"] = step_line(bench_batched(args, ctx), args)
+                args.steps = steps
+            elif leg == "served":
+                legs["served"] = served_line(bench_served(args, ctx))
+        out.update(legs)
+    print(json.dumps(out), flush=True)
+    ctx.close()
+
+
+def step_line(res, args):
+    """The JSON fields of a stepped workload: throughput, step latency, the
+    exactness check, kernel times, roofline and the plain-path comparison."""
     lat = np.array(res["lat"]) * 1e3
-    ms = res["total_s"] / args.steps * 1e3
+    steps = len(res["lat"])
+    ms = res["total_s"] / steps * 1e3
     value = res["src_bytes"] / (ms * 1e-3) / 1e9
-    out = {
-        "metric": METRIC, "value": round(value, 3), "unit": "GB/s", "n_gpus": 1, "steps": args.steps,
-        "warmup": args.warmup, "ms_per_step": round(ms, 4), "higher_is_better": True, "scaling": "weak",
-        "vs_baseline": None, "dtype": "u16", "data": "synthetic (seeded uniform random bytes)",
-        "config": res["config"],
-        "p50_ms": round(float(np.percentile(lat, 50)), 4), "p99_ms": round(float(np.percentile(lat, 99)), 4),
-        "bit_exact_sample": res["exact"],
-        "device": ctx.device_info(),
-        "kernel_ms": {k: round(v, 4) for k, v in res["kernel_ms"].items()},
-    }
+    out = {"value": round(value, 3), "unit": "GB/s", "steps": steps, "warmup": args.warmup,
+           "ms_per_step": round(ms, 4), "config": res["config"],
+           "p50_ms": round(float(np.percentile(lat, 50)), 4), "p99_ms": round(float(np.percentile(lat, 99)), 4),
+           "bit_exact_sample": res["exact"],
+           "kernel_ms": {k: round(v, 4) for k, v in res["kernel_ms"].items()}}
     if "roofline" in res:
         out["roofline"] = res["roofline"]
     if "eager_ms" in res:
@@ -946,22 +991,35 @@ def main():
                              "unit": "GB/s", "speedup_batched": round(value / res["plain_gbs"], 3)}
     if "cpu_baseline" in res:
         out["cpu_baseline"] = res["cpu_baseline"]
-    if not args.no_cpu and "cpu" in res:
-        threads = min(16, os.cpu_count() or 1)
-        n, ss, r = res["cpu"]
-        log("cpu baseline", n, ss, r, "threads", threads)
-        if args.workload in ("encode", "encode_host"):
-            gbs, dt = cpu_baseline_encode(n, ss, r, threads)
-            what = (f"{n} slices x {ss} B, R={r}: AVX2 nibble-shuffle encode (StreamingEncoder batching) "
-                    f"+ MD5/CRC32 per slice, one pass, {dt:.2f} s wall")
-        else:
-            gbs, dt = cpu_baseline_repair(n, ss, r, threads)
-            what = (f"{n} slices x {ss} B: MD5/CRC32 verify of every slice + rebuild contraction of {r} slices "
-                    f"(AVX2 nibble-shuffle), one pass, {dt:.2f} s wall")
-        out["cpu_baseline"] = {"value": round(gbs, 4), "unit": "GB/s", "cores": threads, "kind": "port",
-                               "sample": what}
-    print(json.dumps(out), flush=True)
-    ctx.close()
+    return out
+
+
+def served_line(res):
+    return {"value": res["value"], "unit": "GB/s", "steps": len(res["served"]), "warmup": 0, "ms_per_step": None,
+            "config": res["config"], "p50_ms": res["p50_ms"], "p99_ms": res["p99_ms"],
+            "bit_exact_sample": res["exact"], "served": res["served"], "server_stats": res["server_stats"],
+            "plain_path": res["plain_path"]}
+
+
+def cpu_leg(args, res):
+    """The reference's CPU method (the oracle's restatement of its AVX2
+    strategy and hashing) on this host's cores for the same workload."""
+    if "cpu" not in res:
+        return None
+    threads, how = host_cpus()
+    build = cpu_port_build()
+    n, ss, r = res["cpu"]
+    log("cpu baseline", n, ss, r, "threads", threads, build)
+    if res["workload"] in ("plain_encode", "encode_host"):
+        gbs, dt = cpu_baseline_encode(n, ss, r, threads)
+        what = (f"{n} slices x {ss} B, R={r}: AVX2 nibble-shuffle encode (StreamingEncoder batching) "
+                f"+ MD5/CRC32 per slice, one pass, {dt:.2f} s wall")
+    else:
+        gbs, dt = cpu_baseline_repair(n, ss, r, threads)
+        what = (f"{n} slices x {ss} B: MD5/CRC32 verify of every slice + rebuild contraction of {r} slices "
+                f"(AVX2 nibble-shuffle), one pass, {dt:.2f} s wall")
+    return {"value": round(gbs, 4), "unit": "GB/s", "cores": threads, "kind": "port", "cores_from": how,
+            "build": build, "sample": what}
 
 
 if __name__ == "__main__":

@@ -33,6 +33,16 @@ def build() -> None:
     subprocess.run(["make", "-s", "-C", HERE], check=True)
 
 
+def use_library(path: str) -> None:
+    """Load the restatement from another build of the same sources (for
+    instance one compiled for the running host's CPU with -march=native);
+    every wrapper below then calls into it."""
+    global _lib, LIB_PATH
+    LIB_PATH = path
+    _lib = None
+    lib()
+
+
 def lib():
     global _lib
     if _lib is None:
