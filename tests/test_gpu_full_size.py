@@ -1,19 +1,22 @@
-"""Round trips at the full sizes of the plain-encode and plain-repair workloads,
-through properties that do not depend on a CPU run of the whole set:
-encode -> erase -> rebuild gives back the erased slices byte for byte, the
-IFSC digests of the encode equal hashlib/zlib on sampled slices, and the
-repair finds exactly the erased slices (par2_rs.rs:785-902 encoder,
-repair.rs:536-604 damaged set and lowest exponents, par2_rs.rs:319-675
-rebuild)."""
+"""The full sizes of the plain-encode and plain-repair workloads, byte for
+byte: every recovery slice of the encode against the oracle's multithreaded
+restatement of the reference's AVX2 strategy (itself pinned against the scalar
+oracle), every IFSC digest against the oracle's MD5/CRC32, then encode -> erase
+-> rebuild gives back every erased slice exactly and the repair finds exactly
+the erased slices (par2_rs.rs:785-902 encoder, repair.rs:536-604 damaged set
+and lowest exponents, par2_rs.rs:319-675 rebuild)."""
 from __future__ import annotations
 
 import hashlib
 import sys
 import zlib
 
+import os
+
 import numpy as np
 import pytest
 
+import oracle
 import par2hip
 
 pytestmark = pytest.mark.gpu
@@ -54,6 +57,16 @@ def test_encode_erase_repair_full_size(ctx, n, s, r):
     for i in (0, n // 3, n - 1):
         assert md5[i].tobytes() == hashlib.md5(data[i].tobytes()).digest(), i
         assert int(crc[i]) == zlib.crc32(data[i].tobytes()), i
+    # all of it against the oracle
+    threads = min(16, len(os.sched_getaffinity(0)))
+    say("oracle: AVX2 encode + hashes of the whole set, threads", threads)
+    want_rec = oracle.encode_avx2(data.view(np.uint16), np.arange(r), threads=threads)
+    rec = d_rec.download().reshape(r, s)
+    bad = [e for e in range(r) if not np.array_equal(rec[e].view(np.uint16), want_rec[e])]
+    assert not bad, f"recovery slices differing from the oracle: {bad[:10]}"
+    del want_rec
+    wm, wc = oracle.hash_slices_mt(data, n, s, threads=threads)
+    assert np.array_equal(md5, wm) and np.array_equal(crc, wc)
     rng = np.random.default_rng(7 * n)
     erased = np.sort(rng.choice(n, r, replace=False))
     damaged = data.copy()

@@ -117,19 +117,33 @@ def test_encode_rejects_bad_sizes(ctx):
         ctx.encode(np.zeros((32769, 2), np.uint8), [0])
 
 
-def test_encode_column_sample_at_full_slice_size(kctx):
-    """1024 x 1 MiB slices, R=102 (the plain-encode workload): the contraction
-    is column independent, so a seeded column window checks against the oracle."""
+_C2 = {}
+
+
+def _c2_set():
+    """1024 x 1 MiB seeded slices and their 102 recovery slices from the
+    oracle's multithreaded AVX2 restatement (pinned against the scalar
+    oracle), computed once per session."""
+    if not _C2:
+        n, s, r = 1024, 1 << 20, 102
+        data = np.random.default_rng(2).integers(0, 256, (n, s), dtype=np.uint8)
+        threads = min(16, len(os.sched_getaffinity(0)))
+        _C2["data"] = data
+        _C2["rec"] = oracle.encode_avx2(data.view(np.uint16), np.arange(r), threads=threads)
+    return _C2["data"], _C2["rec"]
+
+
+def test_encode_full_size_every_byte(kctx):
+    """1024 x 1 MiB slices, R=102 (the plain-encode workload), with each
+    contraction variant: all 102 recovery slices byte for byte."""
     ctx = kctx
-    n, s, r = 1024, 1 << 20, 102
-    rng = np.random.default_rng(2)
-    data = rng.integers(0, 256, (n, s), dtype=np.uint8)
+    data, want = _c2_set()
+    n, s = data.shape
+    r = want.shape[0]
     say("encode full size", n, s, r)
     rec = ctx.encode(data, np.arange(r))
-    for w0 in (0, 77777, (s // 2) - 64):
-        cols = data.view(np.uint16)[:, w0:w0 + 64]
-        want = oracle.encode(np.ascontiguousarray(cols), np.arange(r))
-        assert np.array_equal(rec.view(np.uint16)[:, w0:w0 + 64], want), w0
+    bad = [e for e in range(r) if not np.array_equal(rec[e].view(np.uint16), want[e])]
+    assert not bad, f"recovery slices differing from the oracle: {bad[:10]}"
 
 
 # --------------------------------------------------------------- hashes --
