@@ -137,6 +137,22 @@ int32_t par2hip_encode_dev(par2hip_ctx *ctx, const void *d_slices, size_t n, siz
                            const uint32_t *exps, size_t r, void *d_recovery, void *d_md5,
                            void *d_crc);
 
+/* par2hip_encode_dev without any allocation during the call.  No counterpart
+ * in the reference: it is for callers that cannot allocate device memory
+ * (work captured into a HIP graph, tight serving loops, harnesses that forbid
+ * hipMalloc).  Same arguments, results and asynchrony as par2hip_encode_dev
+ * (recovery slices bit-identical; the contraction is always the LDS-table
+ * form).  Its working memory is one 64 MiB device array per device, held
+ * until the process ends: an encode needs 256-byte-rounded n*4 + r*4 + n*8 +
+ * r*8 + r*n*2 + ceil(r/8)*n*1024 bytes of it (the 1 GiB / 1024-slice set with
+ * r = 102 needs ~14 MB), and PAR2HIP_EINVAL is returned, before anything is
+ * queued, when that exceeds 64 MiB.  Every call on a device shares that
+ * array: calls that overlap on one device (through different contexts or
+ * threads) race; calls queued on one context run in order. */
+int32_t par2hip_encode_dev_noalloc(par2hip_ctx *ctx, const void *d_slices, size_t n, size_t slice_size,
+                                   const uint32_t *exps, size_t r, void *d_recovery, void *d_md5,
+                                   void *d_crc);
+
 /* ------------------------------------------------------------ repair -- */
 
 /* Gauss-Jordan inverse of an m x m GF(2^16) matrix on the device

@@ -96,6 +96,7 @@ struct par2hip_ctx {
   bool profile = false;
   bool autotune = false;  // time candidate tile shapes on first use of a shape class
   int forced_shape = -1;  // tile shape used by every LDS contraction (-1: none)
+  void *noalloc_ws = nullptr;  // this device's par2hip_encode_dev_noalloc workspace
   std::vector<par2hip::KernelStat> stats;
   std::vector<par2hip::PendingTiming> pending;
   std::vector<hipEvent_t> event_pool;
@@ -267,6 +268,20 @@ struct InvJob {
 int32_t launch_inverse_batch(par2hip_ctx *ctx, hipStream_t s, const InvJob *d_jobs, int njobs, int max_m,
                              long max_vand);
 int32_t run_gf_lds(par2hip_ctx *ctx, hipStream_t s, const GfProblem &p);
+// Variant T with its nibble tables in a caller-provided workspace of at least
+// gf_tab_entries(ndst, nsrc) * 16 bytes and the fixed heuristic's (or the
+// forced) tile shape: no allocation, no tuning runs.
+int32_t run_gf_lds_fixed(par2hip_ctx *ctx, hipStream_t s, const GfProblem &p, uint64_t *tabs);
+// The main stream forks into the side stream (MD5 chains) and the rest
+// stream (ctx->cur), and joins them back (par2hip_api.hip).
+int32_t fork_overlap(par2hip_ctx *ctx);
+int32_t join_overlap(par2hip_ctx *ctx);
+int align_words_of(uintptr_t p);
+// Bytes of the per-device workspace of par2hip_encode_dev_noalloc (a
+// file-scope device array in noalloc.hip) and its set-up: the array's device
+// address on the context's device, looked up once per device.
+constexpr size_t kNoAllocScratchBytes = size_t(64) << 20;
+int32_t noalloc_init(par2hip_ctx *ctx);
 int32_t run_gf_mfma(par2hip_ctx *ctx, hipStream_t s, const GfProblem &p, bool int8);
 // Kernel used when the caller leaves the choice to the library and the call
 // has no plan of its own (an encode beside MD5 chains plans kernel and CUs):

@@ -202,18 +202,31 @@ static int32_t tune_shape(par2hip_ctx *ctx, hipStream_t s, const GfProblem &p, c
   return rc;
 }
 
+static int32_t build_nibble_tables(par2hip_ctx *ctx, hipStream_t s, const GfProblem &p, uint64_t *tabs) {
+  const long pairs = (long)gf_tab_groups(p.ndst) * p.nsrc;
+  ScopedTiming t(ctx, "nibble_tables", s);
+  hipLaunchKernelGGL(k_nibble_tables, dim3((unsigned)((pairs + 3) / 4)), dim3(256), 0, s, tabs, p.coeff, p.ndst,
+                     p.nsrc, p.ldc ? p.ldc : p.nsrc);
+  P2H_CHECK(ctx, hipGetLastError());
+  return PAR2HIP_OK;
+}
+
+int32_t run_gf_lds_fixed(par2hip_ctx *ctx, hipStream_t s, const GfProblem &p, uint64_t *tabs) {
+  if (p.ndst <= 0 || p.words == 0 || p.nsrc <= 0) return PAR2HIP_OK;
+  int32_t rc = build_nibble_tables(ctx, s, p, tabs);
+  if (rc) return rc;
+  const int vmax = p.align_words % 8 == 0 ? 8 : p.align_words % 4 == 0 ? 4 : p.align_words % 2 == 0 ? 2 : 1;
+  const int shape = ctx->forced_shape >= 0 && kShapes[ctx->forced_shape].v <= vmax ? ctx->forced_shape
+                                                                                    : default_shape(ctx, p, vmax);
+  return launch_shape(ctx, s, p, tabs, shape);
+}
+
 int32_t run_gf_lds(par2hip_ctx *ctx, hipStream_t s, const GfProblem &p) {
   const size_t tab_bytes = (size_t)gf_tab_entries(p.ndst, p.nsrc) * 16;
   void *tabs = nullptr;
   int32_t rc = scratch(ctx, kScratchTables, tab_bytes, &tabs);
   if (rc) return rc;
-  {
-    const long pairs = (long)gf_tab_groups(p.ndst) * p.nsrc;
-    ScopedTiming t(ctx, "nibble_tables", s);
-    hipLaunchKernelGGL(k_nibble_tables, dim3((unsigned)((pairs + 3) / 4)), dim3(256), 0, s, (uint64_t *)tabs, p.coeff,
-                       p.ndst, p.nsrc, p.ldc ? p.ldc : p.nsrc);
-    P2H_CHECK(ctx, hipGetLastError());
-  }
+  if ((rc = build_nibble_tables(ctx, s, p, (uint64_t *)tabs))) return rc;
   const int vmax = p.align_words % 8 == 0 ? 8 : p.align_words % 4 == 0 ? 4 : p.align_words % 2 == 0 ? 2 : 1;
   int shape = ctx->forced_shape >= 0 && kShapes[ctx->forced_shape].v <= vmax ? ctx->forced_shape : -1;
   if (shape < 0 && ctx->autotune) {

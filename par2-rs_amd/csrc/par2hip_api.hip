@@ -350,11 +350,15 @@ static GfPlan plan_overlapped_gf(const par2hip_ctx *ctx, size_t n, size_t slice_
   return g;
 }
 
-int align_words_of(uintptr_t p) {
+}  // namespace
+
+int par2hip::align_words_of(uintptr_t p) {
   int a = 8;
   while (a > 1 && (p % (2 * (uintptr_t)a)) != 0) a >>= 1;
   return a;
 }
+
+namespace {
 
 // Common encode body: device buffers in, queued on ctx->stream.  Recovery
 // row i goes to d_rec + i * rec_stride (rec_stride 0: slice_size).
@@ -713,6 +717,7 @@ int32_t par2hip::ctx_create(int32_t device, bool narrow_streams, par2hip_ctx **o
   P2H_CHECK(ctx, hipMalloc(&ctx->d_status, 256));
   int32_t rc = crc_tables_init(ctx);
   if (rc) return rc;
+  if ((rc = noalloc_init(ctx))) return rc;
   *out = c.release();
   return PAR2HIP_OK;
 }

@@ -152,6 +152,7 @@ def lib():
                 "par2hip_gf_muladd_region": ([P, P, S, P, S, P, P, S, S], I32),
                 "par2hip_encode": ([P, P, S, S, P, S, P, P, P], I32),
                 "par2hip_encode_dev": ([P, P, S, S, P, S, P, P, P], I32),
+                "par2hip_encode_dev_noalloc": ([P, P, S, S, P, S, P, P, P], I32),
                 "par2hip_gf_invert": ([P, P, S, P], I32),
                 "par2hip_reconstruct": ([P, P, S, S, P, S, P, P, P], I32),
                 "par2hip_reconstruct_dev": ([P, P, S, S, P, S, P, P, P, I32], I32),
@@ -343,6 +344,15 @@ class Context:
         self._check(lib().par2hip_encode_dev(self.handle, d_slices.ptr, n, slice_size, _p(exps), len(exps),
                                              d_rec.ptr, d_md5.ptr if d_md5 else None,
                                              d_crc.ptr if d_crc else None))
+
+    def encode_dev_noalloc(self, d_slices: DeviceBuffer, n: int, slice_size: int, exponents, d_rec: DeviceBuffer,
+                           d_md5: DeviceBuffer | None = None, d_crc: DeviceBuffer | None = None) -> None:
+        """encode_dev through par2hip_encode_dev_noalloc (no allocation during
+        the call; working memory in the library's per-device array)."""
+        exps = np.ascontiguousarray(exponents, np.uint32)
+        self._check(lib().par2hip_encode_dev_noalloc(self.handle, d_slices.ptr, n, slice_size, _p(exps), len(exps),
+                                                     d_rec.ptr if d_rec else None, d_md5.ptr if d_md5 else None,
+                                                     d_crc.ptr if d_crc else None))
 
     def reconstruct(self, slices: np.ndarray, missing, recovery: np.ndarray, rec_exps) -> np.ndarray:
         slices = np.ascontiguousarray(slices, np.uint8)
