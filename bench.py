@@ -676,6 +676,56 @@ def bench_served(args, ctx):
     }
 
 
+def bench_inverse(args, ctx):
+    """The device inverse of m x m erasure matrices with gapped exponents
+    (par2hip_gf_invert: the blocked Gauss-Jordan above 256, one workgroup
+    below), one call per size, its kernel times from HIP events."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle
+
+    glog, gexp = oracle.tables()
+    out = {}
+    for m in args.inverse_sizes:
+        rng = np.random.default_rng(m)
+        c = oracle.par2_constants(32768)
+        lc = glog[c[rng.choice(32768, m, replace=False)]].astype(np.uint64)
+        # gapped exponents as a repair with lost volumes has them; a generalised
+        # Vandermonde matrix with arbitrary exponents can be singular, so they
+        # are drawn from a range that leaves few gaps (consecutive at the top
+        # size, where 3m exceeds the exponent range)
+        top = 3 * m + 100
+        exps = (np.sort(rng.choice(top, m, replace=False)) if top <= 65535 else np.arange(m)).astype(np.uint64)
+        A = np.empty((m, m), np.uint16)
+        for i0 in range(0, m, 1024):  # row blocks: no m x m 64-bit temporary
+            A[i0:i0 + 1024] = gexp[(exps[i0:i0 + 1024, None] * lc[None, :]) % 65535]
+        ctx.profile(True)
+        ctx.profile_read()
+        t0 = time.perf_counter()
+        try:
+            inv = ctx.gf_invert(A)
+        except par2hip.RepairFailed:
+            out[str(m)] = {"singular": True}
+            ctx.profile(False)
+            continue
+        wall = time.perf_counter() - t0
+        st = ctx.profile_read()
+        ctx.profile(False)
+        # row 0 of A * A^-1 must be e_0 (table product; the tests use the carry-less one)
+        la = glog[A[0]].astype(np.int64)
+        row = np.zeros(m, np.uint16)
+        for j0 in range(0, m, 1024):
+            li = glog[inv[:, j0:j0 + 1024]].astype(np.int64)
+            prod = np.where((A[0][:, None] != 0) & (inv[:, j0:j0 + 1024] != 0), gexp[(la[:, None] + li) % 65535], 0)
+            row[j0:j0 + 1024] = np.bitwise_xor.reduce(prod.astype(np.uint16), axis=0)
+        e0 = np.zeros(m, np.uint16)
+        e0[0] = 1
+        out[str(m)] = {"wall_ms": round(wall * 1e3, 2), "row0_identity": bool(np.array_equal(row, e0)),
+                       **{k: round(v[1], 3) for k, v in st.items()}}
+        log("inverse", m, out[str(m)])
+        del A, inv
+    return out
+
+
 def bench_create(args, ctx):
     """End-to-end PAR2 creation (Par2Creator::create, creator.rs:261-531)
     from host memory to written .par2 files, exponential volumes.  Default:
@@ -856,7 +906,7 @@ def main():
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--workload", choices=["encode", "repair", "micro", "batched", "served", "copy", "create",
-                                           "repair_files", "encode_host"],
+                                           "repair_files", "encode_host", "inverse"],
                     default="encode")
     ap.add_argument("--hbm-budget", type=int, default=200 * 10**9, help="served: server HBM budget in bytes")
     ap.add_argument("--lanes", type=int, default=2, help="served: concurrent batches")
@@ -868,6 +918,8 @@ def main():
     ap.add_argument("--create-file-mib", type=int, default=64, help="create / repair_files: MiB per input file")
     ap.add_argument("--create-block", type=int, default=0, help="create / repair_files: block size in bytes (0: automatic)")
     ap.add_argument("--create-redundancy", type=float, default=10.0, help="create / repair_files: redundancy percent")
+    ap.add_argument("--inverse-sizes", type=int, nargs="+", default=[1000, 5000, 12000],
+                    help="inverse: erasure counts m (up to 32768)")
     ap.add_argument("--seed", type=int, default=4)
     ap.add_argument("--kernel", type=int, default=0, help="0 auto, 1 LDS tables, 2 MFMA FP4, 3 MFMA int8")
     ap.add_argument("--autotune", type=int, default=None,
@@ -913,6 +965,10 @@ def main():
         dt = (time.perf_counter() - t0) / args.steps
         print(json.dumps({"copy": {"bytes_read": nb, "bytes_written": nb, "ms": round(dt * 1e3, 4),
                                    "GBps_rw": round(2 * nb / dt / 1e9, 1)}}), flush=True)
+        ctx.close()
+        return
+    if args.workload == "inverse":
+        print(json.dumps({"inverse": bench_inverse(args, ctx)}), flush=True)
         ctx.close()
         return
     if args.workload == "micro":

@@ -87,7 +87,7 @@ struct par2hip_ctx {
   uint32_t *d_crc_tab = nullptr;      // CRC-32 slice-by-4 tables T0..T3 (4 x 256)
   int *d_status = nullptr;            // device status word (singular flag)
   par2hip::ParamArena arena;
-  par2hip::DevBuf scratch[11];        // grow-only device scratch, by role
+  par2hip::DevBuf scratch[12];        // grow-only device scratch, by role
   uint8_t *host_stage = nullptr;      // grow-only pinned host staging for file I/O
   size_t host_stage_bytes = 0;
   hipEvent_t stage_ev[2] = {nullptr, nullptr};  // the staging halves' last copies (h2d_staged / d2h_staged)
@@ -116,6 +116,7 @@ enum ScratchRole {
   kScratchVand = 8,    // repair: V rows c_present^e (fold sources)
   kScratchHashD = 9,   // repair: digests of the verify pass
   kScratchBatch = 10,  // batched runs: all per-set workspaces in one arena
+  kScratchGjWork = 11, // blocked Gauss-Jordan: panel, pivots, pointer tables
 };
 
 int32_t fail(par2hip_ctx *ctx, int32_t code, const std::string &msg);
@@ -200,6 +201,9 @@ constexpr double kFp4MacPerCuSec = 1.09e13;
 // time, at the 2.4 GHz peak shader clock.
 double md5_chains_ms(size_t n, uint64_t bytes, int simds);
 int32_t run_gf(par2hip_ctx *ctx, hipStream_t s, const GfProblem &p);
+// Largest per-coefficient operand block (tables) one contraction launch gets;
+// larger problems are split by output rows.
+constexpr long kGfChunkBytes = 1L << 30;
 // The process-wide table of tuned tile shapes as text (one "key v g d" line each).
 int32_t tune_table_save(const char *path);
 // Index of tile shape (v, g, d) among the candidates, or -1.
@@ -358,7 +362,7 @@ int32_t launch_build_aug(par2hip_ctx *ctx, hipStream_t s, uint16_t *d_aug, const
                          const uint32_t *d_exps, int m);
 // Closed-form inverse for consecutive exponents e0, e0+1, ...: writes A^-1
 // into the right half of the m x 2m workspace.
-constexpr int kVandMaxM = 512;
+constexpr int kVandMaxM = 1024;  // [tables | P] of 130 m bytes in LDS
 int32_t launch_vandermonde_inverse(par2hip_ctx *ctx, hipStream_t s, uint16_t *d_aug, const uint32_t *d_miss_log,
                                    uint32_t e0, int m);
 // vand[i][j] = c_{present_j}^{e_i} (row stride vs words).

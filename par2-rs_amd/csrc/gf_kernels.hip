@@ -286,6 +286,25 @@ int32_t run_gf(par2hip_ctx *ctx, hipStream_t s, const GfProblem &p) {
     if (p.accumulate) return PAR2HIP_OK;
     return fail(ctx, PAR2HIP_EINVAL, "contraction with no sources and overwrite semantics");
   }
+  // The per-coefficient operands (nibble tables: 128 B per coefficient; packed
+  // bit matrices: 32 B) grow as ndst * nsrc; beyond kGfChunkBytes of them the
+  // output rows are taken in chunks (each an independent contraction over the
+  // same sources), so a large inverse or fold never needs more.
+  const long per_row = 128L * p.nsrc;
+  const long chunk_rows = std::max(64L, (kGfChunkBytes / per_row) & ~63L);
+  if (p.ndst > chunk_rows) {
+    const int ldc = p.ldc ? p.ldc : p.nsrc;
+    for (long r0 = 0; r0 < p.ndst; r0 += chunk_rows) {
+      GfProblem c = p;
+      c.dsts = p.dsts + r0;
+      c.coeff = p.coeff + r0 * ldc;
+      c.ldc = ldc;
+      c.ndst = (int)std::min<long>(chunk_rows, p.ndst - r0);
+      int32_t rc = run_gf(ctx, s, c);
+      if (rc) return rc;
+    }
+    return PAR2HIP_OK;
+  }
   // 1 = LDS tables, 2 / 3 = bit-sliced MFMA with FP4 / int8 operands (needs
   // 16-byte aligned operands), 0 = automatic
   const bool mfma_ok = p.align_words >= 8;
