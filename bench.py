@@ -476,37 +476,59 @@ def bench_batched(args, ctx):
             lat.append(time.perf_counter() - t0)
         return lat
 
+    def outputs_equal(ref):
+        return all(np.array_equal(a, (j["recovery"].download() if j["kind"] == "encode" else j["out"].download()))
+                   and np.array_equal(b, j["md5"].download()) and np.array_equal(c, j["crc"].download())
+                   for j, (a, b, c) in zip(jobs, ref))
+
+    def clear():
+        for j in jobs:
+            (j["recovery"] if j["kind"] == "encode" else j["out"]).upload(np.zeros(16, np.uint8))
+
+    tuning = {}
+
+    def graph_replay():
+        plan = ctx.batch_plan(jobs, graph=True)
+        tuning.update(plan.tuning())
+
+        def replay():
+            plan.launch()
+            ctx.sync()
+
+        clear()
+        replay()
+        ok = outputs_equal(ref)
+        lat = timed(replay, args.steps)
+        plan.close()
+        return ok, lat
+
+    was_tuned = ctx.set_autotune(False)
     # bit-exactness: batched outputs equal the plain path's for every job
     plain()
     ref = [(j["recovery"].download() if j["kind"] == "encode" else j["out"].download(), j["md5"].download(),
             j["crc"].download()) for j in jobs]
-    for j in jobs:
-        (j["recovery"] if j["kind"] == "encode" else j["out"]).upload(np.zeros(16, np.uint8))
+    clear()
     batched()
-    exact = all(np.array_equal(a, (j["recovery"].download() if j["kind"] == "encode" else j["out"].download()))
-                and np.array_equal(b, j["md5"].download()) and np.array_equal(c, j["crc"].download())
-                for j, (a, b, c) in zip(jobs, ref))
+    exact = outputs_equal(ref)
     ctx.profile(True)
     ctx.profile_read()
     lat_e = timed(batched, args.steps)
     stats = ctx.profile_read()
     ctx.profile(False)
-    # the same batch planned once and replayed as one HIP graph per step
-    plan = ctx.batch_plan(jobs, graph=True)
-
-    def replay():
-        plan.launch()
-        ctx.sync()
-
-    for j in jobs:
-        (j["recovery"] if j["kind"] == "encode" else j["out"]).upload(np.zeros(16, np.uint8))
-    replay()
-    exact_graph = all(np.array_equal(a, (j["recovery"].download() if j["kind"] == "encode" else j["out"].download()))
-                      and np.array_equal(b, j["md5"].download()) and np.array_equal(c, j["crc"].download())
-                      for j, (a, b, c) in zip(jobs, ref))
-    lat_b = timed(replay, args.steps)
-    plan.close()
+    # the same batch planned once and replayed as one HIP graph per step, with
+    # the fixed tile shapes, then planned with autotuning: the single-set
+    # tuner first fills the tuned table on the sets themselves (one of the
+    # policies the plan times), then the plan times every policy on the batch
+    # and keeps the fastest
+    exact_fixed, lat_fixed = graph_replay()
+    ctx.set_autotune(True)
+    plain()
+    exact_plain_tuned = outputs_equal(ref)
+    exact_tuned, lat_b = graph_replay()
+    lat_tuned = lat_b
     lat_p = timed(plain, max(1, args.steps // 2))
+    ctx.set_autotune(was_tuned)
+    exact_graph = exact_fixed and exact_tuned and exact_plain_tuned
     for j in jobs:  # the sets' device memory goes back before the next workload
         for k in ("slices", "md5", "crc", "recovery", "out"):
             if k in j:
@@ -521,6 +543,11 @@ def bench_batched(args, ctx):
         "kernel_ms": {k: v[1] / max(v[0], 1) for k, v in stats.items()},
         "eager_ms": float(np.mean(lat_e) * 1e3),
         "plain_ms": float(np.mean(lat_p) * 1e3), "plain_gbs": total / float(np.mean(lat_p)) / 1e9,
+        "tile_shapes": {"tuned_ms": round(float(np.mean(lat_tuned) * 1e3), 3),
+                        "fixed_ms": round(float(np.mean(lat_fixed) * 1e3), 3),
+                        "tuned_speedup": round(float(np.mean(lat_fixed) / np.mean(lat_tuned)), 4),
+                        "policy_chosen": ["default", "tuned_table", "one_group", "small_v"][tuning["policy"]],
+                        "policy_ms": dict(zip(["default", "tuned_table", "one_group", "small_v"], tuning["ms"]))},
     }
 
 
@@ -1042,6 +1069,8 @@ def step_line(res, args):
     if "eager_ms" in res:
         out["batch_eager_ms"] = round(res["eager_ms"], 3)
         out["step"] = "one replay of the batch captured as a HIP graph (planned once)"
+    if "tile_shapes" in res:
+        out["tile_shapes"] = res["tile_shapes"]
     if "plain_ms" in res:
         out["plain_path"] = {"ms_per_batch": round(res["plain_ms"], 3), "value": round(res["plain_gbs"], 3),
                              "unit": "GB/s", "speedup_batched": round(value / res["plain_gbs"], 3)}

@@ -241,6 +241,20 @@ int32_t par2hip_batch_run_dev(par2hip_ctx *ctx, const par2hip_job *jobs, size_t 
 typedef struct par2hip_batch_plan par2hip_batch_plan;
 int32_t par2hip_batch_plan_create(par2hip_ctx *ctx, const par2hip_job *jobs, size_t njobs,
                                   int32_t use_graph, par2hip_batch_plan **out);
+/* With tile-shape autotuning on (par2hip_set_autotune(ctx, 1)) and no forced
+ * shape, par2hip_batch_plan_create plans the batch under each tile-shape
+ * policy -- 0: 8 words per thread and one 8-row group per item for at most 8
+ * outputs, two above; 1: the tuned table's shape for each task's class (the
+ * single-set tuner's); 2: one group everywhere; 3: 4 words per thread for
+ * slices of at most 64 Ki words -- times two launches of each on the batch
+ * itself and keeps the fastest (policy 0 unless another is at least 2%
+ * faster).  No counterpart in the reference.  The timings are readable: */
+typedef struct {
+    int32_t policy;       /* the policy the plan uses */
+    int32_t policies;     /* policies timed (0 when the plan was not tuned) */
+    double ms[4];         /* mean launch time of each policy */
+} par2hip_batch_plan_tuning;
+int32_t par2hip_batch_plan_tuning_info(par2hip_batch_plan *plan, par2hip_batch_plan_tuning *out);
 int32_t par2hip_batch_plan_launch(par2hip_batch_plan *plan);
 int32_t par2hip_batch_plan_destroy(par2hip_batch_plan *plan);
 

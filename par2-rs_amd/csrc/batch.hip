@@ -76,7 +76,7 @@ struct BatchPlan {
 };
 
 static int32_t plan_build(par2hip_ctx *ctx, const par2hip_job *jobs, size_t njobs, uint8_t *ext_arena,
-                          size_t ext_bytes, bool own, BatchPlan &P) {
+                          size_t ext_bytes, bool own, BatchPlan &P, int policy = kBatchShapeDefault) {
   // ---- validate and classify
   std::vector<size_t> enc, rep, rep_slow;
   for (size_t j = 0; j < njobs; j++) {
@@ -126,7 +126,13 @@ static int32_t plan_build(par2hip_ctx *ctx, const par2hip_job *jobs, size_t njob
     rp[k].coeff = ar.take(m * rp[k].cs * 2);
   }
   // tasks: one fold per repair job (if any present slice), one main per job
-  struct TaskPlan { size_t tabs; GfTaskDev t; size_t srcs_at, dsts_at; };
+  // align: byte alignment all of the task's rows share (rows b + k*s of a
+  // span share that of b and s), which bounds the words per thread
+  struct TaskPlan { size_t tabs; GfTaskDev t; size_t srcs_at, dsts_at; int align; };
+  auto span_align = [](const void *base, size_t stride, size_t count) {
+    const void *p[2] = {base, (const uint8_t *)base + stride};
+    return row_align(p, count > 1 ? 2 : 1);
+  };
   std::vector<TaskPlan> fold_tasks, main_tasks;
   auto tab_bytes = [](int ndst, int nsrc) { return (size_t)gf_tab_entries(ndst, nsrc) * 16; };
   for (size_t k = 0; k < rep.size(); k++) {
@@ -136,6 +142,7 @@ static int32_t plan_build(par2hip_ctx *ctx, const par2hip_job *jobs, size_t njob
       tp.t.nsrc = (int)J.count;
       tp.t.ndst = (int)J.count;
       tp.tabs = ar.take(tab_bytes(tp.t.ndst, tp.t.nsrc));
+      tp.align = 16;  // arena rows: 256-byte aligned blocks, strides of 8 entries
       fold_tasks.push_back(tp);
     }
   }
@@ -145,6 +152,7 @@ static int32_t plan_build(par2hip_ctx *ctx, const par2hip_job *jobs, size_t njob
     tp.t.nsrc = (int)J.n;
     tp.t.ndst = (int)J.count;
     tp.tabs = ar.take(tab_bytes(tp.t.ndst, tp.t.nsrc));
+    tp.align = std::min(span_align(J.d_slices, J.slice_size, J.n), span_align(J.d_recovery, J.slice_size, J.count));
     main_tasks.push_back(tp);
   }
   for (size_t k = 0; k < rep.size(); k++) {
@@ -153,6 +161,8 @@ static int32_t plan_build(par2hip_ctx *ctx, const par2hip_job *jobs, size_t njob
     tp.t.nsrc = (int)J.n;
     tp.t.ndst = (int)J.count;
     tp.tabs = ar.take(tab_bytes(tp.t.ndst, tp.t.nsrc));
+    tp.align = std::min({span_align(J.d_slices, J.slice_size, J.n), span_align(J.d_recovery, J.slice_size, J.count),
+                         span_align(J.d_out, J.slice_size, J.count)});
     main_tasks.push_back(tp);
   }
   const size_t cursor_off = ar.take(512);
@@ -169,15 +179,20 @@ static int32_t plan_build(par2hip_ctx *ctx, const par2hip_job *jobs, size_t njob
   uint8_t *arena = (uint8_t *)arena_v;
   P.slow = rep_slow;
 
-  // ---- work items (sorted by cost: sources per task, descending)
+  // ---- work items (sorted by cost: sources per task, descending); each
+  // task's tile shape sets its items' word tiles (kTpb * v words) and row
+  // blocks (8g rows)
   auto make_items = [&](std::vector<TaskPlan> &tasks, std::vector<GfItem> &items, const std::vector<long> &words) {
     std::vector<int> order(tasks.size());
     for (size_t i = 0; i < tasks.size(); i++) order[i] = (int)i;
     std::stable_sort(order.begin(), order.end(),
                      [&](int a, int b) { return (long)tasks[a].t.nsrc * words[a] > (long)tasks[b].t.nsrc * words[b]; });
     for (int ti : order) {
-      const int ntiles = (int)((words[ti] + kBatchWordsPerWg - 1) / kBatchWordsPerWg);
-      const int nrb = gf_rowblks(tasks[ti].t.ndst, gf_batch_groups(tasks[ti].t.ndst));
+      GfTaskDev &t = tasks[ti].t;
+      batch_tile_shape(ctx, policy, t.ndst, t.nsrc, words[ti], tasks[ti].align, &t.v, &t.groups);
+      const long per_wg = (long)kBatchTpb * t.v;
+      const int ntiles = (int)((words[ti] + per_wg - 1) / per_wg);
+      const int nrb = gf_rowblks(t.ndst, t.groups);
       for (int tile = 0; tile < ntiles; tile++)
         for (int rb = 0; rb < nrb; rb++) items.push_back(GfItem{ti, tile, rb, 0});
     }
@@ -272,7 +287,6 @@ static int32_t plan_build(par2hip_ctx *ctx, const par2hip_job *jobs, size_t njob
     t.tabs = (uint64_t *)(arena + main_tasks[k].tabs);
     t.accumulate = 0;
     t.align = std::min(row_align((const void *const *)h_srcs, J.n), row_align((const void *const *)h_dsts, J.count));
-    t.groups = gf_batch_groups(t.ndst);
   }
   std::vector<InvJob> ijobs;
   int max_m = 0;
@@ -332,7 +346,6 @@ static int32_t plan_build(par2hip_ctx *ctx, const par2hip_job *jobs, size_t njob
         t.tabs = (uint64_t *)(arena + fold_tasks[f].tabs);
         t.accumulate = 0;
         t.align = std::min(row_align((const void *const *)h_fsrc, m), row_align((const void *const *)h_fdst, m));
-        t.groups = gf_batch_groups(t.ndst);
         f++;
       }
       GfTaskDev &t = main_tasks[enc.size() + k].t;
@@ -343,7 +356,6 @@ static int32_t plan_build(par2hip_ctx *ctx, const par2hip_job *jobs, size_t njob
       t.tabs = (uint64_t *)(arena + main_tasks[enc.size() + k].tabs);
       t.accumulate = 0;
       t.align = std::min(row_align((const void *const *)h_srcs, np + m), row_align((const void *const *)h_dsts, m));
-      t.groups = gf_batch_groups(t.ndst);
     }
   }
   auto put = [&](const void *src, size_t bytes) -> void * {
@@ -471,6 +483,7 @@ struct par2hip_batch_plan {
   par2hip::BatchPlan plan;
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
+  par2hip_batch_plan_tuning tuning{};
 };
 
 static void plan_free(par2hip_batch_plan *p) {
@@ -481,15 +494,12 @@ static void plan_free(par2hip_batch_plan *p) {
   delete p;
 }
 
-extern "C" int32_t par2hip_batch_plan_create(par2hip_ctx *ctx, const par2hip_job *jobs, size_t njobs,
-                                             int32_t use_graph, par2hip_batch_plan **out) {
-  if (!ctx || !out || njobs == 0 || !jobs) return PAR2HIP_EINVAL;
-  *out = nullptr;
-  std::lock_guard<std::recursive_mutex> lk(ctx->mu);
-  (void)hipSetDevice(ctx->device);
+// One plan under one tile-shape policy, captured into a graph when asked.
+static int32_t plan_create_one(par2hip_ctx *ctx, const par2hip_job *jobs, size_t njobs, int32_t use_graph, int policy,
+                               par2hip_batch_plan **out) {
   auto *p = new par2hip_batch_plan();
   p->ctx = ctx;
-  int32_t rc = par2hip::plan_build(ctx, jobs, njobs, nullptr, 0, true, p->plan);
+  int32_t rc = par2hip::plan_build(ctx, jobs, njobs, nullptr, 0, true, p->plan, policy);
   if (rc == PAR2HIP_OK && !p->plan.slow.empty())
     rc = par2hip::fail(ctx, PAR2HIP_EINVAL, "a planned batch needs consecutive recovery exponents in every repair job");
   if (rc == PAR2HIP_OK && use_graph) {
@@ -519,7 +529,78 @@ extern "C" int32_t par2hip_batch_plan_create(par2hip_ctx *ctx, const par2hip_job
     plan_free(p);
     return rc;
   }
+  p->tuning.policy = policy;
   *out = p;
+  return PAR2HIP_OK;
+}
+
+// Mean time of `reps` launches after a warm-up one (HIP events on the ctx
+// stream; the plan's outputs are rewritten with the same values each time).
+static int32_t time_plan(par2hip_batch_plan *p, int reps, double *ms) {
+  par2hip_ctx *ctx = p->ctx;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  P2H_CHECK(ctx, hipEventCreate(&e0));
+  P2H_CHECK(ctx, hipEventCreate(&e1));
+  int32_t rc = par2hip_batch_plan_launch(p);
+  (void)hipEventRecord(e0, ctx->stream);
+  for (int i = 0; i < reps && rc == PAR2HIP_OK; i++) rc = par2hip_batch_plan_launch(p);
+  (void)hipEventRecord(e1, ctx->stream);
+  float t = 0;
+  if (rc == PAR2HIP_OK && hipEventSynchronize(e1) == hipSuccess && hipEventElapsedTime(&t, e0, e1) == hipSuccess)
+    *ms = t / reps;
+  else if (rc == PAR2HIP_OK)
+    rc = par2hip::fail(ctx, PAR2HIP_EHIP, "plan timing");
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  return rc;
+}
+
+extern "C" int32_t par2hip_batch_plan_create(par2hip_ctx *ctx, const par2hip_job *jobs, size_t njobs,
+                                             int32_t use_graph, par2hip_batch_plan **out) {
+  if (!ctx || !out || njobs == 0 || !jobs) return PAR2HIP_EINVAL;
+  *out = nullptr;
+  std::lock_guard<std::recursive_mutex> lk(ctx->mu);
+  (void)hipSetDevice(ctx->device);
+  if (!ctx->autotune || ctx->forced_shape >= 0) return plan_create_one(ctx, jobs, njobs, use_graph, par2hip::kBatchShapeDefault, out);
+  // autotuning: every shape policy planned and timed on the batch itself
+  // (the plan is for repeated launches, so its tuning is paid once); the
+  // fastest is kept, the default unless another is at least 2% faster
+  par2hip_batch_plan *best = nullptr;
+  par2hip_batch_plan_tuning info{};
+  info.policies = par2hip::kBatchShapePolicies;
+  double best_ms = 0, ms0 = 0;
+  for (int pol = 0; pol < par2hip::kBatchShapePolicies; pol++) {
+    par2hip_batch_plan *p = nullptr;
+    int32_t rc = plan_create_one(ctx, jobs, njobs, use_graph, pol, &p);
+    double ms = 0;
+    if (rc == PAR2HIP_OK) rc = time_plan(p, 2, &ms);
+    if (rc != PAR2HIP_OK) {
+      if (p) plan_free(p);
+      if (best) plan_free(best);
+      return rc;
+    }
+    info.ms[pol] = ms;
+    (void)par2hip_sync(ctx);  // nothing of the timed launches is left in flight
+    if (pol == 0) {
+      best = p;
+      best_ms = ms0 = ms;
+    } else if (ms < 0.98 * ms0 && ms < best_ms) {
+      plan_free(best);
+      best = p;
+      best_ms = ms;
+    } else {
+      plan_free(p);
+    }
+  }
+  info.policy = best->tuning.policy;
+  best->tuning = info;
+  *out = best;
+  return PAR2HIP_OK;
+}
+
+extern "C" int32_t par2hip_batch_plan_tuning_info(par2hip_batch_plan *p, par2hip_batch_plan_tuning *out) {
+  if (!p || !out) return PAR2HIP_EINVAL;
+  *out = p->tuning;
   return PAR2HIP_OK;
 }
 

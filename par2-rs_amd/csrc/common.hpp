@@ -232,13 +232,30 @@ struct GfTaskDev {
   long words;
   int nsrc, ndst, ldc, accumulate;
   int align;   // byte alignment (2..16) shared by every source and destination row
-  int groups;  // 8-row groups per workgroup (gf_batch_groups)
+  int groups;  // 8-row groups per workgroup (1 or 2)
+  int v;       // words per thread (8, 4, 2 or 1): a work item covers kTpb * v words
   long pair0;  // first (8-row group, source) pair of this task in the batch's table numbering
 };
 struct GfItem {
   int task, tile, rowblk, pad_;
 };
-constexpr int kBatchWordsPerWg = 2048;  // 256 threads x 8 words
+// Tile shape (words per thread v, 8-row groups g; 16 sources per LDS chunk)
+// of one batched task under a policy: a forced shape always wins.  The persistent batch kernel has the shapes with
+// v in {8, 4} and g <= 2 (its LDS holds 16 sources x 2 groups; more variants
+// inlined into the one persistent kernel spill registers): a tuned g = 4 or
+// 32-source shape maps to the same v with g = 2, a narrower v to the default.
+// Shape policies a batch plan is tuned over (par2hip_batch_plan_create with
+// autotuning on times each on the batch itself and keeps the fastest).
+enum BatchShapePolicy {
+  kBatchShapeDefault = 0,  // forced shape, else v = 8 and gf_batch_groups(ndst)
+  kBatchShapeTable = 1,    // the tuned table's shape for each task's class (single-set tuning)
+  kBatchShapeOneGroup = 2, // v = 8, one 8-row group per item
+  kBatchShapeSmallV = 3,   // v = 4 for tasks of at most 64 Ki words (more items per small slice)
+  kBatchShapePolicies = 4
+};
+void batch_tile_shape(const par2hip_ctx *ctx, int policy, int ndst, int nsrc, long words, int align_bytes, int *v,
+                      int *g);
+constexpr int kBatchTpb = 256;  // threads of a batch contraction workgroup
 // Many jobs as one batch.  arena: device working memory for the batch (at
 // least the sum of batch_job_arena_bytes over the jobs plus
 // kBatchArenaExtra), or nullptr for the context's own scratch.

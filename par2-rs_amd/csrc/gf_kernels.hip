@@ -155,6 +155,36 @@ static uint64_t shape_key(const GfProblem &p, int vmax) {
          (uint64_t)bit_length(p.words) << 24;
 }
 
+void batch_tile_shape(const par2hip_ctx *ctx, int policy, int ndst, int nsrc, long words, int align_bytes, int *v,
+                      int *g) {
+  const int vmax = align_bytes >= 16 ? 8 : align_bytes >= 8 ? 4 : align_bytes >= 4 ? 2 : 1;
+  int idx = -1;
+  if (ctx->forced_shape >= 0 && kShapes[ctx->forced_shape].v <= vmax) {
+    idx = ctx->forced_shape;
+  } else if (policy == kBatchShapeTable) {
+    GfProblem p;
+    p.ndst = ndst;
+    p.nsrc = nsrc;
+    p.words = (size_t)words;
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    auto it = g_tuned.find(shape_key(p, vmax));
+    if (it != g_tuned.end()) idx = it->second;
+  } else if (policy == kBatchShapeOneGroup) {
+    idx = shape_index(8, 1, 16);
+  } else if (policy == kBatchShapeSmallV && words <= 65536 && vmax >= 4) {
+    idx = shape_index(4, gf_batch_groups(ndst), 16);
+  }
+  // the batch kernel has v = 8 and 4 (narrower rows take v = 8 with word
+  // accesses, as the default does)
+  if (idx < 0 || kShapes[idx].v < 4) {
+    *v = 8;
+    *g = gf_batch_groups(ndst);
+    return;
+  }
+  *v = kShapes[idx].v;
+  *g = std::min(kShapes[idx].g, 2);
+}
+
 // Times every candidate the alignment allows on the problem itself (the
 // fastest of three launches after a warm-up; the outputs are overwritten with
 // the same values each time).  A shape replaces the default only when it is
@@ -367,6 +397,7 @@ int32_t launch_tables_batch(par2hip_ctx *ctx, hipStream_t s, const GfTaskDev *d_
 }
 
 __global__ void k_reset_cursor(unsigned *cursor) { *cursor = 0u; }
+static_assert(kBatchTpb == kTpb, "batch work items are planned for kTpb threads");
 
 // Persistent: every workgroup takes the next work item from the cursor until
 // none is left (every wave reaches the exit because the cursor only grows).
@@ -384,12 +415,17 @@ __global__ __launch_bounds__(kTpb, 4) void k_gf_lds_batch(const GfTaskDev *__res
     if (it >= nitems) break;
     const GfItem w = items[it];
     const GfTaskDev t = tasks[w.task];
-    if (t.groups == 1)
-      gf_lds_tile<8, 1, 16, true>(lds, t.srcs, t.nsrc, t.dsts, t.ndst, t.tabs, t.words, t.accumulate, w.tile,
-                                  w.rowblk, t.align);
-    else
-      gf_lds_tile<8, 2, 16, true>(lds, t.srcs, t.nsrc, t.dsts, t.ndst, t.tabs, t.words, t.accumulate, w.tile,
-                                  w.rowblk, t.align);
+#define P2H_BATCH_TILE(V, G)                                                                                  \
+  gf_lds_tile<V, G, 16, true>(lds, t.srcs, t.nsrc, t.dsts, t.ndst, t.tabs, t.words, t.accumulate, w.tile, w.rowblk, \
+                              t.align)
+    // the task's tile shape (uniform per item, so the branch is too)
+    switch (t.v * 4 + t.groups) {
+      case 8 * 4 + 1: P2H_BATCH_TILE(8, 1); break;
+      case 4 * 4 + 2: P2H_BATCH_TILE(4, 2); break;
+      case 4 * 4 + 1: P2H_BATCH_TILE(4, 1); break;
+      default: P2H_BATCH_TILE(8, 2); break;
+    }
+#undef P2H_BATCH_TILE
   }
 }
 

@@ -113,6 +113,10 @@ class DeviceInfo(C.Structure):
     _fields_ = [("compute_units", C.c_int32), ("hash_cus", C.c_int32), ("arch", C.c_char * 32)]
 
 
+class PlanTuning(C.Structure):
+    _fields_ = [("policy", C.c_int32), ("policies", C.c_int32), ("ms", C.c_double * 4)]
+
+
 class KernelStat(C.Structure):
     _fields_ = [("name", C.c_char * 48), ("launches", C.c_uint64), ("total_ms", C.c_double)]
 
@@ -175,6 +179,7 @@ def lib():
                 "par2hip_batch_plan_create": ([P, P, S, I32, C.POINTER(P)], I32),
                 "par2hip_batch_plan_launch": ([P], I32),
                 "par2hip_batch_plan_destroy": ([P], I32),
+                "par2hip_batch_plan_tuning_info": ([P, P], I32),
                 "par2hip_server_create": ([I32, C.c_uint64, I32, I32, C.POINTER(P)], I32),
                 "par2hip_server_destroy": ([P], I32),
                 "par2hip_server_now_ms": ([P], C.c_double),
@@ -546,6 +551,13 @@ class BatchPlan:
 
     def launch(self) -> None:
         self.ctx._check(lib().par2hip_batch_plan_launch(self.handle))
+
+    def tuning(self) -> dict:
+        """The tile-shape policy the plan uses and, when it was tuned, the
+        timed launch of every policy (ms)."""
+        t = PlanTuning()
+        self.ctx._check(lib().par2hip_batch_plan_tuning_info(self.handle, C.byref(t)))
+        return {"policy": int(t.policy), "policies": int(t.policies), "ms": [round(float(x), 3) for x in t.ms]}
 
     def close(self) -> None:
         if self.handle is not None:

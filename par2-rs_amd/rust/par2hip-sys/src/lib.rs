@@ -67,6 +67,14 @@ pub struct par2hip_job {
 
 #[repr(C)]
 #[derive(Clone, Copy)]
+pub struct par2hip_batch_plan_tuning {
+    pub policy: i32,
+    pub policies: i32,
+    pub ms: [f64; 4],
+}
+
+#[repr(C)]
+#[derive(Clone, Copy)]
 pub struct par2hip_request {
     pub job: par2hip_job,
     pub hashes: i32,
@@ -270,6 +278,7 @@ extern "C" {
     ) -> i32;
     pub fn par2hip_batch_plan_launch(plan: *mut par2hip_batch_plan) -> i32;
     pub fn par2hip_batch_plan_destroy(plan: *mut par2hip_batch_plan) -> i32;
+    pub fn par2hip_batch_plan_tuning_info(plan: *mut par2hip_batch_plan, out: *mut par2hip_batch_plan_tuning) -> i32;
 
     // ---- serving
     pub fn par2hip_server_create(

@@ -189,3 +189,91 @@ def test_batch_plan_rejects_gapped_repair(ctx):
     # the context still works afterwards
     ctx.batch_run_dev(jobs)
     ctx.sync()
+
+
+SHAPES = [(8, 2, 16), (8, 1, 16), (4, 2, 16), (4, 1, 16), (4, 4, 16), (8, 2, 32), (2, 2, 16), (1, 1, 16)]
+
+
+@pytest.mark.parametrize("shape", SHAPES, ids=lambda s: "v%d_g%d_d%d" % s)
+def test_batch_every_tile_shape(ctx, shape):
+    """Each tile shape the batched contraction runs (v in {8, 4} x g in {1, 2};
+    g = 4 and 32-source shapes map to g = 2, narrower v to the default), forced
+    on a mixed batch (ragged tiles, 2-, 4- and 16-byte aligned rows, folds and
+    main contractions), eager and as a planned graph: the oracle's bytes."""
+    rng = np.random.default_rng(sum(shape))
+    specs = [
+        ("encode", 150, 4096 + 64, 20, True, True),
+        ("encode", 37, 1026, 9, False, True),    # 2-byte aligned rows
+        ("encode", 70, 8196, 17, True, True),    # 4-byte aligned rows
+        ("repair", 90, 6144, 12, True, True),
+        ("repair", 12, 2048, 12, True, True),    # every data slice lost: recovery rows only
+        ("repair", 200, 3000, 9, "shift", True),
+    ]
+    jobs, host = make_jobs(ctx, rng, specs)
+    ctx.force_tile_shape(shape)
+    try:
+        say("batch shape", shape)
+        ctx.batch_run_dev(jobs)
+        ctx.sync()
+        check(ctx, jobs, host)
+        clear_outputs(jobs)
+        plan = ctx.batch_plan(jobs, graph=True)
+        try:
+            plan.launch()
+            ctx.sync()
+            check(ctx, jobs, host)
+        finally:
+            plan.close()
+    finally:
+        ctx.force_tile_shape(None)
+
+
+def test_batch_uses_tuned_shapes(ctx, tmp_path):
+    """With autotuning on, a batched task takes the tuned table's shape for its
+    class (loaded from text here), and the bytes stay the oracle's."""
+    rng = np.random.default_rng(77)
+    specs = [("encode", 300, 16384, 24, True, True), ("repair", 120, 8192, 10, True, True)]
+    jobs, host = make_jobs(ctx, rng, specs)
+    # key = vmax | row groups << 8 | bit_length(nsrc) << 16 | bit_length(words) << 24
+    keys = [8 | 3 << 8 | (300).bit_length() << 16 | (8192).bit_length() << 24,
+            8 | 2 << 8 | (120).bit_length() << 16 | (4096).bit_length() << 24]
+    table = tmp_path / "tuned.txt"
+    table.write_text("".join("%x 4 1 16\n" % k for k in keys))
+    assert par2hip.autotune_load(str(table)) == 2
+    was = ctx.set_autotune(True)
+    try:
+        say("batch tuned shapes")
+        ctx.batch_run_dev(jobs)
+        ctx.sync()
+        check(ctx, jobs, host)
+    finally:
+        ctx.set_autotune(was)
+
+
+def test_batch_plan_autotuned_policies(ctx):
+    """With autotuning on, the plan times every tile-shape policy on the batch
+    itself and keeps the fastest; each policy's plan gives the oracle's bytes."""
+    rng = np.random.default_rng(78)
+    specs = [("encode", 200, 8192, 30, True, True), ("encode", 40, 4096, 5, True, True),
+             ("repair", 150, 16384, 12, True, True), ("repair", 30, 2048, 4, "shift", True)]
+    jobs, host = make_jobs(ctx, rng, specs)
+    was = ctx.set_autotune(True)
+    try:
+        say("plan autotuned")
+        plan = ctx.batch_plan(jobs, graph=True)
+        try:
+            t = plan.tuning()
+            assert t["policies"] == 4 and 0 <= t["policy"] < 4 and all(ms > 0 for ms in t["ms"])
+            clear_outputs(jobs)
+            plan.launch()
+            ctx.sync()
+            check(ctx, jobs, host)
+        finally:
+            plan.close()
+    finally:
+        ctx.set_autotune(was)
+    plan = ctx.batch_plan(jobs, graph=False)  # untuned: no policies timed
+    try:
+        assert plan.tuning()["policies"] == 0
+    finally:
+        plan.close()
