@@ -350,6 +350,47 @@ def bench_micro(args, ctx):
     return out
 
 
+def bench_small_r(args, ctx):
+    """Few recovery slices over the plain-encode input (1024 x 1 MiB in HBM):
+    the contraction alone for R = 1, 2, 4, 8 with the LDS-table and the FP4
+    matrix-core forms.  With so few outputs each launch is bound by reading
+    the 1 GiB once, so the roofline is HBM: (N*S + R*S) / kernel time."""
+    n, s = 1024, 1 << 20
+    data = random_bytes(1234, n * s)
+    d_sl = ctx.alloc(n * s)
+    d_sl.upload(data)
+    d_rec = ctx.alloc(8 * s)
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle
+
+    out = {}
+    prev = ctx.set_gf_kernel(0)
+    for r in args.small_r:
+        exps = np.arange(r, dtype=np.uint32)
+        cols = np.ascontiguousarray(data.reshape(n, s).view(np.uint16)[:, 1000:1032])
+        want = oracle.encode(cols, exps)
+        for k, name in ((1, "lds"), (2, "mfma_fp4")):
+            ctx.set_gf_kernel(k)
+            ctx.encode_dev(d_sl, n, s, exps, d_rec)
+            ctx.sync()
+            exact = bool(np.array_equal(d_rec.download(r * s).reshape(r, s).view(np.uint16)[:, 1000:1032], want))
+            ctx.profile(True)
+            ctx.profile_read()
+            for _ in range(args.steps):
+                ctx.encode_dev(d_sl, n, s, exps, d_rec)
+            ctx.sync()
+            st = ctx.profile_read()
+            ctx.profile(False)
+            gf = next(v for key, v in st.items() if key.startswith("gf_muladd"))
+            ms = gf[1] / max(gf[0], 1)
+            algo = n * s + r * s
+            out[f"R{r}_{name}"] = {"kernel_ms": round(ms, 4), "GBps": round(algo / (ms * 1e-3) / 1e9, 1),
+                                   "hbm_frac": round(algo / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4), "exact": exact}
+            log("small R", r, name, out[f"R{r}_{name}"])
+    ctx.set_gf_kernel(prev)
+    return out
+
+
 def bench_repair(args, ctx):
     """Plain repair: 2000 x 512 KiB slices (~1 GiB), 100 erased at seeded
     positions, 100 recovery slices present.  One step = par2hip_repair_set_dev:
@@ -933,7 +974,7 @@ def main():
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--workload", choices=["encode", "repair", "micro", "batched", "served", "copy", "create",
-                                           "repair_files", "encode_host", "inverse"],
+                                           "repair_files", "encode_host", "inverse", "small_r"],
                     default="encode")
     ap.add_argument("--hbm-budget", type=int, default=200 * 10**9, help="served: server HBM budget in bytes")
     ap.add_argument("--lanes", type=int, default=2, help="served: concurrent batches")
@@ -945,6 +986,7 @@ def main():
     ap.add_argument("--create-file-mib", type=int, default=64, help="create / repair_files: MiB per input file")
     ap.add_argument("--create-block", type=int, default=0, help="create / repair_files: block size in bytes (0: automatic)")
     ap.add_argument("--create-redundancy", type=float, default=10.0, help="create / repair_files: redundancy percent")
+    ap.add_argument("--small-r", type=int, nargs="+", default=[1, 2, 4, 8], help="small_r: recovery counts")
     ap.add_argument("--inverse-sizes", type=int, nargs="+", default=[1000, 5000, 12000],
                     help="inverse: erasure counts m (up to 32768)")
     ap.add_argument("--seed", type=int, default=4)
@@ -992,6 +1034,10 @@ def main():
         dt = (time.perf_counter() - t0) / args.steps
         print(json.dumps({"copy": {"bytes_read": nb, "bytes_written": nb, "ms": round(dt * 1e3, 4),
                                    "GBps_rw": round(2 * nb / dt / 1e9, 1)}}), flush=True)
+        ctx.close()
+        return
+    if args.workload == "small_r":
+        print(json.dumps({"small_r": bench_small_r(args, ctx)}), flush=True)
         ctx.close()
         return
     if args.workload == "inverse":

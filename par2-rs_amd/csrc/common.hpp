@@ -306,7 +306,8 @@ int32_t noalloc_init(par2hip_ctx *ctx);
 int32_t run_gf_mfma(par2hip_ctx *ctx, hipStream_t s, const GfProblem &p, bool int8);
 // Kernel used when the caller leaves the choice to the library and the call
 // has no plan of its own (an encode beside MD5 chains plans kernel and CUs):
-// the LDS-table contraction, the faster one alone.
+// the LDS-table contraction, the faster one alone above 4 outputs (run_gf
+// takes the FP4 form's 4-output tile at or below).
 constexpr int kAutoGfKernel = 1;
 
 // Slice hashes.

@@ -47,11 +47,15 @@ typedef int v16i __attribute__((ext_vector_type(16)));
 typedef float v16f __attribute__((ext_vector_type(16)));
 
 constexpr int kMfWaves = 8;              // waves per workgroup (two per SIMD)
-constexpr int kMfRows = 4;               // A fragments (output pairs) per workgroup -> 8 outputs
-constexpr int kMfCols = 2;               // B fragments (32 words) per wave
-constexpr int kMfWgWords = kMfWaves * kMfCols * 32;  // 512 words per workgroup
-constexpr int kMfKc = 8;                 // K-steps (4 sources each) per LDS stage -> 32 sources
 constexpr int kMfThreads = 64 * kMfWaves;
+// Workgroup tile shapes: kRows A fragments (output pairs), kCols B fragments
+// (32 words) per wave, kKc K-steps (4 sources each) per LDS stage.  Every
+// shape holds kRows * kCols = 8 accumulator tiles (128 registers) per lane.
+//   default: 4 x 2 x 8 -> 8 outputs x 512 words, 32 sources per stage
+//   small:   2 x 4 x 4 -> 4 outputs x 1024 words, 16 sources per stage: at
+//            most 4 outputs fill its rows, so no MFMA multiplies padded zero
+//            rows, and each input word is read by one workgroup only
+constexpr int kMfRows = 4, kMfCols = 2, kMfKc = 8;
 constexpr uint32_t kNib = 0x22222222u;
 
 __device__ __forceinline__ uint16_t mul_x(uint16_t v) {
@@ -122,16 +126,25 @@ __device__ __forceinline__ v8i expand_fp4(uint32_t y) {
 // tiles tile0.. whose words may run past the end of the rows (only the last
 // tile column), loaded word by word with checks; the main launch has no such
 // code, so its loads carry no branches.
-template <bool kI8, bool kRagged>
+template <bool kI8, bool kRagged, int kRows = kMfRows, int kCols = kMfCols, int kKc = kMfKc>
 __global__ __launch_bounds__(kMfThreads, 1) void k_gf_mfma(const uint16_t *const *__restrict__ srcs, int nsrc,
                                                            uint16_t *const *__restrict__ dsts, int ndst,
                                                            const uint32_t *__restrict__ apk, int nmf, int nks,
                                                            long words, int accumulate, int ntiles, int nrowblk,
                                                            int tile0) {
+  constexpr int kWgWords = kMfWaves * kCols * 32;  // words per workgroup
+  constexpr int kUnits = kWgWords / 8;             // 16-byte units of one source row of the tile
+  constexpr int kSrcPass = kMfThreads / kUnits;    // sources one pass of the threads stages
+  constexpr int kSrcStage = 4 * kKc;               // sources per stage
+  constexpr int kPasses = kSrcStage / kSrcPass;
+  constexpr int kApt = kKc * kRows * 64 / kMfThreads;  // packed A words per thread and stage
+  static_assert(kRows * kCols == 8 && kUnits >= 64 && kPasses * kSrcPass == kSrcStage && kPasses == 4 &&
+                    (kApt == 1 || kApt == 2 || kApt == 4),
+                "tile shape");
   using AUnit = typename std::conditional<kI8, uint32_t, uint4>::type;
   using Acc = typename std::conditional<kI8, v16i, v16f>::type;
-  __shared__ uint4 sB[2][4 * kMfKc][kMfWgWords / 8];   // [buf][source][16-byte unit], 64 KiB
-  __shared__ AUnit sA[2][kMfKc][kMfRows][64];         // [buf][ks][fragment][lane]: FP4 64 KiB, packed 8 KiB
+  __shared__ uint4 sB[2][kSrcStage][kUnits];  // [buf][source][16-byte unit], 64 KiB
+  __shared__ AUnit sA[2][kKc][kRows][64];     // [buf][ks][fragment][lane]: FP4 64 / 16 KiB, packed 8 KiB
 
   const int L = blockIdx.x;
   const int xcd = L & 7, q = L >> 3;
@@ -144,13 +157,13 @@ __global__ __launch_bounds__(kMfThreads, 1) void k_gf_mfma(const uint16_t *const
   const int tid = threadIdx.x;
   const int wave = tid >> 6, lane = tid & 63;
   const int n = lane & 31, g = lane >> 5;
-  const int mf0 = rowblk * kMfRows;
-  const long tile_w0 = (long)tile * kMfWgWords;
-  const int nchunks = (nks + kMfKc - 1) / kMfKc;
+  const int mf0 = rowblk * kRows;
+  const long tile_w0 = (long)tile * kWgWords;
+  const int nchunks = (nks + kKc - 1) / kKc;
 
-  // staging roles.  B: unit tid + 512k -> source (tid>>6) + 8k, 16-byte column tid&63.
-  // A: thread tid owns packed words 4*tid..4*tid+3 of the stage: fragment tid>>7,
-  // K-step (tid>>4)&7, lanes 4*(tid&15)..+3.
+  // staging roles.  B: unit tid + 512k -> source tid/kUnits + kSrcPass*k,
+  // 16-byte column tid%kUnits.  A: thread tid owns packed words
+  // kApt*tid..+kApt-1 of the stage, ordered (fragment, K-step, lane).
   //
   // The loads of stage c+1 are issued before stage c's MFMAs and consumed only
   // by store_stage after them, so they must not be waited for in between: no
@@ -159,19 +172,20 @@ __global__ __launch_bounds__(kMfThreads, 1) void k_gf_mfma(const uint16_t *const
   // sources past the last one and A words past the last fragment or K-step
   // are loaded from clamped (valid) addresses and zeroed in store_stage.  Only
   // a ragged last word tile takes the checked per-word path.
-  const int wv = __builtin_amdgcn_readfirstlane(wave);
+  // this thread's staging source (wave-uniform: kUnits is a multiple of 64)
+  const int wsrc = __builtin_amdgcn_readfirstlane(tid / kUnits);
   const P2H_GLOBAL uint16_t *pn[4];  // source rows of the next stage to load
   auto fetch_ptrs = [&](int c) {
-    pn[0] = gptr(gptr(srcs)[min(32 * c + wv, nsrc - 1)]);
-    pn[1] = gptr(gptr(srcs)[min(32 * c + wv + 8, nsrc - 1)]);
-    pn[2] = gptr(gptr(srcs)[min(32 * c + wv + 16, nsrc - 1)]);
-    pn[3] = gptr(gptr(srcs)[min(32 * c + wv + 24, nsrc - 1)]);
+#pragma unroll
+    for (int k = 0; k < 4; k++) pn[k] = gptr(gptr(srcs)[min(kSrcStage * c + wsrc + kSrcPass * k, nsrc - 1)]);
   };
-  u32x4 rb0, rb1, rb2, rb3, ra;
+  using AVec = typename std::conditional<kApt == 4, u32x4, typename std::conditional<kApt == 2, u32x2, uint32_t>::type>::type;
+  u32x4 rb0, rb1, rb2, rb3;
+  AVec ra;
   uint32_t ok_mask;  // bit k: source row k of the stage exists; bit 4: the A words exist
   auto load_b = [&](int c, int k, const P2H_GLOBAL uint16_t *src) -> u32x4 {
-    const long w = tile_w0 + 8 * (tid & 63);
-    if (32 * c + wv + 8 * k < nsrc) ok_mask |= 1u << k;
+    const long w = tile_w0 + 8 * (tid % kUnits);
+    if (kSrcStage * c + wsrc + kSrcPass * k < nsrc) ok_mask |= 1u << k;
     if constexpr (!kRagged) {
       return *reinterpret_cast<const P2H_GLOBAL u32x4 *>(src + w);
     } else {
@@ -190,11 +204,11 @@ __global__ __launch_bounds__(kMfThreads, 1) void k_gf_mfma(const uint16_t *const
     rb2 = load_b(c, 2, pn[2]);
     rb3 = load_b(c, 3, pn[3]);
     {
-      const int frag = tid >> 7, ksl = (tid >> 4) & 7, l4 = 4 * (tid & 15);
-      const int ks = kMfKc * c + ksl;
+      const int e = kApt * tid, frag = e / (kKc * 64), ksl = (e / 64) % kKc, l0 = e % 64;
+      const int ks = kKc * c + ksl;
       if (mf0 + frag < nmf && ks < nks) ok_mask |= 1u << 4;
-      const long idx = ((long)min(mf0 + frag, nmf - 1) * nks + min(ks, nks - 1)) * 64 + l4;
-      ra = *reinterpret_cast<const P2H_GLOBAL u32x4 *>(gptr(apk) + idx);
+      const long idx = ((long)min(mf0 + frag, nmf - 1) * nks + min(ks, nks - 1)) * 64 + l0;
+      ra = *reinterpret_cast<const P2H_GLOBAL AVec *>(gptr(apk) + idx);
     }
     fetch_ptrs(c + 1);
   };
@@ -203,25 +217,27 @@ __global__ __launch_bounds__(kMfThreads, 1) void k_gf_mfma(const uint16_t *const
     const u32x4 b[4] = {(ok_mask & 1u) ? rb0 : zero, (ok_mask & 2u) ? rb1 : zero, (ok_mask & 4u) ? rb2 : zero,
                         (ok_mask & 8u) ? rb3 : zero};
 #pragma unroll
-    for (int k = 0; k < 4; k++) sB[buf][(tid >> 6) + 8 * k][tid & 63] = make_uint4(b[k].x, b[k].y, b[k].z, b[k].w);
-    const int frag = tid >> 7, ksl = (tid >> 4) & 7, l4 = 4 * (tid & 15);
-    const u32x4 a = (ok_mask & 16u) ? ra : zero;
-    const uint32_t y[4] = {a.x, a.y, a.z, a.w};
+    for (int k = 0; k < 4; k++)
+      sB[buf][tid / kUnits + kSrcPass * k][tid % kUnits] = make_uint4(b[k].x, b[k].y, b[k].z, b[k].w);
+    const int e0 = kApt * tid, frag = e0 / (kKc * 64), ksl = (e0 / 64) % kKc, l0 = e0 % 64;
+    uint32_t y[kApt];
+    __builtin_memcpy(y, &ra, sizeof y);
 #pragma unroll
-    for (int e = 0; e < 4; e++) {
+    for (int e = 0; e < kApt; e++) {
+      if (!(ok_mask & 16u)) y[e] = 0;
       if constexpr (kI8)
-        sA[buf][ksl][frag][l4 + e] = y[e];
+        sA[buf][ksl][frag][l0 + e] = y[e];
       else
-        sA[buf][ksl][frag][l4 + e] =
+        sA[buf][ksl][frag][l0 + e] =
             make_uint4((y[e] << 1) & kNib, y[e] & kNib, (y[e] >> 1) & kNib, (y[e] >> 2) & kNib);
     }
   };
 
-  Acc acc[kMfRows][kMfCols];
+  Acc acc[kRows][kCols];
 #pragma unroll
-  for (int i = 0; i < kMfRows; i++)
+  for (int i = 0; i < kRows; i++)
 #pragma unroll
-    for (int f = 0; f < kMfCols; f++)
+    for (int f = 0; f < kCols; f++)
 #pragma unroll
       for (int k = 0; k < 16; k++) acc[i][f][k] = 0;
 
@@ -229,16 +245,17 @@ __global__ __launch_bounds__(kMfThreads, 1) void k_gf_mfma(const uint16_t *const
   load_stage(0);
   store_stage(0);
   __syncthreads();
-  const int bcol = wave * (kMfCols * 32) + 2 * n;  // this lane's first word inside the tile
+  const int bcol = wave * (kCols * 32) + kCols * n;  // this lane's first word inside the tile
   for (int c = 0; c < nchunks; c++) {
     const int buf = c & 1;
     if (c + 1 < nchunks) load_stage(c + 1);
-    // all 8 K-steps of the stage (K-steps past the last source carry zero
+    // all K-steps of the stage (K-steps past the last source carry zero
     // operands); each wave expands its own B columns, A comes pre-expanded.
     // Kept rolled: unrolled, the operands of 8 steps spill at 2 waves/SIMD.
 #pragma unroll 1
-    for (int ksl = 0; ksl < kMfKc; ksl++) {
+    for (int ksl = 0; ksl < kKc; ksl++) {
       if constexpr (kI8) {
+        static_assert(!kI8 || (kRows == kMfRows && kCols == kMfCols && kKc == kMfKc), "int8: default shape");
 #pragma unroll
         for (int hh = 0; hh < 2; hh++) {
           const uint32_t x0 = reinterpret_cast<const uint32_t *>(&sB[buf][4 * ksl + 2 * hh][0])[bcol >> 1];
@@ -255,17 +272,25 @@ __global__ __launch_bounds__(kMfThreads, 1) void k_gf_mfma(const uint16_t *const
           }
         }
       } else {
-        const uint32_t x0 = reinterpret_cast<const uint32_t *>(&sB[buf][4 * ksl + 2 * g][0])[bcol >> 1];
-        const uint32_t x1 = reinterpret_cast<const uint32_t *>(&sB[buf][4 * ksl + 2 * g + 1][0])[bcol >> 1];
-        v8i b[kMfCols];
-        b[0] = expand_fp4(__builtin_amdgcn_perm(x1, x0, 0x05040100u));  // word 0: src 2g low, 2g+1 high
-        b[1] = expand_fp4(__builtin_amdgcn_perm(x1, x0, 0x07060302u));  // word 1
+        // words bcol..bcol+kCols-1 of sources 4ksl+2g (x0) and 4ksl+2g+1 (x1)
+        uint32_t x0[kCols / 2], x1[kCols / 2];
 #pragma unroll
-        for (int i = 0; i < kMfRows; i++) {
+        for (int h = 0; h < kCols / 2; h++) {
+          x0[h] = reinterpret_cast<const uint32_t *>(&sB[buf][4 * ksl + 2 * g][0])[(bcol >> 1) + h];
+          x1[h] = reinterpret_cast<const uint32_t *>(&sB[buf][4 * ksl + 2 * g + 1][0])[(bcol >> 1) + h];
+        }
+        v8i b[kCols];
+#pragma unroll
+        for (int h = 0; h < kCols / 2; h++) {
+          b[2 * h] = expand_fp4(__builtin_amdgcn_perm(x1[h], x0[h], 0x05040100u));      // even word: src 2g low, 2g+1 high
+          b[2 * h + 1] = expand_fp4(__builtin_amdgcn_perm(x1[h], x0[h], 0x07060302u));  // odd word
+        }
+#pragma unroll
+        for (int i = 0; i < kRows; i++) {
           const uint4 t = sA[buf][ksl][i][lane];
           const v8i a = v8i{(int)t.x, (int)t.y, (int)t.z, (int)t.w, 0, 0, 0, 0};
 #pragma unroll
-          for (int f = 0; f < kMfCols; f++)
+          for (int f = 0; f < kCols; f++)
             acc[i][f] = __builtin_amdgcn_mfma_scale_f32_32x32x64_f8f6f4(a, b[f], acc[i][f], 4, 4, 0, 127, 0, 127);
         }
       }
@@ -276,28 +301,33 @@ __global__ __launch_bounds__(kMfThreads, 1) void k_gf_mfma(const uint16_t *const
     __syncthreads();
   }
 
-  // parity of each accumulator = bit q of output slice 2*(mf0+i) + g at words w, w+1
+  // parity of each accumulator = bit q of output slice 2*(mf0+i) + g at words w..w+kCols-1
   const long w = tile_w0 + bcol;
   if (w >= words) return;
 #pragma unroll
-  for (int i = 0; i < kMfRows; i++) {
+  for (int i = 0; i < kRows; i++) {
     const int r = 2 * (mf0 + i) + g;
     if (r >= ndst) continue;
-    uint32_t v[kMfCols];
+    uint32_t v[kCols];
 #pragma unroll
-    for (int f = 0; f < kMfCols; f++) {
+    for (int f = 0; f < kCols; f++) {
       uint32_t word = 0;
 #pragma unroll
       for (int k = 0; k < 16; k++) word |= ((uint32_t)acc[i][f][k] & 1u) << k;  // exact integer sums
       v[f] = word;
     }
     P2H_GLOBAL uint16_t *p = gptr(gptr(dsts)[r]);
-    if (w + 2 <= words) {
-      uint32_t out = v[0] | (v[1] << 16);
-      if (accumulate) out ^= *reinterpret_cast<const P2H_GLOBAL uint32_t *>(p + w);
-      *reinterpret_cast<P2H_GLOBAL uint32_t *>(p + w) = out;
+    if (w + kCols <= words) {  // rows of the matrix-core form are 16-byte aligned
+#pragma unroll
+      for (int h = 0; h < kCols / 2; h++) {
+        uint32_t out = v[2 * h] | (v[2 * h + 1] << 16);
+        if (accumulate) out ^= *reinterpret_cast<const P2H_GLOBAL uint32_t *>(p + w + 2 * h);
+        *reinterpret_cast<P2H_GLOBAL uint32_t *>(p + w + 2 * h) = out;
+      }
     } else {
-      p[w] = accumulate ? (uint16_t)(p[w] ^ v[0]) : (uint16_t)v[0];
+#pragma unroll
+      for (int f = 0; f < kCols; f++)
+        if (w + f < words) p[w + f] = accumulate ? (uint16_t)(p[w + f] ^ v[f]) : (uint16_t)v[f];
     }
   }
 }
@@ -316,9 +346,13 @@ int32_t run_gf_mfma(par2hip_ctx *ctx, hipStream_t s, const GfProblem &p, bool in
                        p.ndst, p.nsrc, p.ldc ? p.ldc : p.nsrc, nmf, nks);
     P2H_CHECK(ctx, hipGetLastError());
   }
-  const int ntiles = (int)((p.words + kMfWgWords - 1) / kMfWgWords);
-  const int nfull = (int)(p.words / kMfWgWords);  // tiles whose words all lie inside the rows
-  const int nrowblk = (nmf + kMfRows - 1) / kMfRows;
+  // at most 4 outputs: the small tile shape (4 outputs x 1024 words)
+  const bool small = !int8 && nmf <= 2;
+  const int rows = small ? 2 : kMfRows;
+  const long wg_words = small ? 1024 : kMfWaves * kMfCols * 32;
+  const int ntiles = (int)((p.words + wg_words - 1) / wg_words);
+  const int nfull = (int)(p.words / wg_words);  // tiles whose words all lie inside the rows
+  const int nrowblk = (nmf + rows - 1) / rows;
   const long nwg = 8L * ((ntiles + 7) / 8) * nrowblk;
   if (nwg > 0x7fffffffL) return fail(ctx, PAR2HIP_EINVAL, "GF problem too large for one launch");
   ScopedTiming t(ctx, p.tag ? p.tag : (int8 ? "gf_muladd_mfma_i8" : "gf_muladd_mfma"), s);
@@ -330,6 +364,9 @@ int32_t run_gf_mfma(par2hip_ctx *ctx, hipStream_t s, const GfProblem &p, bool in
   if (int8) {
     if (nfull) launch(k_gf_mfma<true, false>, nfull, 0);
     if (ntiles > nfull) launch(k_gf_mfma<true, true>, ntiles - nfull, nfull);
+  } else if (small) {
+    if (nfull) launch(k_gf_mfma<false, false, 2, 4, 4>, nfull, 0);
+    if (ntiles > nfull) launch(k_gf_mfma<false, true, 2, 4, 4>, ntiles - nfull, nfull);
   } else {
     if (nfull) launch(k_gf_mfma<false, false>, nfull, 0);
     if (ntiles > nfull) launch(k_gf_mfma<false, true>, ntiles - nfull, nfull);

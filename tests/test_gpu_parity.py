@@ -532,6 +532,33 @@ def test_auto_choice_short_chains_keep_lds(ctx):
     assert np.array_equal(rec.view(np.uint16)[:, 1000:1064], oracle.encode(cols, np.arange(r)))
 
 
+def test_auto_choice_few_outputs_take_fp4_tile(ctx):
+    """At most 4 outputs with 16-byte aligned rows: the automatic choice is the
+    FP4 form's 4-output tile (each input word read once), beside the MD5
+    chains too; bit-exact against the oracle."""
+    rng = np.random.default_rng(23)
+    for n, s, r in [(300, 65536, 4), (64, 8192, 1)]:
+        data = rng.integers(0, 256, (n, s), dtype=np.uint8)
+        say("auto choice, few outputs", n, s, r)
+        d_sl, d_rec = ctx.alloc(n * s), ctx.alloc(r * s)
+        d_sl.upload(data.reshape(-1))
+        prev = ctx.set_gf_kernel(0)
+        ctx.profile(True)
+        ctx.profile_read()
+        try:
+            ctx.encode_dev(d_sl, n, s, np.arange(r, dtype=np.uint32), d_rec)
+            ctx.sync()
+            stats = ctx.profile_read()
+        finally:
+            ctx.profile(False)
+            ctx.set_gf_kernel(prev)
+        assert "gf_muladd_mfma" in stats and "gf_muladd_lds" not in stats, stats
+        rec = d_rec.download().reshape(r, s)
+        assert np.array_equal(rec, oracle.encode(data.view(np.uint16), np.arange(r)).view(np.uint8).reshape(r, s))
+        rec2, _, _ = _encode_dev_profiled(ctx, data, r)  # with the hashes beside it
+        assert np.array_equal(rec2, rec)
+
+
 def test_max_sources_32768(kctx):
     """The largest set PAR2's constants allow (32768 input slices, par2_rs.rs:64-76
     with n up to 32768): encode with exponents up to the field's order against the

@@ -75,3 +75,41 @@ def test_autotune_stays_exact_and_fills_table(ctx, tmp_path):
         assert par2hip.autotune_load(str(path)) == len(rows)
     finally:
         ctx.set_autotune(prev)
+
+
+# (n, slice bytes, r) for the matrix-core form's tile shapes: at most 4
+# outputs take the 4-output x 1024-word shape (16 sources per stage), more
+# the 8-output x 512-word one; ragged last tiles, sources not a multiple of a
+# stage, one source, and the exact tile sizes
+MFMA_PROBLEMS = [(1, 2048, 1), (5, 2048, 2), (16, 4096, 3), (17, 2080, 4), (100, 6176, 4), (33, 1040, 3),
+                 (40, 3088, 5), (7, 1024, 8), (50, 4112, 9)]
+
+
+@pytest.mark.parametrize("kernel", [2, 3], ids=["fp4", "int8"])
+def test_mfma_tile_shapes_match_oracle(ctx, kernel):
+    rng = np.random.default_rng(kernel)
+    prev = ctx.set_gf_kernel(kernel)
+    try:
+        for n, s, r in MFMA_PROBLEMS:
+            data = rng.integers(0, 256, (n, s), dtype=np.uint8)
+            exps = np.sort(rng.choice(400, r, replace=False)).astype(np.uint32)
+            say("mfma", kernel, "problem", n, s, r)
+            got = ctx.encode(data, exps)
+            want = oracle.encode(data.view(np.uint16), exps).view(np.uint8).reshape(r, s)
+            assert np.array_equal(got, want), f"kernel {kernel} problem {(n, s, r)}"
+            if n > r:
+                missing = np.sort(rng.choice(n, r, replace=False)).astype(np.uint32)
+                out = ctx.reconstruct(data, missing, got, exps)
+                assert np.array_equal(out, data[missing]), f"kernel {kernel} repair {(n, s, r)}"
+        # accumulating contraction (the strategy's region muladd) with <= 4 rows
+        s = par2hip.HipStrategy(ctx)
+        srcs = [rng.integers(0, 65536, 3000).astype(np.uint16) for _ in range(19)]
+        dsts = [rng.integers(0, 65536, 3000).astype(np.uint16) for _ in range(3)]
+        coeff = rng.integers(0, 65536, (3, 19)).astype(np.uint16)
+        want = [d.copy() for d in dsts]
+        oracle.muladd_region(want, srcs, coeff, 8, 2056)
+        s.muladd_region(dsts, srcs, list(coeff), 8, 2056)
+        for a, b in zip(dsts, want):
+            assert np.array_equal(a, b)
+    finally:
+        ctx.set_gf_kernel(prev)
