@@ -191,6 +191,7 @@ struct GfProblem {
   int align_words;              // every pointer and `words` stride aligned to this many words
   const char *tag = nullptr;    // kernel name for timing (default: gf_muladd_<variant>)
   int kernel = 0;               // 1 LDS tables, 2 / 3 MFMA FP4 / int8; 0: the context's choice
+  bool beside_chains = false;   // runs beside MD5 chains on the other CUs (FP4 tile shape choice)
 };
 // Rate of the bit-sliced FP4 contraction per CU (binary multiply-adds per
 // second) measured on the 1 GiB / R=102 encode: 1.40e13 over 5.02 ms on 256

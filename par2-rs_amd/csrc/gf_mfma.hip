@@ -48,13 +48,24 @@ typedef float v16f __attribute__((ext_vector_type(16)));
 
 constexpr int kMfWaves = 8;              // waves per workgroup (two per SIMD)
 constexpr int kMfThreads = 64 * kMfWaves;
-// Workgroup tile shapes: kRows A fragments (output pairs), kCols B fragments
-// (32 words) per wave, kKc K-steps (4 sources each) per LDS stage.  Every
+// Workgroup tile shapes: the 8 waves form kWR row groups x 8/kWR column
+// groups; each wave holds kRows A fragments (output pairs) x kCols B
+// fragments (32 words), kKc K-steps (4 sources each) per LDS stage.  Every
 // shape holds kRows * kCols = 8 accumulator tiles (128 registers) per lane.
-//   default: 4 x 2 x 8 -> 8 outputs x 512 words, 32 sources per stage
-//   small:   2 x 4 x 4 -> 4 outputs x 1024 words, 16 sources per stage: at
-//            most 4 outputs fill its rows, so no MFMA multiplies padded zero
-//            rows, and each input word is read by one workgroup only
+//   wide  (1, 4x2, 8): 8 outputs x 512 words, 32 sources per stage; the int8
+//         form's only shape
+//   small (1, 2x4, 4): 4 outputs x 1024 words, 16 sources per stage: at most
+//         4 outputs fill its rows, so no MFMA multiplies padded zero rows, and
+//         each input word is read by one workgroup only
+//   split (2, 2x4, 8): 8 outputs x 512 words; a wave reads 2 A fragments per
+//         K-step instead of 4, half the LDS traffic of A per MFMA.  Alone the
+//         fastest: 1024 x 1 MiB, R=102 in 5.17 ms (wide 5.71 ms), 2000 x
+//         512 KiB, R=100 in 4.89 ms (wide 5.46 ms)
+//   tall  (4, 2x4, 4): 16 outputs x 256 words, 16 sources per stage; the
+//         slowest alone (6.20 ms on 1024 x 1 MiB, R=102) but the one that
+//         draws the least from the MD5 chains beside it: that encode step
+//         9.49 ms (wide 9.54 ms, split 9.93 ms, the chains slowing with the
+//         contraction's clock demand)
 constexpr int kMfRows = 4, kMfCols = 2, kMfKc = 8;
 constexpr uint32_t kNib = 0x22222222u;
 
@@ -126,25 +137,31 @@ __device__ __forceinline__ v8i expand_fp4(uint32_t y) {
 // tiles tile0.. whose words may run past the end of the rows (only the last
 // tile column), loaded word by word with checks; the main launch has no such
 // code, so its loads carry no branches.
-template <bool kI8, bool kRagged, int kRows = kMfRows, int kCols = kMfCols, int kKc = kMfKc>
+template <bool kI8, bool kRagged, int kWR = 1, int kRows = kMfRows, int kCols = kMfCols, int kKc = kMfKc>
 __global__ __launch_bounds__(kMfThreads, 1) void k_gf_mfma(const uint16_t *const *__restrict__ srcs, int nsrc,
                                                            uint16_t *const *__restrict__ dsts, int ndst,
                                                            const uint32_t *__restrict__ apk, int nmf, int nks,
                                                            long words, int accumulate, int ntiles, int nrowblk,
                                                            int tile0) {
-  constexpr int kWgWords = kMfWaves * kCols * 32;  // words per workgroup
+  // the 8 waves form kWR row groups x kWC column groups; each wave computes
+  // kRows A fragments x kCols B fragments
+  constexpr int kWC = kMfWaves / kWR;
+  constexpr int kFrags = kWR * kRows;              // A fragments (output pairs) per workgroup
+  constexpr int kWgWords = kWC * kCols * 32;       // words per workgroup
   constexpr int kUnits = kWgWords / 8;             // 16-byte units of one source row of the tile
   constexpr int kSrcPass = kMfThreads / kUnits;    // sources one pass of the threads stages
   constexpr int kSrcStage = 4 * kKc;               // sources per stage
   constexpr int kPasses = kSrcStage / kSrcPass;
-  constexpr int kApt = kKc * kRows * 64 / kMfThreads;  // packed A words per thread and stage
-  static_assert(kRows * kCols == 8 && kUnits >= 64 && kPasses * kSrcPass == kSrcStage && kPasses == 4 &&
+  constexpr int kSrcWave = kUnits >= 64 ? 1 : 64 / kUnits;  // sources one wave stages per pass
+  constexpr int kApt = kKc * kFrags * 64 / kMfThreads;      // packed A words per thread and stage
+  static_assert(kRows * kCols == 8 && kWR * kWC == kMfWaves && (kUnits == 32 || kUnits % 64 == 0) &&
+                    kPasses * kSrcPass == kSrcStage && kPasses >= 1 && kPasses <= 4 &&
                     (kApt == 1 || kApt == 2 || kApt == 4),
                 "tile shape");
   using AUnit = typename std::conditional<kI8, uint32_t, uint4>::type;
   using Acc = typename std::conditional<kI8, v16i, v16f>::type;
-  __shared__ uint4 sB[2][kSrcStage][kUnits];  // [buf][source][16-byte unit], 64 KiB
-  __shared__ AUnit sA[2][kKc][kRows][64];     // [buf][ks][fragment][lane]: FP4 64 / 16 KiB, packed 8 KiB
+  __shared__ uint4 sB[2][kSrcStage][kUnits];  // [buf][source][16-byte unit]
+  __shared__ AUnit sA[2][kKc][kFrags][64];    // [buf][ks][fragment][lane]
 
   const int L = blockIdx.x;
   const int xcd = L & 7, q = L >> 3;
@@ -157,7 +174,8 @@ __global__ __launch_bounds__(kMfThreads, 1) void k_gf_mfma(const uint16_t *const
   const int tid = threadIdx.x;
   const int wave = tid >> 6, lane = tid & 63;
   const int n = lane & 31, g = lane >> 5;
-  const int mf0 = rowblk * kRows;
+  const int wr = wave / kWC, wc = wave % kWC;
+  const int mf0 = rowblk * kFrags;
   const long tile_w0 = (long)tile * kWgWords;
   const int nchunks = (nks + kKc - 1) / kKc;
 
@@ -167,25 +185,31 @@ __global__ __launch_bounds__(kMfThreads, 1) void k_gf_mfma(const uint16_t *const
   //
   // The loads of stage c+1 are issued before stage c's MFMAs and consumed only
   // by store_stage after them, so they must not be waited for in between: no
-  // load sits behind a branch or feeds a select before store_stage.  The four
-  // source pointers of a stage are wave-uniform and fetched one stage ahead;
+  // load sits behind a branch or feeds a select before store_stage.  The
+  // source pointers of a stage are wave-uniform (one or two per wave and
+  // pass, the lane half picking between two) and fetched one stage ahead;
   // sources past the last one and A words past the last fragment or K-step
   // are loaded from clamped (valid) addresses and zeroed in store_stage.  Only
   // a ragged last word tile takes the checked per-word path.
-  // this thread's staging source (wave-uniform: kUnits is a multiple of 64)
-  const int wsrc = __builtin_amdgcn_readfirstlane(tid / kUnits);
-  const P2H_GLOBAL uint16_t *pn[4];  // source rows of the next stage to load
+  const int wsrc = __builtin_amdgcn_readfirstlane(wave * 64 / kUnits);  // the wave's first staging source
+  const int lsrc = kSrcWave == 2 ? (lane >> 5) : 0;                    // this lane's source within the wave
+  const P2H_GLOBAL uint16_t *pn[kPasses][kSrcWave];  // source rows of the next stage to load
   auto fetch_ptrs = [&](int c) {
 #pragma unroll
-    for (int k = 0; k < 4; k++) pn[k] = gptr(gptr(srcs)[min(kSrcStage * c + wsrc + kSrcPass * k, nsrc - 1)]);
+    for (int k = 0; k < kPasses; k++)
+#pragma unroll
+      for (int j = 0; j < kSrcWave; j++)
+        pn[k][j] = gptr(gptr(srcs)[min(kSrcStage * c + wsrc + j + kSrcPass * k, nsrc - 1)]);
   };
   using AVec = typename std::conditional<kApt == 4, u32x4, typename std::conditional<kApt == 2, u32x2, uint32_t>::type>::type;
-  u32x4 rb0, rb1, rb2, rb3;
+  u32x4 rb[kPasses];
   AVec ra;
-  uint32_t ok_mask;  // bit k: source row k of the stage exists; bit 4: the A words exist
-  auto load_b = [&](int c, int k, const P2H_GLOBAL uint16_t *src) -> u32x4 {
+  uint32_t ok_mask;  // bit k: this thread's source row of pass k exists; bit 8: its A words exist
+  auto load_b = [&](int c, int k) -> u32x4 {
+    const P2H_GLOBAL uint16_t *src = pn[k][0];
+    if constexpr (kSrcWave == 2) src = lsrc ? pn[k][1] : pn[k][0];
     const long w = tile_w0 + 8 * (tid % kUnits);
-    if (kSrcStage * c + wsrc + kSrcPass * k < nsrc) ok_mask |= 1u << k;
+    if (kSrcStage * c + wsrc + lsrc + kSrcPass * k < nsrc) ok_mask |= 1u << k;
     if constexpr (!kRagged) {
       return *reinterpret_cast<const P2H_GLOBAL u32x4 *>(src + w);
     } else {
@@ -199,14 +223,12 @@ __global__ __launch_bounds__(kMfThreads, 1) void k_gf_mfma(const uint16_t *const
   };
   auto load_stage = [&](int c) {
     ok_mask = 0;
-    rb0 = load_b(c, 0, pn[0]);
-    rb1 = load_b(c, 1, pn[1]);
-    rb2 = load_b(c, 2, pn[2]);
-    rb3 = load_b(c, 3, pn[3]);
+#pragma unroll
+    for (int k = 0; k < kPasses; k++) rb[k] = load_b(c, k);
     {
       const int e = kApt * tid, frag = e / (kKc * 64), ksl = (e / 64) % kKc, l0 = e % 64;
       const int ks = kKc * c + ksl;
-      if (mf0 + frag < nmf && ks < nks) ok_mask |= 1u << 4;
+      if (mf0 + frag < nmf && ks < nks) ok_mask |= 1u << 8;
       const long idx = ((long)min(mf0 + frag, nmf - 1) * nks + min(ks, nks - 1)) * 64 + l0;
       ra = *reinterpret_cast<const P2H_GLOBAL AVec *>(gptr(apk) + idx);
     }
@@ -214,17 +236,17 @@ __global__ __launch_bounds__(kMfThreads, 1) void k_gf_mfma(const uint16_t *const
   };
   auto store_stage = [&](int buf) {
     const u32x4 zero = {0u, 0u, 0u, 0u};
-    const u32x4 b[4] = {(ok_mask & 1u) ? rb0 : zero, (ok_mask & 2u) ? rb1 : zero, (ok_mask & 4u) ? rb2 : zero,
-                        (ok_mask & 8u) ? rb3 : zero};
 #pragma unroll
-    for (int k = 0; k < 4; k++)
-      sB[buf][tid / kUnits + kSrcPass * k][tid % kUnits] = make_uint4(b[k].x, b[k].y, b[k].z, b[k].w);
+    for (int k = 0; k < kPasses; k++) {
+      const u32x4 v = (ok_mask & (1u << k)) ? rb[k] : zero;
+      sB[buf][tid / kUnits + kSrcPass * k][tid % kUnits] = make_uint4(v.x, v.y, v.z, v.w);
+    }
     const int e0 = kApt * tid, frag = e0 / (kKc * 64), ksl = (e0 / 64) % kKc, l0 = e0 % 64;
     uint32_t y[kApt];
     __builtin_memcpy(y, &ra, sizeof y);
 #pragma unroll
     for (int e = 0; e < kApt; e++) {
-      if (!(ok_mask & 16u)) y[e] = 0;
+      if (!(ok_mask & 256u)) y[e] = 0;
       if constexpr (kI8)
         sA[buf][ksl][frag][l0 + e] = y[e];
       else
@@ -245,7 +267,7 @@ __global__ __launch_bounds__(kMfThreads, 1) void k_gf_mfma(const uint16_t *const
   load_stage(0);
   store_stage(0);
   __syncthreads();
-  const int bcol = wave * (kCols * 32) + kCols * n;  // this lane's first word inside the tile
+  const int bcol = wc * (kCols * 32) + kCols * n;  // this lane's first word inside the tile
   for (int c = 0; c < nchunks; c++) {
     const int buf = c & 1;
     if (c + 1 < nchunks) load_stage(c + 1);
@@ -255,7 +277,7 @@ __global__ __launch_bounds__(kMfThreads, 1) void k_gf_mfma(const uint16_t *const
 #pragma unroll 1
     for (int ksl = 0; ksl < kKc; ksl++) {
       if constexpr (kI8) {
-        static_assert(!kI8 || (kRows == kMfRows && kCols == kMfCols && kKc == kMfKc), "int8: default shape");
+        static_assert(!kI8 || (kWR == 1 && kRows == kMfRows && kCols == kMfCols && kKc == kMfKc), "int8: default shape");
 #pragma unroll
         for (int hh = 0; hh < 2; hh++) {
           const uint32_t x0 = reinterpret_cast<const uint32_t *>(&sB[buf][4 * ksl + 2 * hh][0])[bcol >> 1];
@@ -287,7 +309,7 @@ __global__ __launch_bounds__(kMfThreads, 1) void k_gf_mfma(const uint16_t *const
         }
 #pragma unroll
         for (int i = 0; i < kRows; i++) {
-          const uint4 t = sA[buf][ksl][i][lane];
+          const uint4 t = sA[buf][ksl][wr * kRows + i][lane];
           const v8i a = v8i{(int)t.x, (int)t.y, (int)t.z, (int)t.w, 0, 0, 0, 0};
 #pragma unroll
           for (int f = 0; f < kCols; f++)
@@ -306,7 +328,7 @@ __global__ __launch_bounds__(kMfThreads, 1) void k_gf_mfma(const uint16_t *const
   if (w >= words) return;
 #pragma unroll
   for (int i = 0; i < kRows; i++) {
-    const int r = 2 * (mf0 + i) + g;
+    const int r = 2 * (mf0 + wr * kRows + i) + g;
     if (r >= ndst) continue;
     uint32_t v[kCols];
 #pragma unroll
@@ -346,13 +368,15 @@ int32_t run_gf_mfma(par2hip_ctx *ctx, hipStream_t s, const GfProblem &p, bool in
                        p.ndst, p.nsrc, p.ldc ? p.ldc : p.nsrc, nmf, nks);
     P2H_CHECK(ctx, hipGetLastError());
   }
-  // at most 4 outputs: the small tile shape (4 outputs x 1024 words)
-  const bool small = !int8 && nmf <= 2;
-  const int rows = small ? 2 : kMfRows;
-  const long wg_words = small ? 1024 : kMfWaves * kMfCols * 32;
+  // tile shape: 0 wide, 1 small, 2 split, 3 tall (the table above)
+  const int shape = int8 ? 0 : nmf <= 2 ? 1 : p.beside_chains ? 3 : 2;
+  static const int kFragsOf[4] = {kMfRows, 2, 4, 8};
+  static const long kWordsOf[4] = {kMfWaves * kMfCols * 32, 1024, 512, 256};
+  const int frags = kFragsOf[shape];
+  const long wg_words = kWordsOf[shape];
   const int ntiles = (int)((p.words + wg_words - 1) / wg_words);
   const int nfull = (int)(p.words / wg_words);  // tiles whose words all lie inside the rows
-  const int nrowblk = (nmf + rows - 1) / rows;
+  const int nrowblk = (nmf + frags - 1) / frags;
   const long nwg = 8L * ((ntiles + 7) / 8) * nrowblk;
   if (nwg > 0x7fffffffL) return fail(ctx, PAR2HIP_EINVAL, "GF problem too large for one launch");
   ScopedTiming t(ctx, p.tag ? p.tag : (int8 ? "gf_muladd_mfma_i8" : "gf_muladd_mfma"), s);
@@ -364,9 +388,15 @@ int32_t run_gf_mfma(par2hip_ctx *ctx, hipStream_t s, const GfProblem &p, bool in
   if (int8) {
     if (nfull) launch(k_gf_mfma<true, false>, nfull, 0);
     if (ntiles > nfull) launch(k_gf_mfma<true, true>, ntiles - nfull, nfull);
-  } else if (small) {
-    if (nfull) launch(k_gf_mfma<false, false, 2, 4, 4>, nfull, 0);
-    if (ntiles > nfull) launch(k_gf_mfma<false, true, 2, 4, 4>, ntiles - nfull, nfull);
+  } else if (shape == 1) {
+    if (nfull) launch(k_gf_mfma<false, false, 1, 2, 4, 4>, nfull, 0);
+    if (ntiles > nfull) launch(k_gf_mfma<false, true, 1, 2, 4, 4>, ntiles - nfull, nfull);
+  } else if (shape == 2) {
+    if (nfull) launch(k_gf_mfma<false, false, 2, 2, 4, 8>, nfull, 0);
+    if (ntiles > nfull) launch(k_gf_mfma<false, true, 2, 2, 4, 8>, ntiles - nfull, nfull);
+  } else if (shape == 3) {
+    if (nfull) launch(k_gf_mfma<false, false, 4, 2, 4, 4>, nfull, 0);
+    if (ntiles > nfull) launch(k_gf_mfma<false, true, 4, 2, 4, 4>, ntiles - nfull, nfull);
   } else {
     if (nfull) launch(k_gf_mfma<false, false>, nfull, 0);
     if (ntiles > nfull) launch(k_gf_mfma<false, true>, ntiles - nfull, nfull);

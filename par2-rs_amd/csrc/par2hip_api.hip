@@ -416,6 +416,7 @@ int32_t encode_body(par2hip_ctx *ctx, const uint8_t *d_slices, size_t n, size_t 
     p.accumulate = false;
     p.align_words = align;
     p.kernel = plan.kernel;
+    p.beside_chains = d_md5 != nullptr;
     if ((rc = run_gf(ctx, ctx->cur, p))) return rc;
   }
   return PAR2HIP_OK;

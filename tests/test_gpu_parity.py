@@ -520,6 +520,24 @@ def test_auto_choice_under_md5_chains(ctx):
         assert md5[i].tobytes() == hashlib.md5(data[i].tobytes()).digest(), i
 
 
+def test_auto_choice_under_md5_chains_ragged(ctx):
+    """512 slices of 512 KiB + 48 bytes, R=60 with the IFSC hashes: the
+    matrix-core contraction beside the chains, its last tile column running
+    past the end of the rows; bit-exact at the start and at the ragged end."""
+    n, s, r = 512, (1 << 19) + 48, 60
+    data = np.random.default_rng(24).integers(0, 256, (n, s), dtype=np.uint8)
+    say("auto choice, ragged encode", n, s, r)
+    rec, md5, stats = _encode_dev_profiled(ctx, data, r)
+    if ctx.device_info()["hash_cus"] > 0:
+        assert "gf_muladd_mfma" in stats and "gf_muladd_lds" not in stats, stats
+    words = s // 2
+    for w0 in (0, words - 300, words - 64):
+        cols = np.ascontiguousarray(data.view(np.uint16)[:, w0:w0 + 64])
+        assert np.array_equal(rec.view(np.uint16)[:, w0:w0 + 64], oracle.encode(cols, np.arange(r))), w0
+    for i in (0, n - 1):
+        assert md5[i].tobytes() == hashlib.md5(data[i].tobytes()).digest(), i
+
+
 def test_auto_choice_short_chains_keep_lds(ctx):
     """2000 x 8 KiB, R=200: the chains are far shorter than a matrix-core
     contraction of this size, so the LDS-table kernel runs."""
