@@ -38,36 +38,327 @@ import par2hip  # noqa: E402
 from workloads import draw_requests, draw_sets, random_bytes  # noqa: E402
 
 HBM_PEAK_GBS = 8000.0
-# HBM bytes per launch of the plain-encode contraction (1024 x 1 MiB, R=102)
-# with the fixed-heuristic tile shape (8 words per thread, 2 groups of 8 rows
-# per workgroup, 16 sources per LDS chunk): rocprofv3 --pmc FETCH_SIZE and
-# --pmc WRITE_SIZE runs of this workload, each scaled by the same counter on a
-# 1 GiB device-to-device copy (--workload copy): FETCH_SIZE reads half the
-# bytes the copy reads (factor 2.0), WRITE_SIZE all it writes (factor 1.0).
-C2_GF_TRAFFIC_BYTES = 2.475e9
-# The same measurement for the bit-sliced FP4 contraction that the automatic
-# choice runs in the encode step (on 192 CUs beside the MD5 chains): FETCH_SIZE
-# 1,647,433.7 KiB x 2.0 + WRITE_SIZE 104,448 KiB per launch.
-C2_GF_MFMA_TRAFFIC_BYTES = 3.481e9
-# The same two counters for the repair step's rebuild contraction (2000 x 512
-# KiB, 100 erasures, default tile shape): FETCH_SIZE 668,054.7 KiB x 2.0 +
-# WRITE_SIZE 51,200 KiB per launch.
-C3_GF_TRAFFIC_BYTES = 1.421e9
-# Unit utilisation of the same contraction (default tile shape, alone on the
-# device) from rocprofv3 --pmc SQ_LDS_IDX_ACTIVE / SQ_ACTIVE_INST_VALU /
-# SQ_LDS_BANK_CONFLICT / GRBM_GUI_ACTIVE runs of this workload: the binding
-# units are the LDS lookups and their XOR folds, not HBM.
-C2_GF_UNITS = {"lds_util": 0.76, "valu_busy": 0.665, "lds_bank_conflict": 0.0, "sclk_ghz": 2.0,
-               "source": "rocprofv3 --pmc, profiles/gf_units_utilisation_pmc.md"}
-# The same for the bit-sliced FP4 matrix-core contraction as the encode step
-# runs it (on 192 of the 256 CUs): utilisations over the whole chip, and the
-# matrix-core one over its own CUs.
-C2_GF_MFMA_UNITS = {"mfma_util": 0.427, "mfma_util_own_cus": 0.570, "cus": 192, "lds_util": 0.267,
-                    "valu_busy": 0.324, "lds_bank_conflict": 0.080, "sclk_ghz": 2.19,
-                    "source": "rocprofv3 --pmc, profiles/gf_units_utilisation_pmc.md"}
+# HBM bytes per launch come from the committed counter summary, never from
+# constants here: rocprofv3 --pmc FETCH_SIZE and --pmc WRITE_SIZE runs of each
+# workload (PMC_RUNS), each counter scaled by what it reads for a 1 GiB
+# device-to-device copy, summarised by `bench.py --pmc-summary`.  A workload
+# whose contraction is no longer the kernel the summary measured reports
+# traffic null and says why.
+PROFILES = os.path.join(ROOT, "profiles")
+TRAFFIC_SUMMARY = os.path.join(PROFILES, "hbm_traffic_pmc.md")
+PMC_OUT = "gpurun_out/pmc"
+PMC_RUNS = {
+    "copy_1GiB": "--workload copy --steps 3",
+    "plain_encode_1GiB": "--steps 3 --warmup 1 --no-cpu --legs",
+    "plain_repair_1GiB": "--workload repair --steps 3 --warmup 1 --no-cpu",
+    "batched_256_sets": "--workload batched --steps 1 --warmup 1 --no-cpu",
+    "small_r_R4_1GiB": "--workload small_r --small-r 4 --steps 3 --warmup 1 --autotune 0",
+}
+# Unit utilisation of the contractions (SQ counters of the same workloads,
+# committed with their commands in profiles/gf_units_utilisation_pmc.md).
+UNITS_SUMMARY = os.path.join(PROFILES, "gf_units_utilisation_pmc.md")
 FP4_DENSE_PEAK_TFLOPS = 10100.0
 INT8_DENSE_PEAK_TOPS = 5000.0
 METRIC = "encode/repair GB/s source data + p50/p99 latency vs plain GPU path, bit-exact"
+
+
+def kernel_tag(symbol: str) -> str:
+    """The library's timing tag of a device kernel symbol."""
+    name = symbol.split("(")[0].replace("par2hip::", "").strip()
+    if name.startswith("void "):
+        name = name[5:]
+    base = name.split("<")[0]
+    if base == "k_gf_mfma":
+        return "gf_muladd_mfma_i8" if name.startswith("k_gf_mfma<true") else "gf_muladd_mfma"
+    return {"k_gf_lds": "gf_muladd_lds", "k_gf_lds_batch": "gf_batch", "k_md5_slices": "md5_slices",
+            "k_crc32_slices": "crc32_slices", "k_md5_list": "md5_list", "k_crc32_list": "crc32_list",
+            "__amd_rocclr_copyBuffer": "copy"}.get(base, base)
+
+
+def summary_rows(path: str, ncells: int) -> list[list[str]]:
+    """Data rows of the markdown tables in `path` with `ncells` cells."""
+    rows = []
+    try:
+        lines = open(path).read().splitlines()
+    except OSError:
+        return rows
+    for ln in lines:
+        ln = ln.strip()
+        if not ln.startswith("|") or set(ln) <= set("|-: "):
+            continue
+        cells = [c.strip().strip("`") for c in ln.strip("|").split("|")]
+        if len(cells) == ncells:
+            rows.append(cells)
+    return rows
+
+
+def profile_traffic(workload: str, tag: str):
+    """HBM bytes per launch of the `tag` kernel in `workload` (the largest
+    such kernel: the main contraction, not a fold) from the committed counter
+    summary, and where the figure came from (or why there is none)."""
+    best, sym = None, None
+    for w, kernel, t, _n, _f, _wr, total in summary_rows(TRAFFIC_SUMMARY, 7):
+        if w == workload and t == tag and (best is None or float(total) > best):
+            best, sym = float(total), kernel
+    if best is None:
+        have = sorted({t for w, _k, t, *_ in summary_rows(TRAFFIC_SUMMARY, 7) if w == workload})
+        return None, f"no {tag} row for {workload} in profiles/hbm_traffic_pmc.md (it holds {have})"
+    return best, f"rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE of {sym}, profiles/hbm_traffic_pmc.md"
+
+
+def profile_units(workload: str, tag: str):
+    """Unit utilisation of the `tag` kernel in `workload` from the committed
+    SQ-counter summary (None when it has no such row)."""
+    keys = ["clock_ghz", "mfma_util", "lds_util", "valu_busy", "lds_bank_conflict", "busy_cu_frac"]
+    for row in summary_rows(UNITS_SUMMARY, 10):
+        w, kernel, t, _ms = row[:4]
+        if w == workload and t == tag:
+            vals = {}
+            for k, v in zip(keys, row[4:]):
+                v = v.rstrip("%")
+                vals[k] = None if v in ("-", "") else round(float(v) / (1 if k == "clock_ghz" else 100), 4)
+            if vals.get("mfma_util") is not None and vals.get("busy_cu_frac"):
+                vals["mfma_util_busy_cus"] = round(vals["mfma_util"] / vals["busy_cu_frac"], 4)
+            vals["kernel"] = kernel
+            vals["source"] = "rocprofv3 --pmc SQ counters, profiles/gf_units_utilisation_pmc.md"
+            return vals
+    return None
+
+
+def pmc_commands() -> list[str]:
+    """The rocprofv3 counter runs behind the two summaries, one counter set
+    per run (FETCH_SIZE and WRITE_SIZE use 3 and 2 of the 4 TCC counters)."""
+    cmds = []
+    for w, a in PMC_RUNS.items():
+        for c in ("FETCH_SIZE", "WRITE_SIZE"):
+            cmds.append(f"rocprofv3 --pmc {c} -d {PMC_OUT}/{w}_{c} -o run --output-format csv -- "
+                        f"python3 bench.py {a}")
+        if w != "copy_1GiB":
+            cmds.append(f"rocprofv3 --kernel-trace --pmc {' '.join(UNIT_COUNTERS)} -d {PMC_OUT}/{w}_units -o run "
+                        f"--output-format csv -- python3 bench.py {a}")
+    return cmds
+
+
+# rocprofv3 --kernel-trace --stats runs: profile name -> bench.py arguments
+TRACE_OUT = "gpurun_out/trace"
+TRACE_RUNS = {
+    "plain_encode_1GiB_R102_default_bench": "",
+    "plain_repair_2000x512KiB": "--workload repair --no-cpu",
+    "batched_256_sets": "--workload batched --steps 5 --warmup 3 --no-cpu",
+    "served_poisson": "--workload served --no-cpu",
+    "small_r_1GiB": "--workload small_r --steps 10",
+    "inverse_m1000_5000_12000": "--workload inverse --steps 3 --warmup 1",
+    "create_1x1GiB_R103": "--workload create --create-files 1 --create-file-mib 1024 --create-block 1048576 "
+                          "--steps 3 --warmup 1",
+    "create_16x64MiB": "--workload create --steps 3 --warmup 1",
+    "repair_files_16x64MiB": "--workload repair_files --steps 3 --warmup 1",
+}
+
+
+def trace_commands() -> list[str]:
+    return [f"rocprofv3 --kernel-trace --stats -d {TRACE_OUT}/{w} -o run --output-format csv -- "
+            f"python3 bench.py {a}".rstrip() for w, a in TRACE_RUNS.items()]
+
+
+def trace_summary() -> None:
+    """Writes profiles/<name>_kernel_trace.md for every run of trace_commands()
+    whose output is under gpurun_out/trace: the kernels by total time, and the
+    bench's own line (its HIP-event kernel averages come from the same run)."""
+    import csv
+    import glob
+
+    for w, a in TRACE_RUNS.items():
+        stats = glob.glob(os.path.join(TRACE_OUT, w, "**", "*kernel_stats.csv"), recursive=True)
+        if not stats:
+            continue
+        rows = list(csv.DictReader(open(stats[0])))
+        rows.sort(key=lambda r: -float(r["TotalDurationNs"]))
+        cmd = f"rocprofv3 --kernel-trace --stats -d {TRACE_OUT}/{w} -o run --output-format csv -- " \
+              f"python3 bench.py {a}".rstrip()
+        lines = [f"# Kernel trace: {w}", "", "Command (repo root, one MI355X), written by "
+                 "`python3 bench.py --trace-summary`:", "", "```", cmd, "```", "",
+                 "`run_kernel_stats.csv`, kernels by total time (all of the run: warm-up, timed steps and the",
+                 "checks):", "", "| kernel | calls | avg ms | min ms | max ms | % of kernel time |",
+                 "|---|---|---|---|---|---|"]
+        for r in rows[:24]:
+            name = r["Name"].split("(")[0]
+            lines.append(f"| `{name}` | {r['Calls']} | {float(r['AverageNs']) / 1e6:.4f} | "
+                         f"{float(r['MinNs']) / 1e6:.4f} | {float(r['MaxNs']) / 1e6:.4f} | "
+                         f"{float(r['Percentage']):.2f} |")
+        log_path = os.path.join(TRACE_OUT, w + ".log")
+        line = None
+        if os.path.exists(log_path):
+            for ln in open(log_path):
+                if ln.startswith("{"):
+                    line = json.loads(ln)
+        # each roofline kernel of the bench line against its launches in the
+        # trace: the main launches (at least half the longest one's duration)
+        # of the symbol with that tag that spends the most time
+        # (the top-level workload only: the legs of the default line have
+        # runs of their own, where their kernels are the largest of their tag)
+        roofs = []
+        if line and isinstance(line.get("roofline"), dict) and line["roofline"].get("kernel"):
+            roofs.append(("step", line["roofline"]["kernel"], line["roofline"].get("kernel_ms")))
+        traces = glob.glob(os.path.join(TRACE_OUT, w, "**", "*kernel_trace.csv"), recursive=True)
+        if roofs and traces:
+            durs = {}
+            for r in csv.DictReader(open(traces[0])):
+                durs.setdefault(r["Kernel_Name"], []).append(
+                    (int(r["End_Timestamp"]) - int(r["Start_Timestamp"])) / 1e6)
+            lines += ["", "Roofline kernels of the bench line against the trace (main launches: at least half the",
+                      "longest launch of that symbol; the trace's launches run alone or beside the same work as in",
+                      "the bench's timed steps):", "",
+                      "| leg | tag | symbol | main launches | trace avg ms | bench HIP-event avg ms |",
+                      "|---|---|---|---|---|---|"]
+            for leg, tag, bench_ms in roofs:
+                cands = [(sum(d), k, d) for k, d in durs.items() if kernel_tag(k) == tag]
+                if not cands:
+                    continue
+                _, sym, d = max(cands)
+                top = max(d)
+                main = [x for x in d if x >= 0.5 * top]
+                lines.append(f"| {leg} | {tag} | `{sym.split('(')[0]}` | {len(main)} | "
+                             f"{sum(main) / len(main):.4f} | {bench_ms} |")
+        if line:
+            lines += ["", "The bench's line from the same run (HIP events on each kernel's own stream):", ""]
+
+            def brief(o, depth=0):
+                keep = ("metric", "value", "unit", "ms_per_step", "p50_ms", "p99_ms", "bit_exact_sample",
+                        "kernel_ms", "roofline", "config", "batch_eager_ms", "plain_path", "cpu_baseline", "exact")
+                out = {}
+                for k, v in o.items():
+                    if k in keep:
+                        out[k] = v
+                    elif isinstance(v, dict) and depth == 0 and ("value" in v or "kernel_ms" in v):
+                        out[k] = brief(v, 1)
+                return out
+
+            lines += ["```", json.dumps(brief(line), indent=1)[:6000], "```"]
+        open(os.path.join(PROFILES, f"{w}_kernel_trace.md"), "w").write("\n".join(lines) + "\n")
+
+
+UNIT_COUNTERS = ["SQ_VALU_MFMA_BUSY_CYCLES", "SQ_LDS_IDX_ACTIVE", "SQ_ACTIVE_INST_VALU", "SQ_LDS_BANK_CONFLICT",
+                 "SQ_BUSY_CU_CYCLES", "GRBM_GUI_ACTIVE"]
+XCCS = 8          # MI355X: 8 XCDs; GRBM_GUI_ACTIVE is summed over them
+CUS, SIMDS = 256, 1024
+
+
+def read_counters(d: str):
+    """Per-dispatch counter sums and kernel names from a rocprofv3 CSV run
+    directory: {dispatch: (kernel, {counter: value}, duration_ns or None)}."""
+    import csv
+    import glob
+
+    out = {}
+    for f in glob.glob(os.path.join(d, "**", "*counter_collection.csv"), recursive=True):
+        for row in csv.DictReader(open(f)):
+            key = (row.get("Agent_Id"), row["Dispatch_Id"])
+            k, cnt, dur = out.setdefault(key, (row["Kernel_Name"], {}, None))
+            cnt[row["Counter_Name"]] = cnt.get(row["Counter_Name"], 0.0) + float(row["Counter_Value"])
+    durs = {}
+    for f in glob.glob(os.path.join(d, "**", "*kernel_trace.csv"), recursive=True):
+        for row in csv.DictReader(open(f)):
+            durs[(row.get("Agent_Id"), row["Dispatch_Id"])] = int(row["End_Timestamp"]) - int(row["Start_Timestamp"])
+    return {k: (v[0], v[1], durs.get(k)) for k, v in out.items()}
+
+
+def pmc_summary() -> None:
+    """Writes profiles/hbm_traffic_pmc.md and profiles/gf_units_utilisation_pmc.md
+    from the counter runs of pmc_commands() (averages per dispatch by kernel)."""
+    def main_launches(vals):
+        """The launches of one kernel symbol that do its main work: at least
+        half the largest one's figure (a fold or a staging copy of the same
+        kernel is far smaller)."""
+        top = max(vals)
+        return [v for v in vals if v >= 0.5 * top]
+
+    def per_kernel(d, counter):
+        acc = {}
+        for kernel, cnt, _ in read_counters(d).values():
+            if counter in cnt:
+                acc.setdefault(kernel, []).append(cnt[counter])
+        out = {}
+        for k, vals in acc.items():
+            main = main_launches(vals)
+            out[k] = (len(main), sum(main) / len(main))
+        return out
+
+    cf = per_kernel(f"{PMC_OUT}/copy_1GiB_FETCH_SIZE", "FETCH_SIZE")
+    cw = per_kernel(f"{PMC_OUT}/copy_1GiB_WRITE_SIZE", "WRITE_SIZE")
+    copy = next(k for k in cf if kernel_tag(k) == "copy")
+    f_scale = (1 << 30) / (cf[copy][1] * 1024)
+    w_scale = (1 << 30) / (cw[copy][1] * 1024)
+    lines = ["# HBM traffic per launch (rocprofv3 --pmc FETCH_SIZE, then WRITE_SIZE)", "",
+             "Each workload's counters in runs of their own (repo root, one MI355X), written by",
+             "`python3 bench.py --pmc-summary` from these commands (`python3 bench.py --pmc-commands`):", "", "```"]
+    lines += [c for c in pmc_commands() if "--kernel-trace" not in c]
+    lines += ["```", "",
+              f"Calibration: a 1 GiB device-to-device copy (`{copy}`, 1,048,576 KiB read and written)",
+              f"reads FETCH_SIZE = {cf[copy][1]:,.1f} KiB and WRITE_SIZE = {cw[copy][1]:,.1f} KiB per dispatch, so",
+              f"fetched bytes = FETCH_SIZE x 1024 x {f_scale:.4f} and written bytes = WRITE_SIZE x 1024 x "
+              f"{w_scale:.4f}.  Counters serialise the dispatches, so every kernel ran alone (CU masks kept).",
+              "Averages per dispatch over each kernel's main launches (those with at least half the largest",
+              "launch's count: a fold or a staging copy through the same kernel is far smaller), for every",
+              "kernel that moves at least 1 MB; bench.py reads its contraction's",
+              "`roofline.traffic` from the `HBM bytes` column (the largest row of its kernel tag).", "",
+              "| workload | kernel | tag | dispatches | FETCH_SIZE KiB | WRITE_SIZE KiB | HBM bytes |",
+              "|---|---|---|---|---|---|---|"]
+    for w in PMC_RUNS:
+        if w == "copy_1GiB":
+            continue
+        fk = per_kernel(f"{PMC_OUT}/{w}_FETCH_SIZE", "FETCH_SIZE")
+        wk = per_kernel(f"{PMC_OUT}/{w}_WRITE_SIZE", "WRITE_SIZE")
+        rows = []
+        for k in fk:
+            if k not in wk:
+                continue
+            total = (fk[k][1] * f_scale + wk[k][1] * w_scale) * 1024
+            if total >= 1e6:
+                rows.append((total, k, fk[k][0], fk[k][1], wk[k][1]))
+        for total, k, n, f, wr in sorted(rows, reverse=True):
+            lines.append(f"| {w} | `{k.split('(')[0]}` | {kernel_tag(k)} | {n} | {f:,.1f} | {wr:,.1f} | {total:.4e} |")
+    open(TRAFFIC_SUMMARY, "w").write("\n".join(lines) + "\n")
+
+    ulines = ["# Unit utilisation of the contractions (rocprofv3 SQ counters)", "",
+              "One counter run per workload (repo root, one MI355X), written by `python3 bench.py --pmc-summary`:",
+              "", "```"]
+    ulines += [c for c in pmc_commands() if "--kernel-trace" in c]
+    ulines += ["```", "",
+               f"Per dispatch, counters summed over the device; G = GRBM_GUI_ACTIVE / {XCCS} (one XCC's busy",
+               f"clocks).  MfmaUtil = SQ_VALU_MFMA_BUSY_CYCLES / (G x {SIMDS} SIMDs); LdsUtil, VALUBusy and",
+               f"LDS bank conflict = SQ_LDS_IDX_ACTIVE, SQ_ACTIVE_INST_VALU, SQ_LDS_BANK_CONFLICT / (G x {CUS}",
+               f"CUs); busy CUs = SQ_BUSY_CU_CYCLES / (G x {CUS}); clock = G / kernel duration.  Averages",
+               "over each kernel's main launches (at least half the longest one's duration); counters",
+               "serialise dispatches (CU masks kept).", "",
+               "| workload | kernel | tag | ms | clock GHz | MfmaUtil | LdsUtil | VALUBusy | LDS bank conflict "
+               "| busy CUs |", "|---|---|---|---|---|---|---|---|---|---|"]
+    for w in PMC_RUNS:
+        if w == "copy_1GiB":
+            continue
+        acc = {}
+        disp = [x for x in read_counters(f"{PMC_OUT}/{w}_units").values()
+                if all(c in x[1] for c in UNIT_COUNTERS) and x[2] and x[1]["GRBM_GUI_ACTIVE"] > 0]
+        longest = {}
+        for kernel, _cnt, dur in disp:
+            longest[kernel] = max(longest.get(kernel, 0), dur)
+        for kernel, cnt, dur in disp:
+            if dur < 0.5 * longest[kernel]:
+                continue
+            g = cnt["GRBM_GUI_ACTIVE"] / XCCS
+            v = [dur / 1e6, g / dur, cnt["SQ_VALU_MFMA_BUSY_CYCLES"] / (g * SIMDS),
+                 cnt["SQ_LDS_IDX_ACTIVE"] / (g * CUS), cnt["SQ_ACTIVE_INST_VALU"] / (g * CUS),
+                 cnt["SQ_LDS_BANK_CONFLICT"] / (g * CUS), cnt["SQ_BUSY_CU_CYCLES"] / (g * CUS)]
+            a = acc.setdefault(kernel, [0, [0.0] * len(v)])
+            a[0] += 1
+            a[1] = [x + y for x, y in zip(a[1], v)]
+        rows = sorted(((a[1][0] / a[0], k, [x / a[0] for x in a[1]]) for k, a in acc.items()), reverse=True)
+        for ms, k, v in rows:
+            if kernel_tag(k) in ("gf_muladd_lds", "gf_muladd_mfma", "gf_muladd_mfma_i8", "gf_batch", "md5_slices",
+                                 "crc32_slices", "md5_list", "crc32_list") and ms >= 0.05:
+                ulines.append(f"| {w} | `{k.split('(')[0]}` | {kernel_tag(k)} | {v[0]:.3f} | {v[1]:.2f} | "
+                              f"{100 * v[2]:.1f}% | {100 * v[3]:.1f}% | {100 * v[4]:.1f}% | {100 * v[5]:.1f}% | "
+                              f"{100 * v[6]:.1f}% |")
+    open(UNITS_SUMMARY, "w").write("\n".join(ulines) + "\n")
 
 
 def log(*a):
@@ -203,13 +494,10 @@ def bench_encode(args, ctx):
     md5_avg = stats.get("md5_slices", (1, 0.0))[1] / max(stats.get("md5_slices", (1, 0))[0], 1)
     crc_avg = stats.get("crc32_slices", (1, 0.0))[1] / max(stats.get("crc32_slices", (1, 0))[0], 1)
     achieved = algo_bytes / (gf_avg * 1e-3) / 1e9 if gf_avg > 0 else 0.0
-    traffic = args.traffic
-    c2 = (n, s, r) == (1024, 1 << 20, 102)
-    default_shape = not args.autotune and gf_name == "gf_muladd_lds" and c2
-    if traffic is None and default_shape:
-        traffic = C2_GF_TRAFFIC_BYTES
-    if traffic is None and gf_name == "gf_muladd_mfma" and c2:
-        traffic = C2_GF_MFMA_TRAFFIC_BYTES
+    traffic, traffic_src = profile_traffic("plain_encode_1GiB", gf_name)
+    if args.traffic is not None:
+        traffic, traffic_src = args.traffic, "--traffic"
+    units = profile_units("plain_encode_1GiB", gf_name)
     muladds = n * r * (s // 2)
     hbm = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
            "frac": round(achieved / HBM_PEAK_GBS, 5)}
@@ -223,10 +511,8 @@ def bench_encode(args, ctx):
         roof = {"bound": "mfma", "achieved": round(ops, 1), "peak": peak, "unit": "TOP/s" if i8 else "TFLOP/s",
                 "frac": round(ops / peak, 4), "hbm_achieved_GBps": round(achieved, 2),
                 "hbm_frac": round(achieved / HBM_PEAK_GBS, 5)}
-        units = C2_GF_MFMA_UNITS if (c2 and not i8) else None
     else:
         roof = dict(hbm)
-        units = C2_GF_UNITS if default_shape else None
     return {
         "workload": "plain_encode",
         "config": {"workload": "plain_encode_1GiB", "n_slices": n, "slice_bytes": s, "recovery_slices": r,
@@ -236,7 +522,7 @@ def bench_encode(args, ctx):
         "kernel_ms": {k: v[1] / max(v[0], 1) for k, v in stats.items()},
         "roofline": {**roof, "traffic": traffic,
                      "traffic_x_algorithmic": round(traffic / algo_bytes, 3) if traffic else None,
-                     "kernel": gf_name, "kernel_ms": round(gf_avg, 4), "algorithmic_bytes": algo_bytes,
+                     "traffic_source": traffic_src, "kernel": gf_name, "kernel_ms": round(gf_avg, 4), "algorithmic_bytes": algo_bytes,
                      "gf_muladds": muladds, "md5_ms": round(md5_avg, 4), "crc32_ms": round(crc_avg, 4),
                      "units": units},
         "cpu": (n, s, r),
@@ -386,6 +672,11 @@ def bench_small_r(args, ctx):
             algo = n * s + r * s
             out[f"R{r}_{name}"] = {"kernel_ms": round(ms, 4), "GBps": round(algo / (ms * 1e-3) / 1e9, 1),
                                    "hbm_frac": round(algo / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4), "exact": exact}
+            if r == 4:  # the counter-profiled case
+                tag = next(key for key in st if key.startswith("gf_muladd"))
+                traffic, src = profile_traffic("small_r_R4_1GiB", tag)
+                out[f"R{r}_{name}"].update(traffic=traffic, traffic_source=src,
+                                           traffic_x_algorithmic=round(traffic / algo, 3) if traffic else None)
             log("small R", r, name, out[f"R{r}_{name}"])
     ctx.set_gf_kernel(prev)
     return out
@@ -442,9 +733,9 @@ def bench_repair(args, ctx):
     gf_avg = gf_ms / max(launches, 1)
     algo_bytes = n * s + m * s + m * s
     achieved = algo_bytes / (gf_avg * 1e-3) / 1e9 if gf_avg > 0 else 0.0
-    traffic = args.traffic
-    if traffic is None and gf_name == "gf_muladd_lds" and not args.autotune:
-        traffic = C3_GF_TRAFFIC_BYTES
+    traffic, traffic_src = profile_traffic("plain_repair_1GiB", gf_name)
+    if args.traffic is not None:
+        traffic, traffic_src = args.traffic, "--traffic"
     return {
         "workload": "plain_repair",
         "config": {"workload": "plain_repair_1GiB", "n_slices": n, "slice_bytes": s, "erasures": m,
@@ -454,8 +745,9 @@ def bench_repair(args, ctx):
         "roofline": {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": traffic,
                      "traffic_x_algorithmic": round(traffic / algo_bytes, 3) if traffic else None,
-                     "kernel": gf_name, "kernel_ms": round(gf_avg, 4), "algorithmic_bytes": algo_bytes,
-                     "gf_muladds": m * n * (s // 2)},
+                     "traffic_source": traffic_src, "kernel": gf_name, "kernel_ms": round(gf_avg, 4),
+                     "algorithmic_bytes": algo_bytes, "gf_muladds": m * n * (s // 2),
+                     "units": profile_units("plain_repair_1GiB", gf_name)},
         "cpu": (n, s, m),
     }
 
@@ -574,8 +866,24 @@ def bench_batched(args, ctx):
         for k in ("slices", "md5", "crc", "recovery", "out"):
             if k in j:
                 j[k].free()
+    # the batch contraction reads every set's sources once and writes its
+    # outputs once (encode: n*s in, r*s out; repair: the n-m present and m
+    # recovery slices in, m rebuilt out)
+    algo = sum(x["n"] * x["s"] + (x["r"] if x["kind"] == "encode" else x["m"]) * x["s"] for x in sets)
+    muladds = sum((x["r"] if x["kind"] == "encode" else x["m"]) * x["n"] * (x["s"] // 2) for x in sets)
+    gf_ms = stats.get("gf_batch", (1, 0.0))[1] / max(stats.get("gf_batch", (1, 0))[0], 1)
+    achieved = algo / (gf_ms * 1e-3) / 1e9 if gf_ms > 0 else 0.0
+    traffic, traffic_src = profile_traffic("batched_256_sets", "gf_batch")
+    roofline = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": traffic,
+                "traffic_x_algorithmic": round(traffic / algo, 3) if traffic else None,
+                "traffic_source": traffic_src, "kernel": "gf_batch", "kernel_ms": round(gf_ms, 4),
+                "algorithmic_bytes": algo, "gf_muladds": muladds,
+                "gf_muladds_per_s": float(f"{muladds / (gf_ms * 1e-3):.4g}") if gf_ms > 0 else None,
+                "units": profile_units("batched_256_sets", "gf_batch")}
     return {
         "workload": "batched",
+        "roofline": roofline,
         "config": {"workload": "batched_256_sets", "sets": len(sets), "source_GiB": round(total / 2**30, 3),
                    "encode_sets": sum(x["kind"] == "encode" for x in sets),
                    "repair_sets": sum(x["kind"] == "repair" for x in sets), "seed": args.seed,
@@ -1001,9 +1309,30 @@ def main():
                     help="encode: other workloads measured in the same run and reported as sub-objects "
                          "(--legs with no value: none)")
     ap.add_argument("--leg-steps", type=int, default=5, help="steps of the batched leg")
+    ap.add_argument("--pmc-commands", action="store_true",
+                    help="print the rocprofv3 counter runs behind profiles/hbm_traffic_pmc.md and "
+                         "profiles/gf_units_utilisation_pmc.md, one per line, and exit")
+    ap.add_argument("--pmc-summary", action="store_true",
+                    help="write those two summaries from the runs' output under gpurun_out/pmc and exit")
+    ap.add_argument("--trace-commands", action="store_true",
+                    help="print the rocprofv3 --kernel-trace --stats runs behind profiles/*_kernel_trace.md")
+    ap.add_argument("--trace-summary", action="store_true",
+                    help="write profiles/<name>_kernel_trace.md from those runs' output under gpurun_out/trace")
     ap.add_argument("--traffic", type=float, default=None,
                     help="HBM bytes per GF launch measured with rocprofv3 --pmc (FETCH_SIZE+WRITE_SIZE)")
     args = ap.parse_args()
+    if args.pmc_commands:
+        print("\n".join(pmc_commands()))
+        return
+    if args.pmc_summary:
+        pmc_summary()
+        return
+    if args.trace_commands:
+        print("\n".join(trace_commands()))
+        return
+    if args.trace_summary:
+        trace_summary()
+        return
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py: measures one GPU only; a multi-GPU run would split every slice's word "
               "range across GPUs with no collective, which this bench does not launch", file=sys.stderr)
