@@ -147,6 +147,7 @@ TRACE_RUNS = {
     "batched_256_sets": "--workload batched --steps 5 --warmup 3 --no-cpu",
     "served_poisson": "--workload served --no-cpu",
     "small_r_1GiB": "--workload small_r --steps 10",
+    "gf_variants_1GiB_R102": "--workload micro --steps 5 --warmup 1",
     "inverse_m1000_5000_12000": "--workload inverse --steps 3 --warmup 1",
     "create_1x1GiB_R103": "--workload create --create-files 1 --create-file-mib 1024 --create-block 1048576 "
                           "--steps 3 --warmup 1",
