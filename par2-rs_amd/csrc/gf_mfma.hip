@@ -56,7 +56,9 @@ constexpr int kMfThreads = 64 * kMfWaves;
 //         form's only shape
 //   small (1, 2x4, 4): 4 outputs x 1024 words, 16 sources per stage: at most
 //         4 outputs fill its rows, so no MFMA multiplies padded zero rows, and
-//         each input word is read by one workgroup only
+//         each input word is read by one workgroup only; its loads run two
+//         stages ahead (kAhead = 2: 2-3% faster, 1 GiB at R=4 in 0.30 ms
+//         against 0.31 on one device)
 //   split (2, 2x4, 8): 8 outputs x 512 words; a wave reads 2 A fragments per
 //         K-step instead of 4, half the LDS traffic of A per MFMA.  Alone the
 //         fastest: 1024 x 1 MiB, R=102 in 5.17 ms (wide 5.71 ms), 2000 x
@@ -137,7 +139,8 @@ __device__ __forceinline__ v8i expand_fp4(uint32_t y) {
 // tiles tile0.. whose words may run past the end of the rows (only the last
 // tile column), loaded word by word with checks; the main launch has no such
 // code, so its loads carry no branches.
-template <bool kI8, bool kRagged, int kWR = 1, int kRows = kMfRows, int kCols = kMfCols, int kKc = kMfKc>
+template <bool kI8, bool kRagged, int kWR = 1, int kRows = kMfRows, int kCols = kMfCols, int kKc = kMfKc,
+          int kAhead = 1>
 __global__ __launch_bounds__(kMfThreads, 1) void k_gf_mfma(const uint16_t *const *__restrict__ srcs, int nsrc,
                                                            uint16_t *const *__restrict__ dsts, int ndst,
                                                            const uint32_t *__restrict__ apk, int nmf, int nks,
@@ -183,8 +186,8 @@ __global__ __launch_bounds__(kMfThreads, 1) void k_gf_mfma(const uint16_t *const
   // 16-byte column tid%kUnits.  A: thread tid owns packed words
   // kApt*tid..+kApt-1 of the stage, ordered (fragment, K-step, lane).
   //
-  // The loads of stage c+1 are issued before stage c's MFMAs and consumed only
-  // by store_stage after them, so they must not be waited for in between: no
+  // The loads of stage c+kAhead are issued before stage c's MFMAs and consumed
+  // only by a later store_stage, so they must not be waited for in between: no
   // load sits behind a branch or feeds a select before store_stage.  The
   // source pointers of a stage are wave-uniform (one or two per wave and
   // pass, the lane half picking between two) and fetched one stage ahead;
@@ -202,14 +205,15 @@ __global__ __launch_bounds__(kMfThreads, 1) void k_gf_mfma(const uint16_t *const
         pn[k][j] = gptr(gptr(srcs)[min(kSrcStage * c + wsrc + j + kSrcPass * k, nsrc - 1)]);
   };
   using AVec = typename std::conditional<kApt == 4, u32x4, typename std::conditional<kApt == 2, u32x2, uint32_t>::type>::type;
-  u32x4 rb[kPasses];
-  AVec ra;
-  uint32_t ok_mask;  // bit k: this thread's source row of pass k exists; bit 8: its A words exist
-  auto load_b = [&](int c, int k) -> u32x4 {
+  // kAhead stages of loads in flight: stage c's registers are set c % kAhead
+  u32x4 rb[kAhead][kPasses];
+  AVec ra[kAhead];
+  uint32_t ok_mask[kAhead];  // bit k: this thread's source row of pass k exists; bit 8: its A words exist
+  auto load_b = [&](int c, int k, uint32_t &ok) -> u32x4 {
     const P2H_GLOBAL uint16_t *src = pn[k][0];
     if constexpr (kSrcWave == 2) src = lsrc ? pn[k][1] : pn[k][0];
     const long w = tile_w0 + 8 * (tid % kUnits);
-    if (kSrcStage * c + wsrc + lsrc + kSrcPass * k < nsrc) ok_mask |= 1u << k;
+    if (kSrcStage * c + wsrc + lsrc + kSrcPass * k < nsrc) ok |= 1u << k;
     if constexpr (!kRagged) {
       return *reinterpret_cast<const P2H_GLOBAL u32x4 *>(src + w);
     } else {
@@ -221,32 +225,35 @@ __global__ __launch_bounds__(kMfThreads, 1) void k_gf_mfma(const uint16_t *const
       return v;
     }
   };
-  auto load_stage = [&](int c) {
-    ok_mask = 0;
+  // the register set is a compile-time index (S), so the sets stay in registers
+  auto load_stage = [&](int c, auto S) {
+    uint32_t &ok = ok_mask[S];
+    ok = 0;
 #pragma unroll
-    for (int k = 0; k < kPasses; k++) rb[k] = load_b(c, k);
+    for (int k = 0; k < kPasses; k++) rb[S][k] = load_b(c, k, ok);
     {
       const int e = kApt * tid, frag = e / (kKc * 64), ksl = (e / 64) % kKc, l0 = e % 64;
       const int ks = kKc * c + ksl;
-      if (mf0 + frag < nmf && ks < nks) ok_mask |= 1u << 8;
+      if (mf0 + frag < nmf && ks < nks) ok |= 1u << 8;
       const long idx = ((long)min(mf0 + frag, nmf - 1) * nks + min(ks, nks - 1)) * 64 + l0;
-      ra = *reinterpret_cast<const P2H_GLOBAL AVec *>(gptr(apk) + idx);
+      ra[S] = *reinterpret_cast<const P2H_GLOBAL AVec *>(gptr(apk) + idx);
     }
     fetch_ptrs(c + 1);
   };
-  auto store_stage = [&](int buf) {
+  auto store_stage = [&](int buf, auto S) {
+    const uint32_t ok = ok_mask[S];
     const u32x4 zero = {0u, 0u, 0u, 0u};
 #pragma unroll
     for (int k = 0; k < kPasses; k++) {
-      const u32x4 v = (ok_mask & (1u << k)) ? rb[k] : zero;
+      const u32x4 v = (ok & (1u << k)) ? rb[S][k] : zero;
       sB[buf][tid / kUnits + kSrcPass * k][tid % kUnits] = make_uint4(v.x, v.y, v.z, v.w);
     }
     const int e0 = kApt * tid, frag = e0 / (kKc * 64), ksl = (e0 / 64) % kKc, l0 = e0 % 64;
     uint32_t y[kApt];
-    __builtin_memcpy(y, &ra, sizeof y);
+    __builtin_memcpy(y, &ra[S], sizeof y);
 #pragma unroll
     for (int e = 0; e < kApt; e++) {
-      if (!(ok_mask & 256u)) y[e] = 0;
+      if (!(ok & 256u)) y[e] = 0;
       if constexpr (kI8)
         sA[buf][ksl][frag][l0 + e] = y[e];
       else
@@ -263,14 +270,23 @@ __global__ __launch_bounds__(kMfThreads, 1) void k_gf_mfma(const uint16_t *const
 #pragma unroll
       for (int k = 0; k < 16; k++) acc[i][f][k] = 0;
 
+  using Set0 = std::integral_constant<int, 0>;
+  using Set1 = std::integral_constant<int, (kAhead > 1 ? 1 : 0)>;
   fetch_ptrs(0);
-  load_stage(0);
-  store_stage(0);
+  load_stage(0, Set0{});
+  store_stage(0, Set0{});
+  if (kAhead > 1 && 1 < nchunks) load_stage(1, Set1{});
   __syncthreads();
   const int bcol = wc * (kCols * 32) + kCols * n;  // this lane's first word inside the tile
-  for (int c = 0; c < nchunks; c++) {
+  // One stage: the loads of stage c + kAhead go out, stage c's MFMAs run from
+  // LDS buffer c % 2, then stage c+1 (loaded kAhead - 1 stages earlier) is
+  // stored to the other buffer, which was last read in stage c-1, which every
+  // wave finished before the previous barrier.  Stage k's registers are set
+  // k % kAhead, a compile-time index, so the loop runs two stages per turn.
+  auto stage = [&](int c, auto S) {
+    constexpr int kS = decltype(S)::value;
     const int buf = c & 1;
-    if (c + 1 < nchunks) load_stage(c + 1);
+    if (c + kAhead < nchunks) load_stage(c + kAhead, std::integral_constant<int, (kAhead > 1 ? kS : 0)>{});
     // all K-steps of the stage (K-steps past the last source carry zero
     // operands); each wave expands its own B columns, A comes pre-expanded.
     // Kept rolled: unrolled, the operands of 8 steps spill at 2 waves/SIMD.
@@ -317,10 +333,12 @@ __global__ __launch_bounds__(kMfThreads, 1) void k_gf_mfma(const uint16_t *const
         }
       }
     }
-    // the other buffer was last read in stage c-1, which every wave finished
-    // before the previous barrier
-    if (c + 1 < nchunks) store_stage(buf ^ 1);
+    if (c + 1 < nchunks) store_stage(buf ^ 1, std::integral_constant<int, (kAhead > 1 ? 1 - kS : 0)>{});
     __syncthreads();
+  };
+  for (int c = 0; c < nchunks; c += 2) {
+    stage(c, Set0{});
+    if (c + 1 < nchunks) stage(c + 1, Set1{});
   }
 
   // parity of each accumulator = bit q of output slice 2*(mf0+i) + g at words w..w+kCols-1
@@ -389,7 +407,7 @@ int32_t run_gf_mfma(par2hip_ctx *ctx, hipStream_t s, const GfProblem &p, bool in
     if (nfull) launch(k_gf_mfma<true, false>, nfull, 0);
     if (ntiles > nfull) launch(k_gf_mfma<true, true>, ntiles - nfull, nfull);
   } else if (shape == 1) {
-    if (nfull) launch(k_gf_mfma<false, false, 1, 2, 4, 4>, nfull, 0);
+    if (nfull) launch(k_gf_mfma<false, false, 1, 2, 4, 4, 2>, nfull, 0);
     if (ntiles > nfull) launch(k_gf_mfma<false, true, 1, 2, 4, 4>, ntiles - nfull, nfull);
   } else if (shape == 2) {
     if (nfull) launch(k_gf_mfma<false, false, 2, 2, 4, 8>, nfull, 0);
