@@ -50,24 +50,29 @@ constexpr int kMfWaves = 8;              // waves per workgroup (two per SIMD)
 constexpr int kMfThreads = 64 * kMfWaves;
 // Workgroup tile shapes: the 8 waves form kWR row groups x 8/kWR column
 // groups; each wave holds kRows A fragments (output pairs) x kCols B
-// fragments (32 words), kKc K-steps (4 sources each) per LDS stage.  Every
-// shape holds kRows * kCols = 8 accumulator tiles (128 registers) per lane.
+// fragments (32 words), kKc K-steps (4 sources each) per LDS stage.  A wave
+// holds 8 accumulator tiles (128 registers, one workgroup per CU) or 4 (64
+// registers, 48 KiB of LDS: two workgroups per CU, so one workgroup's barrier
+// and staging overlap the other's MFMAs).
 //   wide  (1, 4x2, 8): 8 outputs x 512 words, 32 sources per stage; the int8
 //         form's only shape
-//   small (1, 2x4, 4): 4 outputs x 1024 words, 16 sources per stage: at most
-//         4 outputs fill its rows, so no MFMA multiplies padded zero rows, and
-//         each input word is read by one workgroup only; its loads run two
-//         stages ahead (kAhead = 2: 2-3% faster, 1 GiB at R=4 in 0.30 ms
-//         against 0.31 on one device)
+//   small (1, 2x2, 4): 4 outputs x 512 words, 16 sources per stage, two
+//         workgroups per CU, loads two stages ahead: at most 4 outputs fill
+//         its rows, so no MFMA multiplies padded zero rows, and each input
+//         word is read by one workgroup only.  1 GiB at R=4 in 0.287-0.295 ms
+//         against 0.305-0.306 for a one-workgroup 4 x 1024 tile on the same
+//         device; 300 x 64 KiB at R=3 in 0.055 against 0.082 ms
 //   split (2, 2x4, 8): 8 outputs x 512 words; a wave reads 2 A fragments per
 //         K-step instead of 4, half the LDS traffic of A per MFMA.  Alone the
-//         fastest: 1024 x 1 MiB, R=102 in 5.17 ms (wide 5.71 ms), 2000 x
-//         512 KiB, R=100 in 4.89 ms (wide 5.46 ms)
+//         fastest: 1024 x 1 MiB, R=102 in 5.04-5.17 ms (wide 5.71 ms), 2000 x
+//         512 KiB, R=100 in 4.89-4.98 ms (wide 5.46 ms); a 4-tile form at two
+//         workgroups per CU took 5.23 ms
 //   tall  (4, 2x4, 4): 16 outputs x 256 words, 16 sources per stage; the
 //         slowest alone (6.20 ms on 1024 x 1 MiB, R=102) but the one that
 //         draws the least from the MD5 chains beside it: that encode step
-//         9.49 ms (wide 9.54 ms, split 9.93 ms, the chains slowing with the
-//         contraction's clock demand)
+//         9.49 ms (wide 9.54 ms, split 9.93 ms, the 4-tile split at two
+//         workgroups per CU 10.6 ms: the chains slow with the contraction's
+//         clock and issue demand)
 constexpr int kMfRows = 4, kMfCols = 2, kMfKc = 8;
 constexpr uint32_t kNib = 0x22222222u;
 
@@ -141,7 +146,7 @@ __device__ __forceinline__ v8i expand_fp4(uint32_t y) {
 // code, so its loads carry no branches.
 template <bool kI8, bool kRagged, int kWR = 1, int kRows = kMfRows, int kCols = kMfCols, int kKc = kMfKc,
           int kAhead = 1>
-__global__ __launch_bounds__(kMfThreads, 1) void k_gf_mfma(const uint16_t *const *__restrict__ srcs, int nsrc,
+__global__ __launch_bounds__(kMfThreads, (kRows * kCols == 4 ? 2 : 1)) void k_gf_mfma(const uint16_t *const *__restrict__ srcs, int nsrc,
                                                            uint16_t *const *__restrict__ dsts, int ndst,
                                                            const uint32_t *__restrict__ apk, int nmf, int nks,
                                                            long words, int accumulate, int ntiles, int nrowblk,
@@ -157,7 +162,7 @@ __global__ __launch_bounds__(kMfThreads, 1) void k_gf_mfma(const uint16_t *const
   constexpr int kPasses = kSrcStage / kSrcPass;
   constexpr int kSrcWave = kUnits >= 64 ? 1 : 64 / kUnits;  // sources one wave stages per pass
   constexpr int kApt = kKc * kFrags * 64 / kMfThreads;      // packed A words per thread and stage
-  static_assert(kRows * kCols == 8 && kWR * kWC == kMfWaves && (kUnits == 32 || kUnits % 64 == 0) &&
+  static_assert((kRows * kCols == 8 || kRows * kCols == 4) && kWR * kWC == kMfWaves && (kUnits == 32 || kUnits % 64 == 0) &&
                     kPasses * kSrcPass == kSrcStage && kPasses >= 1 && kPasses <= 4 &&
                     (kApt == 1 || kApt == 2 || kApt == 4),
                 "tile shape");
@@ -389,7 +394,7 @@ int32_t run_gf_mfma(par2hip_ctx *ctx, hipStream_t s, const GfProblem &p, bool in
   // tile shape: 0 wide, 1 small, 2 split, 3 tall (the table above)
   const int shape = int8 ? 0 : nmf <= 2 ? 1 : p.beside_chains ? 3 : 2;
   static const int kFragsOf[4] = {kMfRows, 2, 4, 8};
-  static const long kWordsOf[4] = {kMfWaves * kMfCols * 32, 1024, 512, 256};
+  static const long kWordsOf[4] = {kMfWaves * kMfCols * 32, 512, 512, 256};
   const int frags = kFragsOf[shape];
   const long wg_words = kWordsOf[shape];
   const int ntiles = (int)((p.words + wg_words - 1) / wg_words);
@@ -407,8 +412,8 @@ int32_t run_gf_mfma(par2hip_ctx *ctx, hipStream_t s, const GfProblem &p, bool in
     if (nfull) launch(k_gf_mfma<true, false>, nfull, 0);
     if (ntiles > nfull) launch(k_gf_mfma<true, true>, ntiles - nfull, nfull);
   } else if (shape == 1) {
-    if (nfull) launch(k_gf_mfma<false, false, 1, 2, 4, 4, 2>, nfull, 0);
-    if (ntiles > nfull) launch(k_gf_mfma<false, true, 1, 2, 4, 4>, ntiles - nfull, nfull);
+    if (nfull) launch(k_gf_mfma<false, false, 1, 2, 2, 4, 2>, nfull, 0);
+    if (ntiles > nfull) launch(k_gf_mfma<false, true, 1, 2, 2, 4>, ntiles - nfull, nfull);
   } else if (shape == 2) {
     if (nfull) launch(k_gf_mfma<false, false, 2, 2, 4, 8>, nfull, 0);
     if (ntiles > nfull) launch(k_gf_mfma<false, true, 2, 2, 4, 8>, ntiles - nfull, nfull);

@@ -78,7 +78,7 @@ def test_autotune_stays_exact_and_fills_table(ctx, tmp_path):
 
 
 # (n, slice bytes, r) for the matrix-core form's tile shapes: at most 4
-# outputs take the 4-output x 1024-word shape (16 sources per stage), more
+# outputs take the 4-output x 512-word shape (16 sources per stage), more
 # the 8-output x 512-word one; ragged last tiles, sources not a multiple of a
 # stage, one source, and the exact tile sizes
 MFMA_PROBLEMS = [(1, 2048, 1), (5, 2048, 2), (16, 4096, 3), (17, 2080, 4), (100, 6176, 4), (33, 1040, 3),
