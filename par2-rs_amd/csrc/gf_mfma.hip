@@ -143,10 +143,13 @@ __device__ __forceinline__ v8i expand_fp4(uint32_t y) {
 // of FP4 (A expanded once by the staging threads).  kRagged: the launch covers
 // tiles tile0.. whose words may run past the end of the rows (only the last
 // tile column), loaded word by word with checks; the main launch has no such
-// code, so its loads carry no branches.
+// code, so its loads carry no branches.  The second launch bound is waves per
+// SIMD (HIP passes it as amdgpu_waves_per_eu): 2 for the 8-tile shapes (one
+// 8-wave workgroup per CU), 4 for the 4-tile shape (two), which caps its
+// registers at 128.
 template <bool kI8, bool kRagged, int kWR = 1, int kRows = kMfRows, int kCols = kMfCols, int kKc = kMfKc,
           int kAhead = 1>
-__global__ __launch_bounds__(kMfThreads, (kRows * kCols == 4 ? 2 : 1)) void k_gf_mfma(const uint16_t *const *__restrict__ srcs, int nsrc,
+__global__ __launch_bounds__(kMfThreads, kRows * kCols == 4 ? 4 : 2) void k_gf_mfma(const uint16_t *const *__restrict__ srcs, int nsrc,
                                                            uint16_t *const *__restrict__ dsts, int ndst,
                                                            const uint32_t *__restrict__ apk, int nmf, int nks,
                                                            long words, int accumulate, int ntiles, int nrowblk,
