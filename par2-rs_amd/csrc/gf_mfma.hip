@@ -57,11 +57,15 @@ constexpr int kMfThreads = 64 * kMfWaves;
 //   wide  (1, 4x2, 8): 8 outputs x 512 words, 32 sources per stage; the int8
 //         form's only shape
 //   small (1, 2x2, 4): 4 outputs x 512 words, 16 sources per stage, two
-//         workgroups per CU, loads two stages ahead: at most 4 outputs fill
+//         workgroups per CU, loads two stages ahead: 3 or 4 outputs fill
 //         its rows, so no MFMA multiplies padded zero rows, and each input
 //         word is read by one workgroup only.  1 GiB at R=4 in 0.287-0.295 ms
 //         against 0.305-0.306 for a one-workgroup 4 x 1024 tile on the same
 //         device; 300 x 64 KiB at R=3 in 0.055 against 0.082 ms
+//   single (1, 1x4, 4): 2 outputs x 1024 words, two workgroups per CU, loads
+//         two stages ahead: one or two outputs (one A fragment), so no MFMA
+//         multiplies a padded fragment.  1 GiB at R=1 or 2 in 0.187 ms
+//         (72% of 8 TB/s) against 0.27-0.28 ms with the small tile
 //   split (2, 2x4, 8): 8 outputs x 512 words; a wave reads 2 A fragments per
 //         K-step instead of 4, half the LDS traffic of A per MFMA.  Alone the
 //         fastest: 1024 x 1 MiB, R=102 in 5.04-5.17 ms (wide 5.71 ms), 2000 x
@@ -164,10 +168,12 @@ __global__ __launch_bounds__(kMfThreads, kRows * kCols == 4 ? 4 : 2) void k_gf_m
   constexpr int kSrcStage = 4 * kKc;               // sources per stage
   constexpr int kPasses = kSrcStage / kSrcPass;
   constexpr int kSrcWave = kUnits >= 64 ? 1 : 64 / kUnits;  // sources one wave stages per pass
-  constexpr int kApt = kKc * kFrags * 64 / kMfThreads;      // packed A words per thread and stage
+  constexpr int kAWords = kKc * kFrags * 64;                // packed A words per stage
+  constexpr int kApt = kAWords >= kMfThreads ? kAWords / kMfThreads : 1;  // per staging thread
+  constexpr bool kAPart = kAWords < kMfThreads;             // only threads tid < kAWords stage A
   static_assert((kRows * kCols == 8 || kRows * kCols == 4) && kWR * kWC == kMfWaves && (kUnits == 32 || kUnits % 64 == 0) &&
                     kPasses * kSrcPass == kSrcStage && kPasses >= 1 && kPasses <= 4 &&
-                    (kApt == 1 || kApt == 2 || kApt == 4),
+                    (kApt == 1 || kApt == 2 || kApt == 4) && kApt * kMfThreads >= kAWords,
                 "tile shape");
   using AUnit = typename std::conditional<kI8, uint32_t, uint4>::type;
   using Acc = typename std::conditional<kI8, v16i, v16f>::type;
@@ -240,7 +246,10 @@ __global__ __launch_bounds__(kMfThreads, kRows * kCols == 4 ? 4 : 2) void k_gf_m
 #pragma unroll
     for (int k = 0; k < kPasses; k++) rb[S][k] = load_b(c, k, ok);
     {
-      const int e = kApt * tid, frag = e / (kKc * 64), ksl = (e / 64) % kKc, l0 = e % 64;
+      // a thread past the stage's A words (kAPart) loads word 0 again and
+      // stores nothing
+      const int e = (kAPart && tid >= kAWords) ? 0 : kApt * tid;
+      const int frag = e / (kKc * 64), ksl = (e / 64) % kKc, l0 = e % 64;
       const int ks = kKc * c + ksl;
       if (mf0 + frag < nmf && ks < nks) ok |= 1u << 8;
       const long idx = ((long)min(mf0 + frag, nmf - 1) * nks + min(ks, nks - 1)) * 64 + l0;
@@ -256,6 +265,7 @@ __global__ __launch_bounds__(kMfThreads, kRows * kCols == 4 ? 4 : 2) void k_gf_m
       const u32x4 v = (ok & (1u << k)) ? rb[S][k] : zero;
       sB[buf][tid / kUnits + kSrcPass * k][tid % kUnits] = make_uint4(v.x, v.y, v.z, v.w);
     }
+    if (kAPart && tid >= kAWords) return;
     const int e0 = kApt * tid, frag = e0 / (kKc * 64), ksl = (e0 / 64) % kKc, l0 = e0 % 64;
     uint32_t y[kApt];
     __builtin_memcpy(y, &ra[S], sizeof y);
@@ -395,9 +405,9 @@ int32_t run_gf_mfma(par2hip_ctx *ctx, hipStream_t s, const GfProblem &p, bool in
     P2H_CHECK(ctx, hipGetLastError());
   }
   // tile shape: 0 wide, 1 small, 2 split, 3 tall (the table above)
-  const int shape = int8 ? 0 : nmf <= 2 ? 1 : p.beside_chains ? 3 : 2;
-  static const int kFragsOf[4] = {kMfRows, 2, 4, 8};
-  static const long kWordsOf[4] = {kMfWaves * kMfCols * 32, 512, 512, 256};
+  const int shape = int8 ? 0 : nmf == 1 ? 4 : nmf == 2 ? 1 : p.beside_chains ? 3 : 2;
+  static const int kFragsOf[5] = {kMfRows, 2, 4, 8, 1};
+  static const long kWordsOf[5] = {kMfWaves * kMfCols * 32, 512, 512, 256, 1024};
   const int frags = kFragsOf[shape];
   const long wg_words = kWordsOf[shape];
   const int ntiles = (int)((p.words + wg_words - 1) / wg_words);
@@ -417,6 +427,9 @@ int32_t run_gf_mfma(par2hip_ctx *ctx, hipStream_t s, const GfProblem &p, bool in
   } else if (shape == 1) {
     if (nfull) launch(k_gf_mfma<false, false, 1, 2, 2, 4, 2>, nfull, 0);
     if (ntiles > nfull) launch(k_gf_mfma<false, true, 1, 2, 2, 4>, ntiles - nfull, nfull);
+  } else if (shape == 4) {
+    if (nfull) launch(k_gf_mfma<false, false, 1, 1, 4, 4, 2>, nfull, 0);
+    if (ntiles > nfull) launch(k_gf_mfma<false, true, 1, 1, 4, 4>, ntiles - nfull, nfull);
   } else if (shape == 2) {
     if (nfull) launch(k_gf_mfma<false, false, 2, 2, 4, 8>, nfull, 0);
     if (ntiles > nfull) launch(k_gf_mfma<false, true, 2, 2, 4, 8>, ntiles - nfull, nfull);
