@@ -77,12 +77,12 @@ def test_autotune_stays_exact_and_fills_table(ctx, tmp_path):
         ctx.set_autotune(prev)
 
 
-# (n, slice bytes, r) for the matrix-core form's tile shapes: at most 4
-# outputs take the 4-output x 512-word shape (16 sources per stage), more
-# the 8-output x 512-word one; ragged last tiles, sources not a multiple of a
-# stage, one source, and the exact tile sizes
-MFMA_PROBLEMS = [(1, 2048, 1), (5, 2048, 2), (16, 4096, 3), (17, 2080, 4), (100, 6176, 4), (33, 1040, 3),
-                 (40, 3088, 5), (7, 1024, 8), (50, 4112, 9)]
+# (n, slice bytes, r) for the matrix-core form's tile shapes: 1-2 outputs
+# take the 2-output x 1024-word shape, 3-4 the 4-output x 512-word one (16
+# sources per stage), more the 8-output x 512-word one; ragged last tiles,
+# sources not a multiple of a stage, one source, and the exact tile sizes
+MFMA_PROBLEMS = [(1, 2048, 1), (5, 2048, 2), (9, 2080, 1), (21, 4128, 2), (16, 4096, 3), (17, 2080, 4),
+                 (100, 6176, 4), (33, 1040, 3), (40, 3088, 5), (7, 1024, 8), (50, 4112, 9)]
 
 
 @pytest.mark.parametrize("kernel", [2, 3], ids=["fp4", "int8"])
