@@ -71,8 +71,7 @@ struct BatchPlan {
   long max_coeff = 0, pairs_ft = 0, pairs_mt = 0;
   int max_m = 0;
   bool aligned4 = true;
-  unsigned *cursors = nullptr;  // fold queues' cursors, then the main contraction's
-  XcdQueues q_fi{}, q_mi{};
+  unsigned *cursors = nullptr;
   std::vector<size_t> slow;  // repair jobs for the single-set path (gapped exponents)
 };
 
@@ -166,8 +165,7 @@ static int32_t plan_build(par2hip_ctx *ctx, const par2hip_job *jobs, size_t njob
                          span_align(J.d_out, J.slice_size, J.count)});
     main_tasks.push_back(tp);
   }
-  const size_t cursor_off = ar.take(2 * kXcds * kCursorStride * sizeof(unsigned));
-  static_assert(2 * kXcds * kCursorStride * sizeof(unsigned) <= kBatchArenaExtra, "cursor space");
+  const size_t cursor_off = ar.take(512);
   void *arena_v = ext_arena;
   int32_t rc;
   if (own) {
@@ -214,10 +212,7 @@ static int32_t plan_build(par2hip_ctx *ctx, const par2hip_job *jobs, size_t njob
   std::vector<GfItem> fold_items, main_items;
   make_items(fold_tasks, fold_items, fold_words);
   make_items(main_tasks, main_items, main_words);
-  if (main_items.size() > 0x7fffffff || fold_items.size() > 0x7fffffff)
-    return fail(ctx, PAR2HIP_EINVAL, "batch too large");
-  P.q_fi = split_by_xcd(fold_items);
-  P.q_mi = split_by_xcd(main_items);
+  if (main_items.size() > 0x7fffffff) return fail(ctx, PAR2HIP_EINVAL, "batch too large");
 
   // ---- hash lists
   std::vector<HashEntry> md5_list, crc_list, crc_small;
@@ -448,7 +443,7 @@ static int32_t plan_launch(par2hip_ctx *ctx, const BatchPlan &P) {
   if ((rc = launch_coeff_batch(ctx, s, P.d_cj, (int)P.n_cj, P.max_coeff))) return rc;
   if ((rc = launch_inverse_batch(ctx, s, P.d_ij, (int)P.n_ij, P.max_m, 0))) return rc;
   if ((rc = launch_tables_batch(ctx, s, P.d_ft, (int)P.n_ft, P.pairs_ft))) return rc;
-  if ((rc = launch_gf_batch(ctx, s, P.d_ft, P.d_fi, P.q_fi, P.cursors, "gf_fold"))) return rc;
+  if ((rc = launch_gf_batch(ctx, s, P.d_ft, P.d_fi, (int)P.n_fi, P.cursors, "gf_fold"))) return rc;
   if ((rc = launch_tables_batch(ctx, s, P.d_mt, (int)P.n_mt, P.pairs_mt))) return rc;
   if (wide) {
     P2H_CHECK(ctx, hipEventRecord(ctx->fork, s));
@@ -457,8 +452,7 @@ static int32_t plan_launch(par2hip_ctx *ctx, const BatchPlan &P) {
     if ((rc = launch_crc32_list(ctx, ctx->wide, P.d_crcl, P.n_crcl, P.d_cp))) return rc;
     if ((rc = launch_crc32_list_lane(ctx, ctx->wide, P.d_crcs, P.n_crcs))) return rc;
   }
-  if ((rc = launch_gf_batch(ctx, s, P.d_mt, P.d_mi, P.q_mi, P.cursors + kXcds * kCursorStride, "gf_batch")))
-    return rc;
+  if ((rc = launch_gf_batch(ctx, s, P.d_mt, P.d_mi, (int)P.n_mi, P.cursors + 64, "gf_batch"))) return rc;
   if (md5_side && (rc = join_overlap(ctx))) return rc;
   if (wide) {
     P2H_CHECK(ctx, hipEventRecord(ctx->join, ctx->wide));

@@ -240,20 +240,6 @@ struct GfTaskDev {
 struct GfItem {
   int task, tile, rowblk, pad_;
 };
-// The persistent batch contraction keeps one work queue per XCD: all row
-// blocks of one (task, word tile) sit in one queue, so the workgroups that
-// share a tile's input words share an L2 (each XCD has its own); queue q
-// holds items[off[q], off[q+1]).  A workgroup works through its own XCD's
-// queue (workgroup i runs on XCD i mod 8), then helps the others.
-constexpr int kXcds = 8;
-constexpr int kCursorStride = 16;  // uints between two queues' cursors (64 bytes)
-struct XcdQueues {
-  int off[kXcds + 1];
-};
-// Reorders items (cost-sorted, each tile's row blocks adjacent) into the
-// per-XCD queues: whole tiles dealt round-robin, so every queue keeps the
-// cost order and about an eighth of the work.
-XcdQueues split_by_xcd(std::vector<GfItem> &items);
 // Tile shape (words per thread v, 8-row groups g; 16 sources per LDS chunk)
 // of one batched task under a policy: a forced shape always wins.  The persistent batch kernel has the shapes with
 // v in {8, 4} and g <= 2 (its LDS holds 16 sources x 2 groups; more variants
@@ -276,14 +262,13 @@ constexpr int kBatchTpb = 256;  // threads of a batch contraction workgroup
 // kBatchArenaExtra), or nullptr for the context's own scratch.
 int32_t batch_run(par2hip_ctx *ctx, const par2hip_job *jobs, size_t njobs, uint8_t *arena, size_t arena_bytes);
 size_t batch_job_arena_bytes(const par2hip_job &job);
-constexpr size_t kBatchArenaExtra = 1024;  // the work-queue cursors of both contractions
+constexpr size_t kBatchArenaExtra = 512;
 // Nibble tables of every task (npairs = all tasks' (8-row group, source)
 // pairs, numbered by GfTaskDev::pair0), then the persistent kernel works
-// through `items` (per-XCD queues, each sorted by cost) with one atomic
-// cursor per queue (d_cursors[q * kCursorStride]).
+// through `items` (sorted by cost) with an atomic cursor.
 int32_t launch_tables_batch(par2hip_ctx *ctx, hipStream_t s, const GfTaskDev *d_tasks, int ntasks, long npairs);
 int32_t launch_gf_batch(par2hip_ctx *ctx, hipStream_t s, const GfTaskDev *d_tasks, const GfItem *d_items,
-                        const XcdQueues &queues, unsigned *d_cursors, const char *tag);
+                        int nitems, unsigned *d_cursor, const char *tag);
 // coefficient matrices of many encode jobs: coeff[r][d] = exp[(log_d * e_r) mod 65535]
 struct CoeffJob {
   uint16_t *coeff;

@@ -399,74 +399,43 @@ int32_t launch_tables_batch(par2hip_ctx *ctx, hipStream_t s, const GfTaskDev *d_
   return PAR2HIP_OK;
 }
 
-XcdQueues split_by_xcd(std::vector<GfItem> &items) {
-  std::vector<GfItem> q[kXcds];
-  int tiles = -1;
-  for (size_t i = 0; i < items.size(); i++) {
-    if (i == 0 || items[i].task != items[i - 1].task || items[i].tile != items[i - 1].tile) tiles++;
-    q[tiles % kXcds].push_back(items[i]);
-  }
-  XcdQueues out;
-  out.off[0] = 0;
-  items.clear();
-  for (int x = 0; x < kXcds; x++) {
-    items.insert(items.end(), q[x].begin(), q[x].end());
-    out.off[x + 1] = (int)items.size();
-  }
-  return out;
-}
-
-__global__ void k_reset_cursors(unsigned *cursors) {
-  if (threadIdx.x < kXcds) cursors[threadIdx.x * kCursorStride] = 0u;
-}
+__global__ void k_reset_cursor(unsigned *cursor) { *cursor = 0u; }
 static_assert(kBatchTpb == kTpb, "batch work items are planned for kTpb threads");
 
-// One work item of the batch contraction in the tile shape of its task
-// (uniform per item, so the branch is too).
-__device__ __forceinline__ void gf_batch_item(u32x4 *lds, const GfTaskDev &t, const GfItem &w) {
+// Persistent: every workgroup takes the next work item from the cursor until
+// none is left (every wave reaches the exit because the cursor only grows).
+// Items are sorted by cost (sources per task, descending) on the host.
+__global__ __launch_bounds__(kTpb, 4) void k_gf_lds_batch(const GfTaskDev *__restrict__ tasks,
+                                                         const GfItem *__restrict__ items, int nitems,
+                                                         unsigned *__restrict__ cursor) {
+  __shared__ u32x4 lds[16 * 2 * 64];  // D = 16 sources x up to 2 groups
+  __shared__ int s_item;
+  for (;;) {
+    if (threadIdx.x == 0) s_item = (int)atomicAdd(cursor, 1u);
+    __syncthreads();
+    const int it = s_item;
+    __syncthreads();  // all lanes hold `it` before lane 0 may overwrite s_item
+    if (it >= nitems) break;
+    const GfItem w = items[it];
+    const GfTaskDev t = tasks[w.task];
 #define P2H_BATCH_TILE(V, G)                                                                                  \
   gf_lds_tile<V, G, 16, true>(lds, t.srcs, t.nsrc, t.dsts, t.ndst, t.tabs, t.words, t.accumulate, w.tile, w.rowblk, \
                               t.align)
-  switch (t.v * 4 + t.groups) {
-    case 8 * 4 + 1: P2H_BATCH_TILE(8, 1); break;
-    case 4 * 4 + 2: P2H_BATCH_TILE(4, 2); break;
-    case 4 * 4 + 1: P2H_BATCH_TILE(4, 1); break;
-    default: P2H_BATCH_TILE(8, 2); break;
-  }
-#undef P2H_BATCH_TILE
-}
-
-// Persistent: every workgroup takes the next work item of its XCD's queue
-// until that is empty, then of the other queues in turn (every wave reaches
-// the exit: the cursors only grow and each queue is finite).  Each queue is
-// sorted by cost (sources per task, descending) on the host.
-__global__ __launch_bounds__(kTpb, 4) void k_gf_lds_batch(const GfTaskDev *__restrict__ tasks,
-                                                         const GfItem *__restrict__ items, XcdQueues queues,
-                                                         unsigned *__restrict__ cursors) {
-  __shared__ u32x4 lds[16 * 2 * 64];  // D = 16 sources x up to 2 groups
-  __shared__ int s_item;
-  const int home = blockIdx.x % kXcds;
-  for (int k = 0; k < kXcds; k++) {
-    const int qx = (home + k) % kXcds;
-    const int q0 = queues.off[qx], qn = queues.off[qx + 1] - q0;
-    for (;;) {
-      if (threadIdx.x == 0) s_item = (int)atomicAdd(cursors + qx * kCursorStride, 1u);
-      __syncthreads();
-      const int it = s_item;
-      __syncthreads();  // all lanes hold `it` before lane 0 may overwrite s_item
-      if (it >= qn) break;
-      const GfItem w = items[q0 + it];
-      const GfTaskDev t = tasks[w.task];
-      gf_batch_item(lds, t, w);
+    // the task's tile shape (uniform per item, so the branch is too)
+    switch (t.v * 4 + t.groups) {
+      case 8 * 4 + 1: P2H_BATCH_TILE(8, 1); break;
+      case 4 * 4 + 2: P2H_BATCH_TILE(4, 2); break;
+      case 4 * 4 + 1: P2H_BATCH_TILE(4, 1); break;
+      default: P2H_BATCH_TILE(8, 2); break;
     }
+#undef P2H_BATCH_TILE
   }
 }
 
 int32_t launch_gf_batch(par2hip_ctx *ctx, hipStream_t s, const GfTaskDev *d_tasks, const GfItem *d_items,
-                        const XcdQueues &queues, unsigned *d_cursors, const char *tag) {
-  const int nitems = queues.off[kXcds];
+                        int nitems, unsigned *d_cursor, const char *tag) {
   if (nitems == 0) return PAR2HIP_OK;
-  hipLaunchKernelGGL(k_reset_cursors, dim3(1), dim3(64), 0, s, d_cursors);
+  hipLaunchKernelGGL(k_reset_cursor, dim3(1), dim3(1), 0, s, d_cursor);
   // as many persistent workgroups as fit on the device at once (registers and
   // LDS decide; the work list absorbs any that start late)
   static const int per_cu = [] {
@@ -479,7 +448,7 @@ int32_t launch_gf_batch(par2hip_ctx *ctx, hipStream_t s, const GfTaskDev *d_task
   }();
   ScopedTiming t(ctx, tag, s);
   const int grid = std::min(nitems, per_cu * ctx->num_cus);
-  hipLaunchKernelGGL(k_gf_lds_batch, dim3(grid), dim3(kTpb), 0, s, d_tasks, d_items, queues, d_cursors);
+  hipLaunchKernelGGL(k_gf_lds_batch, dim3(grid), dim3(kTpb), 0, s, d_tasks, d_items, nitems, d_cursor);
   P2H_CHECK(ctx, hipGetLastError());
   return PAR2HIP_OK;
 }
