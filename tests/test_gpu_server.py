@@ -160,6 +160,26 @@ def test_server_rejects_bad_requests(ctx):
         srv.close()
 
 
+def test_server_md5_slice_limit(ctx):
+    """Hashed slices of 128 GiB or more are refused at submission (the MD5
+    kernel's block counter and bit length stay exact below); one just under
+    the limit passes that check and is refused only because it can never fit
+    the 1 MiB budget.  Every case is refused before anything is launched."""
+    srv = par2hip.Server(0, hbm_budget=1 << 20, lanes=1, max_batch=4)
+    try:
+        d = ctx.alloc(4096)
+        limit = 1 << 37
+        with pytest.raises(par2hip.InvalidFormat):
+            srv.submit({"kind": "encode", "slices": d, "n": 1, "slice_size": limit, "exps": []}, hashes=True)
+        with pytest.raises(par2hip.DeviceError):  # unhashed: only the budget refuses it
+            srv.submit({"kind": "encode", "slices": d, "n": 1, "slice_size": limit, "exps": []}, hashes=False)
+        with pytest.raises(par2hip.DeviceError):  # just under the limit: the budget refuses it
+            srv.submit({"kind": "encode", "slices": d, "n": 1, "slice_size": limit - 2, "exps": []}, hashes=True)
+        assert srv.stats()["submitted"] == 0
+    finally:
+        srv.close()
+
+
 def test_server_wait_timeout_on_budget_blocked_ticket(ctx):
     """FIFO admission: with the budget held by an unreleased result, a later
     request cannot be admitted; waiting on it with a limit times out instead
