@@ -177,11 +177,12 @@ __global__ __launch_bounds__(kMfThreads, kRows * kCols == 4 ? 4 : 2) void k_gf_m
                 "tile shape");
   using AUnit = typename std::conditional<kI8, uint32_t, uint4>::type;
   using Acc = typename std::conditional<kI8, v16i, v16f>::type;
-  // source rows of the 8-tile shapes padded by 32 bytes: the two lane halves
-  // of a B read hit rows two apart, which without the pad start on the same
-  // LDS bank (C2 alone 5.04-5.07 -> 4.99-5.01 ms, the tall shape in the
-  // encode step 7.14 -> 6.93 ms); the two-workgroup shapes, within 1-2% either
-  // way, keep their LDS
+  // source rows of the 8-tile shapes padded by 32 bytes, so that the rows two
+  // apart that the two lane halves of a B read address do not start on the
+  // same LDS bank: measured faster (C2 alone 5.04-5.07 -> 4.99-5.01 ms, the
+  // tall shape in the encode step 7.14 -> 6.93 ms), though the bank-conflict
+  // counter stays at 16-17%, so not all of the gain is that; the two-workgroup
+  // shapes, within 1-2% either way, keep their LDS
   constexpr int kPad = kRows * kCols == 8 ? 2 : 0;
   __shared__ uint4 sB[2][kSrcStage][kUnits + kPad];  // [buf][source][16-byte unit]
   __shared__ AUnit sA[2][kKc][kFrags][64];    // [buf][ks][fragment][lane]
