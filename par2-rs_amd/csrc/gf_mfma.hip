@@ -8,8 +8,8 @@
 // with A = the stacked bit matrices (16R x 16N) and B = the bits of the input
 // words (16N x W).  Integer sums of 0/1 products are exact in fp32 up to
 // 2^24 > 16N, so the product runs on v_mfma_scale_f32_32x32x64_f8f6f4 with
-// FP4 (e2m1) operands, 1.0 = 0b0010, and the parity is read off the fp32
-// accumulator at the end.
+// FP4 (e2m1) operands whose products are 0 or exactly 1.0, and the parity is
+// read off the fp32 accumulator at the end.
 //
 // One 32x32x64 MFMA covers 32 A-rows = 2 output slices x 16 bits, 64 K = 4
 // sources x 16 bits, and 32 words.  Lane layout (checked on the device):
@@ -21,10 +21,14 @@
 // h = (m/4)%2, j = 4*(m/8) + m%4, so accumulator q is bit q.
 //
 // K order inside one 64-wide step (sources 4ks..4ks+3): lane half g holds
-// sources 4ks+2g+s, s in {0,1}; with y = x_s0 | x_s1 << 16 the four operand
-// registers are ((y << 1), y, (y >> 1), (y >> 2)) & 0x22222222, i.e. nibble p
-// of register t carries bit 4*(p%4)+t of source s = p/4.  A is pre-packed in
-// the same shape (one u32 per lane per fragment per step, expanded on chip).
+// sources 4ks+2g+s, s in {0,1}; with y = x_s0 | x_s1 << 16, nibble p of
+// operand register t carries bit 4*(p%4)+t of source s = p/4.  B leaves each
+// bit where it lies in its nibble as far as e2m1 allows: registers
+// y & 0x11111111 (a set bit reads 0.5), y & 0x22222222 (1.0), y & 0x44444444
+// (2.0) and (y >> 1) & 0x44444444 (bit 3 is the sign, so it moves to 2.0):
+// five VALU operations per operand.  A is pre-packed in the same shape (one
+// u32 per lane per fragment per step) and expanded on chip to the reciprocal
+// values 2.0, 1.0, 0.5, 0.5, so every product of two set bits is exactly 1.0.
 //
 // The int8 form runs the same contraction on v_mfma_i32_32x32x32_i8 (0/1
 // bytes, int32 accumulators, parity of the integer sum): a 64-wide step is
@@ -78,7 +82,7 @@ constexpr int kMfThreads = 64 * kMfWaves;
 //         workgroups per CU 10.6 ms: the chains slow with the contraction's
 //         clock and issue demand)
 constexpr int kMfRows = 4, kMfCols = 2, kMfKc = 8;
-constexpr uint32_t kNib = 0x22222222u;
+constexpr uint32_t kNib1 = 0x11111111u, kNib2 = 0x22222222u, kNib4 = 0x44444444u;
 
 __device__ __forceinline__ uint16_t mul_x(uint16_t v) {
   return (uint16_t)((v << 1) ^ ((v & 0x8000u) ? (kPoly & 0xFFFFu) : 0u));
@@ -126,13 +130,21 @@ __device__ __forceinline__ v4i expand_i8(uint32_t y, int g) {
   return o;
 }
 
+// B operand (input bits) of a packed pair y: e2m1 values 0.5, 1.0, 2.0, 2.0
+// for bits 4p, 4p+1, 4p+2, 4p+3 of y
 __device__ __forceinline__ v8i expand_fp4(uint32_t y) {
   v8i o = {0, 0, 0, 0, 0, 0, 0, 0};
-  o[0] = (int)((y << 1) & kNib);
-  o[1] = (int)(y & kNib);
-  o[2] = (int)((y >> 1) & kNib);
-  o[3] = (int)((y >> 2) & kNib);
+  o[0] = (int)(y & kNib1);
+  o[1] = (int)(y & kNib2);
+  o[2] = (int)(y & kNib4);
+  o[3] = (int)((y >> 1) & kNib4);
   return o;
+}
+
+// A operand (bit-matrix entries) of a packed pair y: the reciprocals 2.0,
+// 1.0, 0.5, 0.5 of expand_fp4's values, so each product is 0 or 1.0
+__device__ __forceinline__ uint4 expand_fp4_a(uint32_t y) {
+  return make_uint4((y << 2) & kNib4, y & kNib2, (y >> 2) & kNib1, (y >> 3) & kNib1);
 }
 
 // Workgroup: 8 outputs (4 A fragments) x 512 words (8 waves x 2 B fragments).
@@ -184,8 +196,13 @@ __global__ __launch_bounds__(kMfThreads, kRows * kCols == 4 ? 4 : 2) void k_gf_m
   // counter stays at 16-17%, so not all of the gain is that; the two-workgroup
   // shapes, within 1-2% either way, keep their LDS
   constexpr int kPad = kRows * kCols == 8 ? 2 : 0;
-  __shared__ uint4 sB[2][kSrcStage][kUnits + kPad];  // [buf][source][16-byte unit]
-  __shared__ AUnit sA[2][kKc][kFrags][64];    // [buf][ks][fragment][lane]
+  // one LDS block, B first: the B reads of a K-step then take their row
+  // offsets as instruction immediates (below 64 KiB) instead of VALU adds
+  constexpr int kBBytes = 2 * kSrcStage * (kUnits + kPad) * 16;
+  constexpr int kABytes = 2 * kKc * kFrags * 64 * (int)sizeof(AUnit);
+  __shared__ uint4 smem[(kBBytes + kABytes) / 16];
+  auto &sB = *reinterpret_cast<uint4(*)[2][kSrcStage][kUnits + kPad]>(smem);             // [buf][source][16-byte unit]
+  auto &sA = *reinterpret_cast<AUnit(*)[2][kKc][kFrags][64]>(smem + kBBytes / 16);  // [buf][ks][fragment][lane]
 
   const int L = blockIdx.x;
   const int xcd = L & 7, q = L >> 3;
@@ -282,8 +299,7 @@ __global__ __launch_bounds__(kMfThreads, kRows * kCols == 4 ? 4 : 2) void k_gf_m
       if constexpr (kI8)
         sA[buf][ksl][frag][l0 + e] = y[e];
       else
-        sA[buf][ksl][frag][l0 + e] =
-            make_uint4((y[e] << 1) & kNib, y[e] & kNib, (y[e] >> 1) & kNib, (y[e] >> 2) & kNib);
+        sA[buf][ksl][frag][l0 + e] = expand_fp4_a(y[e]);
     }
   };
 
