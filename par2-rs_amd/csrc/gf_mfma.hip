@@ -90,8 +90,10 @@ __device__ __forceinline__ uint16_t mul_x(uint16_t v) {
 
 // Apk[mf][ks][lane]: the A operand of fragment mf (outputs 2mf, 2mf+1) at
 // K-step ks, packed as two 16-bit rows of the transposed bit matrices.
+// perm: bits 0 and 2 of every nibble swapped (the ring form's packing, so
+// that each wave expands A with five operations, see expand_fp4_a_ring).
 __global__ void k_pack_a(uint32_t *__restrict__ apk, const uint16_t *__restrict__ coeff, int nrow,
-                         int nsrc, int ldc, int nmf, int nks) {
+                         int nsrc, int ldc, int nmf, int nks, int perm) {
   const long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (idx >= (long)nmf * nks * 64) return;
   const int lane = (int)(idx & 63);
@@ -115,6 +117,7 @@ __global__ void k_pack_a(uint32_t *__restrict__ apk, const uint16_t *__restrict_
     }
     out |= t << (16 * s);
   }
+  if (perm) out = (out & 0xAAAAAAAAu) | ((out & kNib1) << 2) | ((out >> 2) & kNib1);
   apk[idx] = out;
 }
 
@@ -145,6 +148,17 @@ __device__ __forceinline__ v8i expand_fp4(uint32_t y) {
 // 1.0, 0.5, 0.5 of expand_fp4's values, so each product is 0 or 1.0
 __device__ __forceinline__ uint4 expand_fp4_a(uint32_t y) {
   return make_uint4((y << 2) & kNib4, y & kNib2, (y >> 2) & kNib1, (y >> 3) & kNib1);
+}
+
+// The same A operand from the ring form's packing (bits 0 and 2 of each
+// nibble swapped by k_pack_a): five operations instead of seven.
+__device__ __forceinline__ v8i expand_fp4_a_ring(uint32_t y) {
+  v8i o = {0, 0, 0, 0, 0, 0, 0, 0};
+  o[0] = (int)(y & kNib4);
+  o[1] = (int)(y & kNib2);
+  o[2] = (int)(y & kNib1);
+  o[3] = (int)((y >> 3) & kNib1);
+  return o;
 }
 
 // Workgroup: 8 outputs (4 A fragments) x 512 words (8 waves x 2 B fragments).
@@ -413,20 +427,206 @@ __global__ __launch_bounds__(kMfThreads, kRows * kCols == 4 ? 4 : 2) void k_gf_m
   }
 }
 
+
+// Ring form of the FP4 contraction: the input words and the packed A of each
+// stage go global -> LDS directly (global_load_lds_dwordx4: no VGPR staging
+// and no store pass) into one of two LDS stage buffers, one stage ahead of
+// the MFMAs; every wave expands its own A and B operands (five VALU
+// operations each).  The loads of stage c+1 go out at the start of stage c
+// into the buffer stage c-1 used, which every wave finished reading before
+// the barrier that ended stage c-1; at the end of stage c every wave waits
+// for all of its loads (vmcnt(0): no assumption about the order in which
+// the LDS loads and the row-pointer load retire) and a plain s_barrier (no
+// fence, whose release would wait for the same loads again) publishes stage
+// c+1.  The two buffers of each role are separate LDS objects selected at
+// compile time (the stage loop turns two stages at a time), so the compiler
+// sees that the loads in flight do not write the buffer being read.  Full
+// word tiles only: the ragged last tile column takes k_gf_mfma<.., true, ..>
+// of the same tile width.
+// A is packed with bits 0 and 2 of each nibble swapped (k_pack_a perm=1) and
+// padded with zeros to whole stages and fragments, so no load needs a
+// bounds check; source rows past the last one re-read the last row, which
+// the zero A entries of those sources cancel (no e2m1 code is NaN or Inf).
+template <int kWR, int kRows, int kCols, int kKc>
+__global__ __launch_bounds__(kMfThreads, 2) void k_gf_mfma_ring(const uint16_t *const *__restrict__ srcs, int nsrc,
+                                                                uint16_t *const *__restrict__ dsts, int ndst,
+                                                                const uint32_t *__restrict__ apk, int nks_pad,
+                                                                long words, int accumulate, int ntiles,
+                                                                int nrowblk) {
+  constexpr int kWC = kMfWaves / kWR;
+  constexpr int kFrags = kWR * kRows;         // A fragments (output pairs) per workgroup
+  constexpr int kWgWords = kWC * kCols * 32;  // words per workgroup
+  constexpr int kRowBytes = kWgWords * 2;     // bytes of one source row of the tile
+  constexpr int kSrcStage = 4 * kKc;          // sources per stage
+  constexpr int kBBytes = kSrcStage * kRowBytes;
+  constexpr int kFragBytes = kKc * 256;  // packed A of one fragment for one stage
+  constexpr int kABytes = kFrags * kFragBytes;
+  constexpr int kRowsPerLoad = 1024 / kRowBytes;  // rows one 64-lane x 16-byte load covers
+  constexpr int kBLoads = kBBytes / 1024, kALoads = kABytes / 1024;
+  constexpr int kBPerWave = kBLoads / kMfWaves, kAPerWave = kALoads / kMfWaves;
+  constexpr int kLoadsPerWave = kBPerWave + kAPerWave;
+  static_assert(kRows * kCols == 8 && kWR * kWC == kMfWaves && (kRowsPerLoad == 1 || kRowsPerLoad == 2) &&
+                    kBLoads % kMfWaves == 0 && kALoads % kMfWaves == 0 && kFragBytes % 1024 == 0 &&
+                    kLoadsPerWave <= 64,
+                "ring tile shape");
+  __shared__ uint4 sB0[kBBytes / 16], sB1[kBBytes / 16];
+  __shared__ uint32_t sA0[kABytes / 4], sA1[kABytes / 4];
+
+  const int L = blockIdx.x;
+  const int xcd = L & 7, q = L >> 3;
+  const int tiles_per_xcd = (ntiles + 7) >> 3;
+  const int local_tile = q / nrowblk, rowblk = q % nrowblk;
+  const int tile = local_tile * 8 + xcd;
+  if (local_tile >= tiles_per_xcd || tile >= ntiles) return;
+
+  const int tid = threadIdx.x;
+  const int wave = __builtin_amdgcn_readfirstlane(tid >> 6), lane = tid & 63;
+  const int n = lane & 31, g = lane >> 5;
+  const int wr = wave / kWC, wc = wave % kWC;
+  const int mf0 = rowblk * kFrags;
+  const long tile_w0 = (long)tile * kWgWords;
+  const int nchunks = nks_pad / kKc;
+
+  // the loads of stage c into one buffer set: B load j (j = wave + 8k) covers
+  // rows kRowsPerLoad*j.. of the stage, A load j bytes 1024j.. of the
+  // stage's fragment-major packed A
+  const int lrow = kRowsPerLoad == 2 ? (lane >> 5) : 0;  // this lane's row within a B load
+  const int lunit = kRowsPerLoad == 2 ? (lane & 31) : lane;
+  // The row pointers of the next stage to issue, fetched one stage before
+  // with a vector load (lane kRowsPerLoad*k + h holds the pointer of row h of
+  // B load k) and read back with readlane: a scalar load in flight would turn
+  // every wait on an LDS read into a wait for it.  The pointer load goes out
+  // before the stage's LDS loads, so it is older than them in the vmcnt order.
+  constexpr int kPtrs = kBPerWave * kRowsPerLoad;
+  static_assert(kPtrs <= 64, "row pointers per wave");
+  uint64_t pv = 0;
+  auto fetch_ptrs = [&](int c) {
+    const int l = min(lane, kPtrs - 1);
+    const int row = kSrcStage * c + kRowsPerLoad * (wave + kMfWaves * (l / kRowsPerLoad)) + l % kRowsPerLoad;
+    pv = reinterpret_cast<const P2H_GLOBAL uint64_t *>(gptr(srcs))[min(row, nsrc - 1)];
+  };
+  auto issue = [&](int c, uint4 *bB, uint32_t *bA) {
+    uint64_t pr[kPtrs];
+#pragma unroll
+    for (int i = 0; i < kPtrs; i++)
+      pr[i] = (uint64_t)(uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)pv, i) |  // zero-extended halves
+              (uint64_t)(uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(pv >> 32), i) << 32;
+    fetch_ptrs(c + 1);
+#pragma unroll
+    for (int k = 0; k < kBPerWave; k++) {
+      const int j = wave + kMfWaves * k;
+      const uint16_t *p = reinterpret_cast<const uint16_t *>(
+          (kRowsPerLoad == 2 && lrow) ? pr[kRowsPerLoad * k + kRowsPerLoad - 1] : pr[kRowsPerLoad * k]);
+      __builtin_amdgcn_global_load_lds((const void *)(p + tile_w0 + 8 * lunit),
+                                       (__attribute__((address_space(3))) void *)(bB + 64 * j), 16, 0, 0);
+    }
+#pragma unroll
+    for (int k = 0; k < kAPerWave; k++) {
+      const int j = wave + kMfWaves * k;
+      const int frag = (1024 * j) / kFragBytes, off = (1024 * j) % kFragBytes;
+      const uint8_t *a = reinterpret_cast<const uint8_t *>(apk) +
+                         ((long)(mf0 + frag) * nks_pad + (long)kKc * c) * 256 + off + 16 * lane;
+      __builtin_amdgcn_global_load_lds((const void *)a,
+                                       (__attribute__((address_space(3))) void *)(bA + 256 * j), 16, 0, 0);
+    }
+  };
+
+  v16f acc[kRows][kCols];
+#pragma unroll
+  for (int i = 0; i < kRows; i++)
+#pragma unroll
+    for (int f = 0; f < kCols; f++)
+#pragma unroll
+      for (int k = 0; k < 16; k++) acc[i][f][k] = 0;
+
+  // wait for this wave's loads, then for every wave's (the barrier's fences
+  // keep the compiler from moving LDS reads across it)
+  auto publish = [&] {
+    // the stage's MFMAs stay above the wait: they touch no memory, so the
+    // compiler would otherwise sink them below the barrier and expose the
+    // full latency of the loads just sent; an empty asm that takes one
+    // element of every accumulator pins them here
+#pragma unroll
+    for (int i = 0; i < kRows; i++)
+#pragma unroll
+      for (int f = 0; f < kCols; f++) asm volatile("" ::"v"(acc[i][f][0]));
+    __builtin_amdgcn_s_waitcnt((7 << 4) | (15 << 8));  // vmcnt(0)
+    __syncthreads();
+  };
+  const int bcol = wc * (kCols * 32) + kCols * n;  // this lane's first word inside the tile
+  auto kloop = [&](const uint4 *bB, const uint32_t *bA) {
+#pragma unroll
+    for (int ksl = 0; ksl < kKc; ksl++) {
+      const uint32_t *r0 = reinterpret_cast<const uint32_t *>(bB) + (4 * ksl + 2 * g) * (kRowBytes / 4);
+      const uint32_t *r1 = r0 + kRowBytes / 4;
+      uint32_t x0[kCols / 2], x1[kCols / 2];
+#pragma unroll
+      for (int h = 0; h < kCols / 2; h++) {
+        x0[h] = r0[(bcol >> 1) + h];
+        x1[h] = r1[(bcol >> 1) + h];
+      }
+      uint32_t ya[kRows];
+#pragma unroll
+      for (int i = 0; i < kRows; i++) ya[i] = bA[((wr * kRows + i) * kKc + ksl) * 64 + lane];
+      v8i b[kCols];
+#pragma unroll
+      for (int h = 0; h < kCols / 2; h++) {
+        b[2 * h] = expand_fp4(__builtin_amdgcn_perm(x1[h], x0[h], 0x05040100u));      // even word
+        b[2 * h + 1] = expand_fp4(__builtin_amdgcn_perm(x1[h], x0[h], 0x07060302u));  // odd word
+      }
+#pragma unroll
+      for (int i = 0; i < kRows; i++) {
+        const v8i a = expand_fp4_a_ring(ya[i]);
+#pragma unroll
+        for (int f = 0; f < kCols; f++)
+          acc[i][f] = __builtin_amdgcn_mfma_scale_f32_32x32x64_f8f6f4(a, b[f], acc[i][f], 4, 4, 0, 127, 0, 127);
+      }
+    }
+  };
+  // stage c computes from (bB, bA) after sending the loads of stage c+1 into
+  // (nB, nA)
+  auto stage = [&](int c, const uint4 *bB, const uint32_t *bA, uint4 *nB, uint32_t *nA) {
+    if (c + 1 < nchunks) issue(c + 1, nB, nA);
+    kloop(bB, bA);
+    if (c + 1 < nchunks) publish();
+  };
+  fetch_ptrs(0);
+  issue(0, sB0, sA0);
+  publish();
+  for (int c = 0; c < nchunks; c += 2) {
+    stage(c, sB0, sA0, sB1, sA1);
+    if (c + 1 < nchunks) stage(c + 1, sB1, sA1, sB0, sA0);
+  }
+  // the row-pointer load of the stage after the last one is still in flight
+  __builtin_amdgcn_s_waitcnt((7 << 4) | (15 << 8));
+
+  // parity of each accumulator = bit q of output slice 2*(mf0+i) + g at words w..w+kCols-1
+  const long w = tile_w0 + bcol;
+#pragma unroll
+  for (int i = 0; i < kRows; i++) {
+    const int r = 2 * (mf0 + wr * kRows + i) + g;
+    if (r >= ndst) continue;
+    uint32_t v[kCols];
+#pragma unroll
+    for (int f = 0; f < kCols; f++) {
+      uint32_t word = 0;
+#pragma unroll
+      for (int k = 0; k < 16; k++) word |= ((uint32_t)acc[i][f][k] & 1u) << k;  // exact integer sums
+      v[f] = word;
+    }
+    P2H_GLOBAL uint16_t *p = gptr(gptr(dsts)[r]);
+#pragma unroll
+    for (int h = 0; h < kCols / 2; h++) {
+      uint32_t out = v[2 * h] | (v[2 * h + 1] << 16);
+      if (accumulate) out ^= *reinterpret_cast<const P2H_GLOBAL uint32_t *>(p + w + 2 * h);
+      *reinterpret_cast<P2H_GLOBAL uint32_t *>(p + w + 2 * h) = out;
+    }
+  }
+}
+
 int32_t run_gf_mfma(par2hip_ctx *ctx, hipStream_t s, const GfProblem &p, bool int8) {
   const int nmf = (p.ndst + 1) / 2;
   const int nks = (p.nsrc + 3) / 4;
-  const size_t apk_bytes = (size_t)nmf * nks * 64 * sizeof(uint32_t);
-  void *apk = nullptr;
-  int32_t rc = scratch(ctx, kScratchTables, apk_bytes, &apk);
-  if (rc) return rc;
-  {
-    const long total = (long)nmf * nks * 64;
-    ScopedTiming t(ctx, "pack_bitmatrices", s);
-    hipLaunchKernelGGL(k_pack_a, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, s, (uint32_t *)apk, p.coeff,
-                       p.ndst, p.nsrc, p.ldc ? p.ldc : p.nsrc, nmf, nks);
-    P2H_CHECK(ctx, hipGetLastError());
-  }
   // tile shape: 0 wide, 1 small, 2 split, 3 tall (the table above)
   const int shape = int8 ? 0 : nmf == 1 ? 4 : nmf == 2 ? 1 : p.beside_chains ? 3 : 2;
   static const int kFragsOf[5] = {kMfRows, 2, 4, 8, 1};
@@ -438,11 +638,48 @@ int32_t run_gf_mfma(par2hip_ctx *ctx, hipStream_t s, const GfProblem &p, bool in
   const int nrowblk = (nmf + frags - 1) / frags;
   const long nwg = 8L * ((ntiles + 7) / 8) * nrowblk;
   if (nwg > 0x7fffffffL) return fail(ctx, PAR2HIP_EINVAL, "GF problem too large for one launch");
+  // the split shape runs its full tiles in the ring form, whose packed A is
+  // permuted and padded to whole fragments and stages; the other shapes and
+  // the ragged tile column take the plain packing.  The tall shape (beside the
+  // MD5 chains) keeps the staged form: its ring form was faster (the 1 GiB,
+  // R=102 contraction on 224 CUs in 5.83-5.89 ms against 6.65-6.73) but drew
+  // the shader clock down under the chains, so the encode step slowed (MD5
+  // 9.46-9.54 -> 10.06-10.13 ms, step 9.57-9.65 -> 10.21-10.27 ms, same box)
+  constexpr int kRingKc = 8;
+  const bool ring = !int8 && shape == 2 && nfull > 0;
+  const bool plain = !ring || ntiles > nfull;
+  const int nmf_pad = nrowblk * frags, nks_pad = (nks + kRingKc - 1) / kRingKc * kRingKc;
+  const size_t apk_bytes = plain ? (size_t)nmf * nks * 64 * sizeof(uint32_t) : 0;
+  const size_t ring_bytes = ring ? (size_t)nmf_pad * nks_pad * 64 * sizeof(uint32_t) : 0;
+  void *buf = nullptr;
+  int32_t rc = scratch(ctx, kScratchTables, apk_bytes + ring_bytes, &buf);
+  if (rc) return rc;
+  uint32_t *apk = (uint32_t *)buf, *apk_ring = (uint32_t *)((char *)buf + apk_bytes);
+  {
+    ScopedTiming t(ctx, "pack_bitmatrices", s);
+    const int ldc = p.ldc ? p.ldc : p.nsrc;
+    if (plain) {
+      const long total = (long)nmf * nks * 64;
+      hipLaunchKernelGGL(k_pack_a, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, s, apk, p.coeff, p.ndst,
+                         p.nsrc, ldc, nmf, nks, 0);
+    }
+    if (ring) {
+      const long total = (long)nmf_pad * nks_pad * 64;
+      hipLaunchKernelGGL(k_pack_a, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, s, apk_ring, p.coeff,
+                         p.ndst, p.nsrc, ldc, nmf_pad, nks_pad, 1);
+    }
+    P2H_CHECK(ctx, hipGetLastError());
+  }
   ScopedTiming t(ctx, p.tag ? p.tag : (int8 ? "gf_muladd_mfma_i8" : "gf_muladd_mfma"), s);
   auto launch = [&](auto kernel, int tiles, int tile0) {
     const long grid = 8L * ((tiles + 7) / 8) * nrowblk;
     hipLaunchKernelGGL(kernel, dim3((unsigned)grid), dim3(kMfThreads), 0, s, p.srcs, p.nsrc, p.dsts, p.ndst,
                        (const uint32_t *)apk, nmf, nks, (long)p.words, p.accumulate ? 1 : 0, tiles, nrowblk, tile0);
+  };
+  auto launch_ring = [&](auto kernel) {
+    const long grid = 8L * ((nfull + 7) / 8) * nrowblk;
+    hipLaunchKernelGGL(kernel, dim3((unsigned)grid), dim3(kMfThreads), 0, s, p.srcs, p.nsrc, p.dsts, p.ndst,
+                       (const uint32_t *)apk_ring, nks_pad, (long)p.words, p.accumulate ? 1 : 0, nfull, nrowblk);
   };
   if (int8) {
     if (nfull) launch(k_gf_mfma<true, false>, nfull, 0);
@@ -454,7 +691,7 @@ int32_t run_gf_mfma(par2hip_ctx *ctx, hipStream_t s, const GfProblem &p, bool in
     if (nfull) launch(k_gf_mfma<false, false, 1, 1, 4, 4, 2>, nfull, 0);
     if (ntiles > nfull) launch(k_gf_mfma<false, true, 1, 1, 4, 4>, ntiles - nfull, nfull);
   } else if (shape == 2) {
-    if (nfull) launch(k_gf_mfma<false, false, 2, 2, 4, 8>, nfull, 0);
+    if (nfull) launch_ring(k_gf_mfma_ring<2, 2, 4, kRingKc>);
     if (ntiles > nfull) launch(k_gf_mfma<false, true, 2, 2, 4, 8>, ntiles - nfull, nfull);
   } else if (shape == 3) {
     if (nfull) launch(k_gf_mfma<false, false, 4, 2, 4, 4>, nfull, 0);
