@@ -70,6 +70,8 @@ def kernel_tag(symbol: str) -> str:
     base = name.split("<")[0]
     if base == "k_gf_mfma":
         return "gf_muladd_mfma_i8" if name.startswith("k_gf_mfma<true") else "gf_muladd_mfma"
+    if base == "k_gf_mfma_ring":
+        return "gf_muladd_mfma"
     return {"k_gf_lds": "gf_muladd_lds", "k_gf_lds_batch": "gf_batch", "k_md5_slices": "md5_slices",
             "k_crc32_slices": "crc32_slices", "k_md5_list": "md5_list", "k_crc32_list": "crc32_list",
             "__amd_rocclr_copyBuffer": "copy"}.get(base, base)

@@ -448,11 +448,9 @@ __global__ __launch_bounds__(kMfThreads, kRows * kCols == 4 ? 4 : 2) void k_gf_m
 // bounds check; source rows past the last one re-read the last row, which
 // the zero A entries of those sources cancel (no e2m1 code is NaN or Inf).
 template <int kWR, int kRows, int kCols, int kKc>
-__global__ __launch_bounds__(kMfThreads, 2) void k_gf_mfma_ring(const uint16_t *const *__restrict__ srcs, int nsrc,
-                                                                uint16_t *const *__restrict__ dsts, int ndst,
-                                                                const uint32_t *__restrict__ apk, int nks_pad,
-                                                                long words, int accumulate, int ntiles,
-                                                                int nrowblk) {
+__global__ __launch_bounds__(kMfThreads, kRows * kCols == 4 ? 4 : 2) void k_gf_mfma_ring(
+    const uint16_t *const *__restrict__ srcs, int nsrc, uint16_t *const *__restrict__ dsts, int ndst,
+    const uint32_t *__restrict__ apk, int nks_pad, long words, int accumulate, int ntiles, int nrowblk) {
   constexpr int kWC = kMfWaves / kWR;
   constexpr int kFrags = kWR * kRows;         // A fragments (output pairs) per workgroup
   constexpr int kWgWords = kWC * kCols * 32;  // words per workgroup
@@ -461,13 +459,19 @@ __global__ __launch_bounds__(kMfThreads, 2) void k_gf_mfma_ring(const uint16_t *
   constexpr int kBBytes = kSrcStage * kRowBytes;
   constexpr int kFragBytes = kKc * 256;  // packed A of one fragment for one stage
   constexpr int kABytes = kFrags * kFragBytes;
-  constexpr int kRowsPerLoad = 1024 / kRowBytes;  // rows one 64-lane x 16-byte load covers
-  constexpr int kBLoads = kBBytes / 1024, kALoads = kABytes / 1024;
-  constexpr int kBPerWave = kBLoads / kMfWaves, kAPerWave = kALoads / kMfWaves;
-  constexpr int kLoadsPerWave = kBPerWave + kAPerWave;
-  static_assert(kRows * kCols == 8 && kWR * kWC == kMfWaves && (kRowsPerLoad == 1 || kRowsPerLoad == 2) &&
-                    kBLoads % kMfWaves == 0 && kALoads % kMfWaves == 0 && kFragBytes % 1024 == 0 &&
-                    kLoadsPerWave <= 64,
+  // Loads are 64 lanes x 16 bytes (the LDS copy of a narrower load puts one
+  // lane per dword, so only whole dwords per lane keep the layout packed).
+  // B load j covers bytes 1024j.. of the stage's row-major B block, A load j
+  // bytes 1024j.. of its fragment-major A.  Every wave issues the same number
+  // of loads, with no branch (a branch around a load makes the compiler's
+  // wait insertion fall back to waiting for every load in flight where the
+  // paths join): when the stage's A is under 8 KiB, wave w loads A piece
+  // w mod kALoads, so several waves write the same bytes to the same place.
+  constexpr int kBLoads = kBBytes / 1024, kBPerWave = kBLoads / kMfWaves;
+  constexpr int kALoads = kABytes / 1024, kANum = (kALoads + kMfWaves - 1) / kMfWaves;
+  static_assert((kRows * kCols == 8 || kRows * kCols == 4) && kWR * kWC == kMfWaves && kRowBytes % 1024 == 0 &&
+                    kBLoads % kMfWaves == 0 && kBPerWave <= 64 && kABytes % 1024 == 0 && kFragBytes % 1024 == 0 &&
+                    (kALoads < kMfWaves || kALoads % kMfWaves == 0),
                 "ring tile shape");
   __shared__ uint4 sB0[kBBytes / 16], sB1[kBBytes / 16];
   __shared__ uint32_t sA0[kABytes / 4], sA1[kABytes / 4];
@@ -487,42 +491,40 @@ __global__ __launch_bounds__(kMfThreads, 2) void k_gf_mfma_ring(const uint16_t *
   const long tile_w0 = (long)tile * kWgWords;
   const int nchunks = nks_pad / kKc;
 
-  // the loads of stage c into one buffer set: B load j (j = wave + 8k) covers
-  // rows kRowsPerLoad*j.. of the stage, A load j bytes 1024j.. of the
-  // stage's fragment-major packed A
-  const int lrow = kRowsPerLoad == 2 ? (lane >> 5) : 0;  // this lane's row within a B load
-  const int lunit = kRowsPerLoad == 2 ? (lane & 31) : lane;
-  // The row pointers of the next stage to issue, fetched one stage before
-  // with a vector load (lane kRowsPerLoad*k + h holds the pointer of row h of
-  // B load k) and read back with readlane: a scalar load in flight would turn
-  // every wait on an LDS read into a wait for it.  The pointer load goes out
-  // before the stage's LDS loads, so it is older than them in the vmcnt order.
-  constexpr int kPtrs = kBPerWave * kRowsPerLoad;
-  static_assert(kPtrs <= 64, "row pointers per wave");
+  // B load k of the wave (load j = wave + 8k) reads one source row (a row
+  // is one or more whole loads), at byte 1024j mod kRowBytes + 16 * lane of
+  // the tile.  The rows' pointers for the next stage to issue are fetched one
+  // stage before with one vector load (lane k holds load k's pointer) and
+  // read back with readlane: no scalar load is in flight to turn every wait
+  // on an LDS read into a wait for it, and a single register carries them
+  // from stage to stage.
+  auto brow = [&](int k) { return (1024 * (wave + kMfWaves * k)) / kRowBytes; };
+  auto boff = [&](int k) { return (1024 * (wave + kMfWaves * k)) % kRowBytes + 16 * lane; };
   uint64_t pv = 0;
   auto fetch_ptrs = [&](int c) {
-    const int l = min(lane, kPtrs - 1);
-    const int row = kSrcStage * c + kRowsPerLoad * (wave + kMfWaves * (l / kRowsPerLoad)) + l % kRowsPerLoad;
-    pv = reinterpret_cast<const P2H_GLOBAL uint64_t *>(gptr(srcs))[min(row, nsrc - 1)];
+    int row = 0;
+#pragma unroll
+    for (int k = 0; k < kBPerWave; k++)
+      if (lane == k) row = brow(k);
+    pv = reinterpret_cast<const P2H_GLOBAL uint64_t *>(gptr(srcs))[min(kSrcStage * c + min(row, kSrcStage - 1), nsrc - 1)];
   };
   auto issue = [&](int c, uint4 *bB, uint32_t *bA) {
-    uint64_t pr[kPtrs];
+    const uint16_t *p[kBPerWave];
 #pragma unroll
-    for (int i = 0; i < kPtrs; i++)
-      pr[i] = (uint64_t)(uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)pv, i) |  // zero-extended halves
-              (uint64_t)(uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(pv >> 32), i) << 32;
+    for (int k = 0; k < kBPerWave; k++)
+      p[k] = reinterpret_cast<const uint16_t *>(
+          (uint64_t)(uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)pv, k) |  // zero-extended halves
+          (uint64_t)(uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(pv >> 32), k) << 32);
     fetch_ptrs(c + 1);
 #pragma unroll
     for (int k = 0; k < kBPerWave; k++) {
       const int j = wave + kMfWaves * k;
-      const uint16_t *p = reinterpret_cast<const uint16_t *>(
-          (kRowsPerLoad == 2 && lrow) ? pr[kRowsPerLoad * k + kRowsPerLoad - 1] : pr[kRowsPerLoad * k]);
-      __builtin_amdgcn_global_load_lds((const void *)(p + tile_w0 + 8 * lunit),
+      __builtin_amdgcn_global_load_lds((const void *)(p[k] + tile_w0 + boff(k) / 2),
                                        (__attribute__((address_space(3))) void *)(bB + 64 * j), 16, 0, 0);
     }
 #pragma unroll
-    for (int k = 0; k < kAPerWave; k++) {
-      const int j = wave + kMfWaves * k;
+    for (int k = 0; k < kANum; k++) {
+      const int j = kALoads < kMfWaves ? wave % kALoads : wave + kMfWaves * k;
       const int frag = (1024 * j) / kFragBytes, off = (1024 * j) % kFragBytes;
       const uint8_t *a = reinterpret_cast<const uint8_t *>(apk) +
                          ((long)(mf0 + frag) * nks_pad + (long)kKc * c) * 256 + off + 16 * lane;
@@ -645,10 +647,10 @@ int32_t run_gf_mfma(par2hip_ctx *ctx, hipStream_t s, const GfProblem &p, bool in
   // R=102 contraction on 224 CUs in 5.83-5.89 ms against 6.65-6.73) but drew
   // the shader clock down under the chains, so the encode step slowed (MD5
   // 9.46-9.54 -> 10.06-10.13 ms, step 9.57-9.65 -> 10.21-10.27 ms, same box)
-  constexpr int kRingKc = 8;
-  const bool ring = !int8 && shape == 2 && nfull > 0;
+  const int ring_kc = shape == 4 ? 4 : 8;  // K-steps per stage: two stage buffers of each role fit the LDS
+  const bool ring = !int8 && (shape == 1 || shape == 2 || shape == 4) && nfull > 0;
   const bool plain = !ring || ntiles > nfull;
-  const int nmf_pad = nrowblk * frags, nks_pad = (nks + kRingKc - 1) / kRingKc * kRingKc;
+  const int nmf_pad = nrowblk * frags, nks_pad = (nks + ring_kc - 1) / ring_kc * ring_kc;
   const size_t apk_bytes = plain ? (size_t)nmf * nks * 64 * sizeof(uint32_t) : 0;
   const size_t ring_bytes = ring ? (size_t)nmf_pad * nks_pad * 64 * sizeof(uint32_t) : 0;
   void *buf = nullptr;
@@ -685,13 +687,13 @@ int32_t run_gf_mfma(par2hip_ctx *ctx, hipStream_t s, const GfProblem &p, bool in
     if (nfull) launch(k_gf_mfma<true, false>, nfull, 0);
     if (ntiles > nfull) launch(k_gf_mfma<true, true>, ntiles - nfull, nfull);
   } else if (shape == 1) {
-    if (nfull) launch(k_gf_mfma<false, false, 1, 2, 2, 4, 2>, nfull, 0);
+    if (nfull) launch_ring(k_gf_mfma_ring<1, 2, 2, 8>);
     if (ntiles > nfull) launch(k_gf_mfma<false, true, 1, 2, 2, 4>, ntiles - nfull, nfull);
   } else if (shape == 4) {
-    if (nfull) launch(k_gf_mfma<false, false, 1, 1, 4, 4, 2>, nfull, 0);
+    if (nfull) launch_ring(k_gf_mfma_ring<1, 1, 4, 4>);
     if (ntiles > nfull) launch(k_gf_mfma<false, true, 1, 1, 4, 4>, ntiles - nfull, nfull);
   } else if (shape == 2) {
-    if (nfull) launch_ring(k_gf_mfma_ring<2, 2, 4, kRingKc>);
+    if (nfull) launch_ring(k_gf_mfma_ring<2, 2, 4, 8>);
     if (ntiles > nfull) launch(k_gf_mfma<false, true, 2, 2, 4, 8>, ntiles - nfull, nfull);
   } else if (shape == 3) {
     if (nfull) launch(k_gf_mfma<false, false, 4, 2, 4, 4>, nfull, 0);

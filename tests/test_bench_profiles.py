@@ -22,6 +22,8 @@ def test_kernel_tags():
     assert bench.kernel_tag("void par2hip::k_gf_mfma<false, false, 4, 2, 4, 4>(unsigned short const* const*)") == \
         "gf_muladd_mfma"
     assert bench.kernel_tag("par2hip::k_gf_mfma<true, true>(int)") == "gf_muladd_mfma_i8"
+    assert bench.kernel_tag("void par2hip::k_gf_mfma_ring<2, 2, 4, 8>(unsigned short const* const*)") == \
+        "gf_muladd_mfma"
     assert bench.kernel_tag("void par2hip::k_gf_lds<8, 2, 16>(x)") == "gf_muladd_lds"
     assert bench.kernel_tag("par2hip::k_gf_lds_batch(x)") == "gf_batch"
     assert bench.kernel_tag("__amd_rocclr_copyBuffer") == "copy"
