@@ -1,5 +1,5 @@
 """The host MD5 behind the creator's whole-file, 16 KiB and packet digests
-(par2-rs_amd/csrc/md5_host.hpp, compiled with g++) against Python's hashlib
+(par2-rs_amd/csrc/md5_host.hpp and md5_host.cpp, built with g++) against Python's hashlib
 on every length 0..300, on split updates and on a 5 MiB string; plus the RFC
 1321 appendix A.5 suite through the same program's first lines."""
 import hashlib
@@ -27,7 +27,8 @@ def test_md5_host_matches_hashlib(tmp_path):
         pytest.skip("no g++")
     exe = tmp_path / "md5_host_check"
     subprocess.run(["g++", "-O2", "-std=c++17", "-I", os.path.join(ROOT, "par2-rs_amd", "csrc"),
-                    os.path.join(ROOT, "tests", "cpp", "md5_host_check.cpp"), "-o", str(exe)], check=True)
+                    os.path.join(ROOT, "tests", "cpp", "md5_host_check.cpp"),
+                    os.path.join(ROOT, "par2-rs_amd", "csrc", "md5_host.cpp"), "-o", str(exe)], check=True)
     out = subprocess.run([str(exe)], capture_output=True, text=True, timeout=120, check=True).stdout.split()
     data = lcg_bytes(5 << 20)
     want = [hashlib.md5(data[:n]).hexdigest() for n in range(301)]

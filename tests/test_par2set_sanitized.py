@@ -19,7 +19,8 @@ def test_par2set_under_sanitizers(tmp_path):
     exe = tmp_path / "par2set_check"
     subprocess.run(["g++", "-O1", "-g", "-std=c++17", "-fsanitize=address,undefined", "-fno-sanitize-recover=all",
                     "-fno-omit-frame-pointer", "-I", CSRC, os.path.join(CSRC, "par2set.cpp"),
-                    os.path.join(ROOT, "tests", "cpp", "par2set_check.cpp"), "-o", str(exe)], check=True)
+                    os.path.join(CSRC, "md5_host.cpp"), os.path.join(ROOT, "tests", "cpp", "par2set_check.cpp"),
+                    "-o", str(exe)], check=True)
     # verify_asan_link_order=0: a library preloaded by the environment may come first
     env = dict(os.environ, ASAN_OPTIONS="detect_leaks=1:verify_asan_link_order=0",
                UBSAN_OPTIONS="print_stacktrace=1")
