@@ -60,21 +60,26 @@ constexpr int kMfThreads = 64 * kMfWaves;
 // and staging overlap the other's MFMAs).
 //   wide  (1, 4x2, 8): 8 outputs x 512 words, 32 sources per stage; the int8
 //         form's only shape
-//   small (1, 2x2, 4): 4 outputs x 512 words, 16 sources per stage, two
-//         workgroups per CU, loads two stages ahead: 3 or 4 outputs fill
-//         its rows, so no MFMA multiplies padded zero rows, and each input
-//         word is read by one workgroup only.  1 GiB at R=4 in 0.287-0.295 ms
-//         against 0.305-0.306 for a one-workgroup 4 x 1024 tile on the same
-//         device; 300 x 64 KiB at R=3 in 0.055 against 0.082 ms
-//   single (1, 1x4, 4): 2 outputs x 1024 words, two workgroups per CU, loads
-//         two stages ahead: one or two outputs (one A fragment), so no MFMA
-//         multiplies a padded fragment.  1 GiB at R=1 or 2 in 0.187 ms
-//         (72% of 8 TB/s) against 0.27-0.28 ms with the small tile
+//   small (1, 2x2, 4): 4 outputs x 512 words, two workgroups per CU: 3 or
+//         4 outputs fill its rows, so no MFMA multiplies padded zero rows,
+//         and each input word is read by one workgroup only.  Staged form
+//         (16 sources per stage, loads two stages ahead): 1 GiB at R=4 in
+//         0.287-0.295 ms against 0.305-0.306 for a one-workgroup 4 x 1024
+//         tile on the same device, 300 x 64 KiB at R=3 in 0.055 against
+//         0.082 ms; the ring form (32 sources per stage, k_gf_mfma_ring
+//         below) runs its full tiles: 1 GiB at R=4 in 0.213-0.234 ms
+//   single (1, 1x4, 4): 2 outputs x 1024 words, two workgroups per CU: one
+//         or two outputs (one A fragment), so no MFMA multiplies a padded
+//         fragment.  1 GiB at R=1 or 2 in 0.187 ms (72% of 8 TB/s) against
+//         0.27-0.28 ms with the small tile (staged form); 0.179-0.190 ms in
+//         the ring form, which runs its full tiles
 //   split (2, 2x4, 8): 8 outputs x 512 words; a wave reads 2 A fragments per
 //         K-step instead of 4, half the LDS traffic of A per MFMA.  Alone the
-//         fastest: 1024 x 1 MiB, R=102 in 5.04-5.17 ms (wide 5.71 ms), 2000 x
-//         512 KiB, R=100 in 4.89-4.98 ms (wide 5.46 ms); a 4-tile form at two
-//         workgroups per CU took 5.23 ms
+//         fastest: 1024 x 1 MiB, R=102 in 5.04-5.17 ms in the staged form
+//         before the five-operation expansion (wide 5.71 ms), 2000 x 512 KiB,
+//         R=100 in 4.89-4.98 ms (wide 5.46 ms); a 4-tile form at two
+//         workgroups per CU took 5.23 ms.  Its full tiles run in the ring
+//         form: 4.26-4.33 ms and 4.16-4.22 ms
 //   tall  (4, 2x4, 4): 16 outputs x 256 words, 16 sources per stage; the
 //         slowest alone (6.20 ms on 1024 x 1 MiB, R=102) but the one that
 //         draws the least from the MD5 chains beside it: that encode step
@@ -436,9 +441,8 @@ __global__ __launch_bounds__(kMfThreads, kRows * kCols == 4 ? 4 : 2) void k_gf_m
 // into the buffer stage c-1 used, which every wave finished reading before
 // the barrier that ended stage c-1; at the end of stage c every wave waits
 // for all of its loads (vmcnt(0): no assumption about the order in which
-// the LDS loads and the row-pointer load retire) and a plain s_barrier (no
-// fence, whose release would wait for the same loads again) publishes stage
-// c+1.  The two buffers of each role are separate LDS objects selected at
+// the LDS loads and the row-pointer load retire) and a barrier publishes
+// stage c+1.  The two buffers of each role are separate LDS objects selected at
 // compile time (the stage loop turns two stages at a time), so the compiler
 // sees that the loads in flight do not write the buffer being read.  Full
 // word tiles only: the ragged last tile column takes k_gf_mfma<.., true, ..>
@@ -506,7 +510,8 @@ __global__ __launch_bounds__(kMfThreads, kRows * kCols == 4 ? 4 : 2) void k_gf_m
 #pragma unroll
     for (int k = 0; k < kBPerWave; k++)
       if (lane == k) row = brow(k);
-    pv = reinterpret_cast<const P2H_GLOBAL uint64_t *>(gptr(srcs))[min(kSrcStage * c + min(row, kSrcStage - 1), nsrc - 1)];
+    const int src = min(kSrcStage * c + min(row, kSrcStage - 1), nsrc - 1);
+    pv = reinterpret_cast<const P2H_GLOBAL uint64_t *>(gptr(srcs))[src];
   };
   auto issue = [&](int c, uint4 *bB, uint32_t *bA) {
     const uint16_t *p[kBPerWave];
@@ -629,7 +634,7 @@ __global__ __launch_bounds__(kMfThreads, kRows * kCols == 4 ? 4 : 2) void k_gf_m
 int32_t run_gf_mfma(par2hip_ctx *ctx, hipStream_t s, const GfProblem &p, bool int8) {
   const int nmf = (p.ndst + 1) / 2;
   const int nks = (p.nsrc + 3) / 4;
-  // tile shape: 0 wide, 1 small, 2 split, 3 tall (the table above)
+  // tile shape: 0 wide, 1 small, 2 split, 3 tall, 4 single (the table above)
   const int shape = int8 ? 0 : nmf == 1 ? 4 : nmf == 2 ? 1 : p.beside_chains ? 3 : 2;
   static const int kFragsOf[5] = {kMfRows, 2, 4, 8, 1};
   static const long kWordsOf[5] = {kMfWaves * kMfCols * 32, 512, 512, 256, 1024};
@@ -640,9 +645,10 @@ int32_t run_gf_mfma(par2hip_ctx *ctx, hipStream_t s, const GfProblem &p, bool in
   const int nrowblk = (nmf + frags - 1) / frags;
   const long nwg = 8L * ((ntiles + 7) / 8) * nrowblk;
   if (nwg > 0x7fffffffL) return fail(ctx, PAR2HIP_EINVAL, "GF problem too large for one launch");
-  // the split shape runs its full tiles in the ring form, whose packed A is
-  // permuted and padded to whole fragments and stages; the other shapes and
-  // the ragged tile column take the plain packing.  The tall shape (beside the
+  // the single, small and split shapes run their full tiles in the ring
+  // form, whose packed A is permuted and padded to whole fragments and
+  // stages; the other shapes and the ragged tile column take the plain
+  // packing.  The tall shape (beside the
   // MD5 chains) keeps the staged form: its ring form was faster (the 1 GiB,
   // R=102 contraction on 224 CUs in 5.83-5.89 ms against 6.65-6.73) but drew
   // the shader clock down under the chains, so the encode step slowed (MD5
