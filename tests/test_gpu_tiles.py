@@ -113,3 +113,28 @@ def test_mfma_tile_shapes_match_oracle(ctx, kernel):
             assert np.array_equal(a, b)
     finally:
         ctx.set_gf_kernel(prev)
+
+
+# The FP4 ring form over many stages: 600 sources (19-38 stages, the last
+# one padded), 64 KiB + 2064 B slices (16-byte rows, as the matrix-core form
+# needs: whole tiles in the ring form and a ragged 8-word tile column in the
+# staged form), one to eight outputs (single, small and split tiles), every
+# byte against the multithreaded AVX2 restatement, then a rebuild of as many
+# erased slices from those outputs.
+@pytest.mark.parametrize("r", [1, 2, 4, 8])
+def test_fp4_ring_many_stages(ctx, r):
+    rng = np.random.default_rng(600 + r)
+    n, s = 600, 65536 + 2064
+    data = rng.integers(0, 256, (n, s), dtype=np.uint8)
+    exps = np.sort(rng.choice(2000, r, replace=False)).astype(np.uint32)
+    prev = ctx.set_gf_kernel(2)
+    try:
+        say("fp4 ring", n, s, r)
+        got = ctx.encode(data, exps)
+        want = oracle.encode_avx2(data.view(np.uint16), exps).view(np.uint8).reshape(r, s)
+        assert np.array_equal(got, want), f"fp4 ring r={r}"
+        missing = np.sort(rng.choice(n, r, replace=False)).astype(np.uint32)
+        out = ctx.reconstruct(data, missing, got, exps)
+        assert np.array_equal(out, data[missing]), f"fp4 ring repair r={r}"
+    finally:
+        ctx.set_gf_kernel(prev)
