@@ -232,6 +232,8 @@ def trace_summary() -> None:
                 for k, v in o.items():
                     if k in keep:
                         out[k] = v
+                    elif isinstance(v, dict) and depth == 0 and k in ("micro", "small_r", "inverse"):
+                        out[k] = v  # these workloads' whole result is one table of per-case timings
                     elif isinstance(v, dict) and depth == 0 and ("value" in v or "kernel_ms" in v):
                         out[k] = brief(v, 1)
                 return out
