@@ -311,8 +311,9 @@ __global__ __launch_bounds__(64) void k_md5_slices(const uint8_t *__restrict__ d
   P2H_GLOBAL uint32_t *o = ln.store ? gptr(out) + 4 * i : nullptr;
   if (!lens) {
     // every slice full: the block count and the prefetch clamp are the same
-    // for all lanes, so they stay in scalar registers (about 6 vector
-    // instructions less per 64-byte block than with per-lane lengths)
+    // for all lanes, so they stay in scalar registers and the loops are
+    // uniform (per-lane lengths cost about 4 instructions more per 64-byte
+    // block in the clamped loop; the main loop takes the same ~330 either way)
     md5_slice<kAlign>(p, slice_size, slice_size, o);
     return;
   }
@@ -336,10 +337,11 @@ __global__ __launch_bounds__(64) void k_md5_list(const HashEntry *__restrict__ l
 }
 
 double md5_chains_ms(size_t n, uint64_t bytes, int simds) {
-  // 324 vector instructions per 64-byte block in md5_compress's loop with
-  // uniform slice lengths (the form every whole-set encode and verify takes;
-  // 330 with per-slice lengths), one issue per 4 clocks per wave (a wave64
-  // VALU instruction occupies its SIMD 4 clocks)
+  // 324 vector instructions per 64-byte block in the main loop (5 per MD5
+  // step plus the four state additions; its 4 loads, one wait and loop
+  // control add about 6 more issue slots, left out of this lower bound), one
+  // issue per 4 clocks per wave (a wave64 VALU instruction occupies its SIMD
+  // 4 clocks)
   const double blocks = (double)((bytes + 8) / 64 + 1);
   const double waves = (double)((n + 63) / 64);
   const double rounds = std::ceil(waves / std::max(simds, 1));
