@@ -456,8 +456,9 @@ class Context:
         return md5[:n], crc[:n]
 
     def hash_slices_dev(self, d_data: DeviceBuffer, n: int, slice_size: int, d_md5: DeviceBuffer | None,
-                        d_crc: DeviceBuffer | None, slice_stride: int = 0) -> None:
-        self._check(lib().par2hip_hash_slices_dev(self.handle, d_data.ptr, n, slice_size, slice_stride, None,
+                        d_crc: DeviceBuffer | None, slice_stride: int = 0, lens=None) -> None:
+        lv = None if lens is None else np.ascontiguousarray(lens, np.uint64)
+        self._check(lib().par2hip_hash_slices_dev(self.handle, d_data.ptr, n, slice_size, slice_stride, _p(lv),
                                                   d_md5.ptr if d_md5 else None, d_crc.ptr if d_crc else None))
 
     # ---- tuning / measurement

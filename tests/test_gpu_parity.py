@@ -215,6 +215,43 @@ def test_hash_ragged_lengths_zero_padded(ctx):
         assert md5[i].tobytes() == hashlib.md5(b).digest() and int(crc[i]) == zlib.crc32(b), ln
 
 
+def test_hash_md5_loop_boundaries(ctx):
+    """Every count of whole 64-byte blocks from 0 to 40, plus 0, 8 or 56
+    bytes, at 16-, 4- and 1-byte aligned bases, with one length for all the
+    slices and with a length per slice: the MD5 chain's unclamped main loop
+    (six blocks per pass while the prefetch stays inside the slice), its
+    clamped loop and its tail hand over at different block counts, and each
+    hand-over is crossed.  Both hashes equal the stdlib's."""
+    rng = np.random.default_rng(64)
+    sizes = [64 * k + r for k in range(41) for r in (0, 8, 56) if 64 * k + r]
+    stride = 64 * 42  # a multiple of 16
+    n = len(sizes)
+    for shift in (0, 4, 1):
+        buf = rng.integers(0, 256, shift + n * stride, dtype=np.uint8)
+        d = ctx.alloc(len(buf))
+        d.upload(buf)
+        d_md5, d_crc = ctx.alloc(16 * n), ctx.alloc(4 * n)
+        say("md5 loop boundaries, a length per slice, base shift", shift)
+        ctx.hash_slices_dev(d.view(shift), n, stride, d_md5, d_crc, slice_stride=stride, lens=sizes)
+        ctx.sync()
+        md5 = d_md5.download().reshape(n, 16)
+        crc = d_crc.download().view(np.uint32)
+        for i, s in enumerate(sizes):
+            b = buf[shift + i * stride:shift + i * stride + s].tobytes() + bytes(stride - s)
+            assert md5[i].tobytes() == hashlib.md5(b).digest(), (shift, s)
+            assert int(crc[i]) == zlib.crc32(b), (shift, s)
+        say("md5 loop boundaries, one length for all, base shift", shift)
+        for s in sizes:
+            ctx.hash_slices_dev(d.view(shift), 3, s, d_md5, d_crc, slice_stride=stride)
+            ctx.sync()
+            md5 = d_md5.download()[:48].reshape(3, 16)
+            crc = d_crc.download()[:12].view(np.uint32)
+            for i in range(3):
+                b = buf[shift + i * stride:shift + i * stride + s].tobytes()
+                assert md5[i].tobytes() == hashlib.md5(b).digest(), (shift, s, i)
+                assert int(crc[i]) == zlib.crc32(b), (shift, s, i)
+
+
 # --------------------------------------------------------------- repair --
 @pytest.mark.parametrize("m", [1, 2, 17, 100, 300])
 def test_gf_invert_vs_oracle(ctx, m):
