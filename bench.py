@@ -59,7 +59,31 @@ PMC_RUNS = {
 UNITS_SUMMARY = os.path.join(PROFILES, "gf_units_utilisation_pmc.md")
 FP4_DENSE_PEAK_TFLOPS = 10100.0
 INT8_DENSE_PEAK_TOPS = 5000.0
+# The MD5 chains that bound the encode and repair steps: MI355X's peak shader
+# clock, and the vector instructions per 64-byte block of one chain (5 per MD5
+# step -- round function, two adds, an add3, a rotate -- and the four state
+# additions: the fewest this instruction set allows).
+SHADER_CLOCK_GHZ = 2.4
+MD5_VALU_PER_BLOCK = 324
 METRIC = "encode/repair GB/s source data + p50/p99 latency vs plain GPU path, bit-exact"
+
+
+def md5_issue_floor_ms(n: int, s: int, hash_simds: int) -> float:
+    """Lower bound on hashing n full s-byte slices with one serial MD5 chain
+    per lane: a wave of 64 chains issues one vector instruction per 4 clocks,
+    so a chain of B blocks (the slice's and the final padding block) takes B x
+    324 x 4 clocks at the peak shader clock; waves beyond one per SIMD of the
+    hash CUs run in further rounds."""
+    blocks = (s + 8) // 64 + 1
+    rounds = -(-((n + 63) // 64) // max(hash_simds, 1))
+    return rounds * blocks * MD5_VALU_PER_BLOCK * 4 / (SHADER_CLOCK_GHZ * 1e9) * 1e3
+
+
+def md5_floor_fields(n: int, s: int, md5_ms: float, ctx) -> dict:
+    """The step's MD5 time beside its issue floor."""
+    floor = md5_issue_floor_ms(n, s, 4 * int(ctx.device_info().get("hash_cus", 0)))
+    return {"md5_ms": round(md5_ms, 4), "md5_issue_floor_ms": round(floor, 4),
+            "md5_floor_frac": round(floor / md5_ms, 4) if md5_ms > 0 else None}
 
 
 def kernel_tag(symbol: str) -> str:
@@ -528,8 +552,8 @@ def bench_encode(args, ctx):
         "roofline": {**roof, "traffic": traffic,
                      "traffic_x_algorithmic": round(traffic / algo_bytes, 3) if traffic else None,
                      "traffic_source": traffic_src, "kernel": gf_name, "kernel_ms": round(gf_avg, 4), "algorithmic_bytes": algo_bytes,
-                     "gf_muladds": muladds, "md5_ms": round(md5_avg, 4), "crc32_ms": round(crc_avg, 4),
-                     "units": units},
+                     "gf_muladds": muladds, **md5_floor_fields(n, s, md5_avg, ctx),
+                     "crc32_ms": round(crc_avg, 4), "units": units},
         "cpu": (n, s, r),
     }
 
@@ -752,6 +776,8 @@ def bench_repair(args, ctx):
                      "traffic_x_algorithmic": round(traffic / algo_bytes, 3) if traffic else None,
                      "traffic_source": traffic_src, "kernel": gf_name, "kernel_ms": round(gf_avg, 4),
                      "algorithmic_bytes": algo_bytes, "gf_muladds": m * n * (s // 2),
+                     **md5_floor_fields(n, s, stats.get("md5_slices", (1, 0.0))[1] /
+                                        max(stats.get("md5_slices", (1, 0))[0], 1), ctx),
                      "units": profile_units("plain_repair_1GiB", gf_name)},
         "cpu": (n, s, m),
     }

@@ -604,7 +604,9 @@ __global__ __launch_bounds__(kMfThreads, kRows * kCols == 4 ? 4 : 2) void k_gf_m
     stage(c, sB0, sA0, sB1, sA1);
     if (c + 1 < nchunks) stage(c + 1, sB1, sA1, sB0, sA0);
   }
-  // the row-pointer load of the stage after the last one is still in flight
+  // every load issued (the last one is the row-pointer load for the stage
+  // after the last, clamped to the last source row) was waited for by the
+  // last publish; this wait only states that no load is in flight past here
   __builtin_amdgcn_s_waitcnt((7 << 4) | (15 << 8));
 
   // parity of each accumulator = bit q of output slice 2*(mf0+i) + g at words w..w+kCols-1
