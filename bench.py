@@ -60,23 +60,24 @@ UNITS_SUMMARY = os.path.join(PROFILES, "gf_units_utilisation_pmc.md")
 FP4_DENSE_PEAK_TFLOPS = 10100.0
 INT8_DENSE_PEAK_TOPS = 5000.0
 # The MD5 chains that bound the encode and repair steps: MI355X's peak shader
-# clock, and the vector instructions per 64-byte block of one chain (5 per MD5
-# step -- round function, two adds, an add3, a rotate -- and the four state
-# additions: the fewest this instruction set allows).
+# clock, and the instructions a chain wave must issue per 64-byte block in the
+# helper-wave form those steps run (its helper wave precomputes each step's
+# message word plus constant): 4 per MD5 step (round function, add3, rotate,
+# add), the four state additions and 16 LDS reads of the 64 precomputed sums.
 SHADER_CLOCK_GHZ = 2.4
-MD5_VALU_PER_BLOCK = 324
+MD5_SLOTS_PER_BLOCK = 276
 METRIC = "encode/repair GB/s source data + p50/p99 latency vs plain GPU path, bit-exact"
 
 
 def md5_issue_floor_ms(n: int, s: int, hash_simds: int) -> float:
     """Lower bound on hashing n full s-byte slices with one serial MD5 chain
-    per lane: a wave of 64 chains issues one vector instruction per 4 clocks,
-    so a chain of B blocks (the slice's and the final padding block) takes B x
-    324 x 4 clocks at the peak shader clock; waves beyond one per SIMD of the
-    hash CUs run in further rounds."""
+    per lane: a wave of 64 chains issues one instruction per 4 clocks, so a
+    chain of B blocks (the slice's and the final padding block) takes B x 276
+    x 4 clocks at the peak shader clock; waves beyond one per SIMD of the hash
+    CUs run in further rounds."""
     blocks = (s + 8) // 64 + 1
     rounds = -(-((n + 63) // 64) // max(hash_simds, 1))
-    return rounds * blocks * MD5_VALU_PER_BLOCK * 4 / (SHADER_CLOCK_GHZ * 1e9) * 1e3
+    return rounds * blocks * MD5_SLOTS_PER_BLOCK * 4 / (SHADER_CLOCK_GHZ * 1e9) * 1e3
 
 
 def md5_floor_fields(n: int, s: int, md5_ms: float, ctx) -> dict:

@@ -197,10 +197,14 @@ struct GfProblem {
 // second) measured on the 1 GiB / R=102 encode: 1.40e13 over 5.02 ms on 256
 // CUs alone and over 8.04 ms on 160 CUs beside the MD5 chains.
 constexpr double kFp4MacPerCuSec = 1.09e13;
-// Lower bound on the time of n MD5 chains of `bytes` each (zero padding and
-// length block included) on `simds` SIMDs, one wave of 64 chains per SIMD at a
-// time, at the 2.4 GHz peak shader clock.
-double md5_chains_ms(size_t n, uint64_t bytes, int simds);
+// Estimated time of n MD5 chains of `bytes` each (zero padding and length
+// block included) on `simds` SIMDs, one wave of 64 chains per SIMD at a time,
+// from the issue slots per 64-byte block of the form that runs (helper: the
+// helper-wave form), at the 2.4 GHz peak shader clock.
+double md5_chains_ms(size_t n, uint64_t bytes, int simds, bool helper);
+// Whether launch_md5 runs the helper-wave form for these arguments.
+bool md5_helper_form(const par2hip_ctx *ctx, hipStream_t s, const void *d_data, size_t n, size_t slice_size,
+                     size_t stride, bool lens);
 int32_t run_gf(par2hip_ctx *ctx, hipStream_t s, const GfProblem &p);
 // Largest per-coefficient operand block (tables) one contraction launch gets;
 // larger problems are split by output rows.

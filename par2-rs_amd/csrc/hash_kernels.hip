@@ -147,15 +147,15 @@ __device__ __forceinline__ void md5_load_partial(const P2H_GLOBAL uint8_t *__res
 
 // MD5 of one slice: bytes [0, len) of p followed by zeros up to `total`
 // (the IFSC zero padding), digest to out[0..3] (nothing stored when out is
-// null).  kAlign: 16 or 4 = p has that byte alignment; 1 = unaligned (byte
-// loads).
+// null), continuing from state st after the first `start` bytes (a multiple
+// of 64 at or below len).  kAlign: 16 or 4 = p has that byte alignment; 1 =
+// unaligned (byte loads).
 template <int kAlign>
-__device__ __forceinline__ void md5_slice(const P2H_GLOBAL uint8_t *__restrict__ p, uint64_t len, uint64_t total,
-                                          P2H_GLOBAL uint32_t *__restrict__ out) {
-  uint32_t st[4] = {0x67452301u, 0xefcdab89u, 0x98badcfeu, 0x10325476u};
+__device__ __forceinline__ void md5_rest(uint32_t st[4], const P2H_GLOBAL uint8_t *__restrict__ p, uint64_t start,
+                                         uint64_t len, uint64_t total, P2H_GLOBAL uint32_t *__restrict__ out) {
   uint32_t M[16];
   const uint64_t full = len & ~(uint64_t)63;  // blocks made entirely of real bytes
-  uint64_t pos = 0;
+  uint64_t pos = start;
   if (kAlign >= 4) {
     // kMd5Sets register sets rotate with a prefetch distance of kMd5Sets-1
     // blocks: the loads of block k+kMd5Sets-1 are issued before block k is
@@ -166,7 +166,8 @@ __device__ __forceinline__ void md5_slice(const P2H_GLOBAL uint8_t *__restrict__
     // every block index and index + kMd5Sets stays below 2^31): the clamp
     // below is then a 32-bit compare, which the scalar unit has for uniform
     // lengths.  Addresses are formed in 64 bits.
-    const uint32_t nblk = (uint32_t)(full >> 6);
+    const uint32_t nblk = (uint32_t)((full - start) >> 6);
+    p += start;  // block indices below count from start
     if (nblk) {
       const uint32_t last = nblk - 1;
       uint32_t S[kMd5Sets][16];
@@ -231,9 +232,10 @@ __device__ __forceinline__ void md5_slice(const P2H_GLOBAL uint8_t *__restrict__
       for (int j = 0; j < kMd5Sets - 1; j++)
         if (blk + j < nblk) md5_compress(st, S[j]);
     }
+    p -= start;
     pos = full;
   } else {
-    for (pos = 0; pos < full; pos += 64) {
+    for (; pos < full; pos += 64) {
 #pragma unroll
       for (int k = 0; k < 16; k++)
         M[k] = (uint32_t)p[pos + 4 * k] | ((uint32_t)p[pos + 4 * k + 1] << 8) |
@@ -278,6 +280,13 @@ __device__ __forceinline__ void md5_slice(const P2H_GLOBAL uint8_t *__restrict__
   if (out) *reinterpret_cast<P2H_GLOBAL u32x4 *>(out) = digest;
 }
 
+template <int kAlign>
+__device__ __forceinline__ void md5_slice(const P2H_GLOBAL uint8_t *__restrict__ p, uint64_t len, uint64_t total,
+                                          P2H_GLOBAL uint32_t *__restrict__ out) {
+  uint32_t st[4] = {0x67452301u, 0xefcdab89u, 0x98badcfeu, 0x10325476u};
+  md5_rest<kAlign>(st, p, 0, len, total, out);
+}
+
 // Chains per wave.  A wave with many idle lanes hashed a 1 MiB slice in
 // 10.3-11.4 ms against 9.3 ms for a wave of 40 or more chains (1 to 32 active
 // lanes, one box, alone on the device), so no wave is left mostly idle: the n
@@ -289,8 +298,8 @@ struct Md5Lane {
   long slice;  // the chain this lane computes
   bool store;  // false: a spare lane's copy
 };
-__device__ __forceinline__ Md5Lane md5_lane(long n) {
-  const long waves = (n + 63) / 64, w = blockIdx.x, lane = threadIdx.x;
+__device__ __forceinline__ Md5Lane md5_lane(long n, long lane) {
+  const long waves = (n + 63) / 64, w = blockIdx.x;
   const long q = n / waves, rem = n % waves;
   const long cnt = q + (w < rem ? 1 : 0), first = w * q + (w < rem ? w : rem);
   return Md5Lane{first + (lane < cnt ? lane : lane % cnt), lane < cnt};
@@ -305,7 +314,7 @@ __global__ __launch_bounds__(64) void k_md5_slices(const uint8_t *__restrict__ d
   // when these few chains bound the caller's step, let them issue ahead of
   // co-resident waves of other kernels
   if (high_priority) __builtin_amdgcn_s_setprio(3);
-  const Md5Lane ln = md5_lane(n);
+  const Md5Lane ln = md5_lane(n, threadIdx.x);
   const long i = ln.slice;
   const P2H_GLOBAL uint8_t *p = gptr(data) + (uint64_t)i * stride;
   P2H_GLOBAL uint32_t *o = ln.store ? gptr(out) + 4 * i : nullptr;
@@ -327,7 +336,7 @@ __global__ __launch_bounds__(64) void k_md5_slices(const uint8_t *__restrict__ d
 __global__ __launch_bounds__(64) void k_md5_list(const HashEntry *__restrict__ list, long n, int aligned4,
                                                  int high_priority) {
   if (high_priority) __builtin_amdgcn_s_setprio(3);
-  const Md5Lane ln = md5_lane(n);  // neighbours in the list have similar lengths
+  const Md5Lane ln = md5_lane(n, threadIdx.x);  // neighbours in the list have similar lengths
   const HashEntry e = list[ln.slice];
   P2H_GLOBAL uint32_t *o = ln.store ? gptr(e.md5) : nullptr;
   if (aligned4)
@@ -336,16 +345,201 @@ __global__ __launch_bounds__(64) void k_md5_list(const HashEntry *__restrict__ l
     md5_slice<1>(gptr(e.data), e.len, e.size, o);
 }
 
-double md5_chains_ms(size_t n, uint64_t bytes, int simds) {
-  // 324 vector instructions per 64-byte block in the main loop (5 per MD5
-  // step plus the four state additions; its 4 loads, one wait and loop
-  // control add about 6 more issue slots, left out of this lower bound), one
-  // issue per 4 clocks per wave (a wave64 VALU instruction occupies its SIMD
-  // 4 clocks)
+// ------------------------------------------- MD5 with a helper wave -----
+// Each MD5 step adds its message word and its constant, M[g(i)] + T[i],
+// which do not depend on the chain.  In this form a second wave of the
+// workgroup (the helper) loads the message blocks and writes those 64 sums
+// per block and lane to LDS; the chain wave reads them back with 16
+// ds_read_b128 per block and issues 4 instructions per step (round function,
+// add3, rotate, add) instead of 5: ~280 issue slots per 64-byte block instead
+// of ~330, and a wave issues one instruction per 4 clocks, so the chain, the
+// step's critical path, shortens by the same fraction.  The sums pass
+// through a ring of kMd5Ring chunks of kMd5Chunk blocks (16 KiB per block):
+// while the chain reads chunk c (and, near its end, the first block of chunk
+// c+1), the helper fills chunk c+2, and one barrier per chunk hands them
+// over.  The helper keeps the message words of kMd5HelperSets chunks in
+// flight in registers (~3 us of the chain's work).  Whole blocks of full,
+// 16-byte aligned slices; the chain hashes the last (< kMd5Chunk) blocks and
+// the padding itself.
+constexpr int kMd5Chunk = 2, kMd5Ring = 3, kMd5HelperSets = 3;
+constexpr int kMd5BlockVecs = 16 * 64;  // u32x4 per block slot: 16 per lane
+constexpr int kMd5RingBytes = kMd5Ring * kMd5Chunk * kMd5BlockVecs * 16;
+static_assert(kMd5Chunk == 2, "the chain's two register sets are one chunk");
+
+// message word of MD5 step i (RFC 1321 section 3.4: rounds take words k,
+// 5k+1, 3k+5 and 7k mod 16)
+__device__ constexpr int md5_word(int i) {
+  return i < 16 ? i : i < 32 ? (5 * i + 1) & 15 : i < 48 ? (3 * i + 5) & 15 : (7 * i) & 15;
+}
+__device__ constexpr int md5_rot(int i) {
+  return i < 16 ? 7 + 5 * (i & 3) : i < 32 ? (int[4]){5, 9, 14, 20}[i & 3]
+       : i < 48 ? (int[4]){4, 11, 16, 23}[i & 3] : (int[4]){6, 10, 15, 21}[i & 3];
+}
+
+// md5_compress with MK[i] = M[md5_word(i)] + T[i] given
+__device__ __forceinline__ void md5_compress_mk(uint32_t st[4], const uint32_t MK[64]) {
+  uint32_t a = st[0], b = st[1], c = st[2], d = st[3];
+#pragma unroll
+  for (int i = 0; i < 64; i++) {
+    const uint32_t f = i < 16   ? __builtin_amdgcn_bitop3_b32(b, c, d, 0xCA)
+                       : i < 32 ? __builtin_amdgcn_bitop3_b32(b, c, d, 0xE4)
+                       : i < 48 ? __builtin_amdgcn_bitop3_b32(b, c, d, 0x96)
+                                : __builtin_amdgcn_bitop3_b32(b, c, d, 0x39);
+    const uint32_t t = b + rotl(a + f + MK[i], md5_rot(i));
+    a = d;
+    d = c;
+    c = b;
+    b = t;
+  }
+  st[0] += a;
+  st[1] += b;
+  st[2] += c;
+  st[3] += d;
+}
+
+// Workgroup of two waves: wave 0 the chains, wave 1 their helper.  Slices are
+// full (slice_size bytes each), 16-byte aligned, with at least kMd5Chunk whole
+// blocks.
+__global__ __launch_bounds__(128) void k_md5_slices_mk(const uint8_t *__restrict__ data, long n,
+                                                       uint64_t slice_size, uint64_t stride,
+                                                       uint32_t *__restrict__ out, int high_priority) {
+  __shared__ u32x4 ring[kMd5RingBytes / 16];
+  if (high_priority) __builtin_amdgcn_s_setprio(3);
+  const int wave = __builtin_amdgcn_readfirstlane((int)threadIdx.x >> 6);
+  const int lane = threadIdx.x & 63;
+  const Md5Lane ln = md5_lane(n, lane);
+  const P2H_GLOBAL uint8_t *p = gptr(data) + (uint64_t)ln.slice * stride;
+  // 32-bit block counts: slices stay under kMd5MaxSliceBytes (128 GiB)
+  const uint32_t nch = (uint32_t)(slice_size >> 6) / kMd5Chunk;  // >= 1
+  // slot of block blk of a chunk at ring index ri
+  auto slot = [&](uint32_t ri, int blk) { return ring + (ri * kMd5Chunk + blk) * kMd5BlockVecs; };
+
+  if (wave == 1) {
+    // the helper: message words of chunk ch -> the ring's sums of chunk ch
+    uint32_t R[kMd5HelperSets][kMd5Chunk][16];
+    auto load = [&](uint32_t (&D)[kMd5Chunk][16], uint32_t ch) {
+      ch = ch < nch ? ch : nch - 1;  // unconditional: past the end re-reads the last chunk
+      const P2H_GLOBAL u32x4 *r =
+          reinterpret_cast<const P2H_GLOBAL u32x4 *>(p + (uint64_t)ch * (64 * kMd5Chunk));
+#pragma unroll
+      for (int b = 0; b < kMd5Chunk; b++)
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+          const u32x4 t = r[4 * b + k];
+          D[b][4 * k] = t.x;
+          D[b][4 * k + 1] = t.y;
+          D[b][4 * k + 2] = t.z;
+          D[b][4 * k + 3] = t.w;
+        }
+      __builtin_amdgcn_sched_barrier(0);  // issue the loads here, not at their first use
+    };
+    auto fill = [&](const uint32_t (&D)[kMd5Chunk][16], uint32_t ch) {
+#pragma unroll
+      for (int b = 0; b < kMd5Chunk; b++) {
+        u32x4 *q = slot(ch % kMd5Ring, b);
+#pragma unroll
+        for (int v = 0; v < 16; v++) {
+          u32x4 t;
+          t.x = D[b][md5_word(4 * v)] + kMd5T[4 * v];
+          t.y = D[b][md5_word(4 * v + 1)] + kMd5T[4 * v + 1];
+          t.z = D[b][md5_word(4 * v + 2)] + kMd5T[4 * v + 2];
+          t.w = D[b][md5_word(4 * v + 3)] + kMd5T[4 * v + 3];
+          q[v * 64 + lane] = t;
+        }
+      }
+    };
+    // chunks 0 and 1 before the first barrier; in step c the helper fills
+    // chunk c+2 (a slot the chain reads neither in step c nor before its
+    // barrier) and loads chunk c+5 into the registers it just used
+    load(R[0], 0);
+    load(R[1], 1);
+    load(R[2], 2);
+    fill(R[0], 0);
+    load(R[0], 3);
+    fill(R[1], 1);
+    load(R[1], 4);
+    __syncthreads();
+    auto step = [&](uint32_t c, uint32_t (&D)[kMd5Chunk][16]) {
+      fill(D, c + 2);
+      load(D, c + 5);
+      __syncthreads();
+    };
+    for (uint32_t c = 0; c < nch; c += kMd5HelperSets) {
+      step(c, R[2]);
+      if (c + 1 < nch) step(c + 1, R[0]);
+      if (c + 2 < nch) step(c + 2, R[1]);
+    }
+    // leave no load in flight when the wave ends
+    __builtin_amdgcn_s_waitcnt((7 << 4) | (15 << 8));
+    return;
+  }
+
+  // the chain
+  uint32_t st[4] = {0x67452301u, 0xefcdab89u, 0x98badcfeu, 0x10325476u};
+  uint32_t A[64], B[64];
+  auto read = [&](uint32_t (&D)[64], uint32_t ch, int blk) {
+    const u32x4 *q = slot(ch, blk);
+#pragma unroll
+    for (int v = 0; v < 16; v++) {
+      const u32x4 t = q[v * 64 + lane];
+      D[4 * v] = t.x;
+      D[4 * v + 1] = t.y;
+      D[4 * v + 2] = t.z;
+      D[4 * v + 3] = t.w;
+    }
+  };
+  // Each block's 16 reads go out a whole block before their use and are
+  // pinned there (the scheduler would sink them to their uses and expose the
+  // LDS latency at every use); one wait per block, where the reads are long
+  // done, tells the compiler so (s_waitcnt lgkmcnt(0), vmcnt and expcnt left
+  // alone), and the barrier's own wait covers the other block.
+  constexpr int kLgkm0 = 15 | (7 << 4) | (0 << 8) | (3 << 14);
+  __syncthreads();  // chunks 0 and 1 are in the ring
+  read(A, 0, 0);
+  __builtin_amdgcn_s_waitcnt(kLgkm0);
+  __builtin_amdgcn_sched_barrier(0);
+  uint32_t ci = 0;  // ring index of chunk c
+  for (uint32_t c = 0; c < nch; c++) {
+    const uint32_t ni = ci + 1 == kMd5Ring ? 0 : ci + 1;
+    read(B, ci, 1);
+    __builtin_amdgcn_sched_barrier(0);
+    md5_compress_mk(st, A);
+    __builtin_amdgcn_s_waitcnt(kLgkm0);
+    __builtin_amdgcn_sched_barrier(0);
+    // the next chunk was handed over at the previous barrier; after the last
+    // chunk this re-reads the current one (a slot the helper no longer writes)
+    read(A, c + 1 < nch ? ni : ci, 0);
+    __builtin_amdgcn_sched_barrier(0);
+    md5_compress_mk(st, B);
+    __syncthreads();
+    ci = ni;
+  }
+  md5_rest<16>(st, p, (uint64_t)nch * (64 * kMd5Chunk), slice_size, slice_size,
+               ln.store ? gptr(out) + 4 * ln.slice : nullptr);
+}
+
+double md5_chains_ms(size_t n, uint64_t bytes, int simds, bool helper) {
+  // issue slots per 64-byte block in the chain's main loop, one issue per 4
+  // clocks per wave (a wave64 VALU instruction occupies its SIMD 4 clocks):
+  // the helper-wave form 282 (256 for the steps, 4 state additions, 16 LDS
+  // reads, a wait, loop control and half a barrier), the plain form 330 (320
+  // for the steps, 4 state additions, 4 loads, a wait, loop control); counted
+  // in the compiled loops
   const double blocks = (double)((bytes + 8) / 64 + 1);
   const double waves = (double)((n + 63) / 64);
   const double rounds = std::ceil(waves / std::max(simds, 1));
-  return rounds * blocks * 324.0 * 4.0 / 2.4e9 * 1e3;
+  return rounds * blocks * (helper ? 282.0 : 330.0) * 4.0 / 2.4e9 * 1e3;
+}
+
+bool md5_helper_form(const par2hip_ctx *ctx, hipStream_t s, const void *d_data, size_t n, size_t slice_size,
+                     size_t stride, bool lens) {
+  if (stride == 0) stride = slice_size;
+  // the form holds 96 KiB of LDS, one workgroup per CU: taken for full
+  // 16-byte aligned slices of at least one chunk while every chain wave has a
+  // CU of its own among the stream's (the hash CUs on the side stream)
+  const long stream_cus = (s == ctx->side && ctx->hash_cus > 0) ? ctx->hash_cus : ctx->num_cus;
+  return !lens && (uintptr_t)d_data % 16 == 0 && stride % 16 == 0 && slice_size >= 64 * kMd5Chunk &&
+         (long)((n + 63) / 64) <= stream_cus;
 }
 
 int32_t launch_md5_list(par2hip_ctx *ctx, hipStream_t s, const HashEntry *d_list, size_t n, bool aligned4,
@@ -367,7 +561,10 @@ int32_t launch_md5(par2hip_ctx *ctx, hipStream_t s, const uint8_t *d_data, size_
   const bool a4 = ((uintptr_t)d_data % 4 == 0) && (stride % 4 == 0);
   ScopedTiming t(ctx, "md5_slices", s);
   dim3 grid((unsigned)((n + 63) / 64));
-  if (a16)
+  if (md5_helper_form(ctx, s, d_data, n, slice_size, stride, d_lens != nullptr))
+    hipLaunchKernelGGL(k_md5_slices_mk, grid, dim3(128), 0, s, d_data, (long)n, (uint64_t)slice_size,
+                       (uint64_t)stride, d_md5, high_priority ? 1 : 0);
+  else if (a16)
     hipLaunchKernelGGL(k_md5_slices<16>, grid, dim3(64), 0, s, d_data, (long)n, (uint64_t)slice_size,
                        (uint64_t)stride, d_lens, d_md5, high_priority ? 1 : 0);
   else if (a4)

@@ -315,28 +315,32 @@ struct Overlap {
 // Kernel and CUs of an encode contraction that overlaps the MD5 chains of
 // the same call on the side CUs.  Alone, the LDS-table contraction is the
 // faster one; beside the chains the chip is power-bound: the LDS form holds
-// every CU busy on lookups and pulls the shader clock to ~2.0-2.2 GHz, and
-// the chains, the step's critical path, slow with it.  The bit-sliced FP4
-// form on only as many CUs as it needs to finish well under the chains
-// (here: the narrowest of the narrow streams and the full complement that
-// brings its estimate under kPlanSlack of the chains' lower-bound time)
-// leaves them the clock.  1 GiB / R=102 with 32 MD5 CUs: the FP4 form on 128
-// CUs 10.88 ms per step (its contraction outlasts the chains), on 160 CUs
-// 9.80 ms, on 192 CUs 9.80 ms with 1.2 ms more slack, on all 224 10.3 ms
-// (the chains lose clock); with 8 MD5 CUs the LDS form on the other 248 CUs
-// took 10.7 ms.  If no width fits, the contraction is the critical path and
-// the LDS form on every other CU runs.
-constexpr double kPlanSlack = 0.75;
+// every CU busy on lookups and pulls the shader clock down, and the chains,
+// the step's critical path, slow with it.  The bit-sliced FP4 form on only as
+// many CUs as it needs to finish under the chains (here: the narrowest of the
+// narrow streams and the full complement that brings its estimate under
+// kPlanSlack of the chains' estimated time) leaves them the clock.  1 GiB /
+// R=102 with 32 MD5 CUs and the helper-wave chains (8.2 ms alone): the FP4
+// form on the 224 other CUs 8.39-8.48 ms per step (chains 8.28-8.37 ms,
+// contraction 6.6 ms, about 1.15x its estimate), the LDS form on the same
+// CUs 8.91-8.96 ms (chains 8.80-8.86 ms), two alternating rounds on one box.
+// With the plain chains (9.4 ms) the FP4 form on 128 CUs took 10.88 ms per
+// step (its contraction outlasted the chains), on 160 or 192 CUs 9.80 ms.  If
+// no width fits, the contraction is the critical path and the LDS form on
+// every other CU runs.
+constexpr double kPlanSlack = 0.85;
 struct GfPlan {
   int kernel = 0;             // 0: the context's choice
   hipStream_t work = nullptr;  // nullptr: rest
 };
-static GfPlan plan_overlapped_gf(const par2hip_ctx *ctx, size_t n, size_t slice_size, size_t r, int align_words) {
+static GfPlan plan_overlapped_gf(const par2hip_ctx *ctx, const void *d_slices, size_t n, size_t slice_size, size_t r,
+                                 int align_words) {
   GfPlan g;
   if (!ctx->masked || ctx->gf_kernel != 0 || r == 0) return g;
   g.kernel = 1;
   if (align_words < 8) return g;
-  const double md5_ms = md5_chains_ms(n, slice_size, 4 * ctx->hash_cus);
+  const double md5_ms = md5_chains_ms(n, slice_size, 4 * ctx->hash_cus,
+                                     md5_helper_form(ctx, ctx->side, d_slices, n, slice_size, slice_size, false));
   const double macs = 256.0 * (double)r * (double)n * (double)(slice_size / 2);
   for (int i = 0; i <= 3; i++) {
     const int cus = i < 3 ? ctx->narrow_cus[i] : ctx->num_cus - ctx->hash_cus;
@@ -392,7 +396,7 @@ int32_t encode_body(par2hip_ctx *ctx, const uint8_t *d_slices, size_t n, size_t 
   // from the start while CRC-32 and the contraction use the other CUs.
   const int align = std::min({align_words_of((uintptr_t)d_slices), align_words_of((uintptr_t)d_rec),
                               align_words_of(slice_size), align_words_of(rec_stride)});
-  const GfPlan plan = d_md5 ? plan_overlapped_gf(ctx, n, slice_size, r, align) : GfPlan{};
+  const GfPlan plan = d_md5 ? plan_overlapped_gf(ctx, d_slices, n, slice_size, r, align) : GfPlan{};
   std::unique_ptr<Overlap> ov;
   if (d_md5) {
     ov.reset(new Overlap(ctx, plan.work));

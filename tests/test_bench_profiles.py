@@ -63,11 +63,11 @@ def test_summaries_name_their_commands():
 
 def test_md5_issue_floor():
     """The MD5 floor counts the blocks MD5 pads a message to (RFC 1321: the
-    bytes, 0x80 and the 8-byte length, rounded up to 64) at 324 instructions
+    bytes, 0x80 and the 8-byte length, rounded up to 64) at 276 issue slots
     x 4 clocks each at 2.4 GHz, one round per SIMD's worth of 64-chain waves."""
     for s in (1, 55, 56, 63, 64, 119, 120, 512 << 10, 1 << 20):
         blocks = -(-(s + 9) // 64)
-        assert bench.md5_issue_floor_ms(64, s, 1) == blocks * 324 * 4 / 2.4e9 * 1e3, s
+        assert bench.md5_issue_floor_ms(64, s, 1) == blocks * 276 * 4 / 2.4e9 * 1e3, s
     one = bench.md5_issue_floor_ms(64 * 128, 4096, 128)
     assert bench.md5_issue_floor_ms(64 * 128 + 1, 4096, 128) == 2 * one
-    assert abs(bench.md5_issue_floor_ms(1024, 1 << 20, 128) - 16385 * 1296 / 2.4e6) < 1e-9
+    assert abs(bench.md5_issue_floor_ms(1024, 1 << 20, 128) - 16385 * 1104 / 2.4e6) < 1e-9

@@ -252,6 +252,29 @@ def test_hash_md5_loop_boundaries(ctx):
                 assert int(crc[i]) == zlib.crc32(b), (shift, s, i)
 
 
+@pytest.mark.parametrize("s", [256, 4160])
+def test_hash_more_chain_waves_than_cus(ctx, s):
+    """16385 full, 16-byte aligned slices: more 64-chain waves than the
+    device has CUs, so the chains run without a helper wave (the form that
+    needs a CU per chain wave is not taken); both hashes equal the stdlib's
+    on a sample of slices, the last one included."""
+    rng = np.random.default_rng(s)
+    n = 64 * ctx.device_info()["compute_units"] + 1
+    buf = rng.integers(0, 256, n * s, dtype=np.uint8)
+    d = ctx.alloc(len(buf))
+    d.upload(buf)
+    d_md5, d_crc = ctx.alloc(16 * n), ctx.alloc(4 * n)
+    say("hash", n, "slices of", s, "bytes")
+    ctx.hash_slices_dev(d, n, s, d_md5, d_crc)
+    ctx.sync()
+    md5 = d_md5.download().reshape(n, 16)
+    crc = d_crc.download().view(np.uint32)
+    for i in list(range(0, n, 997)) + [n - 1]:
+        b = buf[i * s:(i + 1) * s].tobytes()
+        assert md5[i].tobytes() == hashlib.md5(b).digest(), i
+        assert int(crc[i]) == zlib.crc32(b), i
+
+
 # --------------------------------------------------------------- repair --
 @pytest.mark.parametrize("m", [1, 2, 17, 100, 300])
 def test_gf_invert_vs_oracle(ctx, m):
