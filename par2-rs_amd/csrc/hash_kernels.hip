@@ -357,14 +357,16 @@ __global__ __launch_bounds__(64) void k_md5_list(const HashEntry *__restrict__ l
 // through a ring of kMd5Ring chunks of kMd5Chunk blocks (16 KiB per block):
 // while the chain reads chunk c (and, near its end, the first block of chunk
 // c+1), the helper fills chunk c+2, and one barrier per chunk hands them
-// over.  The helper keeps the message words of kMd5HelperSets chunks in
-// flight in registers (~3 us of the chain's work).  Whole blocks of full,
-// 16-byte aligned slices; the chain hashes the last (< kMd5Chunk) blocks and
-// the padding itself.
-constexpr int kMd5Chunk = 2, kMd5Ring = 3, kMd5HelperSets = 3;
+// over.  A barrier costs the chain ~120 clocks beyond its issue slot, so the
+// chunks are as long as the LDS allows: three blocks (a 144 KiB ring; with
+// two, 96 KiB, MD5 of 1024 x 1 MiB alone took 8.19 instead of 8.05 ms).  The
+// helper keeps the message words of kMd5HelperSets chunks in flight in
+// registers (~3 us of the chain's work).  Whole blocks of full, 16-byte
+// aligned slices; the chain hashes the last (< kMd5Chunk) blocks and the
+// padding itself.
+constexpr int kMd5Chunk = 3, kMd5Ring = 3, kMd5HelperSets = 3;
 constexpr int kMd5BlockVecs = 16 * 64;  // u32x4 per block slot: 16 per lane
 constexpr int kMd5RingBytes = kMd5Ring * kMd5Chunk * kMd5BlockVecs * 16;
-static_assert(kMd5Chunk == 2, "the chain's two register sets are one chunk");
 
 // message word of MD5 step i (RFC 1321 section 3.4: rounds take words k,
 // 5k+1, 3k+5 and 7k mod 16)
@@ -476,7 +478,7 @@ __global__ __launch_bounds__(128) void k_md5_slices_mk(const uint8_t *__restrict
 
   // the chain
   uint32_t st[4] = {0x67452301u, 0xefcdab89u, 0x98badcfeu, 0x10325476u};
-  uint32_t A[64], B[64];
+  uint32_t S[kMd5Chunk][64];  // one register set per block of a chunk
   auto read = [&](uint32_t (&D)[64], uint32_t ch, int blk) {
     const u32x4 *q = slot(ch, blk);
 #pragma unroll
@@ -490,27 +492,32 @@ __global__ __launch_bounds__(128) void k_md5_slices_mk(const uint8_t *__restrict
   };
   // Each block's 16 reads go out a whole block before their use and are
   // pinned there (the scheduler would sink them to their uses and expose the
-  // LDS latency at every use); one wait per block, where the reads are long
-  // done, tells the compiler so (s_waitcnt lgkmcnt(0), vmcnt and expcnt left
-  // alone), and the barrier's own wait covers the other block.
+  // LDS latency at every use); one wait before each later read of a chunk,
+  // where the previous block's reads are long done, tells the compiler so
+  // (s_waitcnt lgkmcnt(0), vmcnt and expcnt left alone), and the barrier's
+  // own wait covers the chunk's last read.
   constexpr int kLgkm0 = 15 | (7 << 4) | (0 << 8) | (3 << 14);
   __syncthreads();  // chunks 0 and 1 are in the ring
-  read(A, 0, 0);
+  read(S[0], 0, 0);
   __builtin_amdgcn_s_waitcnt(kLgkm0);
   __builtin_amdgcn_sched_barrier(0);
   uint32_t ci = 0;  // ring index of chunk c
   for (uint32_t c = 0; c < nch; c++) {
     const uint32_t ni = ci + 1 == kMd5Ring ? 0 : ci + 1;
-    read(B, ci, 1);
-    __builtin_amdgcn_sched_barrier(0);
-    md5_compress_mk(st, A);
-    __builtin_amdgcn_s_waitcnt(kLgkm0);
-    __builtin_amdgcn_sched_barrier(0);
-    // the next chunk was handed over at the previous barrier; after the last
-    // chunk this re-reads the current one (a slot the helper no longer writes)
-    read(A, c + 1 < nch ? ni : ci, 0);
-    __builtin_amdgcn_sched_barrier(0);
-    md5_compress_mk(st, B);
+#pragma unroll
+    for (int b = 0; b < kMd5Chunk; b++) {
+      if (b > 0) {
+        __builtin_amdgcn_s_waitcnt(kLgkm0);
+        __builtin_amdgcn_sched_barrier(0);
+      }
+      // the next chunk was handed over at the previous barrier; after the
+      // last chunk this re-reads the current one (a slot the helper no longer
+      // writes)
+      if (b + 1 < kMd5Chunk) read(S[b + 1], ci, b + 1);
+      else read(S[0], c + 1 < nch ? ni : ci, 0);
+      __builtin_amdgcn_sched_barrier(0);
+      md5_compress_mk(st, S[b]);
+    }
     __syncthreads();
     ci = ni;
   }
@@ -521,20 +528,20 @@ __global__ __launch_bounds__(128) void k_md5_slices_mk(const uint8_t *__restrict
 double md5_chains_ms(size_t n, uint64_t bytes, int simds, bool helper) {
   // issue slots per 64-byte block in the chain's main loop, one issue per 4
   // clocks per wave (a wave64 VALU instruction occupies its SIMD 4 clocks):
-  // the helper-wave form 282 (256 for the steps, 4 state additions, 16 LDS
-  // reads, a wait, loop control and half a barrier), the plain form 330 (320
+  // the helper-wave form 281 (256 for the steps, 4 state additions, 16 LDS
+  // reads, a wait, loop control and a third of a barrier), the plain form 330 (320
   // for the steps, 4 state additions, 4 loads, a wait, loop control); counted
   // in the compiled loops
   const double blocks = (double)((bytes + 8) / 64 + 1);
   const double waves = (double)((n + 63) / 64);
   const double rounds = std::ceil(waves / std::max(simds, 1));
-  return rounds * blocks * (helper ? 282.0 : 330.0) * 4.0 / 2.4e9 * 1e3;
+  return rounds * blocks * (helper ? 281.0 : 330.0) * 4.0 / 2.4e9 * 1e3;
 }
 
 bool md5_helper_form(const par2hip_ctx *ctx, hipStream_t s, const void *d_data, size_t n, size_t slice_size,
                      size_t stride, bool lens) {
   if (stride == 0) stride = slice_size;
-  // the form holds 96 KiB of LDS, one workgroup per CU: taken for full
+  // the form holds 144 KiB of LDS, one workgroup per CU: taken for full
   // 16-byte aligned slices of at least one chunk while every chain wave has a
   // CU of its own among the stream's (the hash CUs on the side stream)
   const long stream_cus = (s == ctx->side && ctx->hash_cus > 0) ? ctx->hash_cus : ctx->num_cus;
