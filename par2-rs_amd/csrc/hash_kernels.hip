@@ -399,20 +399,16 @@ __device__ __forceinline__ void md5_compress_mk(uint32_t st[4], const uint32_t M
   st[3] += d;
 }
 
-// Workgroup of two waves: wave 0 the chains, wave 1 their helper.  Slices are
-// full (slice_size bytes each), 16-byte aligned, with at least kMd5Chunk whole
-// blocks.
-__global__ __launch_bounds__(128) void k_md5_slices_mk(const uint8_t *__restrict__ data, long n,
-                                                       uint64_t slice_size, uint64_t stride,
-                                                       uint32_t *__restrict__ out, int high_priority) {
+// The chunk loop of the helper-wave form, shared by the two kernels below:
+// the first nch chunks of the lane's slice p (nch the same for the whole
+// workgroup: wave 0 the chains, wave 1 their helper).  The helper returns
+// false; the chain returns true with st advanced past those chunks.  kAlign:
+// 16 or 4 = p has that byte alignment.
+template <int kAlign>
+__device__ __forceinline__ bool md5_mk_chunks(const P2H_GLOBAL uint8_t *__restrict__ p, uint32_t nch, int wave,
+                                              int lane, uint32_t st[4]) {
+  if (nch == 0) return wave == 0;  // uniform: no chunk, no barrier
   __shared__ u32x4 ring[kMd5RingBytes / 16];
-  if (high_priority) __builtin_amdgcn_s_setprio(3);
-  const int wave = __builtin_amdgcn_readfirstlane((int)threadIdx.x >> 6);
-  const int lane = threadIdx.x & 63;
-  const Md5Lane ln = md5_lane(n, lane);
-  const P2H_GLOBAL uint8_t *p = gptr(data) + (uint64_t)ln.slice * stride;
-  // 32-bit block counts: slices stay under kMd5MaxSliceBytes (128 GiB)
-  const uint32_t nch = (uint32_t)(slice_size >> 6) / kMd5Chunk;  // >= 1
   // slot of block blk of a chunk at ring index ri
   auto slot = [&](uint32_t ri, int blk) { return ring + (ri * kMd5Chunk + blk) * kMd5BlockVecs; };
 
@@ -421,18 +417,26 @@ __global__ __launch_bounds__(128) void k_md5_slices_mk(const uint8_t *__restrict
     uint32_t R[kMd5HelperSets][kMd5Chunk][16];
     auto load = [&](uint32_t (&D)[kMd5Chunk][16], uint32_t ch) {
       ch = ch < nch ? ch : nch - 1;  // unconditional: past the end re-reads the last chunk
-      const P2H_GLOBAL u32x4 *r =
-          reinterpret_cast<const P2H_GLOBAL u32x4 *>(p + (uint64_t)ch * (64 * kMd5Chunk));
+      const P2H_GLOBAL uint8_t *b0 = p + (uint64_t)ch * (64 * kMd5Chunk);
+      if (kAlign == 16) {
+        const P2H_GLOBAL u32x4 *r = reinterpret_cast<const P2H_GLOBAL u32x4 *>(b0);
 #pragma unroll
-      for (int b = 0; b < kMd5Chunk; b++)
+        for (int b = 0; b < kMd5Chunk; b++)
 #pragma unroll
-        for (int k = 0; k < 4; k++) {
-          const u32x4 t = r[4 * b + k];
-          D[b][4 * k] = t.x;
-          D[b][4 * k + 1] = t.y;
-          D[b][4 * k + 2] = t.z;
-          D[b][4 * k + 3] = t.w;
-        }
+          for (int k = 0; k < 4; k++) {
+            const u32x4 t = r[4 * b + k];
+            D[b][4 * k] = t.x;
+            D[b][4 * k + 1] = t.y;
+            D[b][4 * k + 2] = t.z;
+            D[b][4 * k + 3] = t.w;
+          }
+      } else {
+        const P2H_GLOBAL uint32_t *r = reinterpret_cast<const P2H_GLOBAL uint32_t *>(b0);
+#pragma unroll
+        for (int b = 0; b < kMd5Chunk; b++)
+#pragma unroll
+          for (int k = 0; k < 16; k++) D[b][k] = r[16 * b + k];
+      }
       __builtin_amdgcn_sched_barrier(0);  // issue the loads here, not at their first use
     };
     auto fill = [&](const uint32_t (&D)[kMd5Chunk][16], uint32_t ch) {
@@ -473,11 +477,10 @@ __global__ __launch_bounds__(128) void k_md5_slices_mk(const uint8_t *__restrict
     }
     // leave no load in flight when the wave ends
     __builtin_amdgcn_s_waitcnt((7 << 4) | (15 << 8));
-    return;
+    return false;
   }
 
   // the chain
-  uint32_t st[4] = {0x67452301u, 0xefcdab89u, 0x98badcfeu, 0x10325476u};
   uint32_t S[kMd5Chunk][64];  // one register set per block of a chunk
   auto read = [&](uint32_t (&D)[64], uint32_t ch, int blk) {
     const u32x4 *q = slot(ch, blk);
@@ -521,8 +524,46 @@ __global__ __launch_bounds__(128) void k_md5_slices_mk(const uint8_t *__restrict
     __syncthreads();
     ci = ni;
   }
+  return true;
+}
+
+// Whole-set form: full slices (slice_size bytes each), 16-byte aligned.
+__global__ __launch_bounds__(128) void k_md5_slices_mk(const uint8_t *__restrict__ data, long n,
+                                                       uint64_t slice_size, uint64_t stride,
+                                                       uint32_t *__restrict__ out, int high_priority) {
+  if (high_priority) __builtin_amdgcn_s_setprio(3);
+  const int wave = __builtin_amdgcn_readfirstlane((int)threadIdx.x >> 6);
+  const int lane = threadIdx.x & 63;
+  const Md5Lane ln = md5_lane(n, lane);
+  const P2H_GLOBAL uint8_t *p = gptr(data) + (uint64_t)ln.slice * stride;
+  // 32-bit block counts: slices stay under kMd5MaxSliceBytes (128 GiB)
+  const uint32_t nch = (uint32_t)(slice_size >> 6) / kMd5Chunk;
+  uint32_t st[4] = {0x67452301u, 0xefcdab89u, 0x98badcfeu, 0x10325476u};
+  if (!md5_mk_chunks<16>(p, nch, wave, lane, st)) return;
   md5_rest<16>(st, p, (uint64_t)nch * (64 * kMd5Chunk), slice_size, slice_size,
                ln.store ? gptr(out) + 4 * ln.slice : nullptr);
+}
+
+// Batched form: one lane per entry of a slice list (4-byte aligned entries,
+// longest first so the lanes of a workgroup have similar lengths).  The
+// helper covers the whole chunks of real bytes that every lane of the
+// workgroup has; each chain hashes the rest of its own entry.
+__global__ __launch_bounds__(128) void k_md5_list_mk(const HashEntry *__restrict__ list, long n,
+                                                     int high_priority) {
+  if (high_priority) __builtin_amdgcn_s_setprio(3);
+  const int wave = __builtin_amdgcn_readfirstlane((int)threadIdx.x >> 6);
+  const int lane = threadIdx.x & 63;
+  const Md5Lane ln = md5_lane(n, lane);
+  const HashEntry e = list[ln.slice];
+  const P2H_GLOBAL uint8_t *p = gptr(e.data);
+  // 32-bit block counts: hashed entries stay under kMd5MaxSliceBytes
+  uint32_t nch = (uint32_t)(e.len >> 6) / kMd5Chunk;
+#pragma unroll
+  for (int o = 32; o; o >>= 1) nch = min(nch, (uint32_t)__shfl_xor((int)nch, o));
+  nch = (uint32_t)__builtin_amdgcn_readfirstlane((int)nch);
+  uint32_t st[4] = {0x67452301u, 0xefcdab89u, 0x98badcfeu, 0x10325476u};
+  if (!md5_mk_chunks<4>(p, nch, wave, lane, st)) return;
+  md5_rest<4>(st, p, (uint64_t)nch * (64 * kMd5Chunk), e.len, e.size, ln.store ? gptr(e.md5) : nullptr);
 }
 
 double md5_chains_ms(size_t n, uint64_t bytes, int simds, bool helper) {
@@ -553,8 +594,16 @@ int32_t launch_md5_list(par2hip_ctx *ctx, hipStream_t s, const HashEntry *d_list
                         bool high_priority) {
   if (n == 0) return PAR2HIP_OK;
   ScopedTiming t(ctx, "md5_slices", s);
-  hipLaunchKernelGGL(k_md5_list, dim3((unsigned)((n + 63) / 64)), dim3(64), 0, s, d_list, (long)n, aligned4 ? 1 : 0,
-                     high_priority ? 1 : 0);
+  const long waves = (long)((n + 63) / 64);
+  // the helper-wave form while every chain wave has a CU of its own (its
+  // ring holds 144 KiB of LDS), as for the whole-set kernel
+  const long stream_cus = (s == ctx->side && ctx->hash_cus > 0) ? ctx->hash_cus : ctx->num_cus;
+  if (aligned4 && waves <= stream_cus)
+    hipLaunchKernelGGL(k_md5_list_mk, dim3((unsigned)waves), dim3(128), 0, s, d_list, (long)n,
+                       high_priority ? 1 : 0);
+  else
+    hipLaunchKernelGGL(k_md5_list, dim3((unsigned)waves), dim3(64), 0, s, d_list, (long)n, aligned4 ? 1 : 0,
+                       high_priority ? 1 : 0);
   P2H_CHECK(ctx, hipGetLastError());
   return PAR2HIP_OK;
 }
