@@ -498,14 +498,25 @@ int32_t launch_gauss_jordan(par2hip_ctx *ctx, hipStream_t s, uint16_t *d_aug, in
 // inverse has a closed form (Lagrange basis): with P(t) = prod_k (t + x_k)
 // and Q_j(t) = P(t) / (t + x_j) = sum_i q_ji t^i, Q_j(x_k) = 0 for k != j, so
 //   (A^-1)[j][i] = q_ji / (Q_j(x_j) * x_j^e0).
-// One workgroup: wave 0 builds P's coefficients in LDS, then thread j divides
-// out (t + x_j), evaluates Q_j(x_j) by Horner and writes row j of A^-1 into the
-// right half of the m x 2m workspace (the layout the Gauss-Jordan path uses).
+// One workgroup: wave 0 builds P's coefficients in LDS, then thread j finds
+// Q_j(x_j) = P'(x_j) (characteristic 2: (t + x_j) Q_j'(t) vanishes at x_j, and
+// P' keeps the odd-degree terms), forms s_j = 1 / (Q_j(x_j) x_j^e0), and divides
+// out (t + x_j), writing s_j q_ji as row j of A^-1 into the right half of the
+// m x 2m workspace (the layout the Gauss-Jordan path uses).  s_j q is formed in
+// registers from s_j x^k (k = 0..15): no table lookup per entry.
 // The x_j are distinct powers of 2, so the matrix is never singular.
 // Multiplication by a fixed x through four 16-entry nibble tables in LDS:
 // x*v = T0[v & 15] ^ T1[(v >> 4) & 15] ^ T2[(v >> 8) & 15] ^ T3[v >> 12].
 __device__ __forceinline__ uint32_t mul_tab(const uint16_t *__restrict__ t, uint32_t v) {
   return (uint32_t)t[v & 15u] ^ t[16u + ((v >> 4) & 15u)] ^ t[32u + ((v >> 8) & 15u)] ^ t[48u + (v >> 12)];
+}
+
+// s * v from bs[k] = s * x^k (k = 0..15): the xor of bs[k] over the set bits of v.
+__device__ __forceinline__ uint32_t mul_bits(const uint32_t (&bs)[16], uint32_t v) {
+  uint32_t r = 0;
+#pragma unroll
+  for (int k = 0; k < 16; k++) r ^= (0u - ((v >> k) & 1u)) & bs[k];
+  return r;
 }
 
 __global__ __launch_bounds__(1024) void k_vandermonde_inverse(uint16_t *__restrict__ aug, const uint32_t *__restrict__ miss_log,
@@ -542,24 +553,26 @@ __global__ __launch_bounds__(1024) void k_vandermonde_inverse(uint16_t *__restri
   __syncthreads();
   for (int j = tid; j < m; j += blockDim.x) {
     const uint16_t *tj = tabs + (size_t)j * 64;
-    // synthetic division of P by (t + x_j): q_{m-1} = p_m, q_{i-1} = p_i + x_j q_i;
-    // Horner for Q_j(x_j) runs alongside
-    uint16_t *row = aug + (size_t)j * 2 * m + m;
-    uint32_t q = poly[m];  // = 1
-    row[m - 1] = (uint16_t)q;
-    uint32_t h = q;
-    for (int i = m - 1; i >= 1; i--) {
-      q = poly[i] ^ mul_tab(tj, q);
-      row[i - 1] = (uint16_t)q;
-      h = mul_tab(tj, h) ^ q;
-    }
-    // h = Q_j(x_j) != 0; scale the row by 1 / (h * x_j^e0)
+    // h = P'(x_j) = sum over odd i of p_i x_j^(i-1), by Horner in x_j^2
+    uint32_t h = 0;
+    for (int i = (m & 1) ? m : m - 1; i >= 1; i -= 2) h = mul_tab(tj, mul_tab(tj, h)) ^ poly[i];
+    // h != 0 (the x_k are distinct); s = 1 / (h * x_j^e0)
     const uint32_t lx = miss_log[j];
     const uint32_t ld = (log_tab[h] + (uint32_t)(((uint64_t)lx * e0) % kOrder)) % kOrder;
-    const uint32_t linv = kOrder - ld;
-    for (int i = 0; i < m; i++) {
-      const uint32_t v = row[i];
-      row[i] = v ? exp_tab[log_tab[v] + linv] : 0;
+    uint32_t bs[16];
+    bs[0] = exp_tab[ld ? kOrder - ld : 0];
+#pragma unroll
+    for (int k = 1; k < 16; k++) {
+      const uint32_t t = bs[k - 1] << 1;
+      bs[k] = (t & 0x10000u) ? (t ^ kPoly) : t;
+    }
+    // synthetic division of P by (t + x_j): q_{m-1} = p_m, q_{i-1} = p_i + x_j q_i
+    uint16_t *row = aug + (size_t)j * 2 * m + m;
+    uint32_t q = poly[m];  // = 1
+    row[m - 1] = (uint16_t)mul_bits(bs, q);
+    for (int i = m - 1; i >= 1; i--) {
+      q = poly[i] ^ mul_tab(tj, q);
+      row[i - 1] = (uint16_t)mul_bits(bs, q);
     }
   }
 }

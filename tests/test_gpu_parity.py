@@ -203,6 +203,28 @@ def test_hash_dev_unaligned_bases(ctx, shift, n, s, stride):
         assert int(crc[i]) == zlib.crc32(b), f"crc slice {i}"
 
 
+@pytest.mark.parametrize("n,s", [(3, 65536), (5, 196608), (2, 1 << 21)])
+def test_hash_whole_lane_chunks_aligned_and_offset(ctx, n, s):
+    """Slices whose 256 CRC-32 lane chunks are whole multiples of 256 bytes
+    (no leading virtual zeros), at a 16-byte aligned base and at a 4-byte
+    offset: both hashes equal the stdlib's."""
+    rng = np.random.default_rng(n * 7 + s)
+    buf = rng.integers(0, 256, 16 + n * s, dtype=np.uint8)
+    d = ctx.alloc(len(buf))
+    d.upload(buf)
+    for shift in (0, 4):
+        d_md5, d_crc = ctx.alloc(16 * n), ctx.alloc(4 * n)
+        say("hash whole lane chunks", n, s, shift)
+        ctx.hash_slices_dev(d.view(shift), n, s, d_md5, d_crc)
+        ctx.sync()
+        md5 = d_md5.download().reshape(n, 16)
+        crc = d_crc.download().view(np.uint32)
+        for i in range(n):
+            b = buf[shift + i * s:shift + (i + 1) * s].tobytes()
+            assert int(crc[i]) == zlib.crc32(b), f"crc slice {i} shift {shift}"
+            assert md5[i].tobytes() == hashlib.md5(b).digest(), f"md5 slice {i} shift {shift}"
+
+
 def test_hash_ragged_lengths_zero_padded(ctx):
     rng = np.random.default_rng(3)
     s = 5376
