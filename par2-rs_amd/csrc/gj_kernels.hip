@@ -519,6 +519,35 @@ __device__ __forceinline__ uint32_t mul_bits(const uint32_t (&bs)[16], uint32_t 
   return r;
 }
 
+// Row j of the closed-form inverse from P's coefficients (poly, LDS) and x_j's
+// nibble tables (tj): h = P'(x_j) = sum over odd i of p_i x_j^(i-1) by Horner
+// in x_j^2; s = 1 / (h x_j^e0) (h != 0: the x_k are distinct); then the
+// synthetic division of P by (t + x_j), q_{m-1} = p_m, q_{i-1} = p_i + x_j q_i,
+// with s q_i written to row[i] (and to col[i] when col is given).
+__device__ __forceinline__ void vand_inverse_row(const uint16_t *__restrict__ tj, const uint16_t *__restrict__ poly,
+                                                 int m, uint32_t lx, uint32_t e0,
+                                                 const uint16_t *__restrict__ log_tab,
+                                                 const uint16_t *__restrict__ exp_tab, uint16_t *__restrict__ row,
+                                                 uint16_t *__restrict__ col) {
+  uint32_t h = 0;
+  for (int i = (m & 1) ? m : m - 1; i >= 1; i -= 2) h = mul_tab(tj, mul_tab(tj, h)) ^ poly[i];
+  const uint32_t ld = (log_tab[h] + (uint32_t)(((uint64_t)lx * e0) % kOrder)) % kOrder;
+  uint32_t bs[16];
+  bs[0] = exp_tab[ld ? kOrder - ld : 0];
+#pragma unroll
+  for (int k = 1; k < 16; k++) {
+    const uint32_t t = bs[k - 1] << 1;
+    bs[k] = (t & 0x10000u) ? (t ^ kPoly) : t;
+  }
+  uint32_t q = poly[m];  // = 1
+  for (int i = m - 1; i >= 0; i--) {
+    const uint16_t w = (uint16_t)mul_bits(bs, q);
+    row[i] = w;
+    if (col) col[i] = w;
+    if (i > 0) q = poly[i] ^ mul_tab(tj, q);
+  }
+}
+
 __global__ __launch_bounds__(1024) void k_vandermonde_inverse(uint16_t *__restrict__ aug, const uint32_t *__restrict__ miss_log,
                                                              uint32_t e0, int m, const uint16_t *__restrict__ log_tab,
                                                              const uint16_t *__restrict__ exp_tab) {
@@ -551,30 +580,9 @@ __global__ __launch_bounds__(1024) void k_vandermonde_inverse(uint16_t *__restri
     }
   }
   __syncthreads();
-  for (int j = tid; j < m; j += blockDim.x) {
-    const uint16_t *tj = tabs + (size_t)j * 64;
-    // h = P'(x_j) = sum over odd i of p_i x_j^(i-1), by Horner in x_j^2
-    uint32_t h = 0;
-    for (int i = (m & 1) ? m : m - 1; i >= 1; i -= 2) h = mul_tab(tj, mul_tab(tj, h)) ^ poly[i];
-    // h != 0 (the x_k are distinct); s = 1 / (h * x_j^e0)
-    const uint32_t lx = miss_log[j];
-    const uint32_t ld = (log_tab[h] + (uint32_t)(((uint64_t)lx * e0) % kOrder)) % kOrder;
-    uint32_t bs[16];
-    bs[0] = exp_tab[ld ? kOrder - ld : 0];
-#pragma unroll
-    for (int k = 1; k < 16; k++) {
-      const uint32_t t = bs[k - 1] << 1;
-      bs[k] = (t & 0x10000u) ? (t ^ kPoly) : t;
-    }
-    // synthetic division of P by (t + x_j): q_{m-1} = p_m, q_{i-1} = p_i + x_j q_i
-    uint16_t *row = aug + (size_t)j * 2 * m + m;
-    uint32_t q = poly[m];  // = 1
-    row[m - 1] = (uint16_t)mul_bits(bs, q);
-    for (int i = m - 1; i >= 1; i--) {
-      q = poly[i] ^ mul_tab(tj, q);
-      row[i - 1] = (uint16_t)mul_bits(bs, q);
-    }
-  }
+  for (int j = tid; j < m; j += blockDim.x)
+    vand_inverse_row(tabs + (size_t)j * 64, poly, m, miss_log[j], e0, log_tab, exp_tab,
+                     aug + (size_t)j * 2 * m + m, nullptr);
 }
 
 int32_t launch_vandermonde_inverse(par2hip_ctx *ctx, hipStream_t s, uint16_t *d_aug, const uint32_t *d_miss_log,
@@ -665,27 +673,10 @@ __global__ __launch_bounds__(1024) void k_inverse_batch(const InvJob *__restrict
     }
   }
   __syncthreads();
-  for (int j = tid; j < m; j += blockDim.x) {
-    const uint16_t *tj = tabs + (size_t)j * 64;
-    uint16_t *row = J.aug + (size_t)j * 2 * m + m;
-    uint32_t q = poly[m];
-    row[m - 1] = (uint16_t)q;
-    uint32_t h = q;
-    for (int i = m - 1; i >= 1; i--) {
-      q = poly[i] ^ mul_tab(tj, q);
-      row[i - 1] = (uint16_t)q;
-      h = mul_tab(tj, h) ^ q;
-    }
-    const uint32_t lx = J.miss_log[j];
-    const uint32_t ld = (log_tab[h] + (uint32_t)(((uint64_t)lx * J.e0) % kOrder)) % kOrder;
-    const uint32_t linv = kOrder - ld;
-    for (int i = 0; i < m; i++) {
-      const uint32_t v = row[i];
-      const uint16_t w = v ? exp_tab[log_tab[v] + linv] : 0;
-      row[i] = w;
-      J.coeff[(size_t)j * J.cs + J.np + i] = w;  // inverse columns of C
-    }
-  }
+  // row j of the inverse, also into the inverse columns of C
+  for (int j = tid; j < m; j += blockDim.x)
+    vand_inverse_row(tabs + (size_t)j * 64, poly, m, J.miss_log[j], J.e0, log_tab, exp_tab,
+                     J.aug + (size_t)j * 2 * m + m, J.coeff + (size_t)j * J.cs + J.np);
   // V rows: V[i][c] = c_{present_c}^{e_i}
   for (long e = tid; e < (long)m * J.np; e += blockDim.x) {
     const int i = (int)(e / J.np), c = (int)(e % J.np);
