@@ -84,7 +84,7 @@ struct par2hip_ctx {
   std::string err;
   uint16_t *d_exp = nullptr;          // device copy of the doubled exp table
   uint16_t *d_log = nullptr;          // device copy of the log table
-  uint32_t *d_crc_tab = nullptr;      // CRC-32 slice-by-4 tables T0..T3 (4 x 256)
+  uint32_t *d_crc_tab = nullptr;      // CRC-32 slice-by-4 tables T0..T3, then the row-skip tables Z0..Z3 (8 x 256)
   int *d_status = nullptr;            // device status word (singular flag)
   par2hip::ParamArena arena;
   par2hip::DevBuf scratch[12];        // grow-only device scratch, by role
@@ -321,12 +321,17 @@ constexpr int kAutoGfKernel = 1;
 // hashes refuses larger slices with PAR2HIP_EINVAL before any launch.
 constexpr uint64_t kMd5MaxSliceBytes = uint64_t(1) << 37;
 constexpr int kCrcLanes = 256;  // CRC-32: lanes (one workgroup) per slice
+// CRC-32 row layout: lane l reads the kCrcPiece bytes at l * kCrcPiece of each
+// kCrcRow-byte row, so neighbouring lanes read neighbouring bytes
+constexpr uint64_t kCrcPiece = 64, kCrcRow = kCrcLanes * kCrcPiece;
 // Per-slice-size constants of the CRC-32 tree combine.
 struct CrcParams {
-  uint64_t chunk;       // bytes per lane (multiple of 16)
-  uint64_t pad;         // leading virtual zero bytes
-  uint32_t level[8];    // x^(8*chunk*2^k) mod P, k = 0..7
-  uint32_t init_term;   // 0xFFFFFFFF * x^(8*slice_size) mod P
+  uint64_t chunk;          // bytes per lane (multiple of 16)
+  uint64_t pad;            // leading virtual zero bytes
+  uint32_t level[8];       // x^(8*chunk*2^k) mod P, k = 0..7
+  uint32_t init_term;      // 0xFFFFFFFF * x^(8*slice_size) mod P
+  uint32_t rows;           // slice_size / kCrcRow when that divides evenly (row layout), else 0
+  uint32_t row_level[8];   // x^(8*kCrcPiece*2^k) mod P, k = 0..7
 };
 CrcParams crc_params_for(uint64_t slice_size);
 // Pointers read from device memory are generic to the compiler, which then

@@ -682,6 +682,42 @@ __device__ __forceinline__ uint32_t crc_run(uint32_t crc, const P2H_GLOBAL uint8
   return crc;
 }
 
+// Row layout (slice_size a multiple of kCrcRow, 16-byte aligned, no short
+// length): lane l's register runs over its kCrcPiece-byte piece of each row,
+// the other lanes' bytes counted as zeros.  Skipping the kCrcRow - kCrcPiece
+// bytes between two of its pieces multiplies the register by
+// x^(8 * (kCrcRow - kCrcPiece)), four lookups in Z (Z_b[v] = (v << 8b) times
+// that power).  By linearity the slice's register is the xor over lanes of
+// each lane's register times x^(8 * kCrcPiece * (lanes to its right)), which
+// the tree join forms with the row levels.  The next row's piece is loaded
+// while the current one is folded (two rows ahead measured the same).
+__device__ __forceinline__ uint32_t crc_rows(const P2H_GLOBAL uint8_t *__restrict__ p, uint32_t rows,
+                                             const uint32_t (*T)[256], const uint32_t (*Z)[256]) {
+  const P2H_GLOBAL u32x4 *q = reinterpret_cast<const P2H_GLOBAL u32x4 *>(p + (uint64_t)threadIdx.x * kCrcPiece);
+  constexpr uint64_t kRowVecs = kCrcRow / 16;
+  u32x4 v[4], w[4];
+#pragma unroll
+  for (int k = 0; k < 4; k++) v[k] = q[k];
+  uint32_t crc = 0u;
+  for (uint32_t r = 0; r < rows; r++) {
+    // the next row's piece (the last row reloads its own: in bounds)
+    const P2H_GLOBAL u32x4 *nq = q + (uint64_t)(r + 1 < rows ? r + 1 : r) * kRowVecs;
+#pragma unroll
+    for (int k = 0; k < 4; k++) w[k] = nq[k];
+    crc = Z[0][crc & 0xFF] ^ Z[1][(crc >> 8) & 0xFF] ^ Z[2][(crc >> 16) & 0xFF] ^ Z[3][crc >> 24];
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+      crc = crc_word(crc, v[k].x, T);
+      crc = crc_word(crc, v[k].y, T);
+      crc = crc_word(crc, v[k].z, T);
+      crc = crc_word(crc, v[k].w, T);
+    }
+#pragma unroll
+    for (int k = 0; k < 4; k++) v[k] = w[k];
+  }
+  return crc;
+}
+
 // Register update over n zero bytes.
 __device__ __forceinline__ uint32_t crc_zeros(uint32_t crc, uint64_t n, const uint32_t (*T)[256]) {
   for (; n >= 4; n -= 4) crc = crc_word(crc, 0u, T);
@@ -689,45 +725,54 @@ __device__ __forceinline__ uint32_t crc_zeros(uint32_t crc, uint64_t n, const ui
   return crc;
 }
 
-// CRC-32 of one slice by the whole 256-lane workgroup (tables T already in
-// LDS): bytes [0, len) of p, zeros up to slice_size; result to *out.
-__device__ __forceinline__ void crc_slice_wg(const uint32_t (*T)[256], uint32_t *part,
+// CRC-32 of one slice by the whole 256-lane workgroup (tables T and Z already
+// in LDS): bytes [0, len) of p, zeros up to slice_size; result to *out.
+__device__ __forceinline__ void crc_slice_wg(const uint32_t (*T)[256], const uint32_t (*Z)[256], uint32_t *part,
                                              const P2H_GLOBAL uint8_t *__restrict__ p, uint64_t len,
                                              uint64_t slice_size, const CrcParams &prm,
                                              P2H_GLOBAL uint32_t *__restrict__ out) {
   const int lane = threadIdx.x;
-  // this lane's bytes in slice coordinates: [a, e) real or zero-padding
-  const int64_t va = (int64_t)lane * (int64_t)prm.chunk - (int64_t)prm.pad;
-  const int64_t ve = va + (int64_t)prm.chunk;
-  const uint64_t a = va < 0 ? 0 : (uint64_t)va;
-  const uint64_t e = ve < 0 ? 0 : (uint64_t)ve;   // e <= slice_size by construction
-  const uint64_t real_end = e < len ? e : len;
-  uint32_t crc = a < real_end ? crc_run(0u, p, a, real_end, T) : 0u;
-  // zero padding inside this lane's range (only when the slice is short)
-  const uint64_t zs = a > real_end ? a : real_end;
-  if (e > zs) crc = crc_zeros(crc, e - zs, T);
+  // the same for the whole workgroup
+  const bool rows = prm.rows != 0 && len == slice_size && ((uintptr_t)p & 15) == 0;
+  const uint32_t *level = rows ? prm.row_level : prm.level;
+  uint32_t crc;
+  if (rows) {
+    crc = crc_rows(p, prm.rows, T, Z);
+  } else {
+    // this lane's bytes in slice coordinates: [a, e) real or zero-padding
+    const int64_t va = (int64_t)lane * (int64_t)prm.chunk - (int64_t)prm.pad;
+    const int64_t ve = va + (int64_t)prm.chunk;
+    const uint64_t a = va < 0 ? 0 : (uint64_t)va;
+    const uint64_t e = ve < 0 ? 0 : (uint64_t)ve;   // e <= slice_size by construction
+    const uint64_t real_end = e < len ? e : len;
+    crc = a < real_end ? crc_run(0u, p, a, real_end, T) : 0u;
+    // zero padding inside this lane's range (only when the slice is short)
+    const uint64_t zs = a > real_end ? a : real_end;
+    if (e > zs) crc = crc_zeros(crc, e - zs, T);
+  }
 
   // tree join: level k merges block [l, l+2^k) with its right neighbour
   uint32_t v = crc;
 #pragma unroll
   for (int k = 0; k < 6; k++) {
     const uint32_t right = __shfl_down(v, 1 << k, 64);
-    if ((lane & ((2 << k) - 1)) == 0) v = gf2_mulmod(v, prm.level[k]) ^ right;
+    if ((lane & ((2 << k) - 1)) == 0) v = gf2_mulmod(v, level[k]) ^ right;
   }
   if ((lane & 63) == 0) part[lane >> 6] = v;
   __syncthreads();
   if (lane == 0) {
-    uint32_t w01 = gf2_mulmod(part[0], prm.level[6]) ^ part[1];
-    uint32_t w23 = gf2_mulmod(part[2], prm.level[6]) ^ part[3];
-    const uint32_t tot = gf2_mulmod(w01, prm.level[7]) ^ w23;
+    uint32_t w01 = gf2_mulmod(part[0], level[6]) ^ part[1];
+    uint32_t w23 = gf2_mulmod(part[2], level[6]) ^ part[3];
+    const uint32_t tot = gf2_mulmod(w01, level[7]) ^ w23;
     *out = ~(prm.init_term ^ tot);
   }
 }
 
-// T_j (j = 0..3, byte followed by j zero bytes) precomputed on the host:
-// one coalesced copy into LDS.
-__device__ __forceinline__ void crc_load_tables(uint32_t (*T)[256], const uint32_t *__restrict__ tab) {
-  for (int k = threadIdx.x; k < 1024; k += blockDim.x) (&T[0][0])[k] = tab[k];
+// T_j (j = 0..3, byte followed by j zero bytes) and, with ntab = 8, the row
+// skip tables Z_j after them, precomputed on the host: one coalesced copy into
+// LDS.
+__device__ __forceinline__ void crc_load_tables(uint32_t (*T)[256], const uint32_t *__restrict__ tab, int ntab = 4) {
+  for (int k = threadIdx.x; k < 256 * ntab; k += blockDim.x) (&T[0][0])[k] = tab[k];
   __syncthreads();
 }
 
@@ -735,29 +780,32 @@ __global__ __launch_bounds__(kCrcLanes) void k_crc32_slices(const uint8_t *__res
                                                             uint64_t stride, const uint64_t *__restrict__ lens,
                                                             const uint32_t *__restrict__ tab, CrcParams prm,
                                                             uint32_t *__restrict__ out) {
-  __shared__ uint32_t T[4][256];
+  __shared__ uint32_t T[8][256];  // T0..T3, Z0..Z3
   __shared__ uint32_t part[kCrcLanes / 64];
-  crc_load_tables(T, tab);
+  crc_load_tables(T, tab, 8);
   const long i = blockIdx.x;
   uint64_t len = lens ? lens[i] : slice_size;
   if (len > slice_size) len = slice_size;
-  crc_slice_wg(T, part, gptr(data) + (uint64_t)i * stride, len, slice_size, prm, gptr(out) + i);
+  crc_slice_wg(T, T + 4, part, gptr(data) + (uint64_t)i * stride, len, slice_size, prm, gptr(out) + i);
 }
 
 // Batched form: one workgroup per entry; entry e uses params[e.crc_params].
 __global__ __launch_bounds__(kCrcLanes) void k_crc32_list(const HashEntry *__restrict__ list,
                                                           const CrcParams *__restrict__ params,
                                                           const uint32_t *__restrict__ tab) {
-  __shared__ uint32_t T[4][256];
+  __shared__ uint32_t T[8][256];  // T0..T3, Z0..Z3
   __shared__ uint32_t part[kCrcLanes / 64];
-  crc_load_tables(T, tab);
+  crc_load_tables(T, tab, 8);
   const HashEntry e = list[blockIdx.x];
   const CrcParams prm = params[e.crc_params];
-  crc_slice_wg(T, part, gptr(e.data), e.len, e.size, prm, gptr(e.crc));
+  crc_slice_wg(T, T + 4, part, gptr(e.data), e.len, e.size, prm, gptr(e.crc));
 }
 
+static uint32_t host_mulmod(uint32_t a, uint32_t b);
+static uint32_t host_x8n(uint64_t nbytes);
+
 int32_t crc_tables_init(par2hip_ctx *ctx) {
-  uint32_t h[4][256];
+  uint32_t h[8][256];
   for (uint32_t k = 0; k < 256; k++) {
     uint32_t c = k;
     for (int b = 0; b < 8; b++) c = (c & 1) ? (c >> 1) ^ 0xEDB88320u : c >> 1;
@@ -765,6 +813,10 @@ int32_t crc_tables_init(par2hip_ctx *ctx) {
   }
   for (int j = 1; j < 4; j++)
     for (uint32_t k = 0; k < 256; k++) h[j][k] = (h[j - 1][k] >> 8) ^ h[0][h[j - 1][k] & 0xFF];
+  // Z_j[v] = (v << 8j) * x^(8 * (kCrcRow - kCrcPiece)): the row layout's skip
+  const uint32_t skip = host_x8n(kCrcRow - kCrcPiece);
+  for (int j = 0; j < 4; j++)
+    for (uint32_t k = 0; k < 256; k++) h[4 + j][k] = host_mulmod(k << (8 * j), skip);
   P2H_CHECK(ctx, hipMalloc(&ctx->d_crc_tab, sizeof h));
   P2H_CHECK(ctx, hipMemcpy(ctx->d_crc_tab, h, sizeof h, hipMemcpyHostToDevice));
   return PAR2HIP_OK;
@@ -811,6 +863,14 @@ CrcParams crc_params_for(uint64_t slice_size) {
     lv = host_mulmod(lv, lv);
   }
   prm.init_term = host_mulmod(0xFFFFFFFFu, host_x8n(slice_size));
+  // row layout when whole rows cover the slice (32-bit row count: slices stay
+  // under 2^44 bytes)
+  prm.rows = slice_size % kCrcRow == 0 && slice_size / kCrcRow < (1ull << 32) ? (uint32_t)(slice_size / kCrcRow) : 0;
+  lv = host_x8n(kCrcPiece);
+  for (int k = 0; k < 8; k++) {
+    prm.row_level[k] = lv;
+    lv = host_mulmod(lv, lv);
+  }
   return prm;
 }
 
