@@ -330,8 +330,13 @@ struct CrcParams {
   uint64_t pad;            // leading virtual zero bytes
   uint32_t level[8];       // x^(8*chunk*2^k) mod P, k = 0..7
   uint32_t init_term;      // 0xFFFFFFFF * x^(8*slice_size) mod P
-  uint32_t rows;           // slice_size / kCrcRow when that divides evenly (row layout), else 0
+  uint32_t rows;           // whole kCrcRow rows of the slice for the row layout (0: none)
   uint32_t row_level[8];   // x^(8*kCrcPiece*2^k) mod P, k = 0..7
+  // the bytes after the rows (fewer than kCrcRow), one chunk per lane:
+  uint64_t t_chunk;        // bytes per lane (multiple of 16; 0: no tail)
+  uint64_t t_pad;          // leading virtual zero bytes of the tail
+  uint32_t t_level[8];     // x^(8*t_chunk*2^k) mod P, k = 0..7
+  uint32_t t_shift;        // x^(8*tail bytes) mod P
 };
 CrcParams crc_params_for(uint64_t slice_size);
 // Pointers read from device memory are generic to the compiler, which then

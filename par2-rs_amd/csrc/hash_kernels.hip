@@ -682,8 +682,9 @@ __device__ __forceinline__ uint32_t crc_run(uint32_t crc, const P2H_GLOBAL uint8
   return crc;
 }
 
-// Row layout (slice_size a multiple of kCrcRow, 16-byte aligned, no short
-// length): lane l's register runs over its kCrcPiece-byte piece of each row,
+// Row layout (the whole kCrcRow rows of a full, 16-byte aligned slice; the
+// bytes after them take one chunk per lane): lane l's register runs over its
+// kCrcPiece-byte piece of each row,
 // the other lanes' bytes counted as zeros.  Skipping the kCrcRow - kCrcPiece
 // bytes between two of its pieces multiplies the register by
 // x^(8 * (kCrcRow - kCrcPiece)), four lookups in Z (Z_b[v] = (v << 8b) times
@@ -732,39 +733,53 @@ __device__ __forceinline__ void crc_slice_wg(const uint32_t (*T)[256], const uin
                                              uint64_t slice_size, const CrcParams &prm,
                                              P2H_GLOBAL uint32_t *__restrict__ out) {
   const int lane = threadIdx.x;
-  // the same for the whole workgroup
-  const bool rows = prm.rows != 0 && len == slice_size && ((uintptr_t)p & 15) == 0;
-  const uint32_t *level = rows ? prm.row_level : prm.level;
-  uint32_t crc;
-  if (rows) {
-    crc = crc_rows(p, prm.rows, T, Z);
-  } else {
-    // this lane's bytes in slice coordinates: [a, e) real or zero-padding
-    const int64_t va = (int64_t)lane * (int64_t)prm.chunk - (int64_t)prm.pad;
-    const int64_t ve = va + (int64_t)prm.chunk;
+  // tree join of the lanes' registers: level k merges block [l, l+2^k) with
+  // its right neighbour; the total lands in lane 0 (part free again after)
+  auto join = [&](uint32_t v, const uint32_t *level) -> uint32_t {
+#pragma unroll
+    for (int k = 0; k < 6; k++) {
+      const uint32_t right = __shfl_down(v, 1 << k, 64);
+      if ((lane & ((2 << k) - 1)) == 0) v = gf2_mulmod(v, level[k]) ^ right;
+    }
+    if ((lane & 63) == 0) part[lane >> 6] = v;
+    __syncthreads();
+    uint32_t tot = 0;
+    if (lane == 0) {
+      const uint32_t w01 = gf2_mulmod(part[0], level[6]) ^ part[1];
+      const uint32_t w23 = gf2_mulmod(part[2], level[6]) ^ part[3];
+      tot = gf2_mulmod(w01, level[7]) ^ w23;
+    }
+    __syncthreads();
+    return tot;
+  };
+  // lane l's bytes of chunk form (chunk, pad) over the n bytes at q, real up
+  // to len, zeros after
+  auto chunk_reg = [&](const P2H_GLOBAL uint8_t *__restrict__ q, uint64_t chunk, uint64_t pad, uint64_t len) {
+    // this lane's bytes: [a, e) real or zero-padding
+    const int64_t va = (int64_t)lane * (int64_t)chunk - (int64_t)pad;
+    const int64_t ve = va + (int64_t)chunk;
     const uint64_t a = va < 0 ? 0 : (uint64_t)va;
-    const uint64_t e = ve < 0 ? 0 : (uint64_t)ve;   // e <= slice_size by construction
+    const uint64_t e = ve < 0 ? 0 : (uint64_t)ve;  // e <= the chunk form's size by construction
     const uint64_t real_end = e < len ? e : len;
-    crc = a < real_end ? crc_run(0u, p, a, real_end, T) : 0u;
+    uint32_t c = a < real_end ? crc_run(0u, q, a, real_end, T) : 0u;
     // zero padding inside this lane's range (only when the slice is short)
     const uint64_t zs = a > real_end ? a : real_end;
-    if (e > zs) crc = crc_zeros(crc, e - zs, T);
-  }
-
-  // tree join: level k merges block [l, l+2^k) with its right neighbour
-  uint32_t v = crc;
-#pragma unroll
-  for (int k = 0; k < 6; k++) {
-    const uint32_t right = __shfl_down(v, 1 << k, 64);
-    if ((lane & ((2 << k) - 1)) == 0) v = gf2_mulmod(v, level[k]) ^ right;
-  }
-  if ((lane & 63) == 0) part[lane >> 6] = v;
-  __syncthreads();
-  if (lane == 0) {
-    uint32_t w01 = gf2_mulmod(part[0], level[6]) ^ part[1];
-    uint32_t w23 = gf2_mulmod(part[2], level[6]) ^ part[3];
-    const uint32_t tot = gf2_mulmod(w01, level[7]) ^ w23;
-    *out = ~(prm.init_term ^ tot);
+    if (e > zs) c = crc_zeros(c, e - zs, T);
+    return c;
+  };
+  // the same for the whole workgroup
+  if (prm.rows != 0 && len == slice_size && ((uintptr_t)p & 15) == 0) {
+    // whole rows, then the tail after them: raw(rows || tail) =
+    // raw(rows) x^(8 tail) + raw(tail)
+    const uint64_t head = (uint64_t)prm.rows * kCrcRow;
+    const uint32_t vr = crc_rows(p, prm.rows, T, Z);
+    const uint32_t vt = prm.t_chunk ? chunk_reg(p + head, prm.t_chunk, prm.t_pad, slice_size - head) : 0u;
+    const uint32_t tr = join(vr, prm.row_level);
+    const uint32_t tt = prm.t_chunk ? join(vt, prm.t_level) : 0u;
+    if (lane == 0) *out = ~(prm.init_term ^ (prm.t_chunk ? gf2_mulmod(tr, prm.t_shift) ^ tt : tr));
+  } else {
+    const uint32_t tot = join(chunk_reg(p, prm.chunk, prm.pad, len), prm.level);
+    if (lane == 0) *out = ~(prm.init_term ^ tot);
   }
 }
 
@@ -863,14 +878,23 @@ CrcParams crc_params_for(uint64_t slice_size) {
     lv = host_mulmod(lv, lv);
   }
   prm.init_term = host_mulmod(0xFFFFFFFFu, host_x8n(slice_size));
-  // row layout when whole rows cover the slice (32-bit row count: slices stay
-  // under 2^44 bytes)
-  prm.rows = slice_size % kCrcRow == 0 && slice_size / kCrcRow < (1ull << 32) ? (uint32_t)(slice_size / kCrcRow) : 0;
+  // row layout over the whole rows (32-bit row count: slices stay under 2^44
+  // bytes), the rest in one chunk per lane
+  prm.rows = slice_size / kCrcRow < (1ull << 32) ? (uint32_t)(slice_size / kCrcRow) : 0;
   lv = host_x8n(kCrcPiece);
   for (int k = 0; k < 8; k++) {
     prm.row_level[k] = lv;
     lv = host_mulmod(lv, lv);
   }
+  const uint64_t tail = slice_size - (uint64_t)prm.rows * kCrcRow;
+  prm.t_chunk = tail ? ((tail + kCrcLanes - 1) / kCrcLanes + 15) & ~(uint64_t)15 : 0;
+  prm.t_pad = tail ? prm.t_chunk * kCrcLanes - tail : 0;
+  lv = host_x8n(prm.t_chunk);
+  for (int k = 0; k < 8; k++) {
+    prm.t_level[k] = lv;
+    lv = host_mulmod(lv, lv);
+  }
+  prm.t_shift = host_x8n(tail);
   return prm;
 }
 

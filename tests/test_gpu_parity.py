@@ -203,12 +203,15 @@ def test_hash_dev_unaligned_bases(ctx, shift, n, s, stride):
         assert int(crc[i]) == zlib.crc32(b), f"crc slice {i}"
 
 
-@pytest.mark.parametrize("n,s", [(4, 16384), (3, 65536), (5, 196608), (2, 1 << 21)])
+@pytest.mark.parametrize("n,s", [(4, 16384), (3, 65536), (5, 196608), (2, 1 << 21), (3, 16388), (4, 70000),
+                                 (2, 3 * 16384 + 10000), (3, 16380)])
 def test_hash_whole_lane_chunks_aligned_and_offset(ctx, n, s):
-    """Slices of whole 16 KiB rows: at a 16-byte aligned base the CRC-32 runs
+    """At a 16-byte aligned base the CRC-32 runs the slice's whole 16 KiB rows
     in the row layout (lane l folds bytes [64 l, 64 l + 64) of every row and
-    skips the rest with the x^(8 * 16320) tables); at a 4-byte offset it runs
-    one contiguous chunk per lane.  Both hashes equal the stdlib's."""
+    skips the rest with the x^(8 * 16320) tables) and the bytes after them in
+    one chunk per lane, joined as raw(rows) x^(8 tail) + raw(tail); a slice
+    shorter than a row, and any slice at a 4-byte offset, runs one chunk per
+    lane.  Both hashes equal the stdlib's."""
     rng = np.random.default_rng(n * 7 + s)
     buf = rng.integers(0, 256, 16 + n * s, dtype=np.uint8)
     d = ctx.alloc(len(buf))
